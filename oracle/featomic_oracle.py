@@ -1,0 +1,683 @@
+"""FP64 CPU oracle for the featomic SOAP / LODE calculators.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker. The product package
+(``rascaline_amd``) never imports it.
+
+The numerical work lives in the C library built from this directory
+(``liboracle.so``, see ``oracle.h``); this module assembles the metatensor-style
+metadata (keys, samples, gradient samples, properties) with numpy, restating
+the reference's label builders:
+
+* SOAP spherical expansion keys / samples / gradient samples:
+  ``featomic/src/calculators/soap/spherical_expansion.rs:662-803``,
+  ``featomic/src/labels/keys.rs:61-107``,
+  ``featomic/src/labels/samples/atom_centered.rs:31-150``;
+* gradient row values: ``soap/spherical_expansion.rs:426-604``;
+* SOAP power spectrum: ``soap/power_spectrum.rs:82-767`` and ``labels/keys.rs:101-170``;
+* LODE: ``lode/spherical_expansion.rs:463-880``, ``labels/samples/long_range.rs:25-78``.
+
+Outputs are plain dictionaries::
+
+    {"keys_names": [...], "keys": int32[K, k],
+     "blocks": [{"values", "samples_names", "samples", "components": [(name, values)],
+                 "properties_names", "properties",
+                 "gradients": {parameter: {same fields}}}]}
+"""
+
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class OrcPair(ctypes.Structure):
+    _fields_ = [
+        ("first", ctypes.c_size_t),
+        ("second", ctypes.c_size_t),
+        ("distance", ctypes.c_double),
+        ("vector", ctypes.c_double * 3),
+        ("shift", ctypes.c_int32 * 3),
+    ]
+
+
+class SoapHypers(ctypes.Structure):
+    _fields_ = [
+        ("cutoff", ctypes.c_double),
+        ("smoothing", ctypes.c_int),
+        ("smoothing_width", ctypes.c_double),
+        ("gaussian_width", ctypes.c_double),
+        ("has_scaling", ctypes.c_int),
+        ("scaling_scale", ctypes.c_double),
+        ("scaling_rate", ctypes.c_double),
+        ("scaling_exponent", ctypes.c_double),
+        ("center_atom_weight", ctypes.c_double),
+        ("max_angular", ctypes.c_int),
+        ("n_radial", ctypes.c_int),
+        ("spline_accuracy", ctypes.c_double),
+    ]
+
+
+class LodeHypers(ctypes.Structure):
+    _fields_ = [
+        ("smearing", ctypes.c_double),
+        ("exponent", ctypes.c_int),
+        ("center_atom_weight", ctypes.c_double),
+        ("max_angular", ctypes.c_int),
+        ("n_radial", ctypes.c_int),
+        ("radius", ctypes.c_double),
+        ("k_cutoff", ctypes.c_double),
+        ("spline_accuracy", ctypes.c_double),
+    ]
+
+
+_LIB = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile next to this file."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        d, i32, sz, vp = ctypes.c_double, ctypes.c_int32, ctypes.c_size_t, ctypes.c_void_p
+        L.orc_gamma.restype = d
+        L.orc_gamma.argtypes = [d]
+        L.orc_ln_gamma.restype = d
+        L.orc_ln_gamma.argtypes = [d]
+        L.orc_hyp1f1.restype = d
+        L.orc_hyp1f1.argtypes = [d, d, d]
+        L.orc_hyp2f1.restype = d
+        L.orc_hyp2f1.argtypes = [d, d, d, d]
+        L.orc_expi.restype = d
+        L.orc_expi.argtypes = [d]
+        L.orc_spherical_harmonics.argtypes = [ctypes.c_int, vp, vp, vp]
+        L.orc_symmetric_eigen.argtypes = [ctypes.c_int, vp, vp]
+        L.orc_neighbors.restype = ctypes.c_int
+        L.orc_neighbors.argtypes = [vp, sz, vp, d, ctypes.POINTER(ctypes.POINTER(OrcPair)), ctypes.POINTER(sz)]
+        L.orc_free.argtypes = [vp]
+        L.orc_soap_radial_new.restype = vp
+        L.orc_soap_radial_new.argtypes = [d, d, ctypes.c_int, ctypes.c_int, d]
+        L.orc_lode_radial_new.restype = vp
+        L.orc_lode_radial_new.argtypes = [d, ctypes.c_int, ctypes.c_int, d, d]
+        L.orc_radial_compute.argtypes = [vp, ctypes.c_int, d, vp, vp]
+        L.orc_radial_n_knots.restype = ctypes.c_int
+        L.orc_radial_n_knots.argtypes = [vp, ctypes.c_int]
+        L.orc_radial_knots.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.orc_radial_orthonormalization.argtypes = [vp, vp]
+        L.orc_radial_free.argtypes = [vp]
+        L.orc_soap_new.restype = vp
+        L.orc_soap_new.argtypes = [ctypes.POINTER(SoapHypers)]
+        L.orc_soap_free.argtypes = [vp]
+        L.orc_soap_self_contribution.argtypes = [vp, vp]
+        L.orc_cutoff_smoothing.restype = d
+        L.orc_cutoff_smoothing.argtypes = [vp, d]
+        L.orc_cutoff_smoothing_gradient.restype = d
+        L.orc_cutoff_smoothing_gradient.argtypes = [vp, d]
+        L.orc_soap_se_system.restype = ctypes.c_int
+        L.orc_soap_se_system.argtypes = [vp, sz, vp, sz, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+        L.orc_soap_ps_batch.restype = ctypes.c_int64
+        L.orc_soap_ps_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(d)]
+        L.orc_lode_new.restype = vp
+        L.orc_lode_new.argtypes = [ctypes.POINTER(LodeHypers)]
+        L.orc_lode_free.argtypes = [vp]
+        L.orc_k_vectors.restype = ctypes.c_int
+        L.orc_k_vectors.argtypes = [vp, d, vp]
+        L.orc_lode_system.restype = ctypes.c_int
+        L.orc_lode_system.argtypes = [vp, sz, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ----------------------------------------------------------------------------
+# systems and hyper-parameters
+
+
+class System:
+    """positions [n, 3] (f64), types [n] (i32), cell [3, 3] (rows = cell vectors, zero = no pbc)."""
+
+    def __init__(self, positions, types, cell):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        self.types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        self.cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+
+    def __len__(self):
+        return len(self.types)
+
+
+def load_input(path):
+    """Read a ``*-input.json`` fixture (featomic/tests/data/save_data.py:15-52)."""
+    with open(path) as fd:
+        data = json.load(fd)
+    systems = [System(s["positions"], s["types"], np.array(s["cell"]).reshape(3, 3)) for s in data["systems"]]
+    return systems, data["hyperparameters"]
+
+
+def soap_hypers(h):
+    """JSON hyper-parameters -> SoapHypers (soap/spherical_expansion_pair.rs:37-60)."""
+    out = SoapHypers()
+    out.cutoff = float(h["cutoff"]["radius"])
+    sm = h["cutoff"]["smoothing"]
+    if sm["type"] == "ShiftedCosine":
+        out.smoothing = 1
+        out.smoothing_width = float(sm["width"])
+    elif sm["type"] == "Step":
+        out.smoothing = 0
+    else:
+        raise ValueError(sm)
+    dens = h["density"]
+    assert dens["type"] == "Gaussian"
+    out.gaussian_width = float(dens["width"])
+    sc = dens.get("scaling")
+    if sc is not None:
+        out.has_scaling = 1
+        out.scaling_scale = float(sc["scale"])
+        out.scaling_rate = float(sc["rate"])
+        out.scaling_exponent = float(sc["exponent"])
+    out.center_atom_weight = float(dens.get("center_atom_weight", 1.0))
+    basis = h["basis"]
+    assert basis["type"] == "TensorProduct"
+    out.max_angular = int(basis["max_angular"])
+    out.n_radial = int(basis["radial"]["max_radial"]) + 1
+    acc = basis.get("spline_accuracy", 1e-8)
+    out.spline_accuracy = -1.0 if acc is None else float(acc)
+    return out
+
+
+def lode_hypers(h):
+    out = LodeHypers()
+    dens = h["density"]
+    assert dens["type"] == "SmearedPowerLaw"
+    out.smearing = float(dens["smearing"])
+    out.exponent = int(dens["exponent"])
+    out.center_atom_weight = float(dens.get("center_atom_weight", 1.0))
+    basis = h["basis"]
+    out.max_angular = int(basis["max_angular"])
+    out.n_radial = int(basis["radial"]["max_radial"]) + 1
+    out.radius = float(basis["radial"]["radius"])
+    k_cutoff = h.get("k_cutoff")
+    out.k_cutoff = -1.0 if k_cutoff is None else float(k_cutoff)
+    acc = basis.get("spline_accuracy", 1e-8)
+    out.spline_accuracy = -1.0 if acc is None else float(acc)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# primitives
+
+
+def neighbors(positions, cell, cutoff):
+    """Half neighbour list, sorted by (first, second, shift). Returns a dict of arrays."""
+    L = lib()
+    positions = np.ascontiguousarray(positions, dtype=np.float64)
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    pp = ctypes.POINTER(OrcPair)()
+    n = ctypes.c_size_t()
+    L.orc_neighbors(_ptr(positions), len(positions), _ptr(cell), float(cutoff), ctypes.byref(pp), ctypes.byref(n))
+    count = n.value
+    dtype = np.dtype([("first", np.uint64), ("second", np.uint64), ("distance", np.float64),
+                      ("vector", np.float64, 3), ("shift", np.int32, 3), ("pad", np.int32)])
+    assert dtype.itemsize == ctypes.sizeof(OrcPair)
+    if count:
+        buf = ctypes.string_at(pp, count * dtype.itemsize)
+        arr = np.frombuffer(buf, dtype=dtype).copy()
+    else:
+        arr = np.zeros(0, dtype=dtype)
+    L.orc_free(ctypes.cast(pp, ctypes.c_void_p))
+    return arr
+
+
+def spherical_harmonics(lmax, direction, gradients=True):
+    direction = np.ascontiguousarray(direction, dtype=np.float64)
+    L2 = (lmax + 1) ** 2
+    values = np.zeros(L2)
+    grads = np.zeros((3, L2)) if gradients else None
+    lib().orc_spherical_harmonics(lmax, _ptr(direction), _ptr(values), _ptr(grads))
+    return values, grads
+
+
+class Radial:
+    def __init__(self, handle, max_angular, n):
+        self.handle = handle
+        self.max_angular = max_angular
+        self.n = n
+
+    @staticmethod
+    def soap(cutoff, width, max_angular, n_radial, spline_accuracy=None):
+        h = lib().orc_soap_radial_new(cutoff, width, max_angular, n_radial, -1.0 if spline_accuracy is None else spline_accuracy)
+        if not h:
+            raise ValueError("radial integral construction failed")
+        return Radial(h, max_angular, n_radial)
+
+    @staticmethod
+    def lode(radius, max_angular, n_radial, k_cutoff, spline_accuracy=None):
+        h = lib().orc_lode_radial_new(radius, max_angular, n_radial, k_cutoff, -1.0 if spline_accuracy is None else spline_accuracy)
+        if not h:
+            raise ValueError("radial integral construction failed")
+        return Radial(h, max_angular, n_radial)
+
+    def compute(self, l, r):
+        v = np.zeros(self.n)
+        g = np.zeros(self.n)
+        lib().orc_radial_compute(self.handle, l, float(r), _ptr(v), _ptr(g))
+        return v, g
+
+    def knots(self, l):
+        k = lib().orc_radial_n_knots(self.handle, l)
+        pos = np.zeros(k)
+        val = np.zeros((k, self.n))
+        der = np.zeros((k, self.n))
+        lib().orc_radial_knots(self.handle, l, _ptr(pos), _ptr(val), _ptr(der))
+        return pos, val, der
+
+    def orthonormalization(self):
+        out = np.zeros((self.n, self.n))
+        lib().orc_radial_orthonormalization(self.handle, _ptr(out))
+        return out
+
+    def __del__(self):
+        if self.handle and _LIB is not None:
+            _LIB.orc_radial_free(self.handle)
+            self.handle = None
+
+
+# ----------------------------------------------------------------------------
+# metadata helpers
+
+
+def _block(values, samples_names, samples, components, properties_names, properties):
+    return {
+        "values": values,
+        "samples_names": list(samples_names),
+        "samples": np.asarray(samples, dtype=np.int32).reshape(len(samples), len(samples_names)),
+        "components": components,
+        "properties_names": list(properties_names),
+        "properties": np.asarray(properties, dtype=np.int32).reshape(len(properties), len(properties_names)),
+        "gradients": {},
+    }
+
+
+def _species(systems):
+    return np.array(sorted({int(t) for s in systems for t in s.types}), dtype=np.int32)
+
+
+def _xyz():
+    return ("xyz", np.arange(3, dtype=np.int32).reshape(3, 1))
+
+
+class _SystemData:
+    """Neighbour list + per-atom neighbour sets for one system."""
+
+    def __init__(self, system, cutoff):
+        self.system = system
+        self.pairs = neighbors(system.positions, system.cell, cutoff)
+        n = len(system)
+        # pairs_containing(atom): the other atom of each pair (self image -> atom itself)
+        self.neighbors = [[] for _ in range(n)]
+        for p, pair in enumerate(self.pairs):
+            f, s = int(pair["first"]), int(pair["second"])
+            self.neighbors[f].append((s, p))
+            if s != f:
+                self.neighbors[s].append((f, p))
+            else:
+                self.neighbors[s].append((f, p))
+
+
+# ----------------------------------------------------------------------------
+# SOAP spherical expansion
+
+
+class SoapComputation:
+    """Runs the C oracle once per system and keeps the dense results."""
+
+    def __init__(self, systems, hypers, gradients=()):
+        self.hypers = hypers
+        self.h = soap_hypers(hypers)
+        self.lmax = self.h.max_angular
+        self.N = self.h.n_radial
+        self.L2 = (self.lmax + 1) ** 2
+        self.species = _species(systems)
+        self.S = len(self.species)
+        self.gradients = set(gradients)
+        handle = lib().orc_soap_new(ctypes.byref(self.h))
+        if not handle:
+            raise ValueError("invalid SOAP hyper-parameters")
+        self.data = []
+        try:
+            for system in systems:
+                sd = _SystemData(system, self.h.cutoff)
+                n, P = len(system), len(sd.pairs)
+                shape = (n, self.S, self.L2, self.N)
+                sd.values = np.zeros(shape)
+                sd.pair_grads = np.zeros((max(P, 1), 3, self.L2, self.N)) if "positions" in self.gradients else None
+                sd.self_grads = np.zeros((n, self.S, 3, self.L2, self.N)) if "positions" in self.gradients else None
+                sd.strain = np.zeros((n, self.S, 3, 3, self.L2, self.N)) if "strain" in self.gradients else None
+                sd.cell_g = np.zeros((n, self.S, 3, 3, self.L2, self.N)) if "cell" in self.gradients else None
+                pairs = np.ascontiguousarray(sd.pairs)
+                status = lib().orc_soap_se_system(
+                    handle, n, _ptr(system.types), P, _ptr(pairs) if P else None, self.S, _ptr(self.species),
+                    _ptr(sd.values), _ptr(sd.pair_grads), _ptr(sd.self_grads), _ptr(sd.strain), _ptr(sd.cell_g))
+                assert status == 0
+                self.data.append(sd)
+        finally:
+            lib().orc_soap_free(handle)
+
+    def tindex(self, t):
+        return int(np.searchsorted(self.species, t))
+
+    def se_position_gradient(self, sd, center, atom, nt_index, l):
+        """Gradient of c_center^{nt} w.r.t. r_atom, [3, 2l+1, N] (spherical_expansion.rs:476-521)."""
+        sl = slice(l * l, (l + 1) * (l + 1))
+        if center == atom:
+            return sd.self_grads[center, nt_index, :, sl, :]
+        out = np.zeros((3, 2 * l + 1, self.N))
+        m1l = -1.0 if l % 2 else 1.0
+        for (other, p) in sd.neighbors[center]:
+            if other != atom:
+                continue
+            pair = sd.pairs[p]
+            factor = 1.0 if int(pair["first"]) == center else -m1l
+            out += factor * sd.pair_grads[p, :, sl, :]
+        return out
+
+
+def soap_spherical_expansion(systems, hypers, gradients=()):
+    comp = SoapComputation(systems, hypers, gradients)
+    lmax, N = comp.lmax, comp.N
+
+    # keys: CenterSingleNeighborsTypesKeys (labels/keys.rs:69-106)
+    pairs_types = set()
+    for sd in comp.data:
+        t = sd.system.types
+        for pair in sd.pairs:
+            a, b = int(t[int(pair["first"])]), int(t[int(pair["second"])])
+            pairs_types.add((a, b))
+            pairs_types.add((b, a))
+        for x in t:
+            pairs_types.add((int(x), int(x)))
+    keys = []
+    blocks = []
+    for (ct, nt) in sorted(pairs_types):
+        # samples: AtomCenteredSamples, Single filters (atom_centered.rs:73-98)
+        samples = []
+        for si, sd in enumerate(comp.data):
+            t = sd.system.types
+            for i in range(len(t)):
+                if t[i] != ct:
+                    continue
+                ok = nt == ct
+                for (other, _) in sd.neighbors[i]:
+                    if t[other] == nt:
+                        ok = True
+                if ok:
+                    samples.append((si, i))
+        # gradient samples (atom_centered.rs:103-150)
+        grad_samples = []
+        for s_i, (si, i) in enumerate(samples):
+            sd = comp.data[si]
+            t = sd.system.types
+            nbrs = set()
+            if t[i] == nt:
+                nbrs.add(i)
+            for (other, _) in sd.neighbors[i]:
+                if t[other] == nt:
+                    nbrs.add(other)
+                    nbrs.add(i)
+            for j in sorted(nbrs):
+                grad_samples.append((s_i, si, j))
+        nti = comp.tindex(nt)
+        for l in range(lmax + 1):
+            keys.append((l, 1, ct, nt))
+            sl = slice(l * l, (l + 1) * (l + 1))
+            values = np.zeros((len(samples), 2 * l + 1, N))
+            for s_i, (si, i) in enumerate(samples):
+                values[s_i] = comp.data[si].values[i, nti, sl, :]
+            mu = ("o3_mu", np.arange(-l, l + 1, dtype=np.int32).reshape(-1, 1))
+            blk = _block(values, ["system", "atom"], samples, [mu], ["n"], np.arange(N).reshape(-1, 1))
+            if "positions" in comp.gradients:
+                gv = np.zeros((len(grad_samples), 3, 2 * l + 1, N))
+                for r, (s_i, si, j) in enumerate(grad_samples):
+                    i = samples[s_i][1]
+                    gv[r] = comp.se_position_gradient(comp.data[si], i, j, nti, l)
+                blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz(), mu], ["n"], np.arange(N).reshape(-1, 1))
+            for param, attr, c0 in (("cell", "cell_g", "abc"), ("strain", "strain", "xyz_1")):
+                if param not in comp.gradients:
+                    continue
+                gv = np.zeros((len(samples), 3, 3, 2 * l + 1, N))
+                for s_i, (si, i) in enumerate(samples):
+                    gv[s_i] = getattr(comp.data[si], attr)[i, nti, :, :, sl, :]
+                if param == "cell":
+                    comps = [_xyz(), ("abc", np.arange(3).reshape(3, 1)), mu]
+                else:
+                    comps = [("xyz_2", np.arange(3).reshape(3, 1)), ("xyz_1", np.arange(3).reshape(3, 1)), mu]
+                blk["gradients"][param] = _block(gv, ["sample"], np.arange(len(samples)).reshape(-1, 1), comps, ["n"], np.arange(N).reshape(-1, 1))
+            blocks.append(blk)
+    return {"keys_names": ["o3_lambda", "o3_sigma", "center_type", "neighbor_type"],
+            "keys": np.array(keys, dtype=np.int32).reshape(len(keys), 4), "blocks": blocks}
+
+
+# ----------------------------------------------------------------------------
+# SOAP power spectrum
+
+
+def soap_power_spectrum(systems, hypers, gradients=()):
+    comp = SoapComputation(systems, hypers, gradients)
+    lmax, N = comp.lmax, comp.N
+
+    # neighbour types around each atom, incl. itself (labels/keys.rs:116-143)
+    ntypes = []
+    for sd in comp.data:
+        t = sd.system.types
+        per_atom = []
+        for i in range(len(t)):
+            s = {int(t[i])}
+            for (other, _) in sd.neighbors[i]:
+                s.add(int(t[other]))
+            per_atom.append(s)
+        ntypes.append(per_atom)
+
+    keyset = set()
+    for si, sd in enumerate(comp.data):
+        t = sd.system.types
+        for i in range(len(t)):
+            for a in ntypes[si][i]:
+                for b in ntypes[si][i]:
+                    if b >= a:
+                        keyset.add((int(t[i]), a, b))
+
+    props = [(l, n1, n2) for l in range(lmax + 1) for n1 in range(N) for n2 in range(N)]
+    norms = np.array([(1.0 if l % 2 == 0 else -1.0) * math.sqrt(2 * l + 1) for l in range(lmax + 1)])
+
+    keys, blocks = [], []
+    for (ct, t1, t2) in sorted(keyset):
+        samples = []
+        for si, sd in enumerate(comp.data):
+            t = sd.system.types
+            for i in range(len(t)):
+                if t[i] == ct and t1 in ntypes[si][i] and t2 in ntypes[si][i]:
+                    samples.append((si, i))
+        kappa = math.sqrt(2.0) if t1 != t2 else 1.0
+        i1, i2 = comp.tindex(t1), comp.tindex(t2)
+        values = np.zeros((len(samples), len(props)))
+        for s_i, (si, i) in enumerate(samples):
+            c1 = comp.data[si].values[i, i1]
+            c2 = comp.data[si].values[i, i2]
+            row = []
+            for l in range(lmax + 1):
+                sl = slice(l * l, (l + 1) * (l + 1))
+                row.append((kappa * np.einsum("ma,mb->ab", c1[sl], c2[sl]) / norms[l]).reshape(-1))
+            values[s_i] = np.concatenate(row)
+        blk = _block(values, ["system", "atom"], samples, [], ["l", "n_1", "n_2"], props)
+
+        if "positions" in comp.gradients:
+            grad_samples = []
+            for s_i, (si, i) in enumerate(samples):
+                sd = comp.data[si]
+                t = sd.system.types
+                nbrs = set()
+                if t[i] in (t1, t2):
+                    nbrs.add(i)
+                for (other, _) in sd.neighbors[i]:
+                    if t[other] in (t1, t2):
+                        nbrs.add(other)
+                        nbrs.add(i)
+                for j in sorted(nbrs):
+                    grad_samples.append((s_i, si, j))
+            gv = np.zeros((len(grad_samples), 3, len(props)))
+            for r, (s_i, si, j) in enumerate(grad_samples):
+                sd = comp.data[si]
+                i = samples[s_i][1]
+                t = sd.system.types
+                c1 = sd.values[i, i1]
+                c2 = sd.values[i, i2]
+                row = []
+                for l in range(lmax + 1):
+                    sl = slice(l * l, (l + 1) * (l + 1))
+                    acc = np.zeros((3, N, N))
+                    # SE gradient sample exists in block t1 iff j == i or type_j == t1
+                    if j == i or t[j] == t1:
+                        g1 = comp.se_position_gradient(sd, i, j, i1, l)
+                        acc += np.einsum("mb,xma->xab", c2[sl], g1)
+                    if j == i or t[j] == t2:
+                        g2 = comp.se_position_gradient(sd, i, j, i2, l)
+                        acc += np.einsum("ma,xmb->xab", c1[sl], g2)
+                    row.append((kappa * acc / norms[l]).reshape(3, -1))
+                gv[r] = np.concatenate(row, axis=1)
+            blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz()], ["l", "n_1", "n_2"], props)
+
+        for param, attr in (("cell", "cell_g"), ("strain", "strain")):
+            if param not in comp.gradients:
+                continue
+            gv = np.zeros((len(samples), 3, 3, len(props)))
+            for s_i, (si, i) in enumerate(samples):
+                sd = comp.data[si]
+                c1, c2 = sd.values[i, i1], sd.values[i, i2]
+                g1, g2 = getattr(sd, attr)[i, i1], getattr(sd, attr)[i, i2]
+                row = []
+                for l in range(lmax + 1):
+                    sl = slice(l * l, (l + 1) * (l + 1))
+                    acc = np.einsum("mb,xyma->xyab", c2[sl], g1[:, :, sl]) + np.einsum("ma,xymb->xyab", c1[sl], g2[:, :, sl])
+                    row.append((kappa * acc / norms[l]).reshape(3, 3, -1))
+                gv[s_i] = np.concatenate(row, axis=2)
+            if param == "cell":
+                comps = [_xyz(), ("abc", np.arange(3).reshape(3, 1))]
+            else:
+                comps = [("xyz_2", np.arange(3).reshape(3, 1)), ("xyz_1", np.arange(3).reshape(3, 1))]
+            blk["gradients"][param] = _block(gv, ["sample"], np.arange(len(samples)).reshape(-1, 1), comps, ["l", "n_1", "n_2"], props)
+
+        keys.append((ct, t1, t2))
+        blocks.append(blk)
+    return {"keys_names": ["center_type", "neighbor_1_type", "neighbor_2_type"],
+            "keys": np.array(keys, dtype=np.int32).reshape(len(keys), 3), "blocks": blocks}
+
+
+def soap_ps_baseline(systems, hypers, n_threads=1):
+    """Timed CPU baseline: returns (number of PS gradient rows, checksum)."""
+    h = soap_hypers(hypers)
+    handle = lib().orc_soap_new(ctypes.byref(h))
+    species = _species(systems)
+    offsets = np.zeros(len(systems) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([len(s) for s in systems])
+    positions = np.ascontiguousarray(np.concatenate([s.positions for s in systems]))
+    types = np.ascontiguousarray(np.concatenate([s.types for s in systems]))
+    cells = np.ascontiguousarray(np.stack([s.cell.reshape(9) for s in systems]))
+    checksum = ctypes.c_double()
+    rows = lib().orc_soap_ps_batch(handle, len(systems), _ptr(offsets), _ptr(positions), _ptr(types), _ptr(cells),
+                                   len(species), _ptr(species), int(n_threads), ctypes.byref(checksum))
+    lib().orc_soap_free(handle)
+    return rows, checksum.value
+
+
+# ----------------------------------------------------------------------------
+# LODE
+
+
+def k_vectors(cell, k_cutoff):
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    n = lib().orc_k_vectors(_ptr(cell), float(k_cutoff), None)
+    out = np.zeros((n, 4))
+    lib().orc_k_vectors(_ptr(cell), float(k_cutoff), _ptr(out))
+    return out
+
+
+def lode_spherical_expansion(systems, hypers, gradients=()):
+    h = lode_hypers(hypers)
+    handle = lib().orc_lode_new(ctypes.byref(h))
+    if not handle:
+        raise ValueError("invalid LODE hyper-parameters")
+    lmax, N = h.max_angular, h.n_radial
+    L2 = (lmax + 1) ** 2
+    species = _species(systems)
+    S = len(species)
+    do_grad = "positions" in gradients
+    dense = []
+    try:
+        for system in systems:
+            n = len(system)
+            values = np.zeros((n, S, L2, N))
+            grads = np.zeros((n, n, 3, L2, N)) if do_grad else None
+            self_grads = np.zeros((n, S, 3, L2, N)) if do_grad else None
+            status = lib().orc_lode_system(handle, n, _ptr(system.positions), _ptr(system.types), _ptr(system.cell.reshape(9)),
+                                           S, _ptr(species), _ptr(values), _ptr(grads), _ptr(self_grads))
+            if status != 0:
+                raise ValueError(f"LODE oracle failed with status {status}")
+            dense.append((values, grads, self_grads))
+    finally:
+        lib().orc_lode_free(handle)
+
+    # AllTypesPairsKeys (labels/keys.rs:36-58)
+    keyset = set()
+    for s in systems:
+        for a in s.types:
+            for b in s.types:
+                keyset.add((int(a), int(b)))
+    keys, blocks = [], []
+    for (ct, nt) in sorted(keyset):
+        samples = []
+        for si, s in enumerate(systems):
+            if np.any(s.types == nt):
+                for i in range(len(s)):
+                    if s.types[i] == ct:
+                        samples.append((si, i))
+        grad_samples = []
+        for s_i, (si, i) in enumerate(samples):
+            for j in range(len(systems[si])):
+                if systems[si].types[j] == nt or j == i:
+                    grad_samples.append((s_i, si, j))
+        nti = int(np.searchsorted(species, nt))
+        for l in range(lmax + 1):
+            sl = slice(l * l, (l + 1) * (l + 1))
+            keys.append((l, 1, ct, nt))
+            values = np.zeros((len(samples), 2 * l + 1, N))
+            for s_i, (si, i) in enumerate(samples):
+                values[s_i] = dense[si][0][i, nti, sl]
+            mu = ("o3_mu", np.arange(-l, l + 1, dtype=np.int32).reshape(-1, 1))
+            blk = _block(values, ["system", "atom"], samples, [mu], ["n"], np.arange(N).reshape(-1, 1))
+            if do_grad:
+                gv = np.zeros((len(grad_samples), 3, 2 * l + 1, N))
+                for r, (s_i, si, j) in enumerate(grad_samples):
+                    i = samples[s_i][1]
+                    if i == j:
+                        gv[r] = dense[si][2][i, nti, :, sl]
+                    else:
+                        gv[r] = dense[si][1][i, j, :, sl]
+                blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz(), mu], ["n"], np.arange(N).reshape(-1, 1))
+            blocks.append(blk)
+    return {"keys_names": ["o3_lambda", "o3_sigma", "center_type", "neighbor_type"],
+            "keys": np.array(keys, dtype=np.int32).reshape(len(keys), 4), "blocks": blocks}
