@@ -1,0 +1,96 @@
+/*
+ * Extension entry points of librascaline_amd.so (no counterpart in the
+ * reference C API, which only exposes `featomic_calculator_compute` with host
+ * systems and a host TensorMap).
+ *
+ *  - precision control: the reference computes in float64 only; these
+ *    calculators can also run their device kernels in float32;
+ *  - a device-resident batch entry point for callers whose positions already
+ *    live in GPU memory and who consume the descriptor on the GPU (training
+ *    loops, benchmarks); outputs stay in library-owned device buffers;
+ *  - the neighbour list itself, in the reference's `featomic_pair_t` layout,
+ *    for callers that want the pairs (and for parity testing against the
+ *    reference's `NeighborsList`, featomic/src/systems/neighbors.rs:349-403).
+ */
+#ifndef FEATOMIC_AMD_H
+#define FEATOMIC_AMD_H
+
+#include <stdint.h>
+
+#include "featomic.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEATOMIC_AMD_FLOAT64 64
+#define FEATOMIC_AMD_FLOAT32 32
+
+/* Select the arithmetic of the device kernels (default FEATOMIC_AMD_FLOAT64).
+ * Outputs of featomic_calculator_compute are float64 arrays either way. */
+featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* calculator, int32_t bits);
+
+/* Half neighbour list of one system (pairs sorted by first, second, then
+ * cell shift), computed on the GPU. `positions` [n_atoms][3] and `cell` [9]
+ * are host arrays; a zero cell means no periodicity. The pairs are allocated
+ * by the library; release them with featomic_amd_free. */
+featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff,
+                                         featomic_pair_t** pairs, uintptr_t* n_pairs);
+
+void featomic_amd_free(void* pointer);
+
+/* A batch of systems whose positions and types are already in device memory. */
+typedef struct featomic_amd_device_systems_t {
+    uintptr_t n_systems;
+    const int64_t* atom_offsets;    /* host, [n_systems + 1] */
+    const double* positions;        /* device, [n_atoms][3] float64 */
+    const int32_t* types;           /* host, [n_atoms] atomic types */
+    const double* cells;            /* host, [n_systems][9] */
+} featomic_amd_device_systems_t;
+
+/* Per-block layout of a device-resident result: block b has n_samples[b]
+ * rows of `properties` values starting at values + value_offsets[b], and (when
+ * positions gradients were requested) n_gradient_rows[b] rows of
+ * 3 * properties values starting at gradients + gradient_offsets[b]. For the
+ * power spectrum, `properties` is the same for every block; for the spherical
+ * expansion one block per (o3_lambda, center_type, neighbor_type) key is
+ * reported and `properties` is (2 o3_lambda + 1) * n for that block. */
+typedef struct featomic_amd_device_output_t {
+    uintptr_t n_blocks;
+    const int32_t* keys;               /* host, [n_blocks][key size] */
+    uintptr_t key_size;
+    const int64_t* n_samples;          /* host */
+    const int64_t* n_gradient_rows;    /* host */
+    const int64_t* value_offsets;      /* host, in elements */
+    const int64_t* gradient_offsets;   /* host, in elements */
+    const int64_t* block_properties;   /* host, values per sample of each block */
+    const void* values;                /* device, float32 or float64 */
+    const void* gradients;             /* device, or NULL */
+    const int32_t* sample_labels;      /* device, [total samples][2] (system, atom) */
+    const int32_t* gradient_labels;    /* device, [total rows][3] (sample, system, atom) or NULL */
+    int32_t precision;                 /* FEATOMIC_AMD_FLOAT32 or FEATOMIC_AMD_FLOAT64 */
+} featomic_amd_device_output_t;
+
+/* Compute the calculator on a device-resident batch. `gradients` may hold
+ * "positions". The result stays valid until the next call on this calculator
+ * or its release. All work is queued on `stream` (a hipStream_t, NULL for the
+ * default stream); the call synchronises the stream once after the metadata
+ * pass (to size the outputs) and returns without waiting for the kernels that
+ * fill values and gradients. */
+featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
+                                              const featomic_amd_device_systems_t* systems,
+                                              const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                              featomic_amd_device_output_t* output);
+
+/* Kernel-level timing of the last featomic_amd_compute_device call, measured
+ * with hipEvents on the compute stream: total device time and the time of the
+ * dominant output kernel (power spectrum gradient rows / values), in ms.
+ * Requires a stream synchronisation, which this call performs. */
+featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
+                                            int64_t* main_kernel_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

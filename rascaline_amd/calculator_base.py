@@ -1,0 +1,159 @@
+"""Python side of the calculators, mirroring featomic's ``CalculatorBase``
+(python/featomic/featomic/calculator_base.py:60-314) over the C API of
+librascaline_amd.so."""
+
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+
+from . import _c_lib as C
+from .systems import PAIR_DTYPE, _SystemAdapter, wrap_system
+from .tensormap import Labels, TensorBlock, TensorMap
+
+
+def _labels(raw):
+    names = [raw.names[k].decode("utf8") for k in range(raw.size)]
+    count = raw.count
+    if count and raw.size:
+        values = np.ctypeslib.as_array(raw.values, shape=(count * raw.size,)).reshape(count, raw.size).copy()
+    else:
+        values = np.zeros((count, raw.size), dtype=np.int32)
+    return Labels(names, values)
+
+
+def _read_block(ptr):
+    L = C.lib()
+    arr = C.mts_array_t()
+    C.check_mts(L.mts_block_data(ptr, ctypes.byref(arr)))
+    shape_ptr = ctypes.POINTER(ctypes.c_size_t)()
+    ndim = ctypes.c_size_t()
+    C.check_mts(arr.shape(arr.ptr, ctypes.byref(shape_ptr), ctypes.byref(ndim)))
+    shape = tuple(shape_ptr[k] for k in range(ndim.value))
+    data_ptr = ctypes.POINTER(ctypes.c_double)()
+    C.check_mts(arr.data(arr.ptr, ctypes.byref(data_ptr)))
+    size = int(np.prod(shape)) if shape else 0
+    if size:
+        values = np.ctypeslib.as_array(data_ptr, shape=(size,)).reshape(shape).copy()
+    else:
+        values = np.zeros(shape)
+    labels = []
+    for axis in range(len(shape)):
+        raw = C.mts_labels_t()
+        C.check_mts(L.mts_block_labels(ptr, axis, ctypes.byref(raw)))
+        labels.append(_labels(raw))
+        L.mts_labels_free(ctypes.byref(raw))
+    block = TensorBlock(values, labels[0], labels[1:-1], labels[-1])
+    names = ctypes.POINTER(ctypes.c_char_p)()
+    count = ctypes.c_size_t()
+    C.check_mts(L.mts_block_gradients_list(ptr, ctypes.byref(names), ctypes.byref(count)))
+    for k in range(count.value):
+        param = names[k]
+        g = ctypes.c_void_p()
+        C.check_mts(L.mts_block_gradient(ptr, param, ctypes.byref(g)))
+        block.add_gradient(param.decode("utf8"), _read_block(g))
+    return block
+
+
+def tensormap_from_ptr(ptr):
+    """Copy a library-owned mts_tensormap_t into Python, then release it."""
+    L = C.lib()
+    try:
+        raw = C.mts_labels_t()
+        C.check_mts(L.mts_tensormap_keys(ptr, ctypes.byref(raw)))
+        keys = _labels(raw)
+        blocks = []
+        for b in range(len(keys)):
+            bp = ctypes.c_void_p()
+            C.check_mts(L.mts_tensormap_block_by_id(ptr, ctypes.byref(bp), b))
+            blocks.append(_read_block(bp))
+        return TensorMap(keys, blocks)
+    finally:
+        L.mts_tensormap_free(ptr)
+
+
+class CalculatorBase:
+    """Base class of all calculators (featomic ``CalculatorBase``)."""
+
+    def __init__(self, name: str, parameters: str):
+        self._lib = C.lib()
+        self._as_parameter_ = None
+        ptr = self._lib.featomic_calculator(name.encode("utf8"), parameters.encode("utf8"))
+        if not ptr:
+            message = self._lib.featomic_last_error().decode("utf8")
+            raise C.FeatomicError(message)
+        self._as_parameter_ = ctypes.c_void_p(ptr)
+
+    def __del__(self):
+        if getattr(self, "_as_parameter_", None) is not None and C._LIB is not None:
+            C._LIB.featomic_calculator_free(self._as_parameter_)
+            self._as_parameter_ = None
+
+    @property
+    def name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        C.check(self._lib.featomic_calculator_name(self._as_parameter_, buf, 256))
+        return buf.value.decode("utf8")
+
+    @property
+    def parameters(self) -> str:
+        size = 1024
+        while True:
+            buf = ctypes.create_string_buffer(size)
+            status = self._lib.featomic_calculator_parameters(self._as_parameter_, buf, size)
+            if status == 254:
+                size *= 2
+                continue
+            C.check(status)
+            return buf.value.decode("utf8")
+
+    @property
+    def cutoffs(self) -> List[float]:
+        ptr = ctypes.POINTER(ctypes.c_double)()
+        count = ctypes.c_size_t()
+        C.check(self._lib.featomic_calculator_cutoffs(self._as_parameter_, ctypes.byref(ptr), ctypes.byref(count)))
+        return [ptr[k] for k in range(count.value)]
+
+    def set_precision(self, bits: int):
+        """Arithmetic of the GPU kernels: 64 (default) or 32 bits."""
+        C.check(self._lib.featomic_amd_calculator_set_precision(self._as_parameter_, int(bits)))
+
+    def compute(self, systems, *, gradients: Optional[List[str]] = None, use_native_system: bool = True,
+                selected_samples=None, selected_properties=None, selected_keys=None) -> TensorMap:
+        """Run the calculation (python/featomic/featomic/calculator_base.py:187-314)."""
+        if not isinstance(systems, list):
+            systems = [systems]
+        adapters = [_SystemAdapter(wrap_system(s)) for s in systems]
+        array = (C.featomic_system_t * len(adapters))(*[a.struct for a in adapters])
+        gradients = list(gradients or [])
+        grads = (ctypes.c_char_p * max(len(gradients), 1))(*[g.encode("utf8") for g in gradients])
+        if selected_samples is not None or selected_properties is not None or selected_keys is not None:
+            raise C.FeatomicError("samples, properties and keys selection are not supported by this GPU implementation", 1)
+        options = C.featomic_calculation_options_t()
+        options.gradients = ctypes.cast(grads, ctypes.POINTER(ctypes.c_char_p))
+        options.gradients_count = len(gradients)
+        options.use_native_system = bool(use_native_system)
+        out = ctypes.c_void_p()
+        C.check(self._lib.featomic_calculator_compute(self._as_parameter_, ctypes.byref(out), array, len(adapters), options))
+        if not out.value:
+            return None
+        return tensormap_from_ptr(out)
+
+
+def neighbors(positions, cell, cutoff):
+    """Half neighbour list computed on the GPU, as ``featomic_pair_t`` records."""
+    L = C.lib()
+    positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    ptr = ctypes.POINTER(C.featomic_pair_t)()
+    count = ctypes.c_size_t()
+    C.check(L.featomic_amd_neighbors(positions.ctypes.data, len(positions), cell.ctypes.data, float(cutoff),
+                                     ctypes.byref(ptr), ctypes.byref(count)))
+    try:
+        n = count.value
+        if n == 0:
+            return np.zeros(0, dtype=PAIR_DTYPE)
+        raw = ctypes.string_at(ptr, n * PAIR_DTYPE.itemsize)
+        return np.frombuffer(raw, dtype=PAIR_DTYPE).copy()
+    finally:
+        L.featomic_amd_free(ptr)

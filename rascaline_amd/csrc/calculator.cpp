@@ -1,0 +1,673 @@
+// Calculator construction and the SOAP device pipeline.
+#include "calculator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "special.hpp"
+
+namespace rascaline_amd {
+
+// ---------------------------------------------------------------------------- workspace
+
+Workspace::~Workspace() { release(); }
+
+void* Workspace::get(const std::string& name, size_t bytes) {
+    Slot& slot = slots_[name];
+    if (bytes == 0) bytes = 16;
+    if (slot.bytes < bytes) {
+        if (slot.ptr) RA_HIP_CHECK(hipFree(slot.ptr));
+        slot.ptr = nullptr;
+        // grow with some slack so that repeated calls with similar sizes reuse the buffer
+        const size_t target = bytes + bytes / 8;
+        RA_HIP_CHECK(hipMalloc(&slot.ptr, target));
+        slot.bytes = target;
+    }
+    return slot.ptr;
+}
+
+void Workspace::release() {
+    for (auto& kv : slots_) {
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    }
+    slots_.clear();
+}
+
+namespace {
+
+int device_lds_limit() {
+    static int limit = -1;
+    if (limit < 0) {
+        int dev = 0;
+        RA_HIP_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        RA_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        limit = static_cast<int>(prop.sharedMemPerBlock);
+        if (limit <= 0) limit = 65536;
+    }
+    return limit;
+}
+
+template <typename T>
+void upload(T* dst, const std::vector<T>& src, hipStream_t stream) {
+    if (!src.empty()) RA_HIP_CHECK(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, stream));
+}
+
+template <typename T>
+void download(std::vector<T>& dst, const T* src, size_t count, hipStream_t stream) {
+    dst.resize(count);
+    if (count) RA_HIP_CHECK(hipMemcpyAsync(dst.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost, stream));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- construction
+
+Calculator::Calculator(const std::string& registry_name, const std::string& parameters)
+    : registry_name_(registry_name), parameters_(parameters) {
+    if (registry_name == "spherical_expansion") {
+        kind_ = Kind::SphericalExpansion;
+        soap_ = parse_soap_params(parameters, true);
+    } else if (registry_name == "soap_power_spectrum") {
+        kind_ = Kind::PowerSpectrum;
+        soap_ = parse_soap_params(parameters, false);
+    } else if (registry_name == "lode_spherical_expansion") {
+        kind_ = Kind::LodeSphericalExpansion;
+        lode_ = parse_lode_params(parameters);
+    } else if (registry_name == "atomic_composition" || registry_name == "dummy_calculator" ||
+               registry_name == "neighbor_list" || registry_name == "sorted_distances" ||
+               registry_name == "spherical_expansion_by_pair" || registry_name == "soap_radial_spectrum") {
+        throw Error::invalid_parameter("the '" + registry_name + "' calculator is not provided by this GPU implementation");
+    } else {
+        throw Error::invalid_parameter("unknown calculator with name '" + registry_name + "'");
+    }
+
+    if (kind_ == Kind::LodeSphericalExpansion) {
+        tables_ = build_lode_radial(lode_);
+        self_contribution_ = lode_center_contribution(lode_, tables_);
+    } else {
+        cutoffs_.push_back(soap_.cutoff);
+        tables_ = build_soap_radial(soap_);
+        // self contribution, soap/spherical_expansion_pair.rs:222-252:
+        // w * f(0) * Y_00 * I_n0(0)
+        std::vector<double> ri(soap_.n_radial), rg(soap_.n_radial);
+        if (tables_.use_spline) {
+            const SplineChannel& ch = tables_.channels[0];
+            for (int k = 0; k < soap_.n_radial; k++) ri[k] = ch.values[k];
+        } else {
+            tables_.exact(0, 0.0, ri.data(), rg.data());
+        }
+        double f0 = 1.0;
+        if (soap_.has_scaling) {
+            f0 = soap_.scaling_rate / (soap_.scaling_rate + std::pow(0.0 / soap_.scaling_scale, soap_.scaling_exponent));
+        }
+        // smoothing(0) is 1 for both cutoff functions (cutoff > width)
+        double fc = 1.0;
+        if (soap_.smoothing == Smoothing::ShiftedCosine && 0.0 > soap_.cutoff - soap_.smoothing_width) {
+            fc = 0.5 * (1.0 + std::cos(RA_PI * (0.0 - soap_.cutoff + soap_.smoothing_width) / soap_.smoothing_width));
+        }
+        const double y00 = 0.3989422804014327 / 1.4142135623730951;
+        self_contribution_.resize(soap_.n_radial);
+        for (int k = 0; k < soap_.n_radial; k++) {
+            self_contribution_[k] = soap_.center_atom_weight * (f0 * fc) * y00 * ri[k];
+        }
+    }
+}
+
+Calculator::~Calculator() {
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    if (ev_stop_) (void)hipEventDestroy(ev_stop_);
+    if (ev_main_start_) (void)hipEventDestroy(ev_main_start_);
+    if (ev_main_stop_) (void)hipEventDestroy(ev_main_stop_);
+}
+
+const char* Calculator::name() const {
+    switch (kind_) {
+    case Kind::SphericalExpansion: return "spherical expansion";
+    case Kind::PowerSpectrum: return "SOAP power spectrum";
+    case Kind::LodeSphericalExpansion: return "lode spherical expansion";
+    }
+    return "";
+}
+
+// ---------------------------------------------------------------------------- device tables
+
+template <typename T>
+struct Calculator::DeviceTables {
+    DeviceRadial<T> radial;
+    T* sh_a = nullptr;
+    T* sh_b = nullptr;
+    T* self_c = nullptr;
+    std::vector<void*> owned;
+
+    ~DeviceTables() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+
+    template <typename U>
+    U* put(const std::vector<U>& v) {
+        void* p = nullptr;
+        RA_HIP_CHECK(hipMalloc(&p, sizeof(U) * std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) RA_HIP_CHECK(hipMemcpy(p, v.data(), sizeof(U) * v.size(), hipMemcpyHostToDevice));
+        owned.push_back(p);
+        return static_cast<U*>(p);
+    }
+};
+
+template <typename T>
+const Calculator::DeviceTables<T>& Calculator::device_tables() {
+    std::shared_ptr<void>& slot = sizeof(T) == 4 ? tables_f32_ : tables_f64_;
+    if (!slot) {
+        auto t = std::make_shared<DeviceTables<T>>();
+        const RadialTables& rt = tables_;
+        const int L = rt.max_angular + 1, n = rt.n;
+        std::vector<int> koff(L, 0), nk(L, 0);
+        std::vector<T> kpos, kval, kder;
+        if (rt.use_spline) {
+            int off = 0;
+            for (int l = 0; l < L; l++) {
+                const SplineChannel& ch = rt.channels[l];
+                koff[l] = off;
+                nk[l] = static_cast<int>(ch.positions.size());
+                off += nk[l];
+                for (double x : ch.positions) kpos.push_back(static_cast<T>(x));
+                for (double x : ch.values) kval.push_back(static_cast<T>(x));
+                for (double x : ch.derivatives) kder.push_back(static_cast<T>(x));
+            }
+        }
+        t->radial.max_angular = rt.max_angular;
+        t->radial.n = n;
+        t->radial.use_spline = rt.use_spline ? 1 : 0;
+        t->radial.stop = rt.stop;
+        t->radial.sigma2_atomic = rt.sigma2_atomic;
+        t->radial.gto = t->put(rt.gto);
+        t->radial.ortho = t->put(rt.ortho);
+        t->radial.knot_offset = t->put(koff);
+        t->radial.n_knots = t->put(nk);
+        t->radial.knot_pos = t->put(kpos);
+        t->radial.knot_val = t->put(kval);
+        t->radial.knot_der = t->put(kder);
+        // Legendre recursion coefficients, featomic/src/math/spherical_harmonics.rs:206-216
+        std::vector<T> A(L * (L + 1) / 2 + 1, 0), B(L * (L + 1) / 2 + 1, 0);
+        for (int l = 2; l < L; l++) {
+            const double ls = l * l, lm1s = (l - 1) * (l - 1);
+            for (int m = 0; m < l - 1; m++) {
+                const double ms = m * m;
+                A[l * (l + 1) / 2 + m] = static_cast<T>(std::sqrt((4.0 * ls - 1.0) / (ls - ms)));
+                B[l * (l + 1) / 2 + m] = static_cast<T>(-std::sqrt((lm1s - ms) / (4.0 * lm1s - 1.0)));
+            }
+        }
+        t->sh_a = t->put(A);
+        t->sh_b = t->put(B);
+        std::vector<T> sc;
+        for (double x : self_contribution_) sc.push_back(static_cast<T>(x));
+        t->self_c = t->put(sc);
+        slot = t;
+    }
+    return *static_cast<DeviceTables<T>*>(slot.get());
+}
+
+// ---------------------------------------------------------------------------- SOAP pipeline
+
+namespace {
+
+struct Species {
+    std::vector<int32_t> types;  // sorted unique atomic types
+    std::vector<int32_t> index;  // per atom
+};
+
+Species species_of(const HostBatch& batch) {
+    Species s;
+    s.types = batch.types;
+    std::sort(s.types.begin(), s.types.end());
+    s.types.erase(std::unique(s.types.begin(), s.types.end()), s.types.end());
+    if (s.types.size() > 32) throw Error::invalid_parameter("at most 32 different atomic types are supported");
+    s.index.resize(batch.types.size());
+    for (size_t i = 0; i < batch.types.size(); i++) {
+        s.index[i] = static_cast<int32_t>(std::lower_bound(s.types.begin(), s.types.end(), batch.types[i]) - s.types.begin());
+    }
+    return s;
+}
+
+std::unique_ptr<mts_block_t> make_block(LabelsData samples, std::vector<LabelsData> components, LabelsData properties,
+                                        std::vector<uintptr_t> shape) {
+    auto b = std::make_unique<mts_block_t>();
+    b->samples = std::move(samples);
+    b->components = std::move(components);
+    b->properties = std::move(properties);
+    b->array.shape = std::move(shape);
+    size_t total = 1;
+    for (auto s : b->array.shape) total *= s;
+    b->array.data.assign(total, 0.0);
+    return b;
+}
+
+LabelsData mu_labels(int l) {
+    std::vector<int32_t> v;
+    for (int m = -l; m <= l; m++) v.push_back(m);
+    return LabelsData({"o3_mu"}, v);
+}
+
+}  // namespace
+
+template <typename T>
+mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& grads, hipStream_t stream, bool to_host,
+                                      featomic_amd_device_output_t* output) {
+    const bool ps = kind_ == Kind::PowerSpectrum;
+    const int n_atoms = static_cast<int>(batch.n_atoms());
+    const Species sp = species_of(batch);
+    const int S = static_cast<int>(sp.types.size());
+    const int L = soap_.L(), n = soap_.n_radial, L2 = soap_.L2();
+    const size_t slots = static_cast<size_t>(L2) * n;
+    const int P = L * n * n;
+
+    if (!ev_start_) {
+        RA_HIP_CHECK(hipEventCreate(&ev_start_));
+        RA_HIP_CHECK(hipEventCreate(&ev_stop_));
+        RA_HIP_CHECK(hipEventCreate(&ev_main_start_));
+        RA_HIP_CHECK(hipEventCreate(&ev_main_stop_));
+    }
+    RA_HIP_CHECK(hipEventRecord(ev_start_, stream));
+
+    auto tm = to_host ? std::make_unique<mts_tensormap_t>() : nullptr;
+    if (tm) {
+        tm->keys = ps ? LabelsData({"center_type", "neighbor_1_type", "neighbor_2_type"}, {})
+                      : LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, {});
+    }
+    if (n_atoms == 0) {
+        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+        main_bytes_ = 0;
+        if (output) {
+            device_result_ = DeviceResult();
+            std::memset(output, 0, sizeof(*output));
+            output->key_size = ps ? 3 : 4;
+            output->precision = sizeof(T) == 4 ? 32 : 64;
+        }
+        return tm.release();
+    }
+
+    // ---- geometry upload
+    std::vector<int32_t> atom_system(n_atoms), atom_local(n_atoms);
+    for (size_t s = 0; s < batch.n_systems(); s++) {
+        for (int64_t a = batch.offsets[s]; a < batch.offsets[s + 1]; a++) {
+            atom_system[a] = static_cast<int32_t>(s);
+            atom_local[a] = static_cast<int32_t>(a - batch.offsets[s]);
+        }
+    }
+    const double* d_positions = batch.device_positions;
+    if (!d_positions) {
+        double* p = ws_.get<double>("positions", 3 * static_cast<size_t>(n_atoms));
+        upload(p, batch.positions, stream);
+        d_positions = p;
+    }
+    int* d_atom_system = ws_.get<int>("atom_system", n_atoms);
+    int* d_atom_local = ws_.get<int>("atom_local", n_atoms);
+    int* d_species = ws_.get<int>("species", n_atoms);
+    upload(d_atom_system, atom_system, stream);
+    upload(d_atom_local, atom_local, stream);
+    upload(d_species, sp.index, stream);
+
+    // ---- neighbour list
+    const DeviceNeighbors nl = build_neighbors(ws_, batch, soap_.cutoff, d_positions, d_atom_system, stream);
+
+    // ---- metadata: block membership, sample and gradient-row offsets
+    unsigned* type_mask = ws_.get<unsigned>("type_mask", n_atoms);
+    int* unique_count = ws_.get<int>("unique_count", static_cast<size_t>(n_atoms) * S);
+    launch_center_meta(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, unique_count);
+    const int npairs = S * (S + 1) / 2;
+    const int NB = ps ? S * npairs : S * S;
+    std::vector<int32_t> block_types;  // (ct, t1, t2) per block, species indices
+    for (int ct = 0; ct < S; ct++) {
+        if (ps) {
+            for (int t1 = 0; t1 < S; t1++)
+                for (int t2 = t1; t2 < S; t2++) block_types.insert(block_types.end(), {ct, t1, t2});
+        } else {
+            for (int nt = 0; nt < S; nt++) block_types.insert(block_types.end(), {ct, nt, nt});
+        }
+    }
+    int* d_block_types = ws_.get<int>("block_types", block_types.size());
+    upload(d_block_types, block_types, stream);
+    const size_t NBN = static_cast<size_t>(NB) * n_atoms;
+    long long* sample_flag = ws_.get<long long>("sample_flag", NBN);
+    long long* row_count = ws_.get<long long>("row_count", NBN);
+    long long* sample_scan = ws_.get<long long>("sample_scan", NBN + 1);
+    long long* row_scan = ws_.get<long long>("row_scan", NBN + 1);
+    launch_block_membership(stream, n_atoms, S, ps ? 1 : 0, NB, d_species, type_mask, unique_count, d_block_types,
+                            sample_flag, row_count);
+    exclusive_scan(stream, sample_flag, sample_scan, static_cast<long long>(NBN));
+    exclusive_scan(stream, row_count, row_scan, static_cast<long long>(NBN));
+    long long* d_totals = ws_.get<long long>("block_totals", 2 * (NB + 1));
+    launch_gather_block_totals(stream, NB, n_atoms, sample_scan, row_scan, d_totals);
+    std::vector<long long> totals;
+    download(totals, d_totals, 2 * (NB + 1), stream);
+    RA_HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<long long> block_sample_base(NB + 1), block_row_base(NB + 1);
+    for (int b = 0; b <= NB; b++) {
+        block_sample_base[b] = totals[2 * b];
+        block_row_base[b] = totals[2 * b + 1];
+    }
+    const long long total_samples = block_sample_base[NB];
+    const long long total_rows = block_row_base[NB];
+    long long* d_block_sample_base = ws_.get<long long>("block_sample_base", NB + 1);
+    upload(d_block_sample_base, block_sample_base, stream);
+
+    // ---- kernel launch parameters
+    const DeviceTables<T>& tab = device_tables<T>();
+    SoapLaunch<T> lp;
+    lp.stream = stream;
+    lp.n_atoms = n_atoms;
+    lp.n_species = S;
+    lp.L = L;
+    lp.n = n;
+    lp.offsets = nl.offsets;
+    lp.entry_j = nl.entry_j;
+    lp.entry_shift = nl.entry_shift;
+    lp.entry_vec = nl.entry_vec;
+    lp.entry_dist = nl.entry_dist;
+    lp.species = d_species;
+    lp.radial = tab.radial;
+    lp.scalars.cutoff = soap_.cutoff;
+    lp.scalars.smoothing = soap_.smoothing == Smoothing::ShiftedCosine ? 1 : 0;
+    lp.scalars.width = soap_.smoothing_width;
+    lp.scalars.has_scaling = soap_.has_scaling ? 1 : 0;
+    lp.scalars.scale = soap_.scaling_scale;
+    lp.scalars.rate = soap_.scaling_rate;
+    lp.scalars.exponent = soap_.scaling_exponent;
+    lp.sh_a = tab.sh_a;
+    lp.sh_b = tab.sh_b;
+    lp.self_contribution = tab.self_c;
+    int* error_flag = ws_.get<int>("soap_error", 1);
+    RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
+    lp.error_flag = error_flag;
+
+    const int lds = device_lds_limit();
+    const size_t per_entry = soap_chunk_bytes<T>(L, n, 1);
+    auto chunk_for = [&](size_t fixed) -> int {
+        const long long budget = static_cast<long long>(lds) - static_cast<long long>(fixed) - 1024;
+        const long long c = budget / static_cast<long long>(per_entry);
+        if (c < 1) {
+            throw Error::invalid_parameter("max_angular / max_radial too large for the GPU kernels' local memory (" +
+                                           std::to_string(fixed + per_entry) + " bytes needed, " + std::to_string(lds) +
+                                           " available)");
+        }
+        return static_cast<int>(std::min<long long>(c, 64));
+    };
+
+    // ---- spherical expansion per center (dense)
+    T* coef = ws_.get<T>("coef", static_cast<size_t>(n_atoms) * S * slots);
+    T* dself = grads.positions ? ws_.get<T>("dself", static_cast<size_t>(n_atoms) * S * 3 * slots) : nullptr;
+    lp.chunk = chunk_for(2048 * 4 + 33 * 4);
+    launch_soap_center<T>(lp, 0, coef, dself);
+    T* dstrain = nullptr;
+    T* dcell = nullptr;
+    if (grads.strain) {
+        dstrain = ws_.get<T>("dstrain", static_cast<size_t>(n_atoms) * S * 9 * slots);
+        launch_soap_center<T>(lp, 1, nullptr, dstrain);
+    }
+    if (grads.cell) {
+        dcell = ws_.get<T>("dcell", static_cast<size_t>(n_atoms) * S * 9 * slots);
+        launch_soap_center<T>(lp, 2, nullptr, dcell);
+    }
+    RA_HIP_CHECK(hipGetLastError());
+
+    // ---- outputs
+    int* sample_labels = ws_.get<int>("sample_labels", 2 * static_cast<size_t>(std::max<long long>(total_samples, 1)));
+    launch_sample_labels(stream, NB, n_atoms, sample_flag, sample_scan, d_atom_system, d_atom_local, sample_labels);
+    int* grad_labels = grads.positions ? ws_.get<int>("grad_labels", 3 * static_cast<size_t>(std::max<long long>(total_rows, 1))) : nullptr;
+
+    // value / gradient regions (elements)
+    std::vector<long long> value_base, grad_base, cs_base;  // PS: per block; SE: per (tb, l)
+    long long values_size = 0, grads_size = 0, cs_size = 0;
+    std::vector<long long> se_grad_adj;  // SE: per (tb, l), base - row_base(tb) * 3w
+    if (ps) {
+        values_size = total_samples * P;
+        grads_size = total_rows * 3 * P;
+        cs_size = total_samples * 9 * P;
+    } else {
+        value_base.resize(static_cast<size_t>(NB) * L);
+        grad_base.resize(static_cast<size_t>(NB) * L);
+        cs_base.resize(static_cast<size_t>(NB) * L);
+        se_grad_adj.resize(static_cast<size_t>(NB) * L);
+        for (int tb = 0; tb < NB; tb++) {
+            const long long ns = block_sample_base[tb + 1] - block_sample_base[tb];
+            const long long nr = block_row_base[tb + 1] - block_row_base[tb];
+            for (int l = 0; l < L; l++) {
+                const long long w = (2LL * l + 1) * n;
+                value_base[tb * L + l] = values_size;
+                values_size += ns * w;
+                grad_base[tb * L + l] = grads_size;
+                se_grad_adj[tb * L + l] = grads_size - block_row_base[tb] * 3 * w;
+                grads_size += nr * 3 * w;
+                cs_base[tb * L + l] = cs_size;
+                cs_size += ns * 9 * w;
+            }
+        }
+    }
+    T* d_values = ws_.get<T>("out_values", std::max<long long>(values_size, 1));
+    T* d_grads = grads.positions ? ws_.get<T>("out_grads", std::max<long long>(grads_size, 1)) : nullptr;
+    T* d_strain = grads.strain ? ws_.get<T>("out_strain", std::max<long long>(cs_size, 1)) : nullptr;
+    T* d_cell = grads.cell ? ws_.get<T>("out_cell", std::max<long long>(cs_size, 1)) : nullptr;
+    long long* d_region = nullptr;
+    long long* d_se_grad = nullptr;
+    if (!ps) {
+        d_region = ws_.get<long long>("se_value_base", value_base.size());
+        d_se_grad = ws_.get<long long>("se_grad_base", se_grad_adj.size());
+        upload(d_region, value_base, stream);
+        upload(d_se_grad, se_grad_adj, stream);
+    }
+
+    const size_t slots_bytes = slots * sizeof(T);
+    if (ps) {
+        if (static_cast<long long>(S * slots_bytes) > lds) throw Error::invalid_parameter("too many atomic types for the GPU kernels' local memory");
+        launch_ps_values<T>(lp, coef, type_mask, sample_flag, sample_scan, d_values);
+    } else {
+        launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
+    }
+    RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+    if (grads.positions) {
+        lp.chunk = chunk_for((ps ? S * slots_bytes : 0) + 6 * slots_bytes + 2 * 528 * 4);
+        launch_gradient_rows<T>(lp, ps, coef, dself, type_mask, sample_flag, sample_scan, row_scan, d_block_sample_base,
+                                d_se_grad, d_atom_system, d_atom_local, d_grads, grad_labels);
+    }
+    RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+    main_bytes_ = grads.positions ? grads_size * static_cast<long long>(sizeof(T)) : 0;
+    for (int which = 0; which < 2; which++) {
+        const T* dense = which == 0 ? dstrain : dcell;
+        T* out = which == 0 ? d_strain : d_cell;
+        if (!dense) continue;
+        if (ps) {
+            if (static_cast<long long>((S + 18) * slots_bytes) > lds) throw Error::invalid_parameter("strain/cell gradients too large for the GPU kernels' local memory");
+            launch_ps_cell_strain<T>(lp, coef, dense, type_mask, sample_flag, sample_scan, out);
+        } else {
+            long long* d_cs = ws_.get<long long>(which == 0 ? "se_strain_base" : "se_cell_base", cs_base.size());
+            upload(d_cs, cs_base, stream);
+            launch_se_gather<T>(lp, 9, dense, sample_flag, sample_scan, d_block_sample_base, d_cs, out);
+        }
+    }
+    RA_HIP_CHECK(hipGetLastError());
+    RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+
+    // ---- describe the blocks
+    std::vector<int32_t> keys;
+    std::vector<int64_t> out_ns, out_nr, out_voff, out_goff, out_props;
+    struct BlockRef {
+        int b, l;
+    };
+    std::vector<BlockRef> refs;
+    for (int b = 0; b < NB; b++) {
+        const long long ns = block_sample_base[b + 1] - block_sample_base[b];
+        if (ns == 0) continue;
+        const long long nr = block_row_base[b + 1] - block_row_base[b];
+        const int ct = block_types[3 * b], t1 = block_types[3 * b + 1], t2 = block_types[3 * b + 2];
+        if (ps) {
+            keys.insert(keys.end(), {sp.types[ct], sp.types[t1], sp.types[t2]});
+            out_ns.push_back(ns);
+            out_nr.push_back(nr);
+            out_voff.push_back(block_sample_base[b] * P);
+            out_goff.push_back(block_row_base[b] * 3 * P);
+            out_props.push_back(P);
+            refs.push_back({b, 0});
+        } else {
+            for (int l = 0; l < L; l++) {
+                keys.insert(keys.end(), {l, 1, sp.types[ct], sp.types[t1]});
+                out_ns.push_back(ns);
+                out_nr.push_back(nr);
+                out_voff.push_back(value_base[b * L + l]);
+                out_goff.push_back(grad_base[b * L + l]);
+                out_props.push_back((2LL * l + 1) * n);
+                refs.push_back({b, l});
+            }
+        }
+    }
+
+    if (output) {
+        device_result_.keys = keys;
+        device_result_.key_size = ps ? 3 : 4;
+        device_result_.n_samples = out_ns;
+        device_result_.n_rows = out_nr;
+        device_result_.value_offsets = out_voff;
+        device_result_.gradient_offsets = out_goff;
+        device_result_.block_properties = out_props;
+        output->n_blocks = refs.size();
+        output->keys = device_result_.keys.data();
+        output->key_size = device_result_.key_size;
+        output->n_samples = device_result_.n_samples.data();
+        output->n_gradient_rows = device_result_.n_rows.data();
+        output->value_offsets = device_result_.value_offsets.data();
+        output->gradient_offsets = device_result_.gradient_offsets.data();
+        output->block_properties = device_result_.block_properties.data();
+        output->values = d_values;
+        output->gradients = d_grads;
+        output->sample_labels = sample_labels;
+        output->gradient_labels = grad_labels;
+        output->precision = sizeof(T) == 4 ? 32 : 64;
+    }
+    if (!to_host) return nullptr;
+
+    // ---- download and assemble the TensorMap
+    std::vector<T> h_values, h_grads, h_strain, h_cell;
+    std::vector<int32_t> h_slabels, h_glabels;
+    download(h_values, d_values, values_size, stream);
+    if (d_grads) download(h_grads, d_grads, grads_size, stream);
+    if (d_strain) download(h_strain, d_strain, cs_size, stream);
+    if (d_cell) download(h_cell, d_cell, cs_size, stream);
+    download(h_slabels, sample_labels, 2 * total_samples, stream);
+    if (grad_labels) download(h_glabels, grad_labels, 3 * total_rows, stream);
+    int flag = 0;
+    RA_HIP_CHECK(hipMemcpyAsync(&flag, error_flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+    RA_HIP_CHECK(hipStreamSynchronize(stream));
+    if (flag) throw Error::invalid_parameter("an atom has more than 2048 neighbours within the cutoff");
+
+    const size_t key_size = ps ? 3 : 4;
+    tm->keys.values = keys;
+    std::vector<int32_t> prop_values;
+    if (ps) {
+        for (int l = 0; l < L; l++)
+            for (int n1 = 0; n1 < n; n1++)
+                for (int n2 = 0; n2 < n; n2++) prop_values.insert(prop_values.end(), {l, n1, n2});
+    } else {
+        for (int k = 0; k < n; k++) prop_values.push_back(k);
+    }
+    const LabelsData properties = ps ? LabelsData({"l", "n_1", "n_2"}, prop_values) : LabelsData({"n"}, prop_values);
+    const LabelsData xyz = LabelsData::range("xyz", 3);
+
+    for (size_t k = 0; k < refs.size(); k++) {
+        const int b = refs[k].b, l = refs[k].l;
+        const long long ns = out_ns[k], nr = out_nr[k];
+        const long long s0 = block_sample_base[b], r0 = block_row_base[b];
+        const long long w = out_props[k];
+        std::vector<int32_t> sl(h_slabels.begin() + 2 * s0, h_slabels.begin() + 2 * (s0 + ns));
+        std::vector<LabelsData> comps;
+        std::vector<uintptr_t> shape;
+        if (ps) {
+            shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(P)};
+        } else {
+            comps.push_back(mu_labels(l));
+            shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(2 * l + 1), static_cast<uintptr_t>(n)};
+        }
+        auto block = make_block(LabelsData({"system", "atom"}, sl), comps, properties, shape);
+        for (long long x = 0; x < ns * w; x++) block->array.data[x] = static_cast<double>(h_values[out_voff[k] + x]);
+
+        if (grads.positions) {
+            std::vector<int32_t> gl(h_glabels.begin() + 3 * r0, h_glabels.begin() + 3 * (r0 + nr));
+            std::vector<LabelsData> gcomps = {xyz};
+            for (auto& c : comps) gcomps.push_back(c);
+            std::vector<uintptr_t> gshape = {static_cast<uintptr_t>(nr), 3};
+            for (size_t d = 1; d < shape.size(); d++) gshape.push_back(shape[d]);
+            auto g = make_block(LabelsData({"sample", "system", "atom"}, gl), gcomps, properties, gshape);
+            for (long long x = 0; x < nr * 3 * w; x++) g->array.data[x] = static_cast<double>(h_grads[out_goff[k] + x]);
+            block->add_gradient("positions", std::move(g));
+        }
+        for (int which = 0; which < 2; which++) {
+            const bool want = which == 0 ? grads.cell : grads.strain;
+            if (!want) continue;
+            const std::vector<T>& src = which == 0 ? h_cell : h_strain;
+            std::vector<LabelsData> gcomps;
+            if (which == 0) {
+                gcomps = {xyz, LabelsData::range("abc", 3)};
+            } else {
+                gcomps = {LabelsData::range("xyz_2", 3), LabelsData::range("xyz_1", 3)};
+            }
+            for (auto& c : comps) gcomps.push_back(c);
+            std::vector<uintptr_t> gshape = {static_cast<uintptr_t>(ns), 3, 3};
+            for (size_t d = 1; d < shape.size(); d++) gshape.push_back(shape[d]);
+            auto g = make_block(LabelsData::range("sample", static_cast<int>(ns)), gcomps, properties, gshape);
+            const long long base = ps ? s0 * 9 * P : cs_base[b * L + l];
+            for (long long x = 0; x < ns * 9 * w; x++) g->array.data[x] = static_cast<double>(src[base + x]);
+            block->add_gradient(which == 0 ? "cell" : "strain", std::move(g));
+        }
+        tm->blocks.push_back(std::move(block));
+    }
+    (void)key_size;
+    return tm.release();
+}
+
+// ---------------------------------------------------------------------------- entry points
+
+mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& grads) {
+    hipStream_t stream = nullptr;
+    if (kind_ == Kind::LodeSphericalExpansion) {
+        if (grads.strain || grads.cell) {
+            throw Error::invalid_parameter("the lode spherical expansion calculator does not support gradients with respect to " +
+                                           std::string(grads.strain ? "strain" : "cell"));
+        }
+        return precision == 32 ? run_lode<float>(batch, grads, stream) : run_lode<double>(batch, grads, stream);
+    }
+    return precision == 32 ? run_soap<float>(batch, grads, stream, true, nullptr)
+                           : run_soap<double>(batch, grads, stream, true, nullptr);
+}
+
+void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
+                                featomic_amd_device_output_t* output) {
+    if (kind_ == Kind::LodeSphericalExpansion) {
+        throw Error::invalid_parameter("featomic_amd_compute_device supports the SOAP calculators only");
+    }
+    if (precision == 32) {
+        run_soap<float>(batch, grads, stream, false, output);
+    } else {
+        run_soap<double>(batch, grads, stream, false, output);
+    }
+}
+
+void Calculator::last_timings(double* total_ms, double* main_ms, int64_t* main_bytes) {
+    if (!ev_start_) throw Error::invalid_parameter("no device computation was run yet");
+    RA_HIP_CHECK(hipEventSynchronize(ev_stop_));
+    float t = 0.f, m = 0.f;
+    RA_HIP_CHECK(hipEventElapsedTime(&t, ev_start_, ev_stop_));
+    RA_HIP_CHECK(hipEventElapsedTime(&m, ev_main_start_, ev_main_stop_));
+    if (total_ms) *total_ms = t;
+    if (main_ms) *main_ms = m;
+    if (main_bytes) *main_bytes = main_bytes_;
+}
+
+template mts_tensormap_t* Calculator::run_soap<float>(const HostBatch&, const Gradients&, hipStream_t, bool, featomic_amd_device_output_t*);
+template mts_tensormap_t* Calculator::run_soap<double>(const HostBatch&, const Gradients&, hipStream_t, bool, featomic_amd_device_output_t*);
+template const Calculator::DeviceTables<float>& Calculator::device_tables<float>();
+template const Calculator::DeviceTables<double>& Calculator::device_tables<double>();
+
+}  // namespace rascaline_amd

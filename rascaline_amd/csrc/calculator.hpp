@@ -1,0 +1,137 @@
+// Calculator objects behind the featomic C API: hyper-parameters, radial
+// tables, device workspace and the host orchestration of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/featomic.h"
+#include "../../include/featomic_amd.h"
+#include "hypers.hpp"
+#include "radial.hpp"
+#include "tensormap.hpp"
+
+namespace rascaline_amd {
+
+// Grow-only device allocations reused across compute calls.
+class Workspace {
+public:
+    ~Workspace();
+    void* get(const std::string& name, size_t bytes);
+    template <typename T>
+    T* get(const std::string& name, size_t count) {
+        return static_cast<T*>(get(name, count * sizeof(T) + 16));
+    }
+    void release();
+
+private:
+    struct Slot {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Slot> slots_;
+};
+
+// Systems gathered from the caller, concatenated.
+struct HostBatch {
+    std::vector<int64_t> offsets;        // n_systems + 1
+    std::vector<double> positions;       // host copy (may stay empty when positions are on device)
+    const double* device_positions = nullptr;
+    std::vector<int32_t> types;          // atomic types
+    std::vector<double> cells;           // n_systems x 9
+    // user-provided half neighbour lists (use_native_system = false)
+    bool has_pairs = false;
+    std::vector<std::vector<featomic_pair_t>> pairs;
+
+    size_t n_systems() const { return offsets.empty() ? 0 : offsets.size() - 1; }
+    int64_t n_atoms() const { return offsets.empty() ? 0 : offsets.back(); }
+};
+
+enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion };
+
+struct Gradients {
+    bool positions = false, strain = false, cell = false;
+};
+
+class Calculator {
+public:
+    Calculator(const std::string& registry_name, const std::string& parameters);
+    ~Calculator();
+
+    const char* name() const;
+    const std::string& parameters() const { return parameters_; }
+    const std::vector<double>& cutoffs() const { return cutoffs_; }
+    Kind kind() const { return kind_; }
+    double nl_cutoff() const { return soap_.cutoff; }
+
+    int precision = 64;
+
+    // host systems in, host TensorMap out
+    mts_tensormap_t* compute(const HostBatch& batch, const Gradients& gradients);
+
+    // device-resident batch (positions on the device), outputs stay on device
+    void compute_device(const HostBatch& batch, const Gradients& gradients, hipStream_t stream,
+                        featomic_amd_device_output_t* output);
+    void last_timings(double* total_ms, double* main_ms, int64_t* main_bytes);
+
+private:
+    std::string registry_name_;
+    std::string parameters_;
+    std::vector<double> cutoffs_;
+    Kind kind_;
+    SoapParams soap_;
+    LodeParams lode_;
+    RadialTables tables_;
+    std::vector<double> self_contribution_;  // SOAP l=0 self term / LODE centre contribution
+    Workspace ws_;
+
+    // host mirror of the last device-resident output
+    struct DeviceResult {
+        std::vector<int32_t> keys;
+        size_t key_size = 0;
+        std::vector<int64_t> n_samples, n_rows, value_offsets, gradient_offsets, block_properties;
+        const void* values = nullptr;
+        const void* gradients = nullptr;
+        const int32_t* sample_labels = nullptr;
+        const int32_t* gradient_labels = nullptr;
+    } device_result_;
+
+    hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr, ev_main_start_ = nullptr, ev_main_stop_ = nullptr;
+    int64_t main_bytes_ = 0;
+
+    template <typename T>
+    mts_tensormap_t* run_soap(const HostBatch& batch, const Gradients& gradients, hipStream_t stream, bool to_host,
+                              featomic_amd_device_output_t* output);
+    template <typename T>
+    mts_tensormap_t* run_lode(const HostBatch& batch, const Gradients& gradients, hipStream_t stream);
+
+    template <typename T>
+    struct DeviceTables;
+    std::shared_ptr<void> tables_f32_, tables_f64_;
+    template <typename T>
+    const DeviceTables<T>& device_tables();
+
+    // neighbour list on the device, shared by all calculators
+    friend struct NeighborRunner;
+};
+
+// Full neighbour list in device memory: per center i, entries offsets[i]..offsets[i+1]
+struct DeviceNeighbors {
+    long long n_entries = 0;
+    long long* offsets = nullptr;   // n_atoms + 1
+    int* entry_j = nullptr;
+    int* entry_shift = nullptr;     // 3 per entry
+    double* entry_vec = nullptr;    // 3 per entry
+    double* entry_dist = nullptr;
+};
+
+// Uploads the batch geometry and builds the full neighbour list (GPU cell list,
+// or the caller's half list when provided).
+DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cutoff, const double* d_positions,
+                                const int* d_atom_system, hipStream_t stream);
+
+}  // namespace rascaline_amd

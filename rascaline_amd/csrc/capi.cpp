@@ -1,0 +1,398 @@
+// featomic C API (featomic/include/featomic.h, implementation in
+// featomic/src/c_api/*.rs) over the GPU calculators, plus the extension entry
+// points declared in include/featomic_amd.h.
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/featomic.h"
+#include "../../include/featomic_amd.h"
+#include "calculator.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+struct featomic_calculator_t {
+    std::unique_ptr<rascaline_amd::Calculator> impl;
+    std::string name_storage;
+};
+
+namespace {
+
+using rascaline_amd::Error;
+
+thread_local std::string LAST_ERROR;
+featomic_logging_callback_t LOGGING_CALLBACK = nullptr;
+
+// time spans for featomic_profiling_*, c_api/profiling.rs:17-91
+struct Profiler {
+    std::mutex mutex;
+    bool enabled = false;
+    struct Span {
+        uint64_t calls = 0;
+        double total_ms = 0.0;
+    };
+    std::map<std::string, Span> spans;
+} PROFILER;
+
+struct ScopedSpan {
+    std::string name;
+    std::chrono::steady_clock::time_point start;
+    explicit ScopedSpan(const char* n) : name(n), start(std::chrono::steady_clock::now()) {}
+    ~ScopedSpan() {
+        if (!PROFILER.enabled) return;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+        std::lock_guard<std::mutex> lock(PROFILER.mutex);
+        auto& s = PROFILER.spans[name];
+        s.calls++;
+        s.total_ms += ms;
+    }
+};
+
+void log_message(int level, const std::string& message) {
+    if (LOGGING_CALLBACK) LOGGING_CALLBACK(level, message.c_str());
+}
+
+// catch every exception at the ABI (c_api/status.rs:82-88)
+template <typename F>
+featomic_status_t guard(F&& f) {
+    try {
+        f();
+        return FEATOMIC_SUCCESS;
+    } catch (const Error& e) {
+        LAST_ERROR = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        LAST_ERROR = std::string("internal featomic error: ") + e.what();
+        return FEATOMIC_INTERNAL_ERROR;
+    } catch (...) {
+        LAST_ERROR = "internal featomic error: unknown exception";
+        return FEATOMIC_INTERNAL_ERROR;
+    }
+}
+
+template <typename P>
+void check_pointer(P p, const char* name) {
+    if (!p) throw Error::invalid_parameter(std::string("got invalid NULL pointer for ") + name);
+}
+
+// copy a string into a caller buffer (c_api/utils.rs:5-17)
+void copy_str(const std::string& s, char* buffer, uintptr_t len) {
+    if (s.size() + 1 > len) {
+        throw Error::buffer_size("string buffer is not big enough: got " + std::to_string(len) + " bytes, need " +
+                                 std::to_string(s.size() + 1));
+    }
+    std::memcpy(buffer, s.c_str(), s.size() + 1);
+}
+
+void system_call(featomic_status_t status, const char* what) {
+    if (status != FEATOMIC_SUCCESS) {
+        throw Error::system(status, std::string("call to featomic_system_t::") + what + " failed");
+    }
+}
+
+rascaline_amd::HostBatch gather_systems(featomic_system_t* systems, uintptr_t count, bool use_native, double cutoff,
+                                        bool need_neighbors) {
+    rascaline_amd::HostBatch batch;
+    batch.offsets.push_back(0);
+    batch.has_pairs = need_neighbors && !use_native;
+    for (uintptr_t s = 0; s < count; s++) {
+        featomic_system_t& sys = systems[s];
+        check_pointer(sys.size, "featomic_system_t.size");
+        check_pointer(sys.types, "featomic_system_t.types");
+        check_pointer(sys.positions, "featomic_system_t.positions");
+        check_pointer(sys.cell, "featomic_system_t.cell");
+        uintptr_t n = 0;
+        system_call(sys.size(sys.user_data, &n), "size");
+        const int32_t* types = nullptr;
+        const double* positions = nullptr;
+        double cell[9];
+        if (n > 0) {
+            system_call(sys.types(sys.user_data, &types), "types");
+            system_call(sys.positions(sys.user_data, &positions), "positions");
+        }
+        system_call(sys.cell(sys.user_data, cell), "cell");
+        batch.types.insert(batch.types.end(), types, types + n);
+        batch.positions.insert(batch.positions.end(), positions, positions + 3 * n);
+        batch.cells.insert(batch.cells.end(), cell, cell + 9);
+        batch.offsets.push_back(batch.offsets.back() + static_cast<int64_t>(n));
+        if (batch.has_pairs) {
+            check_pointer(sys.compute_neighbors, "featomic_system_t.compute_neighbors");
+            check_pointer(sys.pairs, "featomic_system_t.pairs");
+            system_call(sys.compute_neighbors(sys.user_data, cutoff), "compute_neighbors");
+            const featomic_pair_t* pairs = nullptr;
+            uintptr_t n_pairs = 0;
+            system_call(sys.pairs(sys.user_data, &pairs, &n_pairs), "pairs");
+            batch.pairs.emplace_back(pairs, pairs + n_pairs);
+        }
+    }
+    return batch;
+}
+
+rascaline_amd::Gradients parse_gradients(const char* const* gradients, uintptr_t count, rascaline_amd::Kind kind) {
+    rascaline_amd::Gradients g;
+    for (uintptr_t k = 0; k < count; k++) {
+        check_pointer(gradients[k], "gradients");
+        const std::string name = gradients[k];
+        if (name == "positions") {
+            g.positions = true;
+        } else if (name == "strain") {
+            g.strain = true;
+        } else if (name == "cell") {
+            g.cell = true;
+        } else {
+            throw Error::invalid_parameter("unexpected gradient \"" + name +
+                                           "\", should be one of \"positions\", \"cell\", or \"strain\"");
+        }
+    }
+    if (kind == rascaline_amd::Kind::LodeSphericalExpansion && (g.strain || g.cell)) {
+        throw Error::invalid_parameter(std::string("the lode spherical expansion calculator does not support gradients with respect to ") +
+                                       (g.strain ? "strain" : "cell"));
+    }
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* featomic_last_error(void) { return LAST_ERROR.c_str(); }
+
+featomic_status_t featomic_set_logging_callback(featomic_logging_callback_t callback) {
+    return guard([&] { LOGGING_CALLBACK = callback; });
+}
+
+featomic_calculator_t* featomic_calculator(const char* name, const char* parameters) {
+    featomic_calculator_t* result = nullptr;
+    const featomic_status_t status = guard([&] {
+        check_pointer(name, "name");
+        check_pointer(parameters, "parameters");
+        auto calc = std::make_unique<featomic_calculator_t>();
+        calc->impl = std::make_unique<rascaline_amd::Calculator>(name, parameters);
+        result = calc.release();
+    });
+    return status == FEATOMIC_SUCCESS ? result : nullptr;
+}
+
+featomic_status_t featomic_calculator_free(featomic_calculator_t* calculator) {
+    return guard([&] { delete calculator; });
+}
+
+featomic_status_t featomic_calculator_name(const featomic_calculator_t* calculator, char* name, uintptr_t bufflen) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(name, "name");
+        copy_str(calculator->impl->name(), name, bufflen);
+    });
+}
+
+featomic_status_t featomic_calculator_parameters(const featomic_calculator_t* calculator, char* parameters, uintptr_t bufflen) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(parameters, "parameters");
+        copy_str(calculator->impl->parameters(), parameters, bufflen);
+    });
+}
+
+featomic_status_t featomic_calculator_cutoffs(const featomic_calculator_t* calculator, const double** cutoffs,
+                                              uintptr_t* cutoffs_count) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(cutoffs, "cutoffs");
+        check_pointer(cutoffs_count, "cutoffs_count");
+        *cutoffs = calculator->impl->cutoffs().data();
+        *cutoffs_count = calculator->impl->cutoffs().size();
+    });
+}
+
+featomic_status_t featomic_calculator_compute(featomic_calculator_t* calculator, mts_tensormap_t** descriptor,
+                                              featomic_system_t* systems, uintptr_t systems_count,
+                                              featomic_calculation_options_t options) {
+    return guard([&] {
+        ScopedSpan span("Calculator::compute");
+        check_pointer(calculator, "calculator");
+        check_pointer(descriptor, "descriptor");
+        if (systems_count == 0) {
+            // c_api/calculator.rs:400-403
+            log_message(FEATOMIC_LOG_LEVEL_WARN, "0 systems given to featomic_calculator_compute, nothing to do");
+            return;
+        }
+        check_pointer(systems, "systems");
+        if (options.gradients_count > 0) check_pointer(options.gradients, "options.gradients");
+        if (options.selected_samples.subset || options.selected_samples.predefined ||
+            options.selected_properties.subset || options.selected_properties.predefined || options.selected_keys) {
+            throw Error::invalid_parameter("samples, properties and keys selection are not supported by this GPU implementation");
+        }
+        auto& impl = *calculator->impl;
+        const auto grads = parse_gradients(options.gradients, options.gradients_count, impl.kind());
+        if (grads.cell && !std::getenv("FEATOMIC_NO_WARN_CELL_GRADIENTS")) {
+            // calculator.rs:350-358
+            log_message(FEATOMIC_LOG_LEVEL_WARN,
+                        "The meaning of \"cell\" gradients has changed recently, you likely want \"strain\" gradients "
+                        "instead. Please review the documentation carefully.");
+        }
+        const bool soap = impl.kind() != rascaline_amd::Kind::LodeSphericalExpansion;
+        auto batch = gather_systems(systems, systems_count, options.use_native_system, soap ? impl.nl_cutoff() : 0.0, soap);
+        *descriptor = impl.compute(batch, grads);
+    });
+}
+
+featomic_status_t featomic_profiling_clear(void) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lock(PROFILER.mutex);
+        PROFILER.spans.clear();
+    });
+}
+
+featomic_status_t featomic_profiling_enable(bool enabled) {
+    return guard([&] { PROFILER.enabled = enabled; });
+}
+
+featomic_status_t featomic_profiling_get(const char* format, char* buffer, uintptr_t bufflen) {
+    return guard([&] {
+        check_pointer(format, "format");
+        check_pointer(buffer, "buffer");
+        const std::string f = format;
+        std::string out;
+        std::lock_guard<std::mutex> lock(PROFILER.mutex);
+        if (f == "json") {
+            out = "{";
+            bool first = true;
+            for (auto& kv : PROFILER.spans) {
+                if (!first) out += ",";
+                first = false;
+                out += "\"" + kv.first + "\":{\"calls\":" + std::to_string(kv.second.calls) +
+                       ",\"total_ms\":" + std::to_string(kv.second.total_ms) + "}";
+            }
+            out += "}";
+        } else if (f == "table" || f == "short_table") {
+            out = "span | calls | total (ms)\n";
+            for (auto& kv : PROFILER.spans) {
+                out += kv.first + " | " + std::to_string(kv.second.calls) + " | " + std::to_string(kv.second.total_ms) + "\n";
+            }
+        } else {
+            throw Error::invalid_parameter("invalid format for profiling data: expected 'table', 'short_table' or 'json', got '" + f + "'");
+        }
+        copy_str(out, buffer, bufflen);
+    });
+}
+
+// ---------------------------------------------------------------------------- extensions
+
+featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* calculator, int32_t bits) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        if (bits != 32 && bits != 64) throw Error::invalid_parameter("precision must be 32 or 64 bits");
+        calculator->impl->precision = bits;
+    });
+}
+
+featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff,
+                                         featomic_pair_t** pairs, uintptr_t* n_pairs) {
+    return guard([&] {
+        check_pointer(cell, "cell");
+        check_pointer(pairs, "pairs");
+        check_pointer(n_pairs, "n_pairs");
+        if (n_atoms > 0) check_pointer(positions, "positions");
+        if (!(cutoff > 0.0)) throw Error::invalid_parameter("cutoff must be positive");
+        rascaline_amd::HostBatch batch;
+        batch.offsets = {0, static_cast<int64_t>(n_atoms)};
+        batch.positions.assign(positions, positions + 3 * n_atoms);
+        batch.types.assign(n_atoms, 0);
+        batch.cells.assign(cell, cell + 9);
+        rascaline_amd::Workspace ws;
+        hipStream_t stream = nullptr;
+        double* d_pos = ws.get<double>("positions", 3 * std::max<uintptr_t>(n_atoms, 1));
+        int* d_sys = ws.get<int>("atom_system", std::max<uintptr_t>(n_atoms, 1));
+        std::vector<int> zeros(n_atoms, 0);
+        if (n_atoms) {
+            RA_HIP_CHECK(hipMemcpy(d_pos, positions, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice));
+            RA_HIP_CHECK(hipMemcpy(d_sys, zeros.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
+        }
+        auto nl = rascaline_amd::build_neighbors(ws, batch, cutoff, d_pos, d_sys, stream);
+        std::vector<long long> offsets(n_atoms + 1);
+        std::vector<int> j(nl.n_entries), s(3 * nl.n_entries);
+        std::vector<double> v(3 * nl.n_entries), d(nl.n_entries);
+        RA_HIP_CHECK(hipMemcpy(offsets.data(), nl.offsets, sizeof(long long) * (n_atoms + 1), hipMemcpyDeviceToHost));
+        if (nl.n_entries) {
+            RA_HIP_CHECK(hipMemcpy(j.data(), nl.entry_j, sizeof(int) * nl.n_entries, hipMemcpyDeviceToHost));
+            RA_HIP_CHECK(hipMemcpy(s.data(), nl.entry_shift, sizeof(int) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
+            RA_HIP_CHECK(hipMemcpy(v.data(), nl.entry_vec, sizeof(double) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
+            RA_HIP_CHECK(hipMemcpy(d.data(), nl.entry_dist, sizeof(double) * nl.n_entries, hipMemcpyDeviceToHost));
+        }
+        // half list: j > i, or j == i with the shift in the positive half-space
+        // (featomic/src/systems/neighbors.rs:253-286)
+        std::vector<featomic_pair_t> out;
+        for (uintptr_t i = 0; i < n_atoms; i++) {
+            for (long long e = offsets[i]; e < offsets[i + 1]; e++) {
+                const int jj = j[e];
+                const int* sh = &s[3 * e];
+                bool keep = jj > static_cast<int>(i);
+                if (jj == static_cast<int>(i)) {
+                    const int sum = sh[0] + sh[1] + sh[2];
+                    keep = !(sum < 0 || (sum == 0 && (sh[2] < 0 || (sh[2] == 0 && sh[1] < 0))));
+                }
+                if (!keep) continue;
+                featomic_pair_t p;
+                p.first = i;
+                p.second = static_cast<uintptr_t>(jj);
+                p.distance = d[e];
+                p.vector[0] = v[3 * e];
+                p.vector[1] = v[3 * e + 1];
+                p.vector[2] = v[3 * e + 2];
+                p.cell_shift_indices[0] = sh[0];
+                p.cell_shift_indices[1] = sh[1];
+                p.cell_shift_indices[2] = sh[2];
+                out.push_back(p);
+                if (p.distance * p.distance < 1e-3) {
+                    log_message(FEATOMIC_LOG_LEVEL_WARN, "atoms " + std::to_string(p.first) + " and " +
+                                                             std::to_string(p.second) + " are very close to one another (" +
+                                                             std::to_string(p.distance) + " A)");
+                }
+            }
+        }
+        *n_pairs = out.size();
+        *pairs = static_cast<featomic_pair_t*>(std::malloc(sizeof(featomic_pair_t) * std::max<size_t>(out.size(), 1)));
+        if (!out.empty()) std::memcpy(*pairs, out.data(), sizeof(featomic_pair_t) * out.size());
+    });
+}
+
+void featomic_amd_free(void* pointer) { std::free(pointer); }
+
+featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
+                                              const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                              featomic_amd_device_output_t* output) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(systems, "systems");
+        check_pointer(output, "output");
+        check_pointer(systems->atom_offsets, "systems.atom_offsets");
+        check_pointer(systems->cells, "systems.cells");
+        auto& impl = *calculator->impl;
+        const auto grads = parse_gradients(gradients, gradients_count, impl.kind());
+        if (grads.strain || grads.cell) throw Error::invalid_parameter("featomic_amd_compute_device supports positions gradients only");
+        rascaline_amd::HostBatch batch;
+        batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
+        const int64_t n_atoms = batch.n_atoms();
+        if (n_atoms > 0) {
+            check_pointer(systems->positions, "systems.positions");
+            check_pointer(systems->types, "systems.types");
+        }
+        batch.device_positions = systems->positions;
+        batch.types.assign(systems->types, systems->types + n_atoms);
+        batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
+        impl.compute_device(batch, grads, static_cast<hipStream_t>(stream), output);
+    });
+}
+
+featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
+                                            int64_t* main_kernel_bytes) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        calculator->impl->last_timings(total_ms, main_kernel_ms, main_kernel_bytes);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// Host-side launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "device.hpp"
+
+namespace rascaline_amd {
+
+// ---- neighbour list and metadata (kernels_nl.hip)
+void launch_nl_assign_bins(hipStream_t stream, int n_atoms, const double* positions, const int* atom_system,
+                           const NLSystem* systems, long long* atom_bin, int* atom_wrap, int* bin_count);
+void launch_nl_fill_bins(hipStream_t stream, int n_atoms, const long long* atom_bin, const long long* bin_start,
+                         int* bin_fill, int* bin_atoms);
+void launch_nl_candidates(hipStream_t stream, bool emit, int n_atoms, const double* positions, const int* atom_system,
+                          const NLSystem* systems, const long long* atom_bin, const int* atom_wrap,
+                          const long long* bin_start, const int* bin_len, const int* bin_atoms, double cutoff2,
+                          long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag);
+void launch_nl_sort_segments(hipStream_t stream, int n_atoms, const long long* offsets, unsigned long long* keys,
+                             int* error_flag);
+void launch_nl_finalize(hipStream_t stream, long long n_entries, int n_atoms, const long long* offsets,
+                        const unsigned long long* keys, const double* positions, const int* atom_system,
+                        const NLSystem* systems, int* entry_center, int* entry_j, int* entry_shift, double* entry_vec,
+                        double* entry_dist);
+void launch_center_meta(hipStream_t stream, int n_atoms, int n_species, const long long* offsets, const int* entry_j,
+                        const int* species, unsigned* type_mask, int* unique_count);
+void launch_block_membership(hipStream_t stream, int n_atoms, int n_species, int ps_mode, int n_blocks,
+                             const int* species, const unsigned* type_mask, const int* unique_count,
+                             const int* block_types, long long* sample_flag, long long* row_count);
+void launch_gather_block_totals(hipStream_t stream, int n_blocks, long long n_atoms, const long long* sample_scan,
+                                const long long* row_scan, long long* totals);
+void launch_sample_labels(hipStream_t stream, int n_blocks, int n_atoms, const long long* sample_flag,
+                          const long long* sample_scan, const int* atom_system, const int* atom_local, int* labels);
+
+// exclusive prefix sum of `n` values into out[0..n] (out[n] = total)
+void exclusive_scan(hipStream_t stream, const long long* in, long long* out, long long n);
+
+// ---- SOAP (kernels_soap.hip)
+template <typename T>
+struct SoapLaunch {
+    hipStream_t stream;
+    int n_atoms, n_species, L, n;
+    const long long* offsets;
+    const int* entry_j;
+    const int* entry_shift;
+    const double* entry_vec;
+    const double* entry_dist;
+    const int* species;
+    DeviceRadial<T> radial;
+    DeviceSoapScalars scalars;
+    const T* sh_a;
+    const T* sh_b;
+    const T* self_contribution;
+    int chunk;
+    int* error_flag;
+};
+
+template <typename T>
+size_t soap_chunk_bytes(int L, int n, int chunk);
+template <typename T>
+void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad);
+template <typename T>
+void launch_ps_values(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
+                      const long long* sample_scan, T* out);
+template <typename T>
+size_t gradient_rows_shmem(const SoapLaunch<T>& p, bool ps);
+template <typename T>
+void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const T* dself, const unsigned* type_mask,
+                          const long long* sample_flag, const long long* sample_scan, const long long* row_scan,
+                          const long long* block_sample_base, const long long* se_grad_base, const int* atom_system,
+                          const int* atom_local, T* out, int* labels);
+template <typename T>
+void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
+                           const long long* sample_flag, const long long* sample_scan, T* out);
+template <typename T>
+void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const long long* sample_flag,
+                      const long long* sample_scan, const long long* block_sample_base, const long long* region_base, T* out);
+
+}  // namespace rascaline_amd

@@ -1,0 +1,629 @@
+// SOAP spherical expansion and power spectrum kernels (gfx950, wave64).
+//
+// Layouts (T = float or double):
+//   coef  [atom][species][L2][n]         c_nlm of every center for every neighbour species
+//   dself [atom][species][3][L2][n]      d c / d r_center (self gradient)
+//   dstrain / dcell [atom][species][9][L2][n]
+// One 64-lane workgroup handles one center. Per-pair quantities (Y_lm, r dY_lm,
+// spline values) are computed lane-parallel for a chunk of neighbour entries
+// into LDS, then every lane accumulates its own (lm, n) slots.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "soap_common.hpp"
+
+namespace rascaline_amd {
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int MAX_SEGMENT = 2048;
+constexpr int MAX_LOCAL_BLOCKS = 528;  // 32 * 33 / 2
+
+// Per-chunk LDS layout of the per-entry quantities.
+struct ChunkLayout {
+    int L, n, L2, leg, chunk;
+    int y, dy, rv, ra, rb, dir, vec, shift, flag, p, dp, pt;  // offsets in elements (per entry)
+    int per_entry;
+
+    __host__ __device__ ChunkLayout(int L_, int n_, int chunk_) : L(L_), n(n_), L2(L_ * L_), leg(L_ * (L_ + 1) / 2), chunk(chunk_) {
+        int o = 0;
+        y = o; o += L2;
+        dy = o; o += 3 * L2;
+        rv = o; o += L * n;
+        ra = o; o += L * n;
+        rb = o; o += L * n;
+        dir = o; o += 3;
+        vec = o; o += 3;
+        shift = o; o += 3;
+        flag = o; o += 1;
+        p = o; o += leg;
+        dp = o; o += leg;
+        pt = o; o += leg;
+        per_entry = o;
+    }
+};
+
+template <typename T>
+struct SoapArgs {
+    int n_atoms, n_species, L, n;
+    const long long* offsets;
+    const int* entry_j;
+    const int* entry_shift;
+    const double* entry_vec;
+    const double* entry_dist;
+    const int* species;
+    DeviceRadial<T> radial;
+    DeviceSoapScalars scalars;
+    const T* sh_a;
+    const T* sh_b;
+    const T* self_contribution;
+    int chunk;
+    int* error_flag;
+};
+
+// Fill the per-entry LDS slot `base` for neighbour entry e of center i.
+template <typename T>
+__device__ void entry_data(const SoapArgs<T>& a, const ChunkLayout& lay, long long e, int i, T* base) {
+    const double r = a.entry_dist[e];
+    double d[3];
+    if (r < 1e-6) {
+        d[0] = 0.0;
+        d[1] = 0.0;
+        d[2] = 1.0;
+    } else {
+        d[0] = a.entry_vec[3 * e] / r;
+        d[1] = a.entry_vec[3 * e + 1] / r;
+        d[2] = a.entry_vec[3 * e + 2] / r;
+    }
+    T dt[3] = {(T)d[0], (T)d[1], (T)d[2]};
+    spherical_harmonics<T>(lay.L, dt, base + lay.y, base + lay.dy, base + lay.p, base + lay.dp, base + lay.pt, a.sh_a, a.sh_b);
+    double f, df;
+    scaling_functions(a.scalars, r, &f, &df);
+    const int n = lay.n;
+    for (int l = 0; l < lay.L; l++) {
+        // R into the rv slots, dR/dr into the ra slots, then combine in place
+        radial_integral<T>(a.radial, l, r, base + lay.rv + l * n, base + lay.ra + l * n);
+        for (int k = 0; k < n; k++) {
+            const T R = base[lay.rv + l * n + k], dR = base[lay.ra + l * n + k];
+            base[lay.rv + l * n + k] = (T)f * R;
+            base[lay.ra + l * n + k] = (T)df * R + (T)f * dR;
+            base[lay.rb + l * n + k] = (T)f * R / (T)r;
+        }
+    }
+    for (int x = 0; x < 3; x++) {
+        base[lay.dir + x] = dt[x];
+        base[lay.vec + x] = (T)a.entry_vec[3 * e + x];
+        base[lay.shift + x] = (T)a.entry_shift[3 * e + x];
+    }
+    base[lay.flag] = (a.entry_j[e] == i) ? (T)1 : (T)0;
+}
+
+__device__ __forceinline__ int l_of_lm(int lm) {
+    int l = (int)sqrtf((float)lm);
+    while ((l + 1) * (l + 1) <= lm) l++;
+    while (l * l > lm) l--;
+    return l;
+}
+
+// MODE 0: coef + dself; MODE 1: dstrain; MODE 2: dcell.
+template <typename T, int MS, int MODE>
+__global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __restrict__ out_coef, T* __restrict__ out_grad) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int list[MAX_SEGMENT];
+    __shared__ int type_start[33];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const int S = a.n_species;
+    if (len > MAX_SEGMENT) {
+        if (lane == 0) atomicOr(a.error_flag, 2);
+        return;
+    }
+    // stable partition of the entries by neighbour species
+    int cursor = 0;
+    for (int t = 0; t < S; t++) {
+        if (lane == 0) type_start[t] = cursor;
+        for (int base = 0; base < len; base += WAVE) {
+            const int e = base + lane;
+            const bool pred = e < len && a.species[a.entry_j[e0 + e]] == t;
+            const unsigned long long mask = __ballot(pred);
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            if (pred) list[cursor + pos] = e;
+            cursor += __popcll(mask);
+        }
+    }
+    if (lane == 0) type_start[S] = len;
+    __syncthreads();
+
+    const ChunkLayout lay(a.L, a.n, a.chunk);
+    T* chunk = reinterpret_cast<T*>(smem);
+    const int n = a.n;
+    const int slots = lay.L2 * n;
+    const int ct = a.species[i];
+    const int groups = (slots + WAVE * MS - 1) / (WAVE * MS);
+
+    for (int g = 0; g < groups; g++) {
+        const int sb = g * WAVE * MS;
+        for (int t = 0; t < S; t++) {
+            T acc[MODE == 0 ? 4 : 9][MS];
+            T accv[MS];
+#pragma unroll
+            for (int k = 0; k < MS; k++) {
+                accv[k] = 0;
+#pragma unroll
+                for (int x = 0; x < (MODE == 0 ? 3 : 9); x++) acc[x][k] = 0;
+            }
+            const int tb = type_start[t], te = type_start[t + 1];
+            for (int cb = tb; cb < te; cb += lay.chunk) {
+                const int cnt = min(lay.chunk, te - cb);
+                if (lane < cnt) entry_data<T>(a, lay, e0 + list[cb + lane], i, chunk + lane * lay.per_entry);
+                __syncthreads();
+                for (int c = 0; c < cnt; c++) {
+                    const T* en = chunk + c * lay.per_entry;
+                    const bool self_image = en[lay.flag] != (T)0;
+#pragma unroll
+                    for (int k = 0; k < MS; k++) {
+                        const int s = sb + lane + WAVE * k;
+                        if (s < slots) {
+                            const int lm = s / n, nn = s - lm * n;
+                            const int l = l_of_lm(lm);
+                            const T y = en[lay.y + lm];
+                            if (MODE == 0) {
+                                accv[k] += en[lay.rv + l * n + nn] * y;
+                                if (out_grad && !self_image) {
+                                    const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+#pragma unroll
+                                    for (int x = 0; x < 3; x++) acc[x][k] -= ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
+                                }
+                            } else {
+                                const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+                                T gx[3];
+#pragma unroll
+                                for (int x = 0; x < 3; x++) gx[x] = ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
+#pragma unroll
+                                for (int p = 0; p < 3; p++) {
+                                    const T w = MODE == 1 ? en[lay.vec + p] : en[lay.shift + p];
+#pragma unroll
+                                    for (int x = 0; x < 3; x++) acc[p * 3 + x][k] += w * gx[x];
+                                }
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            // write this species' accumulators
+#pragma unroll
+            for (int k = 0; k < MS; k++) {
+                const int s = sb + lane + WAVE * k;
+                if (s < slots) {
+                    if (MODE == 0) {
+                        T v = accv[k];
+                        if (t == ct && s < n) v += a.self_contribution[s];
+                        out_coef[((long long)i * S + t) * slots + s] = v;
+                        if (out_grad) {
+#pragma unroll
+                            for (int x = 0; x < 3; x++) out_grad[(((long long)i * S + t) * 3 + x) * slots + s] = acc[x][k];
+                        }
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 9; x++) out_grad[(((long long)i * S + t) * 9 + x) * slots + s] = acc[x][k];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Local PS block index of (t1 <= t2) among S species.
+__device__ __forceinline__ int pair_index(int t1, int t2, int S) { return t1 * S - t1 * (t1 - 1) / 2 + (t2 - t1); }
+
+// Σ_m C2[m][n2] D1[x][m][n1] (if use1) + Σ_m C1[m][n1] D2[x][m][n2] (if use2) for all
+// (x, l, n1, n2), scaled by kappa / ((-1)^l sqrt(2l+1)) (soap/power_spectrum.rs:340-348,604-675).
+template <typename T>
+__device__ void ps_contract(int ncomp, int L, int n, const T* C1, const T* C2, const T* D1, const T* D2, bool use1,
+                            bool use2, T kappa, int slots, T* out) {
+    const int P = L * n * n;
+    for (int o = threadIdx.x; o < ncomp * P; o += WAVE) {
+        const int x = o / P, p = o - x * P;
+        const int l = p / (n * n), n1 = (p / n) % n, n2 = p % n;
+        T sum = 0;
+        for (int lm = l * l; lm < (l + 1) * (l + 1); lm++) {
+            if (use1) sum += C2[lm * n + n2] * D1[x * slots + lm * n + n1];
+            if (use2) sum += C1[lm * n + n1] * D2[x * slots + lm * n + n2];
+        }
+        const T norm = (l % 2 == 0 ? (T)1 : (T)-1) * (T)sqrt((double)(2 * l + 1));
+        out[o] = (sum * kappa) / norm;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(WAVE) ps_values_kernel(int n_atoms, int S, int L, int n, const T* __restrict__ coef,
+                                                         const int* __restrict__ species, const unsigned* __restrict__ type_mask,
+                                                         const long long* __restrict__ sample_flag,
+                                                         const long long* __restrict__ sample_scan, T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* C = reinterpret_cast<T*>(smem);
+    const int i = blockIdx.x;
+    const int slots = L * L * n;
+    for (int k = threadIdx.x; k < S * slots; k += WAVE) C[k] = coef[(long long)i * S * slots + k];
+    __syncthreads();
+    const int ct = species[i];
+    const unsigned mask = type_mask[i];
+    const int P = L * n * n;
+    const int npairs = S * (S + 1) / 2;
+    for (int t1 = 0; t1 < S; t1++) {
+        if (!((mask >> t1) & 1u)) continue;
+        for (int t2 = t1; t2 < S; t2++) {
+            if (!((mask >> t2) & 1u)) continue;
+            const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
+            if (!sample_flag[idx]) continue;
+            const long long s = sample_scan[idx];
+            const T kappa = t1 != t2 ? (T)1.4142135623730951 : (T)1;
+            ps_contract<T>(1, L, n, C + t1 * slots, C + t2 * slots, C + t1 * slots, C + t2 * slots, true, false, kappa,
+                           slots, out + s * P);
+        }
+    }
+}
+
+// Gradients w.r.t. positions for the power spectrum (PS = true) or the spherical
+// expansion (PS = false). Iterates over the center's sorted neighbour entries,
+// sums the images of each neighbour atom into DC = d c^{type_j} / d r_j, and
+// emits one gradient row per block containing type_j; the center's own rows use
+// the self gradients.
+template <typename T, bool PS>
+__global__ void __launch_bounds__(WAVE) gradient_rows_kernel(SoapArgs<T> a, const T* __restrict__ coef,
+                                                             const T* __restrict__ dself, const unsigned* __restrict__ type_mask,
+                                                             const long long* __restrict__ sample_flag,
+                                                             const long long* __restrict__ sample_scan,
+                                                             const long long* __restrict__ row_scan,
+                                                             const long long* __restrict__ block_sample_base,
+                                                             const long long* __restrict__ se_grad_base,
+                                                             const int* __restrict__ atom_system, const int* __restrict__ atom_local,
+                                                             T* __restrict__ out, int* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int counters[MAX_LOCAL_BLOCKS];
+    __shared__ int snapshot[MAX_LOCAL_BLOCKS];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L, slots = L2 * n;
+    const int P = L * n * n;
+    const int npairs = S * (S + 1) / 2;
+    const int nloc = PS ? npairs : S;
+    const ChunkLayout lay(L, n, a.chunk);
+    T* C = reinterpret_cast<T*>(smem);          // S * slots (PS only)
+    T* DC = C + (PS ? S * slots : 0);            // 3 * slots
+    T* D2 = DC + 3 * slots;                      // 3 * slots (center rows)
+    T* chunk = D2 + 3 * slots;
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const int ct = a.species[i];
+    const unsigned mask = type_mask[i];
+    const int n_atoms = a.n_atoms;
+
+    if (PS) {
+        for (int k = lane; k < S * slots; k += WAVE) C[k] = coef[(long long)i * S * slots + k];
+    }
+    for (int k = lane; k < 3 * slots; k += WAVE) DC[k] = 0;
+    for (int k = lane; k < nloc; k += WAVE) {
+        counters[k] = 0;
+        snapshot[k] = -1;
+    }
+    __syncthreads();
+
+    auto global_block = [&](int local) -> int { return PS ? ct * npairs + local : ct * S + local; };
+
+    // emit all rows of neighbour j (or of the center when j == i, with D = dself)
+    auto emit = [&](int j, int tj, bool center) {
+        if (PS) {
+            for (int t1 = 0; t1 < S; t1++) {
+                if (!((mask >> t1) & 1u)) continue;
+                for (int t2 = t1; t2 < S; t2++) {
+                    if (!((mask >> t2) & 1u)) continue;
+                    if (!center && t1 != tj && t2 != tj) continue;
+                    const int local = pair_index(t1, t2, S);
+                    const long long idx = (long long)global_block(local) * n_atoms + i;
+                    const long long row = row_scan[idx] + (center ? snapshot[local] : counters[local] + (i < j ? 1 : 0));
+                    const T kappa = t1 != t2 ? (T)1.4142135623730951 : (T)1;
+                    const T *D1v, *D2v;
+                    bool use1, use2;
+                    if (center) {
+                        // self gradients of both species, loaded into DC (t1) and D2 (t2)
+                        for (int k = lane; k < 3 * slots; k += WAVE) {
+                            DC[k] = dself[((long long)i * S + t1) * 3 * slots + k];
+                            D2[k] = dself[((long long)i * S + t2) * 3 * slots + k];
+                        }
+                        __syncthreads();
+                        D1v = DC;
+                        D2v = D2;
+                        use1 = use2 = true;
+                    } else {
+                        D1v = DC;
+                        D2v = DC;
+                        use1 = t1 == tj;
+                        use2 = t2 == tj;
+                    }
+                    ps_contract<T>(3, L, n, C + t1 * slots, C + t2 * slots, D1v, D2v, use1, use2, kappa, slots,
+                                   out + row * 3 * P);
+                    if (lane == 0) {
+                        labels[3 * row] = (int)(sample_scan[idx] - block_sample_base[global_block(local)]);
+                        labels[3 * row + 1] = atom_system[i];
+                        labels[3 * row + 2] = atom_local[j];
+                    }
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (int nt = 0; nt < S; nt++) {
+                if (!((mask >> nt) & 1u)) continue;
+                if (!center && nt != tj) continue;
+                const long long idx = (long long)global_block(nt) * n_atoms + i;
+                const long long row = row_scan[idx] + (center ? snapshot[nt] : counters[nt] + (i < j ? 1 : 0));
+                const T* D = center ? dself + ((long long)i * S + nt) * 3 * slots : DC;
+                const long long tb = global_block(nt);
+                for (int l = 0; l < L; l++) {
+                    const int w = (2 * l + 1) * n;
+                    T* o = out + se_grad_base[tb * L + l] + row * 3 * w;
+                    for (int k = lane; k < 3 * w; k += WAVE) {
+                        const int x = k / w, r = k - x * w;
+                        o[k] = D[x * slots + l * l * n + r];
+                    }
+                }
+                if (lane == 0) {
+                    labels[3 * row] = (int)(sample_scan[idx] - block_sample_base[tb]);
+                    labels[3 * row + 1] = atom_system[i];
+                    labels[3 * row + 2] = atom_local[j];
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    auto bump = [&](int tj) {
+        // count neighbour tj in every block that contains it
+        if (lane == 0) {
+            if (PS) {
+                for (int t = 0; t < S; t++) {
+                    if (!((mask >> t) & 1u)) continue;
+                    const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
+                    counters[pair_index(lo, hi, S)] += 1;
+                }
+            } else {
+                counters[tj] += 1;
+            }
+        }
+        __syncthreads();
+    };
+
+    auto take_snapshot = [&]() {
+        for (int k = lane; k < nloc; k += WAVE) snapshot[k] = counters[k];
+        __syncthreads();
+    };
+
+    bool snapped = false;
+    for (int cb = 0; cb < len; cb += lay.chunk) {
+        const int cnt = min(lay.chunk, len - cb);
+        if (lane < cnt) entry_data<T>(a, lay, e0 + cb + lane, i, chunk + lane * lay.per_entry);
+        __syncthreads();
+        for (int c = 0; c < cnt; c++) {
+            const long long e = e0 + cb + c;
+            const int j = a.entry_j[e];
+            if (j == i) continue;  // periodic self image: no gradient row of its own
+            if (!snapped && j > i) {
+                take_snapshot();
+                snapped = true;
+            }
+            const T* en = chunk + c * lay.per_entry;
+            for (int s = lane; s < slots; s += WAVE) {
+                const int lm = s / n, nn = s - lm * n;
+                const int l = l_of_lm(lm);
+                const T y = en[lay.y + lm];
+                const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+                for (int x = 0; x < 3; x++) DC[x * slots + s] += ra * en[lay.dir + x] + rb * en[lay.dy + x * L2 + lm];
+            }
+            const bool last_image = (cb + c + 1 >= len) || a.entry_j[e + 1] != j;
+            if (last_image) {
+                __syncthreads();
+                const int tj = a.species[j];
+                emit(j, tj, false);
+                bump(tj);
+                for (int k = lane; k < 3 * slots; k += WAVE) DC[k] = 0;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (!snapped) take_snapshot();
+    emit(i, ct, true);
+}
+
+// Power spectrum strain / cell gradients: one row of [3][3][P] per sample.
+template <typename T>
+__global__ void __launch_bounds__(WAVE) ps_cell_strain_kernel(int n_atoms, int S, int L, int n, const T* __restrict__ coef,
+                                                               const T* __restrict__ dcs, const int* __restrict__ species,
+                                                               const unsigned* __restrict__ type_mask,
+                                                               const long long* __restrict__ sample_flag,
+                                                               const long long* __restrict__ sample_scan, T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int i = blockIdx.x;
+    const int slots = L * L * n;
+    T* C = reinterpret_cast<T*>(smem);
+    T* D1 = C + S * slots;
+    T* D2 = D1 + 9 * slots;
+    for (int k = threadIdx.x; k < S * slots; k += WAVE) C[k] = coef[(long long)i * S * slots + k];
+    __syncthreads();
+    const int ct = species[i];
+    const unsigned mask = type_mask[i];
+    const int P = L * n * n;
+    const int npairs = S * (S + 1) / 2;
+    for (int t1 = 0; t1 < S; t1++) {
+        if (!((mask >> t1) & 1u)) continue;
+        for (int t2 = t1; t2 < S; t2++) {
+            if (!((mask >> t2) & 1u)) continue;
+            const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
+            if (!sample_flag[idx]) continue;
+            const long long s = sample_scan[idx];
+            for (int k = threadIdx.x; k < 9 * slots; k += WAVE) {
+                D1[k] = dcs[((long long)i * S + t1) * 9 * slots + k];
+                D2[k] = dcs[((long long)i * S + t2) * 9 * slots + k];
+            }
+            __syncthreads();
+            const T kappa = t1 != t2 ? (T)1.4142135623730951 : (T)1;
+            ps_contract<T>(9, L, n, C + t1 * slots, C + t2 * slots, D1, D2, true, true, kappa, slots, out + s * 9 * P);
+            __syncthreads();
+        }
+    }
+}
+
+// Spherical expansion values / cell / strain: gather from the dense layouts into
+// per-(type block, l) regions.
+template <typename T>
+__global__ void se_gather_kernel(int n_atoms, int S, int L, int n, int ncomp, const T* __restrict__ dense,
+                                 const int* __restrict__ species, const long long* __restrict__ sample_flag,
+                                 const long long* __restrict__ sample_scan, const long long* __restrict__ block_sample_base,
+                                 const long long* __restrict__ region_base, T* __restrict__ out) {
+    // one workgroup per (type block, atom)
+    const long long idx = blockIdx.x;
+    if (!sample_flag[idx]) return;
+    const int tb = (int)(idx / n_atoms), i = (int)(idx % n_atoms);
+    const int nt = tb % S;
+    const long long s = sample_scan[idx] - block_sample_base[tb];
+    const int slots = L * L * n;
+    for (int l = 0; l < L; l++) {
+        const int w = (2 * l + 1) * n;
+        T* o = out + region_base[(long long)tb * L + l] + s * ncomp * w;
+        for (int k = threadIdx.x; k < ncomp * w; k += blockDim.x) {
+            const int x = k / w, r = k - x * w;
+            o[k] = dense[(((long long)i * S + nt) * ncomp + x) * slots + l * l * n + r];
+        }
+    }
+}
+
+template <typename T>
+SoapArgs<T> make_args(const SoapLaunch<T>& p) {
+    SoapArgs<T> a;
+    a.n_atoms = p.n_atoms;
+    a.n_species = p.n_species;
+    a.L = p.L;
+    a.n = p.n;
+    a.offsets = p.offsets;
+    a.entry_j = p.entry_j;
+    a.entry_shift = p.entry_shift;
+    a.entry_vec = p.entry_vec;
+    a.entry_dist = p.entry_dist;
+    a.species = p.species;
+    a.radial = p.radial;
+    a.scalars = p.scalars;
+    a.sh_a = p.sh_a;
+    a.sh_b = p.sh_b;
+    a.self_contribution = p.self_contribution;
+    a.chunk = p.chunk;
+    a.error_flag = p.error_flag;
+    return a;
+}
+
+template <typename T, int MS, int MODE>
+void launch_center_ms(const SoapLaunch<T>& p, T* coef, T* grad) {
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = (size_t)ChunkLayout(p.L, p.n, p.chunk).per_entry * p.chunk * sizeof(T);
+    soap_center_kernel<T, MS, MODE><<<p.n_atoms, WAVE, shmem, p.stream>>>(a, coef, grad);
+}
+
+}  // namespace
+
+template <typename T>
+size_t soap_chunk_bytes(int L, int n, int chunk) {
+    return (size_t)ChunkLayout(L, n, chunk).per_entry * chunk * sizeof(T);
+}
+template size_t soap_chunk_bytes<float>(int, int, int);
+template size_t soap_chunk_bytes<double>(int, int, int);
+
+template <typename T>
+void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad) {
+    if (p.n_atoms == 0) return;
+    if (mode == 0) {
+        launch_center_ms<T, 16, 0>(p, coef, grad);
+    } else if (mode == 1) {
+        launch_center_ms<T, 8, 1>(p, coef, grad);
+    } else {
+        launch_center_ms<T, 8, 2>(p, coef, grad);
+    }
+}
+
+template <typename T>
+void launch_ps_values(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
+                      const long long* sample_scan, T* out) {
+    if (p.n_atoms == 0) return;
+    const size_t shmem = (size_t)p.n_species * p.L * p.L * p.n * sizeof(T);
+    ps_values_kernel<T><<<p.n_atoms, WAVE, shmem, p.stream>>>(p.n_atoms, p.n_species, p.L, p.n, coef, p.species,
+                                                              type_mask, sample_flag, sample_scan, out);
+}
+
+template <typename T>
+size_t gradient_rows_shmem(const SoapLaunch<T>& p, bool ps) {
+    const size_t slots = (size_t)p.L * p.L * p.n;
+    return ((ps ? p.n_species * slots : 0) + 6 * slots) * sizeof(T) + soap_chunk_bytes<T>(p.L, p.n, p.chunk);
+}
+
+template <typename T>
+void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const T* dself, const unsigned* type_mask,
+                          const long long* sample_flag, const long long* sample_scan, const long long* row_scan,
+                          const long long* block_sample_base, const long long* se_grad_base, const int* atom_system,
+                          const int* atom_local, T* out, int* labels) {
+    if (p.n_atoms == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = gradient_rows_shmem<T>(p, ps);
+    if (ps) {
+        gradient_rows_kernel<T, true><<<p.n_atoms, WAVE, shmem, p.stream>>>(a, coef, dself, type_mask, sample_flag,
+                                                                            sample_scan, row_scan, block_sample_base,
+                                                                            se_grad_base, atom_system, atom_local, out, labels);
+    } else {
+        gradient_rows_kernel<T, false><<<p.n_atoms, WAVE, shmem, p.stream>>>(a, coef, dself, type_mask, sample_flag,
+                                                                             sample_scan, row_scan, block_sample_base,
+                                                                             se_grad_base, atom_system, atom_local, out, labels);
+    }
+}
+
+template <typename T>
+void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
+                           const long long* sample_flag, const long long* sample_scan, T* out) {
+    if (p.n_atoms == 0) return;
+    const size_t slots = (size_t)p.L * p.L * p.n;
+    const size_t shmem = (p.n_species + 18) * slots * sizeof(T);
+    ps_cell_strain_kernel<T><<<p.n_atoms, WAVE, shmem, p.stream>>>(p.n_atoms, p.n_species, p.L, p.n, coef, dcs, p.species,
+                                                                   type_mask, sample_flag, sample_scan, out);
+}
+
+template <typename T>
+void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const long long* sample_flag,
+                      const long long* sample_scan, const long long* block_sample_base, const long long* region_base, T* out) {
+    const long long total = (long long)p.n_species * p.n_species * p.n_atoms;
+    if (total == 0) return;
+    se_gather_kernel<T><<<(unsigned)total, 64, 0, p.stream>>>(p.n_atoms, p.n_species, p.L, p.n, ncomp, dense, p.species,
+                                                              sample_flag, sample_scan, block_sample_base, region_base, out);
+}
+
+#define RA_INSTANTIATE(T)                                                                                              \
+    template void launch_soap_center<T>(const SoapLaunch<T>&, int, T*, T*);                                            \
+    template void launch_ps_values<T>(const SoapLaunch<T>&, const T*, const unsigned*, const long long*,              \
+                                      const long long*, T*);                                                           \
+    template size_t gradient_rows_shmem<T>(const SoapLaunch<T>&, bool);                                                \
+    template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
+                                          const long long*, const long long*, const long long*, const long long*,      \
+                                          const long long*, const int*, const int*, T*, int*);                         \
+    template void launch_ps_cell_strain<T>(const SoapLaunch<T>&, const T*, const T*, const unsigned*,                 \
+                                           const long long*, const long long*, T*);                                    \
+    template void launch_se_gather<T>(const SoapLaunch<T>&, int, const T*, const long long*, const long long*,        \
+                                      const long long*, const long long*, T*);
+
+RA_INSTANTIATE(float)
+RA_INSTANTIATE(double)
+
+}  // namespace rascaline_amd

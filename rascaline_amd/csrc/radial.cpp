@@ -1,0 +1,252 @@
+#include "radial.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "special.hpp"
+
+namespace rascaline_amd {
+
+namespace {
+
+// Jacobi eigen-decomposition of a small symmetric matrix; returns eigenvalues,
+// `a` is replaced by the eigenvectors (columns).
+std::vector<double> jacobi_eigen(std::vector<double>& a, int n) {
+    std::vector<double> v(n * n, 0.0);
+    for (int i = 0; i < n; i++) v[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) off += a[p * n + q] * a[p * n + q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; p++) {
+            for (int q = p + 1; q < n; q++) {
+                const double apq = a[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    const double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double vkp = v[k * n + p], vkq = v[k * n + q];
+                    v[k * n + p] = c * vkp - s * vkq;
+                    v[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    std::vector<double> w(n);
+    for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+    a = v;
+    return w;
+}
+
+void hermite(const SplineChannel& ch, int n, double x, double* values, double* grads) {
+    const int K = static_cast<int>(ch.positions.size()) - 1;
+    // last knot <= x; evaluating exactly at the last knot uses the previous interval
+    int k = static_cast<int>(std::upper_bound(ch.positions.begin(), ch.positions.end(), x) - ch.positions.begin()) - 1;
+    if (k < 0) k = 0;
+    if (k >= K) k = K - 1;
+    const double x0 = ch.positions[k], x1 = ch.positions[k + 1];
+    const double delta = x1 - x0;
+    const double t = (x - x0) / delta, t2 = t * t, t3 = t2 * t;
+    const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+    const double h01 = -2.0 * t3 + 3.0 * t2, h11 = t3 - t2;
+    const double* p0 = &ch.values[k * n];
+    const double* p1 = &ch.values[(k + 1) * n];
+    const double* m0 = &ch.derivatives[k * n];
+    const double* m1 = &ch.derivatives[(k + 1) * n];
+    for (int i = 0; i < n; i++) values[i] = h00 * p0[i] + h10 * delta * m0[i] + h01 * p1[i] + h11 * delta * m1[i];
+    if (grads) {
+        const double d00 = 6.0 * (t2 - t), d10 = 3.0 * t2 - 4.0 * t + 1.0, d01 = -d00, d11 = 3.0 * t2 - 2.0 * t;
+        for (int i = 0; i < n; i++) grads[i] = d00 * p0[i] / delta + d10 * m0[i] + d01 * p1[i] / delta + d11 * m1[i];
+    }
+}
+
+SplineChannel build_channel(const RadialTables& t, int l, double accuracy) {
+    const int n = t.n;
+    SplineChannel ch;
+    const double step = t.stop / 10.0;
+    for (int k = 0; k <= 10; k++) {
+        const double x = k < 10 ? k * step : t.stop;
+        ch.positions.push_back(x);
+        ch.values.resize((k + 1) * n);
+        ch.derivatives.resize((k + 1) * n);
+        t.exact(l, x, &ch.values[k * n], &ch.derivatives[k * n]);
+    }
+    std::vector<double> interp(n), nv, nd;
+    for (;;) {
+        const int count = static_cast<int>(ch.positions.size());
+        const int n_new = count - 1;
+        std::vector<double> npos(n_new);
+        nv.assign(n_new * n, 0.0);
+        nd.assign(n_new * n, 0.0);
+        double mean_abs = 0.0, mean_rel = 0.0;
+        for (int k = 0; k < n_new; k++) {
+            const double x = (ch.positions[k] + ch.positions[k + 1]) / 2.0;
+            npos[k] = x;
+            t.exact(l, x, &nv[k * n], &nd[k * n]);
+            hermite(ch, n, x, interp.data(), nullptr);
+            for (int i = 0; i < n; i++) {
+                const double v = nv[k * n + i];
+                mean_abs += std::fabs(interp[i] - v);
+                mean_rel += std::fabs((interp[i] - v) / v);
+            }
+        }
+        mean_abs /= static_cast<double>(n_new * n);
+        mean_rel /= static_cast<double>(n_new * n);
+        if (mean_abs < accuracy || mean_rel < accuracy) break;
+        if (count + n_new > 10000) {
+            throw Error::internal("failed to reach requested accuracy (" + std::to_string(accuracy) +
+                                  ") in spline interpolation, mean absolute error is " + std::to_string(mean_abs) +
+                                  " and mean relative error is " + std::to_string(mean_rel));
+        }
+        SplineChannel next;
+        next.positions.resize(count + n_new);
+        next.values.resize((count + n_new) * n);
+        next.derivatives.resize((count + n_new) * n);
+        for (int k = 0; k < count; k++) {
+            next.positions[2 * k] = ch.positions[k];
+            std::copy(&ch.values[k * n], &ch.values[k * n] + n, &next.values[2 * k * n]);
+            std::copy(&ch.derivatives[k * n], &ch.derivatives[k * n] + n, &next.derivatives[2 * k * n]);
+            if (k < n_new) {
+                next.positions[2 * k + 1] = npos[k];
+                std::copy(&nv[k * n], &nv[k * n] + n, &next.values[(2 * k + 1) * n]);
+                std::copy(&nd[k * n], &nd[k * n] + n, &next.derivatives[(2 * k + 1) * n]);
+            }
+        }
+        ch = std::move(next);
+    }
+    return ch;
+}
+
+}  // namespace
+
+std::vector<double> gto_orthonormalization(double radius, int n) {
+    std::vector<double> sigma(n);
+    for (int i = 0; i < n; i++) sigma[i] = gto_width(radius, i, n);
+    std::vector<double> S(n * n);
+    for (int a = 0; a < n; a++) {
+        for (int b = a; b < n; b++) {
+            const double sa = sigma[a], sb = sigma[b];
+            const double e = 0.5 * (3.0 + a + b);
+            const double v = std::pow(0.5 / (sa * sa) + 0.5 / (sb * sb), -e) / (std::pow(sa, a) * std::pow(sb, b)) *
+                             std::tgamma(e) / (std::pow(sa * sb, 1.5) * std::sqrt(std::tgamma(a + 1.5) * std::tgamma(b + 1.5)));
+            S[a * n + b] = v;
+            S[b * n + a] = v;
+        }
+    }
+    std::vector<double> w = jacobi_eigen(S, n);
+    for (int i = 0; i < n; i++) {
+        if (w[i] <= 2.220446049250313e-16) {
+            throw Error::internal("radial overlap matrix is singular, try with a lower max_radial (current value is " +
+                                  std::to_string(n - 1) + ")");
+        }
+        w[i] = 1.0 / std::sqrt(w[i]);
+    }
+    std::vector<double> O(n * n);
+    for (int a = 0; a < n; a++) {
+        for (int b = 0; b < n; b++) {
+            const double norm_b = std::sqrt(2.0 / (std::pow(sigma[b], 2 * b + 3) * std::tgamma(b + 1.5)));
+            double acc = 0.0;
+            for (int k = 0; k < n; k++) acc += S[a * n + k] * w[k] * S[b * n + k];
+            O[a * n + b] = acc * norm_b;
+        }
+    }
+    return O;
+}
+
+void RadialTables::exact(int l, double x, double* values, double* grads) const {
+    double raw_v[64], raw_g[64];
+    for (int i = 0; i < n; i++) {
+        if (lode) {
+            lode_gto_raw(l, i, x, gto[i], &raw_v[i], &raw_g[i]);
+        } else {
+            soap_gto_raw(l, i, x, sigma2_atomic, gto[i], &raw_v[i], &raw_g[i]);
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        double v = 0.0, g = 0.0;
+        for (int k = 0; k < n; k++) {
+            v += raw_v[k] * ortho[j * n + k];
+            g += raw_g[k] * ortho[j * n + k];
+        }
+        values[j] = v;
+        if (grads) grads[j] = g;
+    }
+}
+
+RadialTables build_soap_radial(const SoapParams& p) {
+    RadialTables t;
+    t.lode = false;
+    t.max_angular = p.max_angular;
+    t.n = p.n_radial;
+    t.stop = p.cutoff;
+    t.sigma2_atomic = p.gaussian_width * p.gaussian_width;
+    t.gto.resize(t.n);
+    for (int i = 0; i < t.n; i++) {
+        const double s = gto_width(p.cutoff, i, t.n);
+        t.gto[i] = 1.0 / (2.0 * s * s);
+    }
+    t.ortho = gto_orthonormalization(p.cutoff, t.n);
+    t.use_spline = p.use_spline;
+    if (t.use_spline) {
+        for (int l = 0; l <= p.max_angular; l++) t.channels.push_back(build_channel(t, l, p.spline_accuracy));
+    }
+    return t;
+}
+
+RadialTables build_lode_radial(const LodeParams& p) {
+    RadialTables t;
+    t.lode = true;
+    t.max_angular = p.max_angular;
+    t.n = p.n_radial;
+    t.stop = p.k_cutoff;
+    t.gto.resize(t.n);
+    for (int i = 0; i < t.n; i++) t.gto[i] = gto_width(p.radius, i, t.n);
+    t.ortho = gto_orthonormalization(p.radius, t.n);
+    t.use_spline = p.use_spline;
+    if (t.use_spline) {
+        for (int l = 0; l <= p.max_angular; l++) t.channels.push_back(build_channel(t, l, p.spline_accuracy));
+    }
+    return t;
+}
+
+std::vector<double> lode_center_contribution(const LodeParams& p, const RadialTables& t) {
+    const int n = t.n;
+    std::vector<double> contrib(n), out(n);
+    const double s = p.smearing, e = static_cast<double>(p.exponent);
+    for (int i = 0; i < n; i++) {
+        const double n_eff = 0.5 * (3.0 + i);
+        const double w = t.gto[i];
+        if (p.exponent == 0) {
+            const double factor = std::pow(RA_PI, -0.25) / std::pow(s * s, 0.75);
+            const double alpha = 0.5 * (1.0 / (s * s) + 1.0 / (w * w));
+            contrib[i] = factor * std::tgamma(n_eff) / std::pow(alpha, n_eff);
+        } else {
+            const double factor = 2.0 * std::sqrt(4.0 * RA_PI) / std::tgamma(e / 2.0) / e;
+            const double r = s / w;
+            const double arg = 1.0 / (1.0 + r * r);
+            contrib[i] = factor * std::pow(2.0, (1.0 + i - e) / 2.0) * std::pow(s, 3.0 + i - e) * std::tgamma(n_eff) *
+                         hyp2f1_series(1.0, n_eff, (e + 2.0) / 2.0, arg) * std::pow(arg, n_eff);
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        double acc = 0.0;
+        for (int k = 0; k < n; k++) acc += contrib[k] * t.ortho[j * n + k];
+        out[j] = acc;
+    }
+    return out;
+}
+
+}  // namespace rascaline_amd

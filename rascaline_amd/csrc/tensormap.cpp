@@ -1,0 +1,137 @@
+// metatensor-core compatible accessors over the host TensorMap.
+#include "tensormap.hpp"
+
+#include <cstring>
+#include <string>
+
+namespace {
+thread_local std::string MTS_LAST_ERROR;
+
+mts_status_t mts_error(const std::string& message) {
+    MTS_LAST_ERROR = message;
+    return MTS_INVALID_PARAMETER_ERROR;
+}
+
+mts_status_t array_origin(const void*, mts_data_origin_t* origin) {
+    *origin = 0;
+    return MTS_SUCCESS;
+}
+
+mts_status_t array_data(void* array, double** data) {
+    auto* a = static_cast<rascaline_amd::ArrayData*>(array);
+    *data = a->data.data();
+    return MTS_SUCCESS;
+}
+
+mts_status_t array_shape(const void* array, const uintptr_t** shape, uintptr_t* count) {
+    auto* a = static_cast<const rascaline_amd::ArrayData*>(array);
+    *shape = a->shape.data();
+    *count = a->shape.size();
+    return MTS_SUCCESS;
+}
+
+mts_status_t array_unsupported() {
+    MTS_LAST_ERROR = "this operation is not supported on arrays owned by the TensorMap";
+    return MTS_INTERNAL_ERROR;
+}
+
+mts_status_t array_reshape(void*, const uintptr_t*, uintptr_t) { return array_unsupported(); }
+mts_status_t array_swap(void*, uintptr_t, uintptr_t) { return array_unsupported(); }
+mts_status_t array_create(const void*, const uintptr_t*, uintptr_t, mts_array_t*) { return array_unsupported(); }
+mts_status_t array_copy(const void*, mts_array_t*) { return array_unsupported(); }
+void array_destroy(void*) {}
+mts_status_t array_move(void*, const void*, const mts_sample_mapping_t*, uintptr_t, uintptr_t, uintptr_t) {
+    return array_unsupported();
+}
+}  // namespace
+
+void rascaline_amd::LabelsData::fill(mts_labels_t* out) const {
+    out->internal_ptr_ = this;
+    out->names = name_ptrs.data();
+    out->values = values.data();
+    out->size = names.size();
+    out->count = count();
+}
+
+extern "C" {
+
+const char* mts_last_error(void) { return MTS_LAST_ERROR.c_str(); }
+
+mts_status_t mts_labels_free(mts_labels_t* labels) {
+    if (labels) {
+        labels->internal_ptr_ = nullptr;
+        labels->names = nullptr;
+        labels->values = nullptr;
+        labels->size = 0;
+        labels->count = 0;
+    }
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_tensormap_free(mts_tensormap_t* tensor) {
+    delete tensor;
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_tensormap_keys(const mts_tensormap_t* tensor, mts_labels_t* keys) {
+    if (!tensor || !keys) return mts_error("got a NULL pointer");
+    tensor->keys.fill(keys);
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_tensormap_block_by_id(mts_tensormap_t* tensor, mts_block_t** block, uintptr_t index) {
+    if (!tensor || !block) return mts_error("got a NULL pointer");
+    if (index >= tensor->blocks.size()) return mts_error("block index out of bounds");
+    *block = tensor->blocks[index].get();
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_block_labels(const mts_block_t* block, uintptr_t axis, mts_labels_t* labels) {
+    if (!block || !labels) return mts_error("got a NULL pointer");
+    const uintptr_t n_axes = block->components.size() + 2;
+    if (axis >= n_axes) return mts_error("axis out of bounds");
+    if (axis == 0) {
+        block->samples.fill(labels);
+    } else if (axis == n_axes - 1) {
+        block->properties.fill(labels);
+    } else {
+        block->components[axis - 1].fill(labels);
+    }
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_block_data(mts_block_t* block, mts_array_t* data) {
+    if (!block || !data) return mts_error("got a NULL pointer");
+    std::memset(data, 0, sizeof(*data));
+    data->ptr = &block->array;
+    data->origin = array_origin;
+    data->data = array_data;
+    data->shape = array_shape;
+    data->reshape = array_reshape;
+    data->swap_axes = array_swap;
+    data->create = array_create;
+    data->copy = array_copy;
+    data->destroy = array_destroy;
+    data->move_samples_from = array_move;
+    return MTS_SUCCESS;
+}
+
+mts_status_t mts_block_gradient(mts_block_t* block, const char* parameter, mts_block_t** gradient) {
+    if (!block || !parameter || !gradient) return mts_error("got a NULL pointer");
+    for (size_t k = 0; k < block->gradient_names.size(); k++) {
+        if (block->gradient_names[k] == parameter) {
+            *gradient = block->gradients[k].get();
+            return MTS_SUCCESS;
+        }
+    }
+    return mts_error(std::string("can not find gradients with respect to '") + parameter + "' in this block");
+}
+
+mts_status_t mts_block_gradients_list(const mts_block_t* block, const char* const** parameters, uintptr_t* count) {
+    if (!block || !parameters || !count) return mts_error("got a NULL pointer");
+    *parameters = block->gradient_name_ptrs.data();
+    *count = block->gradient_name_ptrs.size();
+    return MTS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Host TensorMap returned through the C API (metatensor-core compatible subset).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/metatensor.h"
+
+namespace rascaline_amd {
+
+struct LabelsData {
+    std::vector<std::string> names;
+    std::vector<const char*> name_ptrs;
+    std::vector<int32_t> values;  // count x size
+
+    LabelsData() = default;
+    LabelsData(std::vector<std::string> n, std::vector<int32_t> v) : names(std::move(n)), values(std::move(v)) { finish(); }
+    void finish() {
+        name_ptrs.clear();
+        for (auto& s : names) name_ptrs.push_back(s.c_str());
+    }
+    size_t size() const { return names.size(); }
+    size_t count() const { return names.empty() ? 0 : values.size() / names.size(); }
+    void fill(mts_labels_t* out) const;
+
+    static LabelsData range(const std::string& name, int n) {
+        std::vector<int32_t> v(n);
+        for (int i = 0; i < n; i++) v[i] = i;
+        return LabelsData({name}, v);
+    }
+};
+
+// Contiguous float64 array owned by a block.
+struct ArrayData {
+    std::vector<uintptr_t> shape;
+    std::vector<double> data;
+};
+
+}  // namespace rascaline_amd
+
+struct mts_block_t {
+    rascaline_amd::LabelsData samples;
+    std::vector<rascaline_amd::LabelsData> components;
+    rascaline_amd::LabelsData properties;
+    rascaline_amd::ArrayData array;
+    std::vector<std::string> gradient_names;
+    std::vector<const char*> gradient_name_ptrs;
+    std::vector<std::unique_ptr<mts_block_t>> gradients;
+
+    void add_gradient(const std::string& name, std::unique_ptr<mts_block_t> g) {
+        gradient_names.push_back(name);
+        gradients.push_back(std::move(g));
+        gradient_name_ptrs.clear();
+        for (auto& s : gradient_names) gradient_name_ptrs.push_back(s.c_str());
+    }
+};
+
+struct mts_tensormap_t {
+    rascaline_amd::LabelsData keys;
+    std::vector<std::unique_ptr<mts_block_t>> blocks;
+};

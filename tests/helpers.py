@@ -81,3 +81,83 @@ def max_rel_err(actual, expected, floor=0.0):
     denom = np.maximum(np.maximum(np.abs(actual), np.abs(expected)), floor)
     denom = np.where(denom == 0, 1.0, denom)
     return float(np.max(np.abs(actual - expected) / denom)) if actual.size else 0.0
+
+
+def random_system(rng, n_atoms, kind="periodic", L=8.0, types=(1, 6, 8), min_dist=0.8):
+    """Random atoms with a minimum-image rejection distance.
+
+    kind: "periodic" (cubic L), "triclinic", "cluster" (no cell), "outside"
+    (cubic, atoms placed outside the unit cell too)."""
+    if kind == "triclinic":
+        cell = np.array([[L, 0, 0], [0.3 * L, 0.9 * L, 0], [-0.2 * L, 0.25 * L, 0.95 * L]])
+    elif kind == "cluster":
+        cell = np.zeros((3, 3))
+    else:
+        cell = np.eye(3) * L
+    pos = []
+    tries = 0
+    while len(pos) < n_atoms and tries < 100000:
+        tries += 1
+        if kind == "cluster":
+            p = rng.uniform(0, L, 3)
+        else:
+            f = rng.uniform(0, 1, 3)
+            p = f @ cell
+            if kind == "outside":
+                p = p + rng.integers(-2, 3, 3) * L
+        ok = True
+        for q in pos:
+            d = p - q
+            if kind != "cluster":
+                frac = np.linalg.solve(cell.T, d)
+                frac -= np.round(frac)
+                d = frac @ cell
+            if np.dot(d, d) < min_dist ** 2:
+                ok = False
+                break
+        if ok:
+            pos.append(p)
+    pos = np.array(pos)
+    t = rng.choice(np.array(types), size=len(pos))
+    return pos, t.astype(np.int32), cell
+
+
+def compare_tensormaps(actual, expected, rtol, atol_scale, gradients=(), check_labels=True):
+    """Block-by-block comparison: identical keys and labels, values within
+    |a - b| <= rtol * |b| + atol_scale * max|b over the whole block|."""
+    assert actual.keys.names == expected.keys.names
+    assert np.array_equal(actual.keys.values, expected.keys.values), (actual.keys.values, expected.keys.values)
+    worst = 0.0
+    for ba, be in zip(actual.blocks(), expected.blocks()):
+        if check_labels:
+            assert ba.samples == be.samples, "samples differ"
+            assert ba.properties == be.properties, "properties differ"
+            assert len(ba.components) == len(be.components)
+            for ca, ce in zip(ba.components, be.components):
+                assert ca == ce
+        worst = max(worst, _close(ba.values, be.values, rtol, atol_scale))
+        for param in gradients:
+            ga, ge = ba.gradient(param), be.gradient(param)
+            if check_labels:
+                assert ga.samples == ge.samples, f"{param} gradient samples differ"
+                for ca, ce in zip(ga.components, ge.components):
+                    assert ca == ce
+            worst = max(worst, _close(ga.values, ge.values, rtol, atol_scale))
+    return worst
+
+
+def _close(a, b, rtol, atol_scale):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    if b.size == 0:
+        return 0.0
+    scale = float(np.max(np.abs(b)))
+    diff = np.abs(a - b)
+    bound = rtol * np.abs(b) + atol_scale * scale
+    bad = diff > bound
+    if np.any(bad):
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError(f"{int(bad.sum())}/{bad.size} elements differ (scale {scale:.3e}): " +
+                             ", ".join(f"{tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}" for i in idx))
+    return float(np.max(diff / np.maximum(np.abs(b) + scale * 1e-300, 1e-300))) if scale > 0 else 0.0
