@@ -1,0 +1,245 @@
+"""Benchmark: SOAP power spectrum + position gradients, atoms/s on one MI355X.
+
+Workload (the batched configuration the headline metric is quoted on): 4096
+synthetic structures with U{8..256} atoms, half periodic (cubic / triclinic,
+0.08-0.12 atoms/A^3) and half open clusters in 15 A boxes, 4 atomic types;
+cutoff 5 A (ShiftedCosine 0.5), Gaussian density 0.3 A, GTO basis with 8 radial
+functions (JSON max_radial 7) and max_angular 6; FP32 device kernels.
+
+One step is one pass of the hot path over the whole batch with positions
+already resident in GPU memory: neighbour list, per-center spherical expansion,
+power spectrum values and position-gradient rows. The batch is processed in
+chunks of structures whose outputs fit in HBM (the full position-gradient
+output is several hundred GB); each chunk's output is produced in device memory
+and left there for the consumer.
+
+Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atoms/s SOAP power spectrum+grads: fast form vs plain FP32 @<=1e-4 rel err"
+HBM_PEAK_GBS = 8000.0
+
+HYPERS = {
+    "cutoff": {"radius": 5.0, "smoothing": {"type": "ShiftedCosine", "width": 0.5}},
+    "density": {"type": "Gaussian", "width": 0.3},
+    "basis": {"type": "TensorProduct", "max_angular": 6, "radial": {"type": "Gto", "max_radial": 7}},
+}
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def make_chunks(systems, max_atoms):
+    chunks, current, atoms = [], [], 0
+    for k, (pos, t, cell) in enumerate(systems):
+        if current and atoms + len(t) > max_atoms:
+            chunks.append(current)
+            current, atoms = [], 0
+        current.append(k)
+        atoms += len(t)
+    if current:
+        chunks.append(current)
+    return chunks
+
+
+class DeviceBatch:
+    """Positions of one chunk resident in device memory (torch allocation)."""
+
+    def __init__(self, systems, indices, torch):
+        from rascaline_amd import _c_lib as C
+
+        self.indices = indices
+        pos = np.concatenate([systems[k][0] for k in indices]).astype(np.float64)
+        self.n_atoms = len(pos)
+        self.positions = torch.from_numpy(pos).to("cuda:0")
+        self.types = np.ascontiguousarray(np.concatenate([systems[k][1] for k in indices]).astype(np.int32))
+        self.cells = np.ascontiguousarray(np.stack([systems[k][2].reshape(9) for k in indices]).astype(np.float64))
+        self.offsets = np.zeros(len(indices) + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum([len(systems[k][1]) for k in indices])
+        self.struct = C.featomic_amd_device_systems_t()
+        self.struct.n_systems = len(indices)
+        self.struct.atom_offsets = self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        self.struct.positions = ctypes.c_void_p(self.positions.data_ptr())
+        self.struct.types = self.types.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self.struct.cells = self.cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def run_chunk(calc, batch, stream_ptr, grads):
+    from rascaline_amd import _c_lib as C
+
+    out = C.featomic_amd_device_output_t()
+    C.check(C.lib().featomic_amd_compute_device(calc._as_parameter_, ctypes.byref(batch.struct), grads, 1,
+                                                ctypes.c_void_p(stream_ptr), ctypes.byref(out)))
+    return out
+
+
+def cpu_baseline(systems, target_seconds=15.0):
+    """The FP64 C restatement of the reference algorithm (oracle) on the host cores,
+    over a bounded sample of the same structures."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import featomic_oracle as orc
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    osys = [orc.System(p, t, c) for (p, t, c) in systems]
+    # calibrate on a few structures, then size the sample to ~target_seconds
+    n = 2 * cores
+    t0 = time.perf_counter()
+    orc.soap_ps_baseline(osys[:n], HYPERS, n_threads=cores)
+    dt = time.perf_counter() - t0
+    per_structure = max(dt / n, 1e-4)
+    n = int(min(len(osys), max(n, target_seconds / per_structure)))
+    t0 = time.perf_counter()
+    rows, checksum = orc.soap_ps_baseline(osys[:n], HYPERS, n_threads=cores)
+    dt = time.perf_counter() - t0
+    atoms = sum(len(s.types) for s in osys[:n])
+    return {
+        "value": atoms / dt,
+        "unit": "atoms/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {n} of the benchmark's structures ({atoms} atoms, {rows} gradient rows), FP64, "
+                  f"{cores} threads, {dt:.1f} s",
+    }
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--structures", type=int, default=4096)
+    parser.add_argument("--chunk-atoms", type=int, default=48000)
+    parser.add_argument("--precision", type=int, default=32)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    args = parser.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md but not built", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import rascaline_amd as ra
+    from rascaline_amd.datasets import batched_uneven
+
+    t0 = time.perf_counter()
+    systems = batched_uneven(seed=0, n_structures=args.structures)
+    n_atoms = sum(len(s[1]) for s in systems)
+    log(f"generated {len(systems)} structures, {n_atoms} atoms in {time.perf_counter() - t0:.1f} s")
+
+    calc = ra.SoapPowerSpectrum(**HYPERS)
+    calc.set_precision(args.precision)
+    chunks = make_chunks(systems, args.chunk_atoms)
+    batches = [DeviceBatch(systems, c, torch) for c in chunks]
+    stream = torch.cuda.current_stream().cuda_stream
+    grads = (ctypes.c_char_p * 1)(b"positions")
+    log(f"{len(batches)} chunks of up to {args.chunk_atoms} atoms")
+
+    def step():
+        for b in batches:
+            run_chunk(calc, b, stream, grads)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup step {w} done")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n_atoms * args.steps / elapsed
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.3e} atoms/s")
+
+    # roofline of the dominant kernel (power-spectrum gradient rows), from hipEvents
+    # on the compute stream, one extra pass
+    from rascaline_amd import _c_lib as C
+
+    total_bytes, kernel_ms, device_ms = 0, 0.0, 0.0
+    rows_total, samples_total = 0, 0
+    for b in batches:
+        out = run_chunk(calc, b, stream, grads)
+        t_ms, k_ms, k_bytes = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
+                                                  ctypes.byref(k_bytes)))
+        rows = sum(out.n_gradient_rows[k] for k in range(out.n_blocks))
+        samples = sum(out.n_samples[k] for k in range(out.n_blocks))
+        rows_total += rows
+        samples_total += samples
+        # algorithmic bytes of the gradient-row kernel, as counted by the library:
+        # gradient values + labels written, per-center coefficients and self gradients
+        # read once, neighbour entries read once
+        total_bytes += k_bytes.value
+        kernel_ms += k_ms.value
+        device_ms += t_ms.value
+    achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": "gradient_rows_kernel<float, PS=true>",
+        "kernel_ms_per_step": kernel_ms,
+        "device_ms_per_step": device_ms,
+        "algorithmic_bytes_per_step": total_bytes,
+    }
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        baseline = cpu_baseline(systems)
+        log(f"cpu baseline: {baseline}")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atoms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(systems)} uneven structures (8-256 atoms; half periodic cubic/triclinic at "
+                        f"0.08-0.12 A^-3, half clusters in 15 A boxes; 4 types), SOAP power spectrum + position "
+                        f"gradients, rc 5.0, n 8, l<=6",
+            "form": "plain FP32" if args.precision == 32 else "FP64",
+            "atoms": n_atoms,
+            "chunks": len(batches),
+            "power_spectrum_samples": samples_total,
+            "gradient_rows": rows_total,
+        },
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -39,35 +39,81 @@ static void gaussian_widths(double radius, int n, double* out) {
     }
 }
 
-/* featomic/src/calculators/shared/basis/radial/gto.rs:17-41,53-77 */
+/* featomic/src/calculators/shared/basis/radial/gto.rs:17-41,53-77.
+ * Deviation from the reference: S and S^{-1/2} are evaluated in long double.
+ * The overlap is ill-conditioned (condition number ~3e4 for 8 functions, ~4e7
+ * for 12), so an FP64 eigen-solver (the reference's JAMA Householder + QL) leaves
+ * noise of order cond(S) * 1e-16 in the orthonormalisation; extended precision
+ * removes it, so that this oracle pins the mathematical value. */
+static void jacobi_ld(int n, long double* a, long double* w) {
+    long double* v = calloc((size_t)(n * n), sizeof(long double));
+    for (int i = 0; i < n; i++) v[i * n + i] = 1.0L;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        long double off = 0.0L;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) off += a[p * n + q] * a[p * n + q];
+        if (off < 1e-4000L) break;
+        for (int p = 0; p < n; p++) {
+            for (int q = p + 1; q < n; q++) {
+                long double apq = a[p * n + q];
+                if (fabsl(apq) < 1e-4000L) continue;
+                long double theta = (a[q * n + q] - a[p * n + p]) / (2.0L * apq);
+                long double t = (theta >= 0 ? 1.0L : -1.0L) / (fabsl(theta) + sqrtl(theta * theta + 1.0L));
+                long double c = 1.0L / sqrtl(t * t + 1.0L), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    long double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    long double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    long double vkp = v[k * n + p], vkq = v[k * n + q];
+                    v[k * n + p] = c * vkp - s * vkq;
+                    v[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+    memcpy(a, v, sizeof(long double) * (size_t)(n * n));
+    free(v);
+}
+
 static int orthonormalization(double radius, int n, double* out) {
-    double* sigma = malloc(sizeof(double) * n);
-    gaussian_widths(radius, n, sigma);
-    double* S = malloc(sizeof(double) * n * n);
+    long double* sigma = malloc(sizeof(long double) * n);
+    for (int i = 0; i < n; i++) {
+        long double s = sqrtl((long double)i);
+        sigma[i] = (long double)radius * (s > 1.0L ? s : 1.0L) / (long double)n;
+    }
+    long double* S = malloc(sizeof(long double) * n * n);
     for (int n1 = 0; n1 < n; n1++) {
-        double s1 = sigma[n1], s1sq = s1 * s1;
+        long double s1 = sigma[n1], s1sq = s1 * s1;
         for (int n2 = n1; n2 < n; n2++) {
-            double s2 = sigma[n2], s2sq = s2 * s2;
-            double e = 0.5 * (3.0 + n1 + n2);
-            double v = pow(0.5 / s1sq + 0.5 / s2sq, -e) / (pow(s1, n1) * pow(s2, n2)) * orc_gamma(e) /
-                       (pow(s1 * s2, 1.5) * sqrt(orc_gamma(n1 + 1.5) * orc_gamma(n2 + 1.5)));
+            long double s2 = sigma[n2], s2sq = s2 * s2;
+            long double e = 0.5L * (3.0L + n1 + n2);
+            long double v = powl(0.5L / s1sq + 0.5L / s2sq, -e) / (powl(s1, n1) * powl(s2, n2)) * tgammal(e) /
+                            (powl(s1 * s2, 1.5L) * sqrtl(tgammal(n1 + 1.5L) * tgammal(n2 + 1.5L)));
             S[n2 * n + n1] = v;
             S[n1 * n + n2] = v;
         }
     }
-    double* w = malloc(sizeof(double) * n);
-    orc_symmetric_eigen(n, S, w);
+    long double* w = malloc(sizeof(long double) * n);
+    jacobi_ld(n, S, w);
     int status = 0;
     for (int i = 0; i < n; i++) {
-        if (w[i] <= 2.220446049250313e-16) status = -1;
-        w[i] = 1.0 / sqrt(w[i]);
+        if (w[i] <= 2.220446049250313e-16L) status = -1;
+        w[i] = 1.0L / sqrtl(w[i]);
     }
     for (int a = 0; a < n; a++) {
         for (int b = 0; b < n; b++) {
-            double norm_b = sqrt(2.0 / (pow(sigma[b], 2 * b + 3) * orc_gamma(b + 1.5)));
-            double acc = 0.0;
+            long double norm_b = sqrtl(2.0L / (powl(sigma[b], 2 * b + 3) * tgammal(b + 1.5L)));
+            long double acc = 0.0L;
             for (int k = 0; k < n; k++) acc += S[a * n + k] * w[k] * S[b * n + k];
-            out[a * n + b] = acc * norm_b;
+            out[a * n + b] = (double)(acc * norm_b);
         }
     }
     free(sigma);

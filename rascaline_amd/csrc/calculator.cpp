@@ -475,7 +475,12 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                                 d_se_grad, d_atom_system, d_atom_local, d_grads, grad_labels);
     }
     RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
-    main_bytes_ = grads.positions ? grads_size * static_cast<long long>(sizeof(T)) : 0;
+    // algorithmic bytes of the gradient-row kernel: gradient values and labels written once,
+    // per-center coefficients and self gradients read once, neighbour entries read once
+    main_bytes_ = grads.positions ? grads_size * static_cast<long long>(sizeof(T)) + total_rows * 12 +
+                                        static_cast<long long>(n_atoms) * S * static_cast<long long>(slots) * 4 * sizeof(T) +
+                                        nl.n_entries * 48
+                                  : 0;
     for (int which = 0; which < 2; which++) {
         const T* dense = which == 0 ? dstrain : dcell;
         T* out = which == 0 ? d_strain : d_cell;

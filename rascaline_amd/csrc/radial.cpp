@@ -10,42 +10,42 @@ namespace rascaline_amd {
 
 namespace {
 
-// Jacobi eigen-decomposition of a small symmetric matrix; returns eigenvalues,
-// `a` is replaced by the eigenvectors (columns).
-std::vector<double> jacobi_eigen(std::vector<double>& a, int n) {
-    std::vector<double> v(n * n, 0.0);
-    for (int i = 0; i < n; i++) v[i * n + i] = 1.0;
+// Jacobi eigen-decomposition of a small symmetric matrix in extended precision;
+// returns eigenvalues, `a` is replaced by the eigenvectors (columns).
+std::vector<long double> jacobi_eigen(std::vector<long double>& a, int n) {
+    std::vector<long double> v(n * n, 0.0L);
+    for (int i = 0; i < n; i++) v[i * n + i] = 1.0L;
     for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0.0;
+        long double off = 0.0L;
         for (int p = 0; p < n; p++)
             for (int q = p + 1; q < n; q++) off += a[p * n + q] * a[p * n + q];
-        if (off < 1e-300) break;
+        if (off < 1e-4000L) break;
         for (int p = 0; p < n; p++) {
             for (int q = p + 1; q < n; q++) {
-                const double apq = a[p * n + q];
-                if (std::fabs(apq) < 1e-300) continue;
-                const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                const long double apq = a[p * n + q];
+                if (fabsl(apq) < 1e-4000L) continue;
+                const long double theta = (a[q * n + q] - a[p * n + p]) / (2.0L * apq);
+                const long double t = (theta >= 0 ? 1.0L : -1.0L) / (fabsl(theta) + sqrtl(theta * theta + 1.0L));
+                const long double c = 1.0L / sqrtl(t * t + 1.0L), sn = t * c;
                 for (int k = 0; k < n; k++) {
-                    const double akp = a[k * n + p], akq = a[k * n + q];
-                    a[k * n + p] = c * akp - s * akq;
-                    a[k * n + q] = s * akp + c * akq;
+                    const long double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - sn * akq;
+                    a[k * n + q] = sn * akp + c * akq;
                 }
                 for (int k = 0; k < n; k++) {
-                    const double apk = a[p * n + k], aqk = a[q * n + k];
-                    a[p * n + k] = c * apk - s * aqk;
-                    a[q * n + k] = s * apk + c * aqk;
+                    const long double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - sn * aqk;
+                    a[q * n + k] = sn * apk + c * aqk;
                 }
                 for (int k = 0; k < n; k++) {
-                    const double vkp = v[k * n + p], vkq = v[k * n + q];
-                    v[k * n + p] = c * vkp - s * vkq;
-                    v[k * n + q] = s * vkp + c * vkq;
+                    const long double vkp = v[k * n + p], vkq = v[k * n + q];
+                    v[k * n + p] = c * vkp - sn * vkq;
+                    v[k * n + q] = sn * vkp + c * vkq;
                 }
             }
         }
     }
-    std::vector<double> w(n);
+    std::vector<long double> w(n);
     for (int i = 0; i < n; i++) w[i] = a[i * n + i];
     a = v;
     return w;
@@ -133,34 +133,39 @@ SplineChannel build_channel(const RadialTables& t, int l, double accuracy) {
 }  // namespace
 
 std::vector<double> gto_orthonormalization(double radius, int n) {
-    std::vector<double> sigma(n);
-    for (int i = 0; i < n; i++) sigma[i] = gto_width(radius, i, n);
-    std::vector<double> S(n * n);
+    // The overlap matrix is ill-conditioned (condition number ~3e4 for 8 functions,
+    // ~4e7 for 12), so S and S^{-1/2} are evaluated in extended precision.
+    std::vector<long double> sigma(n);
+    for (int i = 0; i < n; i++) {
+        const long double s = sqrtl(static_cast<long double>(i));
+        sigma[i] = static_cast<long double>(radius) * (s > 1.0L ? s : 1.0L) / static_cast<long double>(n);
+    }
+    std::vector<long double> S(n * n);
     for (int a = 0; a < n; a++) {
         for (int b = a; b < n; b++) {
-            const double sa = sigma[a], sb = sigma[b];
-            const double e = 0.5 * (3.0 + a + b);
-            const double v = std::pow(0.5 / (sa * sa) + 0.5 / (sb * sb), -e) / (std::pow(sa, a) * std::pow(sb, b)) *
-                             std::tgamma(e) / (std::pow(sa * sb, 1.5) * std::sqrt(std::tgamma(a + 1.5) * std::tgamma(b + 1.5)));
+            const long double sa = sigma[a], sb = sigma[b];
+            const long double e = 0.5L * (3.0L + a + b);
+            const long double v = powl(0.5L / (sa * sa) + 0.5L / (sb * sb), -e) / (powl(sa, a) * powl(sb, b)) * tgammal(e) /
+                                  (powl(sa * sb, 1.5L) * sqrtl(tgammal(a + 1.5L) * tgammal(b + 1.5L)));
             S[a * n + b] = v;
             S[b * n + a] = v;
         }
     }
-    std::vector<double> w = jacobi_eigen(S, n);
+    std::vector<long double> w = jacobi_eigen(S, n);
     for (int i = 0; i < n; i++) {
-        if (w[i] <= 2.220446049250313e-16) {
+        if (w[i] <= 2.220446049250313e-16L) {
             throw Error::internal("radial overlap matrix is singular, try with a lower max_radial (current value is " +
                                   std::to_string(n - 1) + ")");
         }
-        w[i] = 1.0 / std::sqrt(w[i]);
+        w[i] = 1.0L / sqrtl(w[i]);
     }
     std::vector<double> O(n * n);
     for (int a = 0; a < n; a++) {
         for (int b = 0; b < n; b++) {
-            const double norm_b = std::sqrt(2.0 / (std::pow(sigma[b], 2 * b + 3) * std::tgamma(b + 1.5)));
-            double acc = 0.0;
+            const long double norm_b = sqrtl(2.0L / (powl(sigma[b], 2 * b + 3) * tgammal(b + 1.5L)));
+            long double acc = 0.0L;
             for (int k = 0; k < n; k++) acc += S[a * n + k] * w[k] * S[b * n + k];
-            O[a * n + b] = acc * norm_b;
+            O[a * n + b] = static_cast<double>(acc * norm_b);
         }
     }
     return O;

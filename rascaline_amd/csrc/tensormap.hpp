@@ -17,6 +17,21 @@ struct LabelsData {
 
     LabelsData() = default;
     LabelsData(std::vector<std::string> n, std::vector<int32_t> v) : names(std::move(n)), values(std::move(v)) { finish(); }
+    // name_ptrs point into `names`: rebuild them on every copy / move
+    LabelsData(const LabelsData& o) : names(o.names), values(o.values) { finish(); }
+    LabelsData(LabelsData&& o) noexcept : names(std::move(o.names)), values(std::move(o.values)) { finish(); }
+    LabelsData& operator=(const LabelsData& o) {
+        names = o.names;
+        values = o.values;
+        finish();
+        return *this;
+    }
+    LabelsData& operator=(LabelsData&& o) noexcept {
+        names = std::move(o.names);
+        values = std::move(o.values);
+        finish();
+        return *this;
+    }
     void finish() {
         name_ptrs.clear();
         for (auto& s : names) name_ptrs.push_back(s.c_str());

@@ -123,10 +123,24 @@ def random_system(rng, n_atoms, kind="periodic", L=8.0, types=(1, 6, 8), min_dis
 
 
 def compare_tensormaps(actual, expected, rtol, atol_scale, gradients=(), check_labels=True):
-    """Block-by-block comparison: identical keys and labels, values within
-    |a - b| <= rtol * |b| + atol_scale * max|b over the whole block|."""
+    """Block-by-block comparison: identical keys and labels, and values within
+    |a - b| <= rtol * |b| + atol_scale * S, where S is the largest |b| of the same
+    array kind (values, or one gradient parameter) over the whole TensorMap, so that
+    elements that vanish by symmetry or cancellation are compared on the scale of
+    the descriptor rather than on their own."""
     assert actual.keys.names == expected.keys.names
     assert np.array_equal(actual.keys.values, expected.keys.values), (actual.keys.values, expected.keys.values)
+
+    def scale_of(get):
+        m = 0.0
+        for blk in expected.blocks():
+            v = np.asarray(get(blk))
+            if v.size:
+                m = max(m, float(np.max(np.abs(v))))
+        return m
+
+    value_scale = scale_of(lambda b: b.values)
+    grad_scale = {p: scale_of(lambda b, p=p: b.gradient(p).values) for p in gradients}
     worst = 0.0
     for ba, be in zip(actual.blocks(), expected.blocks()):
         if check_labels:
@@ -135,29 +149,29 @@ def compare_tensormaps(actual, expected, rtol, atol_scale, gradients=(), check_l
             assert len(ba.components) == len(be.components)
             for ca, ce in zip(ba.components, be.components):
                 assert ca == ce
-        worst = max(worst, _close(ba.values, be.values, rtol, atol_scale))
+        worst = max(worst, _close(ba.values, be.values, rtol, atol_scale * value_scale))
         for param in gradients:
             ga, ge = ba.gradient(param), be.gradient(param)
             if check_labels:
                 assert ga.samples == ge.samples, f"{param} gradient samples differ"
                 for ca, ce in zip(ga.components, ge.components):
                     assert ca == ce
-            worst = max(worst, _close(ga.values, ge.values, rtol, atol_scale))
+            worst = max(worst, _close(ga.values, ge.values, rtol, atol_scale * grad_scale[param]))
     return worst
 
 
-def _close(a, b, rtol, atol_scale):
+def _close(a, b, rtol, atol):
+    """Returns the largest |a - b| / (|b| + atol / rtol) ratio after checking the bound."""
     a = np.asarray(a)
     b = np.asarray(b)
     assert a.shape == b.shape, (a.shape, b.shape)
     if b.size == 0:
         return 0.0
-    scale = float(np.max(np.abs(b)))
     diff = np.abs(a - b)
-    bound = rtol * np.abs(b) + atol_scale * scale
+    bound = rtol * np.abs(b) + atol
     bad = diff > bound
     if np.any(bad):
         idx = np.argwhere(bad)[:5]
-        raise AssertionError(f"{int(bad.sum())}/{bad.size} elements differ (scale {scale:.3e}): " +
+        raise AssertionError(f"{int(bad.sum())}/{bad.size} elements differ (atol {atol:.3e}): " +
                              ", ".join(f"{tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}" for i in idx))
-    return float(np.max(diff / np.maximum(np.abs(b) + scale * 1e-300, 1e-300))) if scale > 0 else 0.0
+    return float(np.max(diff / np.maximum(np.abs(b) + atol / max(rtol, 1e-300), 1e-300)))
