@@ -401,7 +401,12 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     // ---- spherical expansion per center (dense)
     T* coef = ws_.get<T>("coef", static_cast<size_t>(n_atoms) * S * slots);
     T* dself = grads.positions ? ws_.get<T>("dself", static_cast<size_t>(n_atoms) * S * 3 * slots) : nullptr;
-    lp.chunk = chunk_for(2048 * 4 + 33 * 4);
+    {
+        // entries per chunk: up to 64, with at most ~48 KB of LDS per workgroup
+        const long long target = std::min<long long>(lds, 49152) - (2048 + 64) * 4 - 1024;
+        const long long c = std::max<long long>(1, std::min<long long>(64, target / static_cast<long long>(per_entry)));
+        lp.chunk = static_cast<int>(c);
+    }
     launch_soap_center<T>(lp, 0, coef, dself);
     T* dstrain = nullptr;
     T* dcell = nullptr;
@@ -470,9 +475,26 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
     if (grads.positions) {
-        lp.chunk = chunk_for((ps ? S * slots_bytes : 0) + 6 * slots_bytes + 2 * 528 * 4);
-        launch_gradient_rows<T>(lp, ps, coef, dself, type_mask, sample_flag, sample_scan, row_scan, d_block_sample_base,
-                                d_se_grad, d_atom_system, d_atom_local, d_grads, grad_labels);
+        bool done = false;
+        if (ps) {
+            // register-blocked kernel; chunk sized for ~48 KB of LDS per workgroup
+            const size_t fixed = ps_gradient_shmem<T>(lp, 0);
+            const size_t per = ps_gradient_shmem<T>(lp, 1) - fixed;
+            // chunk: at most 8 entries (so that ~5 workgroups fit per CU) and L * chunk
+            // radial threads in the last three waves
+            const long long target = std::min<long long>(lds, 32768) - 1024;
+            long long c = (target - static_cast<long long>(fixed)) / static_cast<long long>(per);
+            c = std::max<long long>(1, std::min<long long>(c, std::min<long long>(8, 192 / L)));
+            if (static_cast<long long>(fixed + per * c) + 1024 <= lds) {
+                done = launch_ps_gradient_fast<T>(lp, static_cast<int>(c), coef, dself, type_mask, sample_scan, row_scan,
+                                                  d_block_sample_base, d_atom_system, d_atom_local, d_grads, grad_labels);
+            }
+        }
+        if (!done) {
+            lp.chunk = chunk_for((ps ? S * slots_bytes : 0) + 6 * slots_bytes + 2 * 528 * 4);
+            launch_gradient_rows<T>(lp, ps, coef, dself, type_mask, sample_flag, sample_scan, row_scan, d_block_sample_base,
+                                    d_se_grad, d_atom_system, d_atom_local, d_grads, grad_labels);
+        }
     }
     RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
     // algorithmic bytes of the gradient-row kernel: gradient values and labels written once,

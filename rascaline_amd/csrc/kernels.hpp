@@ -72,6 +72,13 @@ void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const 
                           const long long* block_sample_base, const long long* se_grad_base, const int* atom_system,
                           const int* atom_local, T* out, int* labels);
 template <typename T>
+size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
+// register-blocked power-spectrum gradient rows; returns false when n > 16
+template <typename T>
+bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, const T* dself, const unsigned* type_mask,
+                             const long long* sample_scan, const long long* row_scan, const long long* block_sample_base,
+                             const int* atom_system, const int* atom_local, T* out, int* labels);
+template <typename T>
 void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
                            const long long* sample_flag, const long long* sample_scan, T* out);
 template <typename T>

@@ -111,11 +111,15 @@ __device__ __forceinline__ int l_of_lm(int lm) {
 }
 
 // MODE 0: coef + dself; MODE 1: dstrain; MODE 2: dcell.
+// The center's entries are taken in neighbour-species order (stable partition),
+// in chunks of up to `chunk` entries whose per-entry data all lanes compute in
+// parallel; each lane then accumulates its own (lm, n) slots, flushing its
+// accumulators to global memory whenever the species changes.
 template <typename T, int MS, int MODE>
 __global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __restrict__ out_coef, T* __restrict__ out_grad) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int list[MAX_SEGMENT];
-    __shared__ int type_start[33];
+    __shared__ int chunk_t[64];
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const long long e0 = a.offsets[i];
@@ -128,7 +132,6 @@ __global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __r
     // stable partition of the entries by neighbour species
     int cursor = 0;
     for (int t = 0; t < S; t++) {
-        if (lane == 0) type_start[t] = cursor;
         for (int base = 0; base < len; base += WAVE) {
             const int e = base + lane;
             const bool pred = e < len && a.species[a.entry_j[e0 + e]] == t;
@@ -138,7 +141,6 @@ __global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __r
             cursor += __popcll(mask);
         }
     }
-    if (lane == 0) type_start[S] = len;
     __syncthreads();
 
     const ChunkLayout lay(a.L, a.n, a.chunk);
@@ -147,58 +149,24 @@ __global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __r
     const int slots = lay.L2 * n;
     const int ct = a.species[i];
     const int groups = (slots + WAVE * MS - 1) / (WAVE * MS);
+    constexpr int NACC = MODE == 0 ? 3 : 9;
 
     for (int g = 0; g < groups; g++) {
         const int sb = g * WAVE * MS;
-        for (int t = 0; t < S; t++) {
-            T acc[MODE == 0 ? 4 : 9][MS];
-            T accv[MS];
+        T acc[NACC][MS];
+        T accv[MS];
+        unsigned written = 0;
+        int cur_t = -1;
+
+        auto reset = [&]() {
 #pragma unroll
             for (int k = 0; k < MS; k++) {
                 accv[k] = 0;
 #pragma unroll
-                for (int x = 0; x < (MODE == 0 ? 3 : 9); x++) acc[x][k] = 0;
+                for (int x = 0; x < NACC; x++) acc[x][k] = 0;
             }
-            const int tb = type_start[t], te = type_start[t + 1];
-            for (int cb = tb; cb < te; cb += lay.chunk) {
-                const int cnt = min(lay.chunk, te - cb);
-                if (lane < cnt) entry_data<T>(a, lay, e0 + list[cb + lane], i, chunk + lane * lay.per_entry);
-                __syncthreads();
-                for (int c = 0; c < cnt; c++) {
-                    const T* en = chunk + c * lay.per_entry;
-                    const bool self_image = en[lay.flag] != (T)0;
-#pragma unroll
-                    for (int k = 0; k < MS; k++) {
-                        const int s = sb + lane + WAVE * k;
-                        if (s < slots) {
-                            const int lm = s / n, nn = s - lm * n;
-                            const int l = l_of_lm(lm);
-                            const T y = en[lay.y + lm];
-                            if (MODE == 0) {
-                                accv[k] += en[lay.rv + l * n + nn] * y;
-                                if (out_grad && !self_image) {
-                                    const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
-#pragma unroll
-                                    for (int x = 0; x < 3; x++) acc[x][k] -= ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
-                                }
-                            } else {
-                                const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
-                                T gx[3];
-#pragma unroll
-                                for (int x = 0; x < 3; x++) gx[x] = ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
-#pragma unroll
-                                for (int p = 0; p < 3; p++) {
-                                    const T w = MODE == 1 ? en[lay.vec + p] : en[lay.shift + p];
-#pragma unroll
-                                    for (int x = 0; x < 3; x++) acc[p * 3 + x][k] += w * gx[x];
-                                }
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            // write this species' accumulators
+        };
+        auto flush = [&](int t) {
 #pragma unroll
             for (int k = 0; k < MS; k++) {
                 const int s = sb + lane + WAVE * k;
@@ -217,6 +185,68 @@ __global__ void __launch_bounds__(WAVE) soap_center_kernel(SoapArgs<T> a, T* __r
                     }
                 }
             }
+        };
+
+        reset();
+        for (int cb = 0; cb < len; cb += lay.chunk) {
+            const int cnt = min(lay.chunk, len - cb);
+            if (lane < cnt) {
+                const int e = list[cb + lane];
+                entry_data<T>(a, lay, e0 + e, i, chunk + lane * lay.per_entry);
+                chunk_t[lane] = a.species[a.entry_j[e0 + e]];
+            }
+            __syncthreads();
+            for (int c = 0; c < cnt; c++) {
+                const int t = chunk_t[c];
+                if (t != cur_t) {
+                    if (cur_t >= 0) {
+                        flush(cur_t);
+                        written |= 1u << cur_t;
+                    }
+                    reset();
+                    cur_t = t;
+                }
+                const T* en = chunk + c * lay.per_entry;
+                const bool self_image = en[lay.flag] != (T)0;
+#pragma unroll
+                for (int k = 0; k < MS; k++) {
+                    const int s = sb + lane + WAVE * k;
+                    if (s < slots) {
+                        const int lm = s / n, nn = s - lm * n;
+                        const int l = l_of_lm(lm);
+                        const T y = en[lay.y + lm];
+                        if (MODE == 0) {
+                            accv[k] += en[lay.rv + l * n + nn] * y;
+                            if (out_grad && !self_image) {
+                                const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+#pragma unroll
+                                for (int x = 0; x < 3; x++) acc[x][k] -= ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
+                            }
+                        } else {
+                            const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+                            T gx[3];
+#pragma unroll
+                            for (int x = 0; x < 3; x++) gx[x] = ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
+#pragma unroll
+                            for (int p = 0; p < 3; p++) {
+                                const T w = MODE == 1 ? en[lay.vec + p] : en[lay.shift + p];
+#pragma unroll
+                                for (int x = 0; x < 3; x++) acc[p * 3 + x][k] += w * gx[x];
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (cur_t >= 0) {
+            flush(cur_t);
+            written |= 1u << cur_t;
+        }
+        // species without neighbours: zeros (plus the self contribution)
+        reset();
+        for (int t = 0; t < S; t++) {
+            if (!((written >> t) & 1u)) flush(t);
         }
     }
 }
@@ -444,6 +474,332 @@ __global__ void __launch_bounds__(WAVE) gradient_rows_kernel(SoapArgs<T> a, cons
     emit(i, ct, true);
 }
 
+// ---------------------------------------------------------------------------
+// Power-spectrum position gradients, register-blocked (n <= NMAX).
+//
+// One 256-thread workgroup per center. Neighbour entries are processed in
+// chunks: one thread per entry computes Y_lm, r dY_lm and the spline values into
+// LDS, then all threads sum the images of neighbour j into DC[x][lm][n] =
+// d c_i^{type_j} / d r_j. Each gradient row [3][L][n][n] of a block containing
+// type_j is produced by items (x, l, n1..n1+R-1) that keep R x n accumulators
+// in registers: per m they read one row of C (or DC) and R scalars, so that a
+// thread writes R x n contiguous outputs and consecutive threads write
+// consecutive addresses.
+
+struct GradLayout {
+    int L, n, L2, leg;
+    int y, dy, ra, rb, dir, p, dp, pt, per_entry;
+    __host__ __device__ GradLayout(int L_, int n_) : L(L_), n(n_), L2(L_ * L_), leg(L_ * (L_ + 1) / 2) {
+        int o = 0;
+        y = o; o += L2;
+        dy = o; o += 3 * L2;
+        ra = o; o += L * n;
+        rb = o; o += L * n;
+        dir = o; o += 4;
+        p = o; o += leg;
+        dp = o; o += leg;
+        pt = o; o += leg;
+        per_entry = o;
+    }
+};
+
+template <typename T>
+__device__ void entry_grad_data(const SoapArgs<T>& a, const GradLayout& lay, long long e, T* base) {
+    const double r = a.entry_dist[e];
+    double d[3];
+    if (r < 1e-6) {
+        d[0] = 0.0;
+        d[1] = 0.0;
+        d[2] = 1.0;
+    } else {
+        d[0] = a.entry_vec[3 * e] / r;
+        d[1] = a.entry_vec[3 * e + 1] / r;
+        d[2] = a.entry_vec[3 * e + 2] / r;
+    }
+    T dt[3] = {(T)d[0], (T)d[1], (T)d[2]};
+    spherical_harmonics<T>(lay.L, dt, base + lay.y, base + lay.dy, base + lay.p, base + lay.dp, base + lay.pt, a.sh_a, a.sh_b);
+    double f, df;
+    scaling_functions(a.scalars, r, &f, &df);
+    const int n = lay.n;
+    for (int l = 0; l < lay.L; l++) {
+        radial_integral<T>(a.radial, l, r, base + lay.ra + l * n, base + lay.rb + l * n);
+        for (int k = 0; k < n; k++) {
+            const T R = base[lay.ra + l * n + k], dR = base[lay.rb + l * n + k];
+            base[lay.ra + l * n + k] = (T)df * R + (T)f * dR;
+            base[lay.rb + l * n + k] = (T)f * R / (T)r;
+        }
+    }
+    base[lay.dir + 0] = dt[0];
+    base[lay.dir + 1] = dt[1];
+    base[lay.dir + 2] = dt[2];
+}
+
+constexpr int GRAD_THREADS = 256;
+constexpr int MAX_ROW_BLOCKS = 32;
+
+template <typename T, int NMAX, int R>
+__device__ __forceinline__ void emit_row_items(int n_blocks, const long long* jb_row, const int* jb_t1, const int* jb_t2,
+                                               const unsigned char* jb_use, const T* C, const T* D1base, const T* D2base,
+                                               long long d_stride_t, int L, int n, int slots, T* __restrict__ out) {
+    const int nR = (n + R - 1) / R;
+    const int per_block = 3 * L * nR;
+    const int P = L * n * n;
+    const int total = n_blocks * per_block;
+    for (int item = threadIdx.x; item < total; item += GRAD_THREADS) {
+        const int k = item / per_block;
+        int rem = item - k * per_block;
+        const int x = rem / (L * nR);
+        rem -= x * (L * nR);
+        const int l = rem / nR;
+        const int n1 = (rem - l * nR) * R;
+        const int t1 = jb_t1[k], t2 = jb_t2[k];
+        const bool use1 = jb_use[k] & 1, use2 = jb_use[k] & 2;
+        // D1 / D2: gradient arrays [3][L2][n] of the terms d c^{t1} and d c^{t2}
+        const T* D1 = D1base + (d_stride_t ? d_stride_t * t1 : 0);
+        const T* D2 = D2base + (d_stride_t ? d_stride_t * t2 : 0);
+        T acc[R][NMAX];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < NMAX; q++) acc[r][q] = 0;
+        for (int m = l * l; m < (l + 1) * (l + 1); m++) {
+            if (use1) {
+                T d[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) d[r] = (n1 + r < n) ? D1[x * slots + m * n + n1 + r] : (T)0;
+                const T* crow = C + t2 * slots + m * n;
+#pragma unroll
+                for (int q = 0; q < NMAX; q++) {
+                    if (q < n) {
+                        const T cv = crow[q];
+#pragma unroll
+                        for (int r = 0; r < R; r++) acc[r][q] += cv * d[r];
+                    }
+                }
+            }
+            if (use2) {
+                T c[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) c[r] = (n1 + r < n) ? C[t1 * slots + m * n + n1 + r] : (T)0;
+                const T* drow = D2 + x * slots + m * n;
+#pragma unroll
+                for (int q = 0; q < NMAX; q++) {
+                    if (q < n) {
+                        const T dv = drow[q];
+#pragma unroll
+                        for (int r = 0; r < R; r++) acc[r][q] += c[r] * dv;
+                    }
+                }
+            }
+        }
+        const T kappa = t1 != t2 ? (T)1.4142135623730951 : (T)1;
+        const T norm = (l % 2 == 0 ? (T)1 : (T)-1) * (T)sqrt((double)(2 * l + 1));
+        T* o = out + jb_row[k] * 3 * P + (long long)x * P + l * n * n + n1 * n;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (n1 + r < n) {
+#pragma unroll
+                for (int q = 0; q < NMAX; q++) {
+                    if (q < n) o[r * n + q] = (acc[r][q] * kappa) / norm;
+                }
+            }
+        }
+    }
+}
+
+// Per-entry data for the gradient kernel, split over threads: threads
+// [0, cnt) compute the spherical harmonics of entry tid, threads
+// [64, 64 + cnt * L) the radial integral of (entry, l).
+template <typename T>
+__device__ void chunk_grad_data(const SoapArgs<T>& a, const GradLayout& lay, long long e_first, int cnt, T* chunk_data) {
+    const int tid = threadIdx.x;
+    if (tid < cnt) {
+        const long long e = e_first + tid;
+        T* base = chunk_data + tid * lay.per_entry;
+        const double r = a.entry_dist[e];
+        double d[3];
+        if (r < 1e-6) {
+            d[0] = 0.0;
+            d[1] = 0.0;
+            d[2] = 1.0;
+        } else {
+            d[0] = a.entry_vec[3 * e] / r;
+            d[1] = a.entry_vec[3 * e + 1] / r;
+            d[2] = a.entry_vec[3 * e + 2] / r;
+        }
+        T dt[3] = {(T)d[0], (T)d[1], (T)d[2]};
+        spherical_harmonics<T>(lay.L, dt, base + lay.y, base + lay.dy, base + lay.p, base + lay.dp, base + lay.pt, a.sh_a, a.sh_b);
+        base[lay.dir + 0] = dt[0];
+        base[lay.dir + 1] = dt[1];
+        base[lay.dir + 2] = dt[2];
+    }
+    const int w = tid - WAVE;
+    if (w >= 0 && w < cnt * lay.L) {
+        const int c = w / lay.L, l = w - c * lay.L;
+        const long long e = e_first + c;
+        T* base = chunk_data + c * lay.per_entry;
+        const double r = a.entry_dist[e];
+        double f, df;
+        scaling_functions(a.scalars, r, &f, &df);
+        const int n = lay.n;
+        radial_integral<T>(a.radial, l, r, base + lay.ra + l * n, base + lay.rb + l * n);
+        for (int k = 0; k < n; k++) {
+            const T Rv = base[lay.ra + l * n + k], dR = base[lay.rb + l * n + k];
+            base[lay.ra + l * n + k] = (T)df * Rv + (T)f * dR;
+            base[lay.rb + l * n + k] = (T)f * Rv / (T)r;
+        }
+    }
+}
+
+template <typename T, int NMAX, int R>
+__global__ void __launch_bounds__(GRAD_THREADS) ps_gradient_kernel(
+    SoapArgs<T> a, int chunk, const T* __restrict__ coef, const T* __restrict__ dself, const unsigned* __restrict__ type_mask,
+    const long long* __restrict__ sample_scan, const long long* __restrict__ row_scan,
+    const long long* __restrict__ block_sample_base, const int* __restrict__ atom_system, const int* __restrict__ atom_local,
+    T* __restrict__ out, int* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int chunk_j[64];
+    __shared__ long long jb_row[MAX_ROW_BLOCKS];
+    __shared__ int jb_t1[MAX_ROW_BLOCKS], jb_t2[MAX_ROW_BLOCKS], jb_local[MAX_ROW_BLOCKS];
+    __shared__ unsigned char jb_use[MAX_ROW_BLOCKS];
+    __shared__ int jb_count;
+
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L, slots = L2 * n;
+    const int npairs = S * (S + 1) / 2;
+    const int n_atoms = a.n_atoms;
+    const GradLayout lay(L, n);
+    // dynamic LDS: per-block row bases and sample indices, then C, DC, chunk data
+    long long* rowbase = reinterpret_cast<long long*>(smem);
+    int* sample_in_block = reinterpret_cast<int*>(rowbase + npairs);
+    int* counters = sample_in_block + npairs;
+    int* snapshot = counters + npairs;
+    const size_t meta_bytes = ((size_t)npairs * (8 + 12) + 15) / 16 * 16;
+    T* C = reinterpret_cast<T*>(smem + meta_bytes);
+    T* DC = C + S * slots;
+    T* chunk_data = DC + 3 * slots;
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const int ct = a.species[i];
+    const unsigned mask = type_mask[i];
+
+    for (int k = tid; k < S * slots; k += GRAD_THREADS) C[k] = coef[(long long)i * S * slots + k];
+    for (int k = tid; k < 3 * slots; k += GRAD_THREADS) DC[k] = 0;
+    for (int k = tid; k < npairs; k += GRAD_THREADS) {
+        const long long idx = (long long)(ct * npairs + k) * n_atoms + i;
+        rowbase[k] = row_scan[idx];
+        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * npairs + k]);
+        counters[k] = 0;
+        snapshot[k] = -1;
+    }
+    const int sys = atom_system[i];
+    __syncthreads();
+
+    bool snapped = false;
+    for (int cb = 0; cb < len; cb += chunk) {
+        const int cnt = min(chunk, len - cb);
+        chunk_grad_data<T>(a, lay, e0 + cb, cnt, chunk_data);
+        if (tid < cnt) chunk_j[tid] = a.entry_j[e0 + cb + tid];
+        __syncthreads();
+        for (int c = 0; c < cnt; c++) {
+            const int j = chunk_j[c];
+            if (j == i) continue;  // periodic self image: contributes to no gradient row
+            if (!snapped && j > i) {
+                for (int k = tid; k < npairs; k += GRAD_THREADS) snapshot[k] = counters[k];
+                snapped = true;
+            }
+            const T* en = chunk_data + c * lay.per_entry;
+            for (int s = tid; s < 3 * slots; s += GRAD_THREADS) {
+                const int x = s / slots;
+                const int r = s - x * slots;
+                const int lm = r / n, nn = r - lm * n;
+                const int l = l_of_lm(lm);
+                DC[s] += en[lay.ra + l * n + nn] * en[lay.y + lm] * en[lay.dir + x] + en[lay.rb + l * n + nn] * en[lay.dy + x * L2 + lm];
+            }
+            const bool last = (cb + c + 1 >= len) || (c + 1 < cnt ? chunk_j[c + 1] != j : a.entry_j[e0 + cb + c + 1] != j);
+            if (last) {
+                const int tj = a.species[j];
+                const int local_j = atom_local[j];
+                // one thread per species t present around the center: block (min, max)(t, tj)
+                if (tid < S && ((mask >> tid) & 1u)) {
+                    const int t = tid;
+                    const int k = __popc(mask & ((1u << t) - 1u));
+                    const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
+                    const int local = pair_index(lo, hi, S);
+                    const long long row = rowbase[local] + counters[local] + (i < j ? 1 : 0);
+                    jb_row[k] = row;
+                    jb_t1[k] = lo;
+                    jb_t2[k] = hi;
+                    jb_local[k] = local;
+                    jb_use[k] = (unsigned char)((lo == tj ? 1 : 0) | (hi == tj ? 2 : 0));
+                    labels[3 * row] = sample_in_block[local];
+                    labels[3 * row + 1] = sys;
+                    labels[3 * row + 2] = local_j;
+                }
+                __syncthreads();
+                emit_row_items<T, NMAX, R>(__popc(mask), jb_row, jb_t1, jb_t2, jb_use, C, DC, DC, 0, L, n, slots, out);
+                __syncthreads();
+                if (tid < S && ((mask >> tid) & 1u)) counters[jb_local[__popc(mask & ((1u << tid) - 1u))]] += 1;
+                for (int k = tid; k < 3 * slots; k += GRAD_THREADS) DC[k] = 0;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (!snapped) {
+        for (int k = tid; k < npairs; k += GRAD_THREADS) snapshot[k] = counters[k];
+    }
+    __syncthreads();
+
+    // center rows of every block: d c^{t} / d r_i are the self gradients
+    const T* self_base = dself + (long long)i * S * 3 * slots;
+    int t1 = 0, t2 = 0;
+    for (;;) {
+        // next group of up to MAX_ROW_BLOCKS blocks in (t1, t2) order
+        if (tid == 0) {
+            int count = 0;
+            int a1 = t1, a2 = t2;
+            while (a1 < S && count < MAX_ROW_BLOCKS) {
+                if (a2 < a1) a2 = a1;
+                if (a2 >= S) {
+                    a1++;
+                    a2 = a1;
+                    continue;
+                }
+                if (((mask >> a1) & 1u) && ((mask >> a2) & 1u)) {
+                    const int local = pair_index(a1, a2, S);
+                    const long long row = rowbase[local] + snapshot[local];
+                    jb_row[count] = row;
+                    jb_t1[count] = a1;
+                    jb_t2[count] = a2;
+                    jb_use[count] = 3;
+                    labels[3 * row] = sample_in_block[local];
+                    labels[3 * row + 1] = sys;
+                    labels[3 * row + 2] = atom_local[i];
+                    count++;
+                }
+                a2++;
+            }
+            jb_count = count;
+            jb_local[0] = a1;
+            jb_local[1] = a2;
+        }
+        __syncthreads();
+        const int count = jb_count;
+        const int next1 = jb_local[0], next2 = jb_local[1];
+        if (count > 0) {
+            emit_row_items<T, NMAX, R>(count, jb_row, jb_t1, jb_t2, jb_use, C, self_base, self_base,
+                                       (long long)3 * slots, L, n, slots, out);
+        }
+        __syncthreads();
+        if (next1 >= S || count == 0) break;
+        t1 = next1;
+        t2 = next2;
+    }
+}
+
 // Power spectrum strain / cell gradients: one row of [3][3][P] per sample.
 template <typename T>
 __global__ void __launch_bounds__(WAVE) ps_cell_strain_kernel(int n_atoms, int S, int L, int n, const T* __restrict__ coef,
@@ -592,6 +948,39 @@ void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const 
 }
 
 template <typename T>
+size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk) {
+    const size_t slots = (size_t)p.L * p.L * p.n;
+    const size_t npairs = (size_t)p.n_species * (p.n_species + 1) / 2;
+    const size_t meta = (npairs * (8 + 12) + 15) / 16 * 16;
+    return meta + ((size_t)p.n_species * slots + 3 * slots + (size_t)GradLayout(p.L, p.n).per_entry * chunk) * sizeof(T);
+}
+
+template <typename T>
+bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, const T* dself, const unsigned* type_mask,
+                             const long long* sample_scan, const long long* row_scan, const long long* block_sample_base,
+                             const int* atom_system, const int* atom_local, T* out, int* labels) {
+    if (p.n_atoms == 0) return true;
+    if (p.n > 16 || p.n_species > MAX_ROW_BLOCKS || chunk > 64 || chunk * p.L > GRAD_THREADS - WAVE) return false;
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = ps_gradient_shmem<T>(p, chunk);
+#define RA_LAUNCH_PSG(NM)                                                                                           \
+    ps_gradient_kernel<T, NM, 2><<<p.n_atoms, GRAD_THREADS, shmem, p.stream>>>(a, chunk, coef, dself, type_mask,    \
+                                                                               sample_scan, row_scan, block_sample_base, \
+                                                                               atom_system, atom_local, out, labels)
+    if (p.n <= 4) {
+        RA_LAUNCH_PSG(4);
+    } else if (p.n <= 8) {
+        RA_LAUNCH_PSG(8);
+    } else if (p.n <= 12) {
+        RA_LAUNCH_PSG(12);
+    } else {
+        RA_LAUNCH_PSG(16);
+    }
+#undef RA_LAUNCH_PSG
+    return true;
+}
+
+template <typename T>
 void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
                            const long long* sample_flag, const long long* sample_scan, T* out) {
     if (p.n_atoms == 0) return;
@@ -615,6 +1004,10 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
     template void launch_ps_values<T>(const SoapLaunch<T>&, const T*, const unsigned*, const long long*,              \
                                       const long long*, T*);                                                           \
     template size_t gradient_rows_shmem<T>(const SoapLaunch<T>&, bool);                                                \
+    template size_t ps_gradient_shmem<T>(const SoapLaunch<T>&, int);                                                   \
+    template bool launch_ps_gradient_fast<T>(const SoapLaunch<T>&, int, const T*, const T*, const unsigned*,          \
+                                             const long long*, const long long*, const long long*, const int*,        \
+                                             const int*, T*, int*);                                                    \
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
                                           const long long*, const long long*, const long long*, const long long*,      \
                                           const long long*, const int*, const int*, T*, int*);                         \
