@@ -190,6 +190,8 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
         t->radial.knot_pos = t->put(kpos);
         t->radial.knot_val = t->put(kval);
         t->radial.knot_der = t->put(kder);
+        t->radial.exact_val = nullptr;
+        t->radial.exact_der = nullptr;
         // Legendre recursion coefficients, featomic/src/math/spherical_harmonics.rs:206-216
         std::vector<T> A(L * (L + 1) / 2 + 1, 0), B(L * (L + 1) / 2 + 1, 0);
         for (int l = 2; l < L; l++) {
@@ -371,6 +373,15 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     lp.entry_dist = nl.entry_dist;
     lp.species = d_species;
     lp.radial = tab.radial;
+    if (!tab.radial.use_spline) {
+        // exact radial integrals: evaluated once per neighbour entry in FP64
+        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
+        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
+        launch_radial_exact<T>(stream, nl.n_entries, L, n, nl.entry_dist, tables_.sigma2_atomic, tab.radial.gto,
+                               tab.radial.ortho, ev, ed);
+        lp.radial.exact_val = ev;
+        lp.radial.exact_der = ed;
+    }
     lp.scalars.cutoff = soap_.cutoff;
     lp.scalars.smoothing = soap_.smoothing == Smoothing::ShiftedCosine ? 1 : 0;
     lp.scalars.width = soap_.smoothing_width;

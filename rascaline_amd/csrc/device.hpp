@@ -47,6 +47,10 @@ struct DeviceRadial {
     const T* knot_pos;         // [total knots]
     const T* knot_val;         // [total knots][n]
     const T* knot_der;         // [total knots][n]
+    // exact mode: radial values / derivatives precomputed per neighbour entry,
+    // [entry][L][n] (filled by launch_radial_exact before the SOAP kernels)
+    const T* exact_val;
+    const T* exact_der;
 };
 
 // SOAP per-pair scalar functions (cutoff smoothing + Willatt2018 scaling).

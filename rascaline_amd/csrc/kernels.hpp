@@ -60,6 +60,9 @@ struct SoapLaunch {
 template <typename T>
 size_t soap_chunk_bytes(int L, int n, int chunk);
 template <typename T>
+void launch_radial_exact(hipStream_t stream, long long n_entries, int L, int n, const double* entry_dist,
+                         double sigma2_atomic, const double* gto, const double* ortho, T* out_val, T* out_der);
+template <typename T>
 void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad);
 template <typename T>
 void launch_ps_values(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,

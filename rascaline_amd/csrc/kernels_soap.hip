@@ -87,7 +87,7 @@ __device__ void entry_data(const SoapArgs<T>& a, const ChunkLayout& lay, long lo
     const int n = lay.n;
     for (int l = 0; l < lay.L; l++) {
         // R into the rv slots, dR/dr into the ra slots, then combine in place
-        radial_integral<T>(a.radial, l, r, base + lay.rv + l * n, base + lay.ra + l * n);
+        radial_integral<T>(a.radial, l, e, r, base + lay.rv + l * n, base + lay.ra + l * n);
         for (int k = 0; k < n; k++) {
             const T R = base[lay.rv + l * n + k], dR = base[lay.ra + l * n + k];
             base[lay.rv + l * n + k] = (T)f * R;
@@ -522,7 +522,7 @@ __device__ void entry_grad_data(const SoapArgs<T>& a, const GradLayout& lay, lon
     scaling_functions(a.scalars, r, &f, &df);
     const int n = lay.n;
     for (int l = 0; l < lay.L; l++) {
-        radial_integral<T>(a.radial, l, r, base + lay.ra + l * n, base + lay.rb + l * n);
+        radial_integral<T>(a.radial, l, e, r, base + lay.ra + l * n, base + lay.rb + l * n);
         for (int k = 0; k < n; k++) {
             const T R = base[lay.ra + l * n + k], dR = base[lay.rb + l * n + k];
             base[lay.ra + l * n + k] = (T)df * R + (T)f * dR;
@@ -642,7 +642,7 @@ __device__ void chunk_grad_data(const SoapArgs<T>& a, const GradLayout& lay, lon
         double f, df;
         scaling_functions(a.scalars, r, &f, &df);
         const int n = lay.n;
-        radial_integral<T>(a.radial, l, r, base + lay.ra + l * n, base + lay.rb + l * n);
+        radial_integral<T>(a.radial, l, e, r, base + lay.ra + l * n, base + lay.rb + l * n);
         for (int k = 0; k < n; k++) {
             const T Rv = base[lay.ra + l * n + k], dR = base[lay.rb + l * n + k];
             base[lay.ra + l * n + k] = (T)df * Rv + (T)f * dR;
@@ -892,6 +892,46 @@ void launch_center_ms(const SoapLaunch<T>& p, T* coef, T* grad) {
     soap_center_kernel<T, MS, MODE><<<p.n_atoms, WAVE, shmem, p.stream>>>(a, coef, grad);
 }
 
+// Exact GTO radial integrals (spline_accuracy = null): one thread per (entry, l),
+// FP64 hypergeometric evaluation, stored per entry for the SOAP kernels.
+template <typename T>
+__global__ void radial_exact_kernel(long long n_entries, int L, int n, const double* __restrict__ entry_dist,
+                                    double sigma2_atomic, const double* __restrict__ gto, const double* __restrict__ ortho,
+                                    T* __restrict__ out_val, T* __restrict__ out_der) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_entries * L) return;
+    const long long e = idx / L;
+    const int l = (int)(idx - e * L);
+    const double r = entry_dist[e];
+    double raw_v[64], raw_g[64];
+    for (int k = 0; k < n; k++) soap_gto_raw(l, k, r, sigma2_atomic, gto[k], &raw_v[k], &raw_g[k]);
+    for (int j = 0; j < n; j++) {
+        double v = 0.0, g = 0.0;
+        for (int k = 0; k < n; k++) {
+            v += raw_v[k] * ortho[j * n + k];
+            g += raw_g[k] * ortho[j * n + k];
+        }
+        out_val[idx * n + j] = (T)v;
+        out_der[idx * n + j] = (T)g;
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void launch_radial_exact(hipStream_t stream, long long n_entries, int L, int n, const double* entry_dist,
+                         double sigma2_atomic, const double* gto, const double* ortho, T* out_val, T* out_der) {
+    const long long total = n_entries * L;
+    if (total == 0) return;
+    radial_exact_kernel<T><<<(unsigned)((total + 127) / 128), 128, 0, stream>>>(n_entries, L, n, entry_dist, sigma2_atomic,
+                                                                               gto, ortho, out_val, out_der);
+}
+template void launch_radial_exact<float>(hipStream_t, long long, int, int, const double*, double, const double*,
+                                         const double*, float*, float*);
+template void launch_radial_exact<double>(hipStream_t, long long, int, int, const double*, double, const double*,
+                                          const double*, double*, double*);
+
+namespace {
 }  // namespace
 
 template <typename T>

@@ -110,9 +110,10 @@ __device__ void spherical_harmonics(int L, const T* d, T* Y, T* grads, T* P, T* 
     }
 }
 
-// Radial integral of channel l at distance r: values / grads of size n.
+// Radial integral of channel l for neighbour entry e at distance r: values /
+// grads of size n, from the spline tables or the per-entry exact values.
 template <typename T>
-__device__ void radial_integral(const DeviceRadial<T>& rad, int l, double r, T* values, T* grads) {
+__device__ void radial_integral(const DeviceRadial<T>& rad, int l, long long e, double r, T* values, T* grads) {
     const int n = rad.n;
     if (rad.use_spline) {
         const int K = rad.n_knots[l] - 1;
@@ -138,18 +139,10 @@ __device__ void radial_integral(const DeviceRadial<T>& rad, int l, double r, T* 
             if (grads) grads[i] = d00 * v0[i] / delta + d10 * m0[i] + d01 * v1[i] / delta + d11 * m1[i];
         }
     } else {
-        double raw_v[64], raw_g[64];
+        const long long base = (e * (rad.max_angular + 1) + l) * n;
         for (int i = 0; i < n; i++) {
-            soap_gto_raw(l, i, r, rad.sigma2_atomic, rad.gto[i], &raw_v[i], &raw_g[i]);
-        }
-        for (int j = 0; j < n; j++) {
-            double v = 0.0, g = 0.0;
-            for (int k = 0; k < n; k++) {
-                v += raw_v[k] * rad.ortho[j * n + k];
-                g += raw_g[k] * rad.ortho[j * n + k];
-            }
-            values[j] = (T)v;
-            if (grads) grads[j] = (T)g;
+            values[i] = rad.exact_val[base + i];
+            if (grads) grads[i] = rad.exact_der[base + i];
         }
     }
 }
