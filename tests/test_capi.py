@@ -1,0 +1,160 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/*.h, and the host-side parts of the API (construction,
+hyper-parameter validation, name / parameters / cutoffs, buffer sizes, error
+codes) behave like featomic/tests/c/calculator.cpp and
+python/featomic/tests/calculators/hypers.py expect. Nothing here launches a
+kernel."""
+
+import ctypes
+import json
+import os
+import re
+
+import pytest
+
+from rascaline_amd import _c_lib as C
+from rascaline_amd import calculators
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+FEATOMIC_SUCCESS = 0
+FEATOMIC_INVALID_PARAMETER_ERROR = 1
+FEATOMIC_BUFFER_SIZE_ERROR = 254
+
+SOAP = {
+    "cutoff": {"radius": 5.0, "smoothing": {"type": "ShiftedCosine", "width": 0.5}},
+    "density": {"type": "Gaussian", "width": 0.3},
+    "basis": {"type": "TensorProduct", "max_angular": 6, "radial": {"type": "Gto", "max_radial": 7}},
+}
+LODE = {
+    "density": {"type": "SmearedPowerLaw", "smearing": 1.0, "exponent": 1},
+    "basis": {"type": "TensorProduct", "max_angular": 4, "radial": {"type": "Gto", "max_radial": 5, "radius": 5.0}},
+}
+
+
+def declared_functions():
+    names = set()
+    pattern = re.compile(r"^[A-Za-z_][A-Za-z0-9_ \*]*[ \*]((?:featomic|mts)[a-z0-9_]*)\(", re.M)
+    for header in ("featomic.h", "metatensor.h", "featomic_amd.h"):
+        with open(os.path.join(ROOT, "include", header)) as fd:
+            names.update(pattern.findall(fd.read()))
+    return sorted(names)
+
+
+def make(name, hypers):
+    text = hypers if isinstance(hypers, str) else json.dumps(hypers)
+    return C.lib().featomic_calculator(name.encode(), text.encode())
+
+
+def last_error():
+    return C.lib().featomic_last_error().decode()
+
+
+def test_every_declared_function_is_exported():
+    names = declared_functions()
+    # the reference API (featomic.h), the metatensor subset and the extensions
+    assert len(names) >= 25
+    handle = ctypes.CDLL(C.LIBRARY_PATH)
+    missing = [name for name in names if not hasattr(handle, name)]
+    assert missing == []
+    # and the Python prototypes cover all of them
+    assert {p[0] for p in C.PROTOTYPES} >= set(names)
+
+
+@pytest.mark.parametrize(
+    "name, hypers, expected",
+    [
+        # names from soap/power_spectrum.rs:412, soap/spherical_expansion.rs:651,
+        # lode/spherical_expansion.rs:454
+        ("soap_power_spectrum", SOAP, "SOAP power spectrum"),
+        ("spherical_expansion", SOAP, "spherical expansion"),
+        ("lode_spherical_expansion", LODE, "lode spherical expansion"),
+    ],
+)
+def test_name_parameters_cutoffs(name, hypers, expected):
+    text = json.dumps(hypers)
+    calc = make(name, text)
+    assert calc, last_error()
+    try:
+        buffer = ctypes.create_string_buffer(256)
+        assert C.lib().featomic_calculator_name(calc, buffer, 256) == FEATOMIC_SUCCESS
+        assert buffer.value.decode() == expected
+
+        buffer = ctypes.create_string_buffer(4096)
+        assert C.lib().featomic_calculator_parameters(calc, buffer, 4096) == FEATOMIC_SUCCESS
+        assert buffer.value.decode() == text
+
+        # too small a buffer: FEATOMIC_BUFFER_SIZE_ERROR (featomic/tests/c/calculator.cpp)
+        small = ctypes.create_string_buffer(8)
+        assert C.lib().featomic_calculator_parameters(calc, small, 8) == FEATOMIC_BUFFER_SIZE_ERROR
+
+        cutoffs = ctypes.POINTER(ctypes.c_double)()
+        count = ctypes.c_size_t()
+        assert C.lib().featomic_calculator_cutoffs(calc, ctypes.byref(cutoffs), ctypes.byref(count)) == FEATOMIC_SUCCESS
+        values = [cutoffs[k] for k in range(count.value)]
+        if name == "lode_spherical_expansion":
+            assert values == []
+        else:
+            assert values == [5.0]
+    finally:
+        C.lib().featomic_calculator_free(calc)
+
+
+def test_unknown_calculator():
+    calc = make("not_a_calculator", "{}")
+    assert not calc
+    assert "not_a_calculator" in last_error()
+
+
+@pytest.mark.parametrize(
+    "hypers, prefix",
+    [
+        ("{", "json error"),
+        (dict(SOAP, extra=1), "json error"),
+        ({k: v for k, v in SOAP.items() if k != "density"}, "json error"),
+        (dict(SOAP, cutoff={"radius": -1.0, "smoothing": {"type": "Step"}}), "invalid parameter"),
+        (dict(SOAP, density={"type": "Gaussian", "width": -0.3}), "invalid parameter"),
+    ],
+)
+def test_invalid_soap_hypers(hypers, prefix):
+    """Errors surface as a null calculator and a message in featomic_last_error
+    whose prefix names the error kind (featomic/src/errors.rs Display); the
+    Python layer raises FeatomicError with that message."""
+    calc = make("spherical_expansion", hypers)
+    assert not calc
+    assert last_error().startswith(prefix), last_error()
+    text = hypers if isinstance(hypers, str) else json.dumps(hypers)
+    with pytest.raises(C.FeatomicError, match=prefix):
+        calculators.CalculatorBase("spherical_expansion", text)
+
+
+def test_null_arguments():
+    calc = make("soap_power_spectrum", SOAP)
+    try:
+        assert C.lib().featomic_calculator_name(calc, None, 10) == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "null" in last_error().lower()
+    finally:
+        C.lib().featomic_calculator_free(calc)
+    # freeing NULL is allowed
+    C.lib().featomic_calculator_free(None)
+
+
+def test_precision_extension():
+    calc = make("soap_power_spectrum", SOAP)
+    try:
+        assert C.lib().featomic_amd_calculator_set_precision(calc, 32) == FEATOMIC_SUCCESS
+        assert C.lib().featomic_amd_calculator_set_precision(calc, 64) == FEATOMIC_SUCCESS
+        assert C.lib().featomic_amd_calculator_set_precision(calc, 16) == FEATOMIC_INVALID_PARAMETER_ERROR
+    finally:
+        C.lib().featomic_calculator_free(calc)
+
+
+def test_profiling_and_logging_entry_points():
+    assert C.lib().featomic_profiling_clear() == FEATOMIC_SUCCESS
+    assert C.lib().featomic_profiling_enable(True) == FEATOMIC_SUCCESS
+    assert C.lib().featomic_profiling_enable(False) == FEATOMIC_SUCCESS
+    buffer = ctypes.create_string_buffer(4096)
+    assert C.lib().featomic_profiling_get(b"short_table", buffer, 4096) == FEATOMIC_SUCCESS
+    assert C.lib().featomic_profiling_get(b"json", buffer, 4096) == FEATOMIC_SUCCESS
+    json.loads(buffer.value.decode())
+    assert C.lib().featomic_profiling_get(b"no-such-format", buffer, 4096) == FEATOMIC_INVALID_PARAMETER_ERROR
