@@ -488,6 +488,20 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     if (grads.positions) {
         bool done = false;
         if (ps) {
+            // windowed tiled kernel: the largest (runs, entries) window that keeps the
+            // workgroup within ~56 KB of LDS (3-4 workgroups per CU)
+            static const int windows[][2] = {{4, 8}, {2, 8}, {2, 4}, {1, 4}, {1, 1}};
+            const size_t budget = std::min<size_t>(lds > 1024 ? lds - 1024 : 0, 57344);
+            for (const auto& w : windows) {
+                if (ps_gradient_tiled_shmem<T>(lp, w[0], w[1]) <= budget) {
+                    done = launch_ps_gradient_tiled<T>(lp, w[0], w[1], coef, dself, type_mask, sample_scan, row_scan,
+                                                       d_block_sample_base, d_atom_system, d_atom_local, d_grads,
+                                                       grad_labels);
+                    break;
+                }
+            }
+        }
+        if (ps && !done) {
             // register-blocked kernel; chunk sized for ~48 KB of LDS per workgroup
             const size_t fixed = ps_gradient_shmem<T>(lp, 0);
             const size_t per = ps_gradient_shmem<T>(lp, 1) - fixed;

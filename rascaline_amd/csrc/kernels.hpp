@@ -75,6 +75,13 @@ void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const 
                           const long long* block_sample_base, const long long* se_grad_base, const int* atom_system,
                           const int* atom_local, T* out, int* labels);
 template <typename T>
+size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries);
+template <typename T>
+bool launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* coef, const T* dself,
+                              const unsigned* type_mask, const long long* sample_scan, const long long* row_scan,
+                              const long long* block_sample_base, const int* atom_system, const int* atom_local, T* out,
+                              int* labels);
+template <typename T>
 size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
 // register-blocked power-spectrum gradient rows; returns false when n > 16
 template <typename T>

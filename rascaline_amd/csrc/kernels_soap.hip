@@ -800,6 +800,435 @@ __global__ void __launch_bounds__(GRAD_THREADS) ps_gradient_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Power-spectrum position gradients, windowed and tiled (n padded to NC, a
+// multiple of 4).
+//
+// One 256-thread workgroup per center. The center's sorted neighbour entries are
+// consumed in windows of up to `max_runs` complete runs (a run = all periodic
+// images of one neighbour atom j) and `max_entries` entries; a run longer than
+// a window is accumulated over several windows. Per window:
+//   1. wave 0 finds the run boundaries (ballot) and thread 0 assigns the output
+//      rows of every (run, present type) in the reference's row order;
+//   2. per-entry Y_lm, r dY_lm and radial values go to LDS;
+//   3. DC_r[x][lm][:] = d c_i^{type j} / d r_j is summed over the run's images,
+//      one (run, x, lm) row of NC values per thread;
+//   4. every gradient row is produced by (l, x, quad of n1) tiles of 4 x NC
+//      register accumulators. For a neighbour of type tj and a present type t,
+//      block (min, max)(t, tj) gets Σ_m DC[m][n1] C_t[m][n2] (t > tj),
+//      Σ_m C_t[m][n1] DC[m][n2] (t < tj) or both (t == tj) -- the operand order
+//      alone handles the transposed case, so each thread always writes 4 x NC
+//      contiguous outputs.
+// The center's own rows (self gradients for both factors) follow the windows.
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_MAX_RUNS = 8;
+constexpr int TILE_MAX_TYPES = 32;
+constexpr int TILE_MAX_L = 16;
+
+// Per-entry LDS layout, radial arrays padded to NC and 4-element aligned.
+struct TileEntryLayout {
+    int L, n, NC, L2, leg;
+    int y, dy, ra, rb, dir, p, dp, pt, per_entry;
+    __host__ __device__ TileEntryLayout(int L_, int n_, int NC_)
+        : L(L_), n(n_), NC(NC_), L2(L_ * L_), leg(L_ * (L_ + 1) / 2) {
+        int o = 0;
+        y = o; o += L2;
+        dy = o; o += 3 * L2;
+        o = (o + 3) / 4 * 4;
+        ra = o; o += L * NC;
+        rb = o; o += L * NC;
+        dir = o; o += 4;
+        p = o; o += leg;
+        dp = o; o += leg;
+        pt = o; o += leg;
+        per_entry = (o + 3) / 4 * 4;
+    }
+};
+
+template <typename T>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4Of<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// Load / store 4 consecutive values (16-byte aligned for float, 16-byte pairs for double).
+template <typename T>
+__device__ __forceinline__ void load4(const T* p, T* v) {
+    typedef typename Vec4Of<T>::type V;
+    if constexpr (sizeof(T) == 4) {
+        const V x = *reinterpret_cast<const V*>(p);
+        v[0] = x[0];
+        v[1] = x[1];
+        v[2] = x[2];
+        v[3] = x[3];
+    } else {
+        const V x = *reinterpret_cast<const V*>(p);
+        const V y = *reinterpret_cast<const V*>(p + 2);
+        v[0] = x[0];
+        v[1] = x[1];
+        v[2] = y[0];
+        v[3] = y[1];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* p, T a0, T a1, T a2, T a3) {
+    typedef typename Vec4Of<T>::type V;
+    if constexpr (sizeof(T) == 4) {
+        V x;
+        x[0] = a0;
+        x[1] = a1;
+        x[2] = a2;
+        x[3] = a3;
+        *reinterpret_cast<V*>(p) = x;
+    } else {
+        V x, y;
+        x[0] = a0;
+        x[1] = a1;
+        y[0] = a2;
+        y[1] = a3;
+        *reinterpret_cast<V*>(p) = x;
+        *reinterpret_cast<V*>(p + 2) = y;
+    }
+}
+
+// acc[r][c] += Σ_m R[m][4q + r] * Cm[m][c] over m in [l², (l+1)²); R and Cm are [L2][NC].
+template <typename T, int NC>
+__device__ __forceinline__ void tile_fma(int l, int q, const T* __restrict__ R, const T* __restrict__ Cm, T (&acc)[4][NC]) {
+    for (int m = l * l; m < (l + 1) * (l + 1); m++) {
+        T rv[4];
+        load4<T>(R + m * NC + 4 * q, rv);
+        T cv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c += 4) load4<T>(Cm + m * NC + c, cv + c);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[r][c] += rv[r] * cv[c];
+    }
+}
+
+// Write rows 4q..4q+3 (those < n) of an [n][n] tile at `o`, scaled by s.
+template <typename T, int NC>
+__device__ __forceinline__ void tile_store(T* __restrict__ o, int q, int n, T s, const T (&acc)[4][NC]) {
+    if (n == NC) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < NC; c += 4)
+                store4<T>(o + (4 * q + r) * NC + c, acc[r][c] * s, acc[r][c + 1] * s, acc[r][c + 2] * s, acc[r][c + 3] * s);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (4 * q + r < n) {
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+                    if (c < n) o[(4 * q + r) * n + c] = acc[r][c] * s;
+            }
+        }
+    }
+}
+
+// Per-entry data of entries [e_first, e_first + cnt): threads [0, cnt) compute
+// the spherical harmonics of one entry each, threads [64, 64 + cnt * L) the
+// radial functions of one (entry, l) each, zero-padded to NC.
+template <typename T>
+__device__ void tile_entry_data(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e_first, int cnt, T* ent) {
+    const int tid = threadIdx.x;
+    if (tid < cnt) {
+        const long long e = e_first + tid;
+        T* base = ent + tid * lay.per_entry;
+        const double r = a.entry_dist[e];
+        double d[3];
+        if (r < 1e-6) {
+            d[0] = 0.0;
+            d[1] = 0.0;
+            d[2] = 1.0;
+        } else {
+            d[0] = a.entry_vec[3 * e] / r;
+            d[1] = a.entry_vec[3 * e + 1] / r;
+            d[2] = a.entry_vec[3 * e + 2] / r;
+        }
+        T dt[3] = {(T)d[0], (T)d[1], (T)d[2]};
+        spherical_harmonics<T>(lay.L, dt, base + lay.y, base + lay.dy, base + lay.p, base + lay.dp, base + lay.pt, a.sh_a, a.sh_b);
+        base[lay.dir + 0] = dt[0];
+        base[lay.dir + 1] = dt[1];
+        base[lay.dir + 2] = dt[2];
+        base[lay.dir + 3] = 0;
+    }
+    const int w = tid - WAVE;
+    if (w >= 0 && w < cnt * lay.L) {
+        const int c = w / lay.L, l = w - c * lay.L;
+        const long long e = e_first + c;
+        T* base = ent + c * lay.per_entry;
+        const double r = a.entry_dist[e];
+        double f, df;
+        scaling_functions(a.scalars, r, &f, &df);
+        const int n = lay.n, NC = lay.NC;
+        T* va = base + lay.ra + l * NC;
+        T* vb = base + lay.rb + l * NC;
+        radial_integral<T>(a.radial, l, e, r, va, vb);
+        for (int k = 0; k < n; k++) {
+            const T Rv = va[k], dR = vb[k];
+            va[k] = (T)df * Rv + (T)f * dR;
+            vb[k] = (T)f * Rv / (T)r;
+        }
+        for (int k = n; k < NC; k++) {
+            va[k] = 0;
+            vb[k] = 0;
+        }
+    }
+}
+
+template <typename T, int NC>
+__global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
+    SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
+    const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_scan,
+    const long long* __restrict__ row_scan, const long long* __restrict__ block_sample_base,
+    const int* __restrict__ atom_system, const int* __restrict__ atom_local, T* __restrict__ out, int* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ long long run_row[TILE_MAX_RUNS][TILE_MAX_TYPES];
+    __shared__ int run_first[TILE_MAX_RUNS + 1];
+    __shared__ int run_valid[TILE_MAX_RUNS];
+    __shared__ int run_tj[TILE_MAX_RUNS];
+    __shared__ int present[TILE_MAX_TYPES];
+    __shared__ int win[4];
+    __shared__ T scale_l[TILE_MAX_L];
+
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L;
+    const int CS = L2 * NC;
+    const int npairs = S * (S + 1) / 2;
+    const TileEntryLayout lay(L, n, NC);
+    long long* rowbase = reinterpret_cast<long long*>(smem);
+    int* sample_in_block = reinterpret_cast<int*>(rowbase + npairs);
+    int* counters = sample_in_block + npairs;
+    int* snapshot = counters + npairs;
+    int* pair_k1 = snapshot + npairs;
+    int* pair_k2 = pair_k1 + npairs;
+    const size_t meta_bytes = ((size_t)npairs * (8 + 4 * 5) + 15) / 16 * 16;
+    T* C = reinterpret_cast<T*>(smem + meta_bytes);
+    T* region = C + (size_t)S * CS;
+    T* DC = region;
+    T* ent = DC + (size_t)max_runs * 3 * CS;
+
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const unsigned mask = type_mask[i];
+    const int n_present = __popc(mask);
+    const int ct = a.species[i];
+    const int sys = atom_system[i];
+    constexpr int nq = NC / 4;
+    const int P = L * n * n;
+
+    for (int k = tid; k < S * CS; k += TILE_THREADS) {
+        const int t = k / CS;
+        const int r = k - t * CS;
+        const int lm = r / NC, c = r - lm * NC;
+        C[k] = c < n ? coef[((long long)i * S + t) * L2 * n + lm * n + c] : (T)0;
+    }
+    for (int k = tid; k < npairs; k += TILE_THREADS) {
+        const long long idx = (long long)(ct * npairs + k) * a.n_atoms + i;
+        rowbase[k] = row_scan[idx];
+        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * npairs + k]);
+        counters[k] = 0;
+        snapshot[k] = -1;
+    }
+    if (tid < L) scale_l[tid] = (T)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
+    if (tid == 0) {
+        int k = 0;
+        for (int t = 0; t < S; t++)
+            if ((mask >> t) & 1u) present[k++] = t;
+        int p = 0;
+        for (int k1 = 0; k1 < n_present; k1++)
+            for (int k2 = k1; k2 < n_present; k2++) {
+                pair_k1[p] = k1;
+                pair_k2[p] = k2;
+                p++;
+            }
+        win[3] = 0;
+    }
+    __syncthreads();
+
+    bool snapped = false;  // thread 0 only
+    bool carry = false;    // thread 0 only: the next window continues a partial run
+    int pos = 0;
+    while (pos < len) {
+        if (tid < WAVE) {
+            const int avail = min(max_entries, len - pos);
+            bool boundary = false;
+            if (tid >= 1 && tid <= avail)
+                boundary = (pos + tid == len) || (a.entry_j[e0 + pos + tid] != a.entry_j[e0 + pos + tid - 1]);
+            const unsigned long long bits = __ballot(boundary);
+            if (tid == 0) {
+                unsigned long long m = bits;
+                int nr = 0;
+                run_first[0] = 0;
+                while (m != 0ull && nr < max_runs) {
+                    const int b = __ffsll((long long)m) - 1;
+                    m &= m - 1ull;
+                    nr++;
+                    run_first[nr] = b;
+                }
+                int ncomplete = nr;
+                if (nr == 0) {
+                    run_first[1] = avail;
+                    nr = 1;
+                }
+                const int wlen = run_first[nr];
+                for (int r = 0; r < nr; r++) {
+                    const int j = a.entry_j[e0 + pos + run_first[r]];
+                    const int tj = a.species[j];
+                    run_tj[r] = tj;
+                    run_valid[r] = j != i;
+                    if (r < ncomplete && j != i) {
+                        if (!snapped && j > i) {
+                            for (int k = 0; k < npairs; k++) snapshot[k] = counters[k];
+                            snapped = true;
+                        }
+                        const int local_j = atom_local[j];
+                        for (int k = 0; k < n_present; k++) {
+                            const int t = present[k];
+                            const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
+                            const int local = pair_index(lo, hi, S);
+                            const long long row = rowbase[local] + counters[local] + (i < j ? 1 : 0);
+                            counters[local] += 1;
+                            run_row[r][k] = row;
+                            labels[3 * row] = sample_in_block[local];
+                            labels[3 * row + 1] = sys;
+                            labels[3 * row + 2] = local_j;
+                        }
+                    }
+                }
+                win[0] = nr;
+                win[1] = ncomplete;
+                win[2] = wlen;
+                win[3] = carry ? 1 : 0;
+                carry = ncomplete == 0;
+            }
+        }
+        __syncthreads();
+        const int nr = win[0], ncomplete = win[1], wlen = win[2];
+        const bool carry_in = win[3] != 0;
+
+        tile_entry_data<T>(a, lay, e0 + pos, wlen, ent);
+        __syncthreads();
+
+        // DC of every run in the window (a carried partial run continues in slot 0)
+        for (int u = tid; u < nr * 3 * L2; u += TILE_THREADS) {
+            const int r = u / (3 * L2);
+            const int rem = u - r * 3 * L2;
+            const int x = rem / L2, lm = rem - x * L2;
+            if (!run_valid[r]) continue;
+            const int l = l_of_lm(lm);
+            T* dst = DC + ((size_t)r * 3 + x) * CS + lm * NC;
+            T acc[NC];
+            if (r == 0 && carry_in) {
+#pragma unroll
+                for (int c = 0; c < NC; c += 4) load4<T>(dst + c, acc + c);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c] = 0;
+            }
+            for (int k = run_first[r]; k < run_first[r + 1]; k++) {
+                const T* en = ent + k * lay.per_entry;
+                const T g = en[lay.y + lm] * en[lay.dir + x];
+                const T dy = en[lay.dy + x * L2 + lm];
+                T ra[NC], rb[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c += 4) {
+                    load4<T>(en + lay.ra + l * NC + c, ra + c);
+                    load4<T>(en + lay.rb + l * NC + c, rb + c);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c] += ra[c] * g + rb[c] * dy;
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c += 4) store4<T>(dst + c, acc[c], acc[c + 1], acc[c + 2], acc[c + 3]);
+        }
+        __syncthreads();
+
+        // gradient rows of the complete runs: units (l descending, x, quad) x (run, type)
+        const int NRK = ncomplete * n_present;
+        for (int u = tid; u < NRK * 3 * L * nq; u += TILE_THREADS) {
+            const int rem = u / NRK;
+            const int rk = u - rem * NRK;
+            const int r = rk / n_present, k = rk - r * n_present;
+            if (!run_valid[r]) continue;
+            const int lrev = rem / (3 * nq);
+            const int l = L - 1 - lrev;
+            const int xq = rem - lrev * 3 * nq;
+            const int x = xq / nq, q = xq - x * nq;
+            const int t = present[k], tj = run_tj[r];
+            const T* D = DC + ((size_t)r * 3 + x) * CS;
+            const T* Ct = C + (size_t)t * CS;
+            T acc[4][NC];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[rr][c] = 0;
+            if (t >= tj) tile_fma<T, NC>(l, q, D, Ct, acc);
+            if (t <= tj) tile_fma<T, NC>(l, q, Ct, D, acc);
+            const T s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+            tile_store<T, NC>(out + run_row[r][k] * 3 * P + (long long)x * P + l * n * n, q, n, s, acc);
+        }
+        __syncthreads();
+        pos += wlen;
+    }
+
+    // the center's own rows: d c^{t} / d r_i are the self gradients
+    if (tid == 0 && !snapped) {
+        for (int k = 0; k < npairs; k++) snapshot[k] = counters[k];
+    }
+    T* DS = region;
+    for (int k = tid; k < n_present * 3 * CS; k += TILE_THREADS) {
+        const int kt = k / (3 * CS);
+        const int r = k - kt * 3 * CS;
+        const int x = r / CS;
+        const int r2 = r - x * CS;
+        const int lm = r2 / NC, c = r2 - lm * NC;
+        DS[k] = c < n ? dself[(((long long)i * S + present[kt]) * 3 + x) * L2 * n + lm * n + c] : (T)0;
+    }
+    __syncthreads();
+    const int npp = n_present * (n_present + 1) / 2;
+    for (int p = tid; p < npp; p += TILE_THREADS) {
+        const int t1 = present[pair_k1[p]], t2 = present[pair_k2[p]];
+        const int local = pair_index(t1, t2, S);
+        const long long row = rowbase[local] + snapshot[local];
+        labels[3 * row] = sample_in_block[local];
+        labels[3 * row + 1] = sys;
+        labels[3 * row + 2] = atom_local[i];
+    }
+    for (int u = tid; u < npp * 3 * L * nq; u += TILE_THREADS) {
+        const int rem = u / npp;
+        const int p = u - rem * npp;
+        const int lrev = rem / (3 * nq);
+        const int l = L - 1 - lrev;
+        const int xq = rem - lrev * 3 * nq;
+        const int x = xq / nq, q = xq - x * nq;
+        const int k1 = pair_k1[p], k2 = pair_k2[p];
+        const int t1 = present[k1], t2 = present[k2];
+        const int local = pair_index(t1, t2, S);
+        const long long row = rowbase[local] + snapshot[local];
+        T acc[4][NC];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[rr][c] = 0;
+        tile_fma<T, NC>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
+        tile_fma<T, NC>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
+        const T s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+        tile_store<T, NC>(out + row * 3 * P + (long long)x * P + l * n * n, q, n, s, acc);
+    }
+}
+
 // Power spectrum strain / cell gradients: one row of [3][3][P] per sample.
 template <typename T>
 __global__ void __launch_bounds__(WAVE) ps_cell_strain_kernel(int n_atoms, int S, int L, int n, const T* __restrict__ coef,
@@ -1021,6 +1450,46 @@ bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, c
 }
 
 template <typename T>
+size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries) {
+    const int NC = (p.n + 3) / 4 * 4;
+    const size_t cs = (size_t)p.L * p.L * NC;
+    const size_t npairs = (size_t)p.n_species * (p.n_species + 1) / 2;
+    const size_t meta = (npairs * (8 + 4 * 5) + 15) / 16 * 16;
+    const size_t window = (size_t)max_runs * 3 * cs + (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries;
+    const size_t self = (size_t)p.n_species * 3 * cs;
+    return meta + ((size_t)p.n_species * cs + (window > self ? window : self)) * sizeof(T);
+}
+
+template <typename T>
+bool launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* coef, const T* dself,
+                              const unsigned* type_mask, const long long* sample_scan, const long long* row_scan,
+                              const long long* block_sample_base, const int* atom_system, const int* atom_local, T* out,
+                              int* labels) {
+    if (p.n_atoms == 0) return true;
+    if (p.n > 16 || p.L > TILE_MAX_L || p.n_species > TILE_MAX_TYPES || max_runs < 1 || max_runs > TILE_MAX_RUNS ||
+        max_entries < 1 || max_entries > WAVE - 1 || max_entries * p.L > TILE_THREADS - WAVE) {
+        return false;
+    }
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries);
+#define RA_LAUNCH_TILED(NCV)                                                                                           \
+    ps_gradient_tiled_kernel<T, NCV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                                   \
+        a, max_runs, max_entries, coef, dself, type_mask, sample_scan, row_scan, block_sample_base, atom_system,      \
+        atom_local, out, labels)
+    if (p.n <= 4) {
+        RA_LAUNCH_TILED(4);
+    } else if (p.n <= 8) {
+        RA_LAUNCH_TILED(8);
+    } else if (p.n <= 12) {
+        RA_LAUNCH_TILED(12);
+    } else {
+        RA_LAUNCH_TILED(16);
+    }
+#undef RA_LAUNCH_TILED
+    return true;
+}
+
+template <typename T>
 void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
                            const long long* sample_flag, const long long* sample_scan, T* out) {
     if (p.n_atoms == 0) return;
@@ -1048,6 +1517,10 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
     template bool launch_ps_gradient_fast<T>(const SoapLaunch<T>&, int, const T*, const T*, const unsigned*,          \
                                              const long long*, const long long*, const long long*, const int*,        \
                                              const int*, T*, int*);                                                    \
+    template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int);                                        \
+    template bool launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, const T*, const T*, const unsigned*,    \
+                                              const long long*, const long long*, const long long*, const int*,       \
+                                              const int*, T*, int*);                                                   \
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
                                           const long long*, const long long*, const long long*, const long long*,      \
                                           const long long*, const int*, const int*, T*, int*);                         \
