@@ -140,6 +140,7 @@ struct Calculator::DeviceTables {
     DeviceRadial<T> radial;
     T* sh_a = nullptr;
     T* sh_b = nullptr;
+    T* sh_c = nullptr;
     T* self_c = nullptr;
     std::vector<void*> owned;
 
@@ -204,6 +205,31 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
         }
         t->sh_a = t->put(A);
         t->sh_b = t->put(B);
+        // per-(l, m) and per-l factors of the lane-parallel evaluation, each rounded
+        // to T the way the serial evaluation rounds it (spherical_harmonics.rs:228-298):
+        // [leg] sqrt((l+m)(l-m+1)), [leg] sqrt((l-m)(l+m+1)), [L] sqrt(2l+1),
+        // [L] sqrt(1 + 1/(2l)) (the l = 1 value is sqrt(3/2)), [L] sqrt(l(l+1)/2),
+        // [L] -1/(l(l+1))
+        {
+            const int leg = L * (L + 1) / 2;
+            std::vector<T> c(2 * leg + 4 * L, static_cast<T>(0));
+            for (int l = 0; l < L; l++) {
+                for (int m = 0; m <= l; m++) {
+                    const int k = l * (l + 1) / 2 + m;
+                    c[k] = static_cast<T>(std::sqrt(static_cast<double>((l + m) * (l - m + 1))));
+                    c[leg + k] = static_cast<T>(std::sqrt(static_cast<double>((l - m) * (l + m + 1))));
+                }
+                c[2 * leg + l] = l == 1 ? static_cast<T>(1.7320508075688772) : static_cast<T>(std::sqrt(2.0 * l + 1.0));
+                if (l == 1) {
+                    c[2 * leg + L + l] = static_cast<T>(1.224744871391589);
+                } else if (l >= 2) {
+                    c[2 * leg + L + l] = static_cast<T>(std::sqrt(1.0 + 0.5 / l));
+                }
+                c[2 * leg + 2 * L + l] = static_cast<T>(std::sqrt(0.5 * static_cast<double>(l * (l + 1))));
+                if (l >= 1) c[2 * leg + 3 * L + l] = -static_cast<T>(1) / static_cast<T>(l * l + l);
+            }
+            t->sh_c = t->put(c);
+        }
         std::vector<T> sc;
         for (double x : self_contribution_) sc.push_back(static_cast<T>(x));
         t->self_c = t->put(sc);
@@ -213,6 +239,10 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
 }
 
 // ---------------------------------------------------------------------------- SOAP pipeline
+
+#ifndef RA_TILED_LDS_BUDGET
+#define RA_TILED_LDS_BUDGET 53248
+#endif
 
 namespace {
 
@@ -391,6 +421,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     lp.scalars.exponent = soap_.scaling_exponent;
     lp.sh_a = tab.sh_a;
     lp.sh_b = tab.sh_b;
+    lp.sh_c = tab.sh_c;
     lp.self_contribution = tab.self_c;
     int* error_flag = ws_.get<int>("soap_error", 1);
     RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
@@ -484,23 +515,42 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     } else {
         launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
     }
-    RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+    bool timed_start = false;
+    if (!grads.positions) {
+        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        timed_start = true;
+    }
     if (grads.positions) {
         bool done = false;
-        if (ps) {
-            // windowed tiled kernel: the largest (runs, entries) window that keeps the
-            // workgroup within ~56 KB of LDS (3-4 workgroups per CU)
-            static const int windows[][2] = {{4, 8}, {2, 8}, {2, 4}, {1, 4}, {1, 1}};
-            const size_t budget = std::min<size_t>(lds > 1024 ? lds - 1024 : 0, 57344);
+        if (ps && total_rows < (1ll << 31)) {
+            // planned, windowed tiled kernel: the first (runs, entries, staged rows)
+            // configuration that keeps the workgroup within the LDS budget (3
+            // workgroups per CU in FP32, 2 in FP64)
+            static const int windows[][3] = {{4, 32, 4}, {4, 32, 2}, {4, 16, 2}, {4, 16, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}};
+            const size_t cap = sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920;
+            const size_t budget = std::min<size_t>(lds > 1024 ? lds - 1024 : 0, cap);
             for (const auto& w : windows) {
-                if (ps_gradient_tiled_shmem<T>(lp, w[0], w[1]) <= budget) {
-                    done = launch_ps_gradient_tiled<T>(lp, w[0], w[1], coef, dself, type_mask, sample_scan, row_scan,
-                                                       d_block_sample_base, d_atom_system, d_atom_local, d_grads,
-                                                       grad_labels);
-                    break;
-                }
+                if (!ps_gradient_tiled_supported(L, n, S, w[0], w[1], w[2])) continue;
+                if (ps_gradient_tiled_shmem<T>(lp, w[0], w[1], w[2]) > budget) continue;
+                const long long ne = std::max<long long>(nl.n_entries, 1);
+                const int npairs = S * (S + 1) / 2;
+                int* d_runs = ws_.get<int>("plan_runs", ne * 4);
+                int* d_run_rows = ws_.get<int>("plan_run_rows", ne * S);
+                int* d_windows = ws_.get<int>("plan_windows", ne * 4);
+                int* d_n_windows = ws_.get<int>("plan_n_windows", std::max<long long>(n_atoms, 1));
+                int* d_self_rows = ws_.get<int>("plan_self_rows", std::max<long long>(n_atoms, 1) * npairs);
+                launch_ps_grad_plan(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, sample_scan,
+                                    row_scan, d_block_sample_base, d_atom_system, d_atom_local, w[0], w[1], d_runs,
+                                    d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels);
+                RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+                timed_start = true;
+                launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows, d_windows,
+                                            d_n_windows, d_self_rows, d_grads);
+                done = true;
+                break;
             }
         }
+        if (!timed_start) RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
         if (ps && !done) {
             // register-blocked kernel; chunk sized for ~48 KB of LDS per workgroup
             const size_t fixed = ps_gradient_shmem<T>(lp, 0);

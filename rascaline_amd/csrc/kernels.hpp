@@ -52,6 +52,7 @@ struct SoapLaunch {
     DeviceSoapScalars scalars;
     const T* sh_a;
     const T* sh_b;
+    const T* sh_c;
     const T* self_contribution;
     int chunk;
     int* error_flag;
@@ -75,12 +76,20 @@ void launch_gradient_rows(const SoapLaunch<T>& p, bool ps, const T* coef, const 
                           const long long* block_sample_base, const long long* se_grad_base, const int* atom_system,
                           const int* atom_local, T* out, int* labels);
 template <typename T>
-size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries);
+size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass);
+bool ps_gradient_tiled_supported(int L, int n, int n_species, int max_runs, int max_entries, int rows_per_pass);
+// Planning pass of the tiled gradient kernel; runs / windows are int4 arrays with
+// one slot per neighbour entry, run_rows [entries][n_species], self_rows
+// [n_atoms][n_species (n_species + 1) / 2].
+void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const long long* offsets, const int* entry_j,
+                         const int* species, const unsigned* type_mask, const long long* sample_scan,
+                         const long long* row_scan, const long long* block_sample_base, const int* atom_system,
+                         const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
+                         int* n_windows, int* self_rows, int* labels);
 template <typename T>
-bool launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* coef, const T* dself,
-                              const unsigned* type_mask, const long long* sample_scan, const long long* row_scan,
-                              const long long* block_sample_base, const int* atom_system, const int* atom_local, T* out,
-                              int* labels);
+void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
+                              const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
+                              const int* windows, const int* n_windows, const int* self_rows, T* out);
 template <typename T>
 size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
 // register-blocked power-spectrum gradient rows; returns false when n > 16

@@ -61,6 +61,7 @@ struct SoapArgs {
     DeviceSoapScalars scalars;
     const T* sh_a;
     const T* sh_b;
+    const T* sh_c;  // spherical-harmonics coefficient table (ShTable layout)
     const T* self_contribution;
     int chunk;
     int* error_flag;
@@ -801,47 +802,66 @@ __global__ void __launch_bounds__(GRAD_THREADS) ps_gradient_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Power-spectrum position gradients, windowed and tiled (n padded to NC, a
-// multiple of 4).
+// Power-spectrum position gradients, planned, windowed and tiled (n padded to NC,
+// a multiple of 4).
 //
-// One 256-thread workgroup per center. The center's sorted neighbour entries are
-// consumed in windows of up to `max_runs` complete runs (a run = all periodic
-// images of one neighbour atom j) and `max_entries` entries; a run longer than
-// a window is accumulated over several windows. Per window:
-//   1. wave 0 finds the run boundaries (ballot) and thread 0 assigns the output
-//      rows of every (run, present type) in the reference's row order;
-//   2. per-entry Y_lm, r dY_lm and radial values go to LDS;
-//   3. DC_r[x][lm][:] = d c_i^{type j} / d r_j is summed over the run's images,
-//      one (run, x, lm) row of NC values per thread;
-//   4. every gradient row is produced by (l, x, quad of n1) tiles of 4 x NC
-//      register accumulators. For a neighbour of type tj and a present type t,
+// A planning pass (one wave per center) splits every center's sorted neighbour
+// entries into runs (all periodic images of one neighbour atom j), assigns every
+// output row in the reference's order (rows sorted by atom within each block,
+// the center's own row before its first neighbour j > i), writes all gradient
+// sample labels, and groups the runs into windows of at most `max_runs` runs and
+// `max_entries` entries (a longer run is split over several windows whose sums
+// are carried). Runs of periodic images of the center itself produce no row and
+// are left out. The gradient kernel (one 256-thread workgroup per center) then
+// processes the windows:
+//   1. one lane per (entry, m) runs the Legendre recursion along l (plus the
+//      direction, trigonometric and cutoff factors on the m = 0 lane), one lane
+//      per (entry, l) evaluates the radial spline;
+//   2. one lane per (x, run, l, m >= 0) forms Y_l(±m) and r dY_l(±m)/dx and sums
+//      DC_r[x][lm][:] = d c_i^{type j} / d r_j over the run's images;
+//   3. every gradient row is produced by (l, x, quad of n1) tiles of 4 x NC
+//      register accumulators: for a neighbour of type tj and a present type t,
 //      block (min, max)(t, tj) gets Σ_m DC[m][n1] C_t[m][n2] (t > tj),
-//      Σ_m C_t[m][n1] DC[m][n2] (t < tj) or both (t == tj) -- the operand order
-//      alone handles the transposed case, so each thread always writes 4 x NC
-//      contiguous outputs.
-// The center's own rows (self gradients for both factors) follow the windows.
+//      Σ_m C_t[m][n1] DC[m][n2] (t < tj) or both (t == tj). The tiles of one wave
+//      leave through its LDS staging area so that consecutive lanes store
+//      consecutive 16-byte pieces.
+// The center's own rows (self gradients in both factors) follow the windows.
 constexpr int TILE_THREADS = 256;
 constexpr int TILE_MAX_RUNS = 8;
 constexpr int TILE_MAX_TYPES = 32;
 constexpr int TILE_MAX_L = 16;
 
-// Per-entry LDS layout, radial arrays padded to NC and 4-element aligned.
+// Per-entry LDS layout: radial values R and dR/dr padded to NC (4-element aligned),
+// direction, trigonometric factors, cutoff factors, Legendre functions.
 struct TileEntryLayout {
     int L, n, NC, L2, leg;
-    int y, dy, ra, rb, dir, p, dp, pt, per_entry;
+    int rv, rd, dir, trig, scal, p, per_entry;
     __host__ __device__ TileEntryLayout(int L_, int n_, int NC_)
         : L(L_), n(n_), NC(NC_), L2(L_ * L_), leg(L_ * (L_ + 1) / 2) {
         int o = 0;
-        y = o; o += L2;
-        dy = o; o += 3 * L2;
-        o = (o + 3) / 4 * 4;
-        ra = o; o += L * NC;
-        rb = o; o += L * NC;
+        rv = o; o += L * NC;
+        rd = o; o += L * NC;
         dir = o; o += 4;
+        trig = o; o += 4;   // cos theta, sin theta, cos phi, sin phi
+        scal = o; o += 4;   // f, df/dr, f / r
         p = o; o += leg;
-        dp = o; o += leg;
-        pt = o; o += leg;
         per_entry = (o + 3) / 4 * 4;
+    }
+};
+
+// Offsets in the per-l / per-(l, m) coefficient table of the spherical harmonics
+// (built on the host, every value rounded to T as the serial evaluation rounds it).
+struct ShTable {
+    int dpa, dpb, sq2l1, diag, gl, il, size;
+    __host__ __device__ explicit ShTable(int L) {
+        const int leg = L * (L + 1) / 2;
+        dpa = 0;
+        dpb = leg;
+        sq2l1 = 2 * leg;
+        diag = sq2l1 + L;
+        gl = diag + L;
+        il = gl + L;
+        size = il + L;
     }
 };
 
@@ -897,6 +917,19 @@ __device__ __forceinline__ void store4(T* p, T a0, T a1, T a2, T a3) {
     }
 }
 
+// q = u / d, r = u % d for 0 <= u < 2^22, 1 <= d < 2^12, with inv = 1.0f / d.
+__device__ __forceinline__ void small_divmod(int u, int d, float inv, int& q, int& r) {
+    q = (int)((float)u * inv);
+    r = u - q * d;
+    if (r < 0) {
+        q--;
+        r += d;
+    } else if (r >= d) {
+        q++;
+        r -= d;
+    }
+}
+
 // acc[r][c] += Σ_m R[m][4q + r] * Cm[m][c] over m in [l², (l+1)²); R and Cm are [L2][NC].
 template <typename T, int NC>
 __device__ __forceinline__ void tile_fma(int l, int q, const T* __restrict__ R, const T* __restrict__ Cm, T (&acc)[4][NC]) {
@@ -934,71 +967,396 @@ __device__ __forceinline__ void tile_store(T* __restrict__ o, int q, int n, T s,
     }
 }
 
-// Per-entry data of entries [e_first, e_first + cnt): threads [0, cnt) compute
-// the spherical harmonics of one entry each, threads [64, 64 + cnt * L) the
-// radial functions of one (entry, l) each, zero-padded to NC.
-template <typename T>
-__device__ void tile_entry_data(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e_first, int cnt, T* ent) {
-    const int tid = threadIdx.x;
-    if (tid < cnt) {
-        const long long e = e_first + tid;
-        T* base = ent + tid * lay.per_entry;
-        const double r = a.entry_dist[e];
-        double d[3];
-        if (r < 1e-6) {
-            d[0] = 0.0;
-            d[1] = 0.0;
-            d[2] = 1.0;
+// Per-wave staged store: every lane holds rows 4q..4q+3 of an [n][n] tile (n ==
+// NC) bound for `tile` (or nothing, tile == nullptr). The rows go through the
+// wave's LDS staging area, RP rows per pass, and are written back so that
+// consecutive lanes store consecutive 16-byte pieces of each lane's rows: a store
+// instruction then covers a few contiguous runs instead of 64 separate lines.
+template <typename T, int NC, int RP>
+__device__ __forceinline__ void staged_tile_store(T* tile, int q, T s, const T (&acc)[4][NC], T* stage, T** stage_ptr) {
+    constexpr int VEC = 16 / sizeof(T);           // values per 16-byte piece
+    constexpr int PER_OWNER = RP * NC / VEC;      // pieces per lane and pass
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int h = 0; h < 4; h += RP) {
+        if (tile != nullptr) {
+#pragma unroll
+            for (int r = 0; r < RP; r++)
+#pragma unroll
+                for (int c = 0; c < NC; c += 4)
+                    store4<T>(stage + lane * RP * NC + r * NC + c, acc[h + r][c] * s, acc[h + r][c + 1] * s,
+                              acc[h + r][c + 2] * s, acc[h + r][c + 3] * s);
+            stage_ptr[lane] = tile + (4 * q + h) * NC;
         } else {
-            d[0] = a.entry_vec[3 * e] / r;
-            d[1] = a.entry_vec[3 * e + 1] / r;
-            d[2] = a.entry_vec[3 * e + 2] / r;
+            stage_ptr[lane] = nullptr;
         }
-        T dt[3] = {(T)d[0], (T)d[1], (T)d[2]};
-        spherical_harmonics<T>(lay.L, dt, base + lay.y, base + lay.dy, base + lay.p, base + lay.dp, base + lay.pt, a.sh_a, a.sh_b);
-        base[lay.dir + 0] = dt[0];
-        base[lay.dir + 1] = dt[1];
-        base[lay.dir + 2] = dt[2];
-        base[lay.dir + 3] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int s2 = 0; s2 < PER_OWNER; s2++) {
+            const int p = s2 * WAVE + lane;
+            const int owner = p / PER_OWNER, sub = p - owner * PER_OWNER;
+            T* dst = stage_ptr[owner];
+            if (dst != nullptr) {
+                typedef typename Vec4Of<T>::type V;
+                *reinterpret_cast<V*>(dst + sub * VEC) =
+                    *reinterpret_cast<const V*>(stage + owner * RP * NC + sub * VEC);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    const int w = tid - WAVE;
-    if (w >= 0 && w < cnt * lay.L) {
-        const int c = w / lay.L, l = w - c * lay.L;
-        const long long e = e_first + c;
-        T* base = ent + c * lay.per_entry;
-        const double r = a.entry_dist[e];
-        double f, df;
-        scaling_functions(a.scalars, r, &f, &df);
-        const int n = lay.n, NC = lay.NC;
-        T* va = base + lay.ra + l * NC;
-        T* vb = base + lay.rb + l * NC;
-        radial_integral<T>(a.radial, l, e, r, va, vb);
-        for (int k = 0; k < n; k++) {
-            const T Rv = va[k], dR = vb[k];
-            va[k] = (T)df * Rv + (T)f * dR;
-            vb[k] = (T)f * Rv / (T)r;
+}
+
+// Planning pass: one 64-lane workgroup per center. Runs go to runs[e0 + r] =
+// {first entry, entry count, neighbour type, -}, their rows (one per present type,
+// in present-type order) to run_rows[(e0 + r) * S + k], windows to windows[e0 + w]
+// = {first entry, entry count, first run, run count}, the center's own rows to
+// self_rows[i * npairs + block], and every row's (sample, system, atom) label to
+// labels. Row indices are global over the batch (from row_scan).
+__global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
+    int n_atoms, int S, const long long* __restrict__ offsets, const int* __restrict__ entry_j,
+    const int* __restrict__ species, const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_scan,
+    const long long* __restrict__ row_scan, const long long* __restrict__ block_sample_base,
+    const int* __restrict__ atom_system, const int* __restrict__ atom_local, int max_runs, int max_entries,
+    int4* __restrict__ runs, int* __restrict__ run_rows, int4* __restrict__ windows, int* __restrict__ n_windows,
+    int* __restrict__ self_rows, int* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int present[TILE_MAX_TYPES];
+    const int npairs = S * (S + 1) / 2;
+    long long* rowbase = reinterpret_cast<long long*>(smem);
+    int* sample_in_block = reinterpret_cast<int*>(rowbase + npairs);
+    int* counters = sample_in_block + npairs;
+    int* snapshot = counters + npairs;
+
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const long long e0 = offsets[i];
+    const int len = (int)(offsets[i + 1] - e0);
+    const int ct = species[i];
+    const unsigned mask = type_mask[i];
+    const int n_present = __popc(mask);
+    const int sys = atom_system[i];
+    for (int k = lane; k < npairs; k += WAVE) {
+        const long long idx = (long long)(ct * npairs + k) * n_atoms + i;
+        rowbase[k] = row_scan[idx];
+        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * npairs + k]);
+        counters[k] = 0;
+    }
+    if (lane == 0) {
+        int k = 0;
+        for (int t = 0; t < S; t++)
+            if ((mask >> t) & 1u) present[k++] = t;
+    }
+    __syncthreads();
+
+    // uniform planning state
+    bool snapped = false;
+    int n_runs = 0, n_win = 0;
+    int cur_start = 0, cur_entries = 0, cur_run_start = 0, cur_runs = 0;
+    auto emit = [&](int start, int entries, int run_start, int count) {
+        if (lane == 0) windows[e0 + n_win] = make_int4(start, entries, run_start, count);
+        n_win++;
+    };
+    auto process_run = [&](int first, int count, int j) {
+        if (j == i) {
+            // periodic images of the center: no row; close the current window around them
+            if (cur_runs > 0) emit(cur_start, cur_entries, cur_run_start, cur_runs);
+            cur_runs = 0;
+            cur_entries = 0;
+            return;
         }
-        for (int k = n; k < NC; k++) {
-            va[k] = 0;
-            vb[k] = 0;
+        const int r = n_runs++;
+        const int tj = species[j];
+        if (lane == 0) runs[e0 + r] = make_int4(first, count, tj, 0);
+        if (!snapped && j > i) {
+            for (int k = lane; k < npairs; k += WAVE) snapshot[k] = counters[k];
+            snapped = true;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < n_present) {
+            const int t = present[lane];
+            const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
+            const int local = pair_index(lo, hi, S);
+            const long long row = rowbase[local] + counters[local] + (i < j ? 1 : 0);
+            counters[local] += 1;
+            run_rows[(e0 + r) * S + lane] = (int)row;
+            labels[3 * row] = sample_in_block[local];
+            labels[3 * row + 1] = sys;
+            labels[3 * row + 2] = atom_local[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        // windows: up to max_runs runs and max_entries entries; a longer run is split
+        int pos = first, rem = count;
+        for (;;) {
+            if (cur_runs > 0 && (cur_runs == max_runs || cur_entries + rem > max_entries)) {
+                emit(cur_start, cur_entries, cur_run_start, cur_runs);
+                cur_runs = 0;
+                cur_entries = 0;
+            }
+            if (cur_runs == 0) {
+                cur_start = pos;
+                cur_run_start = r;
+                if (rem > max_entries) {
+                    emit(pos, max_entries, r, 1);
+                    pos += max_entries;
+                    rem -= max_entries;
+                    continue;
+                }
+            }
+            cur_runs++;
+            cur_entries += rem;
+            break;
+        }
+    };
+
+    int open_first = -1, open_j = -1;
+    int prev_last = -1;
+    for (int base = 0; base < len; base += WAVE) {
+        const int e = base + lane;
+        const int j = e < len ? entry_j[e0 + e] : -1;
+        int prev = __shfl_up(j, 1);
+        if (lane == 0) prev = prev_last;
+        const bool start = e < len && (e == 0 || j != prev);
+        unsigned long long bits = __ballot(start);
+        prev_last = __shfl(j, WAVE - 1);
+        while (bits != 0ull) {
+            const int b = __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
+            const int jb = __shfl(j, b);
+            if (open_first >= 0) process_run(open_first, base + b - open_first, open_j);
+            open_first = base + b;
+            open_j = jb;
+        }
+    }
+    if (open_first >= 0) process_run(open_first, len - open_first, open_j);
+    if (cur_runs > 0) emit(cur_start, cur_entries, cur_run_start, cur_runs);
+    if (lane == 0) n_windows[i] = n_win;
+
+    // the center's own rows: before its first neighbour j > i in every block
+    if (!snapped) {
+        for (int k = lane; k < npairs; k += WAVE) snapshot[k] = counters[k];
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int p = lane; p < n_present * n_present; p += WAVE) {
+        const int k1 = p / n_present, k2 = p - k1 * n_present;
+        if (k2 < k1) continue;
+        const int local = pair_index(present[k1], present[k2], S);
+        const long long row = rowbase[local] + snapshot[local];
+        self_rows[(long long)i * npairs + local] = (int)row;
+        labels[3 * row] = sample_in_block[local];
+        labels[3 * row + 1] = sys;
+        labels[3 * row + 2] = atom_local[i];
+    }
+}
+
+// Stage 1 for the entries [e_first, e_first + cnt) of a window. Items [0, cnt L):
+// one (entry, m) each, the associated Legendre functions P_lm for l >= m (the
+// recursion of featomic/src/math/spherical_harmonics.rs:228-251 run along l for
+// this m, the diagonal chain recomputed per lane); the m = 0 lane also stores the
+// direction, trigonometric factors and the cutoff / scaling factors. Items
+// [cnt L, 2 cnt L): one (entry, l) each, the radial spline R_nl and dR_nl/dr,
+// zero-padded to NC.
+template <typename T>
+__device__ void plan_entry_stage1(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e_first, int cnt, T* ent) {
+    const int L = lay.L;
+    const ShTable tb(L);
+    for (int item = threadIdx.x; item < 2 * cnt * L; item += TILE_THREADS) {
+        if (item < cnt * L) {
+            const int c = item / L, m = item - c * L;
+            const long long e = e_first + c;
+            T* base = ent + c * lay.per_entry;
+            const double r = a.entry_dist[e];
+            double d[3];
+            if (r < 1e-6) {
+                d[0] = 0.0;
+                d[1] = 0.0;
+                d[2] = 1.0;
+            } else {
+                d[0] = a.entry_vec[3 * e] / r;
+                d[1] = a.entry_vec[3 * e + 1] / r;
+                d[2] = a.entry_vec[3 * e + 2] / r;
+            }
+            const T x = (T)d[0], y = (T)d[1], z = (T)d[2];
+            const T sqrt_xy = (T)hypot((double)x, (double)y);
+            const T cos_theta = z, sin_theta = sqrt_xy;
+            if (m == 0) {
+                T cos_phi = 1, sin_phi = 0;
+                if ((double)sqrt_xy > (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07)) {
+                    cos_phi = x / sqrt_xy;
+                    sin_phi = y / sqrt_xy;
+                }
+                base[lay.dir + 0] = x;
+                base[lay.dir + 1] = y;
+                base[lay.dir + 2] = z;
+                base[lay.dir + 3] = 0;
+                base[lay.trig + 0] = cos_theta;
+                base[lay.trig + 1] = sin_theta;
+                base[lay.trig + 2] = cos_phi;
+                base[lay.trig + 3] = sin_phi;
+                double f, df;
+                scaling_functions(a.scalars, r, &f, &df);
+                base[lay.scal + 0] = (T)f;
+                base[lay.scal + 1] = (T)df;
+                base[lay.scal + 2] = (T)f / (T)r;
+                base[lay.scal + 3] = 0;
+            }
+            T value = (T)0.3989422804014327;
+            for (int l = 1; l <= m; l++) value *= -a.sh_c[tb.diag + l] * sin_theta;
+            T* P = base + lay.p;
+            P[leg_index(m, m)] = value;
+            if (m + 1 < L) {
+                T p2 = value;
+                T p1 = cos_theta * a.sh_c[tb.sq2l1 + m + 1] * value;
+                P[leg_index(m + 1, m)] = p1;
+                for (int l = m + 2; l < L; l++) {
+                    const int k = leg_index(l, m);
+                    const T p0 = a.sh_a[k] * (cos_theta * p1 + a.sh_b[k] * p2);
+                    P[k] = p0;
+                    p2 = p1;
+                    p1 = p0;
+                }
+            }
+        } else {
+            const int w = item - cnt * L;
+            const int c = w / L, l = w - c * L;
+            const long long e = e_first + c;
+            T* base = ent + c * lay.per_entry;
+            const int n = lay.n, NC = lay.NC;
+            T* va = base + lay.rv + l * NC;
+            T* vb = base + lay.rd + l * NC;
+            radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
+            for (int k = n; k < NC; k++) {
+                va[k] = 0;
+                vb[k] = 0;
+            }
         }
     }
 }
 
+// Stage 2: one item per (x, run, l, m >= 0) forms Y_l(±m) and r dY_l(±m)/dx of
+// every image of the run (spherical_harmonics.rs:256-298, :304-424) and sums
+// DC_run[x][l² + l ± m][:] += (df R + f dR) Y d_x + (f / r) R r dY/dx
+// (soap/spherical_expansion_pair.rs:382-419). A run carried from the previous
+// window (slot 0) continues its sums.
 template <typename T, int NC>
+__device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& lay, int nr, const int* run_lo,
+                                  const int* run_hi, bool carry_in, const int* leg_l, const int* leg_m, const T* ent,
+                                  T* DC) {
+    const int leg = lay.leg, L2 = lay.L2;
+    const int CS = L2 * NC;
+    const ShTable tb(lay.L);
+    const T SQRT_2 = (T)1.4142135623730951;
+    const int per_x = nr * leg;
+    for (int item = threadIdx.x; item < 3 * per_x; item += TILE_THREADS) {
+        const int x = item / per_x;
+        const int rest = item - x * per_x;
+        const int r = rest / leg, k = rest - r * leg;
+        const int l = leg_l[k], m = leg_m[k];
+        const int lm0 = l * l + l;
+        T* dp_ = DC + ((size_t)r * 3 + x) * CS + (lm0 + m) * NC;
+        T* dm_ = DC + ((size_t)r * 3 + x) * CS + (lm0 - m) * NC;
+        T accp[NC], accm[NC];
+        if (r == 0 && carry_in) {
+#pragma unroll
+            for (int c = 0; c < NC; c += 4) {
+                load4<T>(dp_ + c, accp + c);
+                load4<T>(dm_ + c, accm + c);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                accp[c] = 0;
+                accm[c] = 0;
+            }
+        }
+        for (int e = run_lo[r]; e < run_hi[r]; e++) {
+            const T* base = ent + e * lay.per_entry;
+            const T* P = base + lay.p;
+            const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+            const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+            const T dx = base[lay.dir + x];
+            T yp, ym = 0, gp, gm = 0;
+            if (m == 0) {
+                yp = P[k] / SQRT_2;
+                if (l == 0) {
+                    gp = 0;
+                } else {
+                    const T lf = a.sh_c[tb.gl + l] * P[leg_index(l, 1)];
+                    gp = x == 0 ? cos_phi * cos_theta * lf : (x == 1 ? sin_phi * cos_theta * lf : -sin_theta * lf);
+                }
+            } else {
+                T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+                const T minus_two_cos = -2 * cos_phi;
+                T sin_m = 0, cos_m = 1;
+                for (int mm = 1; mm <= m; mm++) {
+                    sin_m = minus_two_cos * sin_1 - sin_2;
+                    cos_m = minus_two_cos * cos_1 - cos_2;
+                    sin_2 = sin_1;
+                    sin_1 = sin_m;
+                    cos_2 = cos_1;
+                    cos_1 = cos_m;
+                }
+                const T p = P[k];
+                yp = p * cos_m;
+                ym = p * sin_m;
+                const T pm1 = P[leg_index(l, m - 1)];
+                const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
+                const T dp = a.sh_c[tb.dpa + k] * pm1 - a.sh_c[tb.dpb + k] * pp1;
+                const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
+                const T s_dp = sin_m * dp, c_dp = cos_m * dp;
+                if (x == 0) {
+                    gp = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
+                    gm = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
+                } else if (x == 1) {
+                    gp = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
+                    gm = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
+                } else {
+                    gp = (T)0.5 * sin_theta * c_dp;
+                    gm = (T)0.5 * sin_theta * s_dp;
+                }
+            }
+            const T f = base[lay.scal + 0], df = base[lay.scal + 1], f_r = base[lay.scal + 2];
+            const T ypd = yp * dx, ymd = ym * dx;
+            T Rv[NC], dR[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c += 4) {
+                load4<T>(base + lay.rv + l * NC + c, Rv + c);
+                load4<T>(base + lay.rd + l * NC + c, dR + c);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const T ra = df * Rv[c] + f * dR[c];
+                const T rb = f_r * Rv[c];
+                accp[c] += ra * ypd + rb * gp;
+                accm[c] += ra * ymd + rb * gm;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c += 4) store4<T>(dp_ + c, accp[c], accp[c + 1], accp[c + 2], accp[c + 3]);
+        if (m > 0) {
+#pragma unroll
+            for (int c = 0; c < NC; c += 4) store4<T>(dm_ + c, accm[c], accm[c + 1], accm[c + 2], accm[c + 3]);
+        }
+    }
+}
+
+template <typename T, int NC, int RP>
 __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
-    const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_scan,
-    const long long* __restrict__ row_scan, const long long* __restrict__ block_sample_base,
-    const int* __restrict__ atom_system, const int* __restrict__ atom_local, T* __restrict__ out, int* __restrict__ labels) {
+    const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
+    const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
+    T* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ long long run_row[TILE_MAX_RUNS][TILE_MAX_TYPES];
-    __shared__ int run_first[TILE_MAX_RUNS + 1];
-    __shared__ int run_valid[TILE_MAX_RUNS];
-    __shared__ int run_tj[TILE_MAX_RUNS];
+    __shared__ int run_lo[TILE_MAX_RUNS], run_hi[TILE_MAX_RUNS], run_tj[TILE_MAX_RUNS], run_complete[TILE_MAX_RUNS];
+    __shared__ int run_row[TILE_MAX_RUNS][TILE_MAX_TYPES];
     __shared__ int present[TILE_MAX_TYPES];
-    __shared__ int win[4];
+    __shared__ int carry_flag;
     __shared__ T scale_l[TILE_MAX_L];
+    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
 
     const int i = blockIdx.x;
     const int tid = threadIdx.x;
@@ -1007,24 +1365,20 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     const int CS = L2 * NC;
     const int npairs = S * (S + 1) / 2;
     const TileEntryLayout lay(L, n, NC);
-    long long* rowbase = reinterpret_cast<long long*>(smem);
-    int* sample_in_block = reinterpret_cast<int*>(rowbase + npairs);
-    int* counters = sample_in_block + npairs;
-    int* snapshot = counters + npairs;
-    int* pair_k1 = snapshot + npairs;
-    int* pair_k2 = pair_k1 + npairs;
-    const size_t meta_bytes = ((size_t)npairs * (8 + 4 * 5) + 15) / 16 * 16;
-    T* C = reinterpret_cast<T*>(smem + meta_bytes);
+    T* C = reinterpret_cast<T*>(smem);
     T* region = C + (size_t)S * CS;
     T* DC = region;
-    T* ent = DC + (size_t)max_runs * 3 * CS;
+    const size_t dc_size = (size_t)(max_runs > S ? max_runs : S) * 3 * CS;
+    T* ent = region + dc_size;
+    // staging of the stores overlays the entry data (free while rows are emitted)
+    const int wave_id = tid / WAVE;
+    T* stage = ent + (size_t)wave_id * WAVE * RP * NC;
+    T** stage_ptrs = reinterpret_cast<T**>(ent + (size_t)(TILE_THREADS / WAVE) * WAVE * RP * NC) + wave_id * WAVE;
+    const bool staged = n == NC;
 
     const long long e0 = a.offsets[i];
-    const int len = (int)(a.offsets[i + 1] - e0);
     const unsigned mask = type_mask[i];
     const int n_present = __popc(mask);
-    const int ct = a.species[i];
-    const int sys = atom_system[i];
     constexpr int nq = NC / 4;
     const int P = L * n * n;
 
@@ -1034,159 +1388,85 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         const int lm = r / NC, c = r - lm * NC;
         C[k] = c < n ? coef[((long long)i * S + t) * L2 * n + lm * n + c] : (T)0;
     }
-    for (int k = tid; k < npairs; k += TILE_THREADS) {
-        const long long idx = (long long)(ct * npairs + k) * a.n_atoms + i;
-        rowbase[k] = row_scan[idx];
-        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * npairs + k]);
-        counters[k] = 0;
-        snapshot[k] = -1;
-    }
     if (tid < L) scale_l[tid] = (T)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
+    for (int k = tid; k < lay.leg; k += TILE_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        leg_l[k] = l;
+        leg_m[k] = k - l * (l + 1) / 2;
+    }
     if (tid == 0) {
         int k = 0;
         for (int t = 0; t < S; t++)
             if ((mask >> t) & 1u) present[k++] = t;
-        int p = 0;
-        for (int k1 = 0; k1 < n_present; k1++)
-            for (int k2 = k1; k2 < n_present; k2++) {
-                pair_k1[p] = k1;
-                pair_k2[p] = k2;
-                p++;
-            }
-        win[3] = 0;
     }
-    __syncthreads();
 
-    bool snapped = false;  // thread 0 only
-    bool carry = false;    // thread 0 only: the next window continues a partial run
-    int pos = 0;
-    while (pos < len) {
-        if (tid < WAVE) {
-            const int avail = min(max_entries, len - pos);
-            bool boundary = false;
-            if (tid >= 1 && tid <= avail)
-                boundary = (pos + tid == len) || (a.entry_j[e0 + pos + tid] != a.entry_j[e0 + pos + tid - 1]);
-            const unsigned long long bits = __ballot(boundary);
-            if (tid == 0) {
-                unsigned long long m = bits;
-                int nr = 0;
-                run_first[0] = 0;
-                while (m != 0ull && nr < max_runs) {
-                    const int b = __ffsll((long long)m) - 1;
-                    m &= m - 1ull;
-                    nr++;
-                    run_first[nr] = b;
-                }
-                int ncomplete = nr;
-                if (nr == 0) {
-                    run_first[1] = avail;
-                    nr = 1;
-                }
-                const int wlen = run_first[nr];
-                for (int r = 0; r < nr; r++) {
-                    const int j = a.entry_j[e0 + pos + run_first[r]];
-                    const int tj = a.species[j];
-                    run_tj[r] = tj;
-                    run_valid[r] = j != i;
-                    if (r < ncomplete && j != i) {
-                        if (!snapped && j > i) {
-                            for (int k = 0; k < npairs; k++) snapshot[k] = counters[k];
-                            snapped = true;
-                        }
-                        const int local_j = atom_local[j];
-                        for (int k = 0; k < n_present; k++) {
-                            const int t = present[k];
-                            const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
-                            const int local = pair_index(lo, hi, S);
-                            const long long row = rowbase[local] + counters[local] + (i < j ? 1 : 0);
-                            counters[local] += 1;
-                            run_row[r][k] = row;
-                            labels[3 * row] = sample_in_block[local];
-                            labels[3 * row + 1] = sys;
-                            labels[3 * row + 2] = local_j;
-                        }
-                    }
-                }
-                win[0] = nr;
-                win[1] = ncomplete;
-                win[2] = wlen;
-                win[3] = carry ? 1 : 0;
-                carry = ncomplete == 0;
-            }
+    const int nw = n_windows[i];
+    for (int w = 0; w < nw; w++) {
+        const int4 wd = windows[e0 + w];
+        const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
+        if (tid < rc) {
+            const int4 ri = runs[e0 + rs + tid];
+            run_lo[tid] = max(ri.x, es) - es;
+            run_hi[tid] = min(ri.x + ri.y, es + ec) - es;
+            run_tj[tid] = ri.z;
+            run_complete[tid] = ri.x + ri.y <= es + ec ? 1 : 0;
+            if (tid == 0) carry_flag = ri.x < es ? 1 : 0;
         }
-        __syncthreads();
-        const int nr = win[0], ncomplete = win[1], wlen = win[2];
-        const bool carry_in = win[3] != 0;
-
-        tile_entry_data<T>(a, lay, e0 + pos, wlen, ent);
-        __syncthreads();
-
-        // DC of every run in the window (a carried partial run continues in slot 0)
-        for (int u = tid; u < nr * 3 * L2; u += TILE_THREADS) {
-            const int r = u / (3 * L2);
-            const int rem = u - r * 3 * L2;
-            const int x = rem / L2, lm = rem - x * L2;
-            if (!run_valid[r]) continue;
-            const int l = l_of_lm(lm);
-            T* dst = DC + ((size_t)r * 3 + x) * CS + lm * NC;
-            T acc[NC];
-            if (r == 0 && carry_in) {
-#pragma unroll
-                for (int c = 0; c < NC; c += 4) load4<T>(dst + c, acc + c);
-            } else {
-#pragma unroll
-                for (int c = 0; c < NC; c++) acc[c] = 0;
-            }
-            for (int k = run_first[r]; k < run_first[r + 1]; k++) {
-                const T* en = ent + k * lay.per_entry;
-                const T g = en[lay.y + lm] * en[lay.dir + x];
-                const T dy = en[lay.dy + x * L2 + lm];
-                T ra[NC], rb[NC];
-#pragma unroll
-                for (int c = 0; c < NC; c += 4) {
-                    load4<T>(en + lay.ra + l * NC + c, ra + c);
-                    load4<T>(en + lay.rb + l * NC + c, rb + c);
-                }
-#pragma unroll
-                for (int c = 0; c < NC; c++) acc[c] += ra[c] * g + rb[c] * dy;
-            }
-#pragma unroll
-            for (int c = 0; c < NC; c += 4) store4<T>(dst + c, acc[c], acc[c + 1], acc[c + 2], acc[c + 3]);
+        for (int it = tid; it < rc * n_present; it += TILE_THREADS) {
+            const int r = it / n_present, k = it - r * n_present;
+            run_row[r][k] = run_rows[(e0 + rs + r) * S + k];
         }
+        plan_entry_stage1<T>(a, lay, e0 + es, ec, ent);
+        __syncthreads();
+        plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
         __syncthreads();
 
-        // gradient rows of the complete runs: units (l descending, x, quad) x (run, type)
-        const int NRK = ncomplete * n_present;
-        for (int u = tid; u < NRK * 3 * L * nq; u += TILE_THREADS) {
-            const int rem = u / NRK;
-            const int rk = u - rem * NRK;
-            const int r = rk / n_present, k = rk - r * n_present;
-            if (!run_valid[r]) continue;
-            const int lrev = rem / (3 * nq);
-            const int l = L - 1 - lrev;
-            const int xq = rem - lrev * 3 * nq;
-            const int x = xq / nq, q = xq - x * nq;
-            const int t = present[k], tj = run_tj[r];
-            const T* D = DC + ((size_t)r * 3 + x) * CS;
-            const T* Ct = C + (size_t)t * CS;
+        // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
+        // so that the nq lanes of one (run, type, x, l) tile are adjacent
+        const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
+        const int NRK = nc_runs * n_present;
+        const int per_lx = NRK * nq;
+        const int total = per_lx * 3 * L;
+        const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
+        const float inv_np = 1.0f / (float)(n_present > 0 ? n_present : 1);
+        for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
+            const int u = ubase + (tid & (WAVE - 1));
+            T* tile = nullptr;
+            int q = 0;
+            T s = 0;
             T acc[4][NC];
 #pragma unroll
             for (int rr = 0; rr < 4; rr++)
 #pragma unroll
                 for (int c = 0; c < NC; c++) acc[rr][c] = 0;
-            if (t >= tj) tile_fma<T, NC>(l, q, D, Ct, acc);
-            if (t <= tj) tile_fma<T, NC>(l, q, Ct, D, acc);
-            const T s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-            tile_store<T, NC>(out + run_row[r][k] * 3 * P + (long long)x * P + l * n * n, q, n, s, acc);
+            if (u < total) {
+                int lx, wq, rk, r, k;
+                small_divmod(u, per_lx, inv_lx, lx, wq);
+                rk = wq / nq;
+                q = wq - rk * nq;
+                small_divmod(rk, n_present, inv_np, r, k);
+                const int lrev = lx / 3;
+                const int x = lx - 3 * lrev;
+                const int l = L - 1 - lrev;
+                const int t = present[k], tj = run_tj[r];
+                const T* D = DC + ((size_t)r * 3 + x) * CS;
+                const T* Ct = C + (size_t)t * CS;
+                if (t >= tj) tile_fma<T, NC>(l, q, D, Ct, acc);
+                if (t <= tj) tile_fma<T, NC>(l, q, Ct, D, acc);
+                s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+                tile = out + (long long)run_row[r][k] * 3 * P + (long long)x * P + l * n * n;
+            }
+            if (staged) {
+                staged_tile_store<T, NC, RP>(tile, q, s, acc, stage, stage_ptrs);
+            } else if (tile != nullptr) {
+                tile_store<T, NC>(tile, q, n, s, acc);
+            }
         }
         __syncthreads();
-        pos += wlen;
     }
 
     // the center's own rows: d c^{t} / d r_i are the self gradients
-    if (tid == 0 && !snapped) {
-        for (int k = 0; k < npairs; k++) snapshot[k] = counters[k];
-    }
     T* DS = region;
     for (int k = tid; k < n_present * 3 * CS; k += TILE_THREADS) {
         const int kt = k / (3 * CS);
@@ -1198,34 +1478,47 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     }
     __syncthreads();
     const int npp = n_present * (n_present + 1) / 2;
-    for (int p = tid; p < npp; p += TILE_THREADS) {
-        const int t1 = present[pair_k1[p]], t2 = present[pair_k2[p]];
-        const int local = pair_index(t1, t2, S);
-        const long long row = rowbase[local] + snapshot[local];
-        labels[3 * row] = sample_in_block[local];
-        labels[3 * row + 1] = sys;
-        labels[3 * row + 2] = atom_local[i];
-    }
-    for (int u = tid; u < npp * 3 * L * nq; u += TILE_THREADS) {
-        const int rem = u / npp;
-        const int p = u - rem * npp;
-        const int lrev = rem / (3 * nq);
-        const int l = L - 1 - lrev;
-        const int xq = rem - lrev * 3 * nq;
-        const int x = xq / nq, q = xq - x * nq;
-        const int k1 = pair_k1[p], k2 = pair_k2[p];
-        const int t1 = present[k1], t2 = present[k2];
-        const int local = pair_index(t1, t2, S);
-        const long long row = rowbase[local] + snapshot[local];
+    const int per_lx = npp * nq;
+    const int total = per_lx * 3 * L;
+    const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
+    for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
+        const int u = ubase + (tid & (WAVE - 1));
+        T* tile = nullptr;
+        int q = 0;
+        T s = 0;
         T acc[4][NC];
 #pragma unroll
         for (int rr = 0; rr < 4; rr++)
 #pragma unroll
             for (int c = 0; c < NC; c++) acc[rr][c] = 0;
-        tile_fma<T, NC>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
-        tile_fma<T, NC>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
-        const T s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-        tile_store<T, NC>(out + row * 3 * P + (long long)x * P + l * n * n, q, n, s, acc);
+        if (u < total) {
+            int lx, wq;
+            small_divmod(u, per_lx, inv_lx, lx, wq);
+            const int p = wq / nq;
+            q = wq - p * nq;
+            const int lrev = lx / 3;
+            const int x = lx - 3 * lrev;
+            const int l = L - 1 - lrev;
+            // p -> (k1 <= k2) in row-major order of the upper triangle
+            int k1 = 0, rem = p;
+            while (rem >= n_present - k1) {
+                rem -= n_present - k1;
+                k1++;
+            }
+            const int k2 = k1 + rem;
+            const int t1 = present[k1], t2 = present[k2];
+            const int local = pair_index(t1, t2, S);
+            const long long row = self_rows[(long long)i * npairs + local];
+            tile_fma<T, NC>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
+            tile_fma<T, NC>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
+            s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+            tile = out + row * 3 * P + (long long)x * P + l * n * n;
+        }
+        if (staged) {
+            staged_tile_store<T, NC, RP>(tile, q, s, acc, stage, stage_ptrs);
+        } else if (tile != nullptr) {
+            tile_store<T, NC>(tile, q, n, s, acc);
+        }
     }
 }
 
@@ -1308,6 +1601,7 @@ SoapArgs<T> make_args(const SoapLaunch<T>& p) {
     a.scalars = p.scalars;
     a.sh_a = p.sh_a;
     a.sh_b = p.sh_b;
+    a.sh_c = p.sh_c;
     a.self_contribution = p.self_contribution;
     a.chunk = p.chunk;
     a.error_flag = p.error_flag;
@@ -1449,44 +1743,67 @@ bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, c
     return true;
 }
 
+// `rows_per_pass`: rows of each 4 x NC tile staged at once (1, 2 or 4)
 template <typename T>
-size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries) {
+size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass) {
     const int NC = (p.n + 3) / 4 * 4;
     const size_t cs = (size_t)p.L * p.L * NC;
-    const size_t npairs = (size_t)p.n_species * (p.n_species + 1) / 2;
-    const size_t meta = (npairs * (8 + 4 * 5) + 15) / 16 * 16;
-    const size_t window = (size_t)max_runs * 3 * cs + (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries;
-    const size_t self = (size_t)p.n_species * 3 * cs;
-    return meta + ((size_t)p.n_species * cs + (window > self ? window : self)) * sizeof(T);
+    const size_t dc = (size_t)(max_runs > p.n_species ? max_runs : p.n_species) * 3 * cs;
+    const size_t entries = (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries * sizeof(T);
+    const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC * sizeof(T) + (size_t)TILE_THREADS * sizeof(void*);
+    return ((size_t)p.n_species * cs + dc) * sizeof(T) + (entries > staging ? entries : staging);
+}
+
+bool ps_gradient_tiled_supported(int L, int n, int n_species, int max_runs, int max_entries, int rows_per_pass) {
+    return n <= 16 && L <= TILE_MAX_L && n_species <= TILE_MAX_TYPES && max_runs >= 1 && max_runs <= TILE_MAX_RUNS &&
+           max_entries >= 1 && (rows_per_pass == 1 || rows_per_pass == 2 || rows_per_pass == 4);
+}
+
+void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const long long* offsets, const int* entry_j,
+                         const int* species, const unsigned* type_mask, const long long* sample_scan,
+                         const long long* row_scan, const long long* block_sample_base, const int* atom_system,
+                         const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
+                         int* n_windows, int* self_rows, int* labels) {
+    if (n_atoms == 0) return;
+    const size_t npairs = (size_t)n_species * (n_species + 1) / 2;
+    const size_t shmem = npairs * (8 + 4 * 3);
+    ps_grad_plan_kernel<<<n_atoms, WAVE, shmem, stream>>>(
+        n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base, atom_system,
+        atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows, reinterpret_cast<int4*>(windows),
+        n_windows, self_rows, labels);
 }
 
 template <typename T>
-bool launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* coef, const T* dself,
-                              const unsigned* type_mask, const long long* sample_scan, const long long* row_scan,
-                              const long long* block_sample_base, const int* atom_system, const int* atom_local, T* out,
-                              int* labels) {
-    if (p.n_atoms == 0) return true;
-    if (p.n > 16 || p.L > TILE_MAX_L || p.n_species > TILE_MAX_TYPES || max_runs < 1 || max_runs > TILE_MAX_RUNS ||
-        max_entries < 1 || max_entries > WAVE - 1 || max_entries * p.L > TILE_THREADS - WAVE) {
-        return false;
-    }
+void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
+                              const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
+                              const int* windows, const int* n_windows, const int* self_rows, T* out) {
+    if (p.n_atoms == 0) return;
     SoapArgs<T> a = make_args(p);
-    const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries);
-#define RA_LAUNCH_TILED(NCV)                                                                                           \
-    ps_gradient_tiled_kernel<T, NCV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                                   \
-        a, max_runs, max_entries, coef, dself, type_mask, sample_scan, row_scan, block_sample_base, atom_system,      \
-        atom_local, out, labels)
-    if (p.n <= 4) {
-        RA_LAUNCH_TILED(4);
-    } else if (p.n <= 8) {
-        RA_LAUNCH_TILED(8);
-    } else if (p.n <= 12) {
-        RA_LAUNCH_TILED(12);
-    } else {
-        RA_LAUNCH_TILED(16);
+    const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass);
+    const int4* rs = reinterpret_cast<const int4*>(runs);
+    const int4* ws = reinterpret_cast<const int4*>(windows);
+#define RA_LAUNCH_TILED(NCV, RPV)                                                                                      \
+    ps_gradient_tiled_kernel<T, NCV, RPV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                              \
+        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, out)
+#define RA_LAUNCH_TILED_RP(NCV)       \
+    if (rows_per_pass == 4) {         \
+        RA_LAUNCH_TILED(NCV, 4);      \
+    } else if (rows_per_pass == 2) {  \
+        RA_LAUNCH_TILED(NCV, 2);      \
+    } else {                          \
+        RA_LAUNCH_TILED(NCV, 1);      \
     }
+    if (p.n <= 4) {
+        RA_LAUNCH_TILED_RP(4)
+    } else if (p.n <= 8) {
+        RA_LAUNCH_TILED_RP(8)
+    } else if (p.n <= 12) {
+        RA_LAUNCH_TILED_RP(12)
+    } else {
+        RA_LAUNCH_TILED_RP(16)
+    }
+#undef RA_LAUNCH_TILED_RP
 #undef RA_LAUNCH_TILED
-    return true;
 }
 
 template <typename T>
@@ -1517,10 +1834,9 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
     template bool launch_ps_gradient_fast<T>(const SoapLaunch<T>&, int, const T*, const T*, const unsigned*,          \
                                              const long long*, const long long*, const long long*, const int*,        \
                                              const int*, T*, int*);                                                    \
-    template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int);                                        \
-    template bool launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, const T*, const T*, const unsigned*,    \
-                                              const long long*, const long long*, const long long*, const int*,       \
-                                              const int*, T*, int*);                                                   \
+    template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int, int);                                   \
+    template void launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, int, const T*, const T*, const unsigned*, \
+                                              const int*, const int*, const int*, const int*, const int*, T*);         \
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
                                           const long long*, const long long*, const long long*, const long long*,      \
                                           const long long*, const int*, const int*, T*, int*);                         \
