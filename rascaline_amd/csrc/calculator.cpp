@@ -449,7 +449,25 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         const long long c = std::max<long long>(1, std::min<long long>(64, target / static_cast<long long>(per_entry)));
         lp.chunk = static_cast<int>(c);
     }
-    launch_soap_center<T>(lp, 0, coef, dself);
+    // per-entry data (Legendre functions, direction, cutoff factors, radial values)
+    // shared by the expansion and the power-spectrum gradient kernels
+    const bool tiled = n <= 16 && L <= 16 && S <= 32;
+    T* entry_data = nullptr;
+    if (tiled) {
+        entry_data = ws_.get<T>("entry_data", std::max<size_t>(entry_prep_size<T>(lp, nl.n_entries), 1));
+        launch_entry_prep<T>(lp, nl.n_entries, entry_data);
+        int max_entries = 64;
+        while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(std::min(lds, 65536))) {
+            max_entries /= 2;
+        }
+        if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
+            launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);
+        } else {
+            launch_soap_center<T>(lp, 0, coef, dself);
+        }
+    } else {
+        launch_soap_center<T>(lp, 0, coef, dself);
+    }
     T* dstrain = nullptr;
     T* dcell = nullptr;
     if (grads.strain) {
@@ -539,13 +557,15 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 int* d_windows = ws_.get<int>("plan_windows", ne * 4);
                 int* d_n_windows = ws_.get<int>("plan_n_windows", std::max<long long>(n_atoms, 1));
                 int* d_self_rows = ws_.get<int>("plan_self_rows", std::max<long long>(n_atoms, 1) * npairs);
+                if (!entry_data) continue;
+                T* d_entry_data = entry_data;
                 launch_ps_grad_plan(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, sample_scan,
                                     row_scan, d_block_sample_base, d_atom_system, d_atom_local, w[0], w[1], d_runs,
                                     d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels);
                 RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
                 timed_start = true;
                 launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows, d_windows,
-                                            d_n_windows, d_self_rows, d_grads);
+                                            d_n_windows, d_self_rows, d_entry_data, d_grads);
                 done = true;
                 break;
             }

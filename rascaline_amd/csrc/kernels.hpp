@@ -86,10 +86,22 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
                          const long long* row_scan, const long long* block_sample_base, const int* atom_system,
                          const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
                          int* n_windows, int* self_rows, int* labels);
+// Per-entry data (Legendre functions, direction, trigonometric and cutoff factors,
+// radial values) of the tiled gradient kernel: entry_prep_size values.
+template <typename T>
+size_t entry_prep_size(const SoapLaunch<T>& p, long long n_entries);
+template <typename T>
+void launch_entry_prep(const SoapLaunch<T>& p, long long n_entries, T* data);
+// c_nlm and self gradients from the per-entry data (n <= 16, l_max < 16)
+template <typename T>
+size_t expansion_tiled_shmem(const SoapLaunch<T>& p, int max_entries);
+template <typename T>
+void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* entry_data, T* coef, T* dself);
 template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
-                              const int* windows, const int* n_windows, const int* self_rows, T* out);
+                              const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
+                              T* out);
 template <typename T>
 size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
 // register-blocked power-spectrum gradient rows; returns false when n > 16

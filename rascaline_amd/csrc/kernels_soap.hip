@@ -1152,86 +1152,86 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     }
 }
 
-// Stage 1 for the entries [e_first, e_first + cnt) of a window. Items [0, cnt L):
-// one (entry, m) each, the associated Legendre functions P_lm for l >= m (the
-// recursion of featomic/src/math/spherical_harmonics.rs:228-251 run along l for
-// this m, the diagonal chain recomputed per lane); the m = 0 lane also stores the
-// direction, trigonometric factors and the cutoff / scaling factors. Items
-// [cnt L, 2 cnt L): one (entry, l) each, the radial spline R_nl and dR_nl/dr,
-// zero-padded to NC.
+// Per-entry data of the tiled gradient kernel, one (entry, slot) per thread over
+// the whole batch. Slots [0, L): m = slot, the associated Legendre functions P_lm
+// for l >= m (the recursion of featomic/src/math/spherical_harmonics.rs:228-251
+// run along l for this m, the diagonal chain recomputed per slot); slot 0 also
+// stores the direction, trigonometric factors and the cutoff / scaling factors.
+// Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
+// to NC.
 template <typename T>
-__device__ void plan_entry_stage1(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e_first, int cnt, T* ent) {
+__global__ void entry_prep_kernel(SoapArgs<T> a, long long n_entries, int NC, T* __restrict__ data) {
+    const TileEntryLayout lay(a.L, a.n, NC);
     const int L = lay.L;
-    const ShTable tb(L);
-    for (int item = threadIdx.x; item < 2 * cnt * L; item += TILE_THREADS) {
-        if (item < cnt * L) {
-            const int c = item / L, m = item - c * L;
-            const long long e = e_first + c;
-            T* base = ent + c * lay.per_entry;
-            const double r = a.entry_dist[e];
-            double d[3];
-            if (r < 1e-6) {
-                d[0] = 0.0;
-                d[1] = 0.0;
-                d[2] = 1.0;
-            } else {
-                d[0] = a.entry_vec[3 * e] / r;
-                d[1] = a.entry_vec[3 * e + 1] / r;
-                d[2] = a.entry_vec[3 * e + 2] / r;
-            }
-            const T x = (T)d[0], y = (T)d[1], z = (T)d[2];
-            const T sqrt_xy = (T)hypot((double)x, (double)y);
-            const T cos_theta = z, sin_theta = sqrt_xy;
-            if (m == 0) {
-                T cos_phi = 1, sin_phi = 0;
-                if ((double)sqrt_xy > (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07)) {
-                    cos_phi = x / sqrt_xy;
-                    sin_phi = y / sqrt_xy;
-                }
-                base[lay.dir + 0] = x;
-                base[lay.dir + 1] = y;
-                base[lay.dir + 2] = z;
-                base[lay.dir + 3] = 0;
-                base[lay.trig + 0] = cos_theta;
-                base[lay.trig + 1] = sin_theta;
-                base[lay.trig + 2] = cos_phi;
-                base[lay.trig + 3] = sin_phi;
-                double f, df;
-                scaling_functions(a.scalars, r, &f, &df);
-                base[lay.scal + 0] = (T)f;
-                base[lay.scal + 1] = (T)df;
-                base[lay.scal + 2] = (T)f / (T)r;
-                base[lay.scal + 3] = 0;
-            }
-            T value = (T)0.3989422804014327;
-            for (int l = 1; l <= m; l++) value *= -a.sh_c[tb.diag + l] * sin_theta;
-            T* P = base + lay.p;
-            P[leg_index(m, m)] = value;
-            if (m + 1 < L) {
-                T p2 = value;
-                T p1 = cos_theta * a.sh_c[tb.sq2l1 + m + 1] * value;
-                P[leg_index(m + 1, m)] = p1;
-                for (int l = m + 2; l < L; l++) {
-                    const int k = leg_index(l, m);
-                    const T p0 = a.sh_a[k] * (cos_theta * p1 + a.sh_b[k] * p2);
-                    P[k] = p0;
-                    p2 = p1;
-                    p1 = p0;
-                }
-            }
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_entries * 2 * L) return;
+    const long long e = idx / (2 * L);
+    const int slot = (int)(idx - e * 2 * L);
+    T* base = data + e * lay.per_entry;
+    if (slot < L) {
+        const ShTable tb(L);
+        const int m = slot;
+        const double r = a.entry_dist[e];
+        double d[3];
+        if (r < 1e-6) {
+            d[0] = 0.0;
+            d[1] = 0.0;
+            d[2] = 1.0;
         } else {
-            const int w = item - cnt * L;
-            const int c = w / L, l = w - c * L;
-            const long long e = e_first + c;
-            T* base = ent + c * lay.per_entry;
-            const int n = lay.n, NC = lay.NC;
-            T* va = base + lay.rv + l * NC;
-            T* vb = base + lay.rd + l * NC;
-            radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
-            for (int k = n; k < NC; k++) {
-                va[k] = 0;
-                vb[k] = 0;
+            d[0] = a.entry_vec[3 * e] / r;
+            d[1] = a.entry_vec[3 * e + 1] / r;
+            d[2] = a.entry_vec[3 * e + 2] / r;
+        }
+        const T x = (T)d[0], y = (T)d[1], z = (T)d[2];
+        const T sqrt_xy = (T)hypot((double)x, (double)y);
+        const T cos_theta = z, sin_theta = sqrt_xy;
+        if (m == 0) {
+            T cos_phi = 1, sin_phi = 0;
+            if ((double)sqrt_xy > (sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920929e-07)) {
+                cos_phi = x / sqrt_xy;
+                sin_phi = y / sqrt_xy;
             }
+            base[lay.dir + 0] = x;
+            base[lay.dir + 1] = y;
+            base[lay.dir + 2] = z;
+            base[lay.dir + 3] = 0;
+            base[lay.trig + 0] = cos_theta;
+            base[lay.trig + 1] = sin_theta;
+            base[lay.trig + 2] = cos_phi;
+            base[lay.trig + 3] = sin_phi;
+            double f, df;
+            scaling_functions(a.scalars, r, &f, &df);
+            base[lay.scal + 0] = (T)f;
+            base[lay.scal + 1] = (T)df;
+            base[lay.scal + 2] = (T)f / (T)r;
+            base[lay.scal + 3] = 0;
+            for (int k = lay.p + lay.leg; k < lay.per_entry; k++) base[k] = 0;
+        }
+        T value = (T)0.3989422804014327;
+        for (int l = 1; l <= m; l++) value *= -a.sh_c[tb.diag + l] * sin_theta;
+        T* P = base + lay.p;
+        P[leg_index(m, m)] = value;
+        if (m + 1 < L) {
+            T p2 = value;
+            T p1 = cos_theta * a.sh_c[tb.sq2l1 + m + 1] * value;
+            P[leg_index(m + 1, m)] = p1;
+            for (int l = m + 2; l < L; l++) {
+                const int k = leg_index(l, m);
+                const T p0 = a.sh_a[k] * (cos_theta * p1 + a.sh_b[k] * p2);
+                P[k] = p0;
+                p2 = p1;
+                p1 = p0;
+            }
+        }
+    } else {
+        const int l = slot - L;
+        const int n = lay.n;
+        T* va = base + lay.rv + l * NC;
+        T* vb = base + lay.rd + l * NC;
+        radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
+        for (int k = n; k < NC; k++) {
+            va[k] = 0;
+            vb[k] = 0;
         }
     }
 }
@@ -1272,6 +1272,7 @@ __device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& l
                 accm[c] = 0;
             }
         }
+        const T c_gl = a.sh_c[tb.gl + l], c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
         for (int e = run_lo[r]; e < run_hi[r]; e++) {
             const T* base = ent + e * lay.per_entry;
             const T* P = base + lay.p;
@@ -1284,7 +1285,7 @@ __device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& l
                 if (l == 0) {
                     gp = 0;
                 } else {
-                    const T lf = a.sh_c[tb.gl + l] * P[leg_index(l, 1)];
+                    const T lf = c_gl * P[leg_index(l, 1)];
                     gp = x == 0 ? cos_phi * cos_theta * lf : (x == 1 ? sin_phi * cos_theta * lf : -sin_theta * lf);
                 }
             } else {
@@ -1304,7 +1305,7 @@ __device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& l
                 ym = p * sin_m;
                 const T pm1 = P[leg_index(l, m - 1)];
                 const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
-                const T dp = a.sh_c[tb.dpa + k] * pm1 - a.sh_c[tb.dpb + k] * pp1;
+                const T dp = c_dpa * pm1 - c_dpb * pp1;
                 const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
                 const T s_dp = sin_m * dp, c_dp = cos_m * dp;
                 if (x == 0) {
@@ -1348,7 +1349,7 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
     const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
     const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
-    T* __restrict__ out) {
+    const T* __restrict__ entry_data, T* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int run_lo[TILE_MAX_RUNS], run_hi[TILE_MAX_RUNS], run_tj[TILE_MAX_RUNS], run_complete[TILE_MAX_RUNS];
     __shared__ int run_row[TILE_MAX_RUNS][TILE_MAX_TYPES];
@@ -1417,7 +1418,15 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             const int r = it / n_present, k = it - r * n_present;
             run_row[r][k] = run_rows[(e0 + rs + r) * S + k];
         }
-        plan_entry_stage1<T>(a, lay, e0 + es, ec, ent);
+        {
+            // the window's entries are contiguous: one coalesced copy into LDS
+            typedef typename Vec4Of<T>::type V;
+            constexpr int VEC = 16 / sizeof(T);
+            const V* src = reinterpret_cast<const V*>(entry_data + (e0 + es) * lay.per_entry);
+            V* dst = reinterpret_cast<V*>(ent);
+            const int count = ec * lay.per_entry / VEC;
+            for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
+        }
         __syncthreads();
         plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
         __syncthreads();
@@ -1518,6 +1527,204 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             staged_tile_store<T, NC, RP>(tile, q, s, acc, stage, stage_ptrs);
         } else if (tile != nullptr) {
             tile_store<T, NC>(tile, q, n, s, acc);
+        }
+    }
+}
+
+// Spherical expansion c_i and self gradients d c_i / d r_i of one center per
+// 256-thread workgroup, from the per-entry data of entry_prep_kernel
+// (soap/spherical_expansion.rs:101-358, spherical_expansion_pair.rs:323-422).
+// The center's entries are read in contiguous chunks; one item per
+// (component, l, m >= 0, half of n) forms Y_l(±m) (component 0) or r dY_l(±m)/dx
+// (components 1..3) and accumulates f R Y (values) or -(ra Y d_x + rb dY/dx)
+// (self gradients, not for periodic images of the center itself) for the
+// neighbour type of each entry. Per-type sums live in LDS; each item keeps its
+// running sums in registers while the neighbour type stays the same.
+template <typename T, int NC, int NH>
+__global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<T> a, int max_entries,
+                                                                       const T* __restrict__ entry_data,
+                                                                       T* __restrict__ out_coef, T* __restrict__ out_grad) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int chunk_t[WAVE];
+    __shared__ int chunk_self[WAVE];
+    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    constexpr int NW = NC / NH;  // n values per item
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L;
+    const int CS = L2 * NC;
+    const TileEntryLayout lay(L, n, NC);
+    const int leg = lay.leg;
+    const bool grads = out_grad != nullptr;
+    const int ncomp = grads ? 4 : 1;
+    T* acc = reinterpret_cast<T*>(smem);                   // [S][4][L2][NC]
+    T* ent = acc + (size_t)S * 4 * CS;                      // [max_entries][per_entry]
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const ShTable tb(L);
+    const T SQRT_2 = (T)1.4142135623730951;
+
+    for (int k = tid; k < S * 4 * CS; k += TILE_THREADS) acc[k] = 0;
+    for (int k = tid; k < leg; k += TILE_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        leg_l[k] = l;
+        leg_m[k] = k - l * (l + 1) / 2;
+    }
+    for (int cb = 0; cb < len; cb += max_entries) {
+        const int cnt = min(max_entries, len - cb);
+        __syncthreads();
+        {
+            typedef typename Vec4Of<T>::type V;
+            constexpr int VEC = 16 / sizeof(T);
+            const V* src = reinterpret_cast<const V*>(entry_data + (e0 + cb) * lay.per_entry);
+            V* dst = reinterpret_cast<V*>(ent);
+            const int count = cnt * lay.per_entry / VEC;
+            for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
+        }
+        for (int c = tid; c < cnt; c += TILE_THREADS) {
+            const int j = a.entry_j[e0 + cb + c];
+            chunk_t[c] = a.species[j];
+            chunk_self[c] = j == i ? 1 : 0;
+        }
+        __syncthreads();
+        for (int item = tid; item < ncomp * leg * NH; item += TILE_THREADS) {
+            const int comp = item / (leg * NH);
+            const int rest = item - comp * leg * NH;
+            const int k = rest / NH, h = rest - k * NH;
+            const int l = leg_l[k], m = leg_m[k];
+            const int lm0 = l * l + l;
+            const int x = comp - 1;
+            const T c_gl = a.sh_c[tb.gl + l], c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
+            T accp[NW], accm[NW];
+            int cur_t = -1;
+            auto flush = [&]() {
+                T* dp_ = acc + ((size_t)cur_t * 4 + comp) * CS + (lm0 + m) * NC + h * NW;
+                T* dm_ = acc + ((size_t)cur_t * 4 + comp) * CS + (lm0 - m) * NC + h * NW;
+#pragma unroll
+                for (int c = 0; c < NW; c += 4) {
+                    T o[4];
+                    load4<T>(dp_ + c, o);
+                    store4<T>(dp_ + c, o[0] + accp[c], o[1] + accp[c + 1], o[2] + accp[c + 2], o[3] + accp[c + 3]);
+                }
+                if (m > 0) {
+#pragma unroll
+                    for (int c = 0; c < NW; c += 4) {
+                        T o[4];
+                        load4<T>(dm_ + c, o);
+                        store4<T>(dm_ + c, o[0] + accm[c], o[1] + accm[c + 1], o[2] + accm[c + 2], o[3] + accm[c + 3]);
+                    }
+                }
+            };
+            for (int e = 0; e < cnt; e++) {
+                const int t = chunk_t[e];
+                if (comp > 0 && chunk_self[e]) continue;
+                if (t != cur_t) {
+                    if (cur_t >= 0) flush();
+                    cur_t = t;
+#pragma unroll
+                    for (int c = 0; c < NW; c++) {
+                        accp[c] = 0;
+                        accm[c] = 0;
+                    }
+                }
+                const T* base = ent + e * lay.per_entry;
+                const T* P = base + lay.p;
+                const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+                const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+                T yp, ym = 0, gp = 0, gm = 0;
+                if (m == 0) {
+                    yp = P[k] / SQRT_2;
+                    if (comp > 0 && l > 0) {
+                        const T lf = c_gl * P[leg_index(l, 1)];
+                        gp = x == 0 ? cos_phi * cos_theta * lf : (x == 1 ? sin_phi * cos_theta * lf : -sin_theta * lf);
+                    }
+                } else {
+                    T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+                    const T minus_two_cos = -2 * cos_phi;
+                    T sin_m = 0, cos_m = 1;
+                    for (int mm = 1; mm <= m; mm++) {
+                        sin_m = minus_two_cos * sin_1 - sin_2;
+                        cos_m = minus_two_cos * cos_1 - cos_2;
+                        sin_2 = sin_1;
+                        sin_1 = sin_m;
+                        cos_2 = cos_1;
+                        cos_1 = cos_m;
+                    }
+                    const T p = P[k];
+                    yp = p * cos_m;
+                    ym = p * sin_m;
+                    if (comp > 0) {
+                        const T pm1 = P[leg_index(l, m - 1)];
+                        const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
+                        const T dp = c_dpa * pm1 - c_dpb * pp1;
+                        const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
+                        const T s_dp = sin_m * dp, c_dp = cos_m * dp;
+                        if (x == 0) {
+                            gp = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
+                            gm = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
+                        } else if (x == 1) {
+                            gp = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
+                            gm = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
+                        } else {
+                            gp = (T)0.5 * sin_theta * c_dp;
+                            gm = (T)0.5 * sin_theta * s_dp;
+                        }
+                    }
+                }
+                const T f = base[lay.scal + 0];
+                T Rv[NW];
+#pragma unroll
+                for (int c = 0; c < NW; c += 4) load4<T>(base + lay.rv + l * NC + h * NW + c, Rv + c);
+                if (comp == 0) {
+#pragma unroll
+                    for (int c = 0; c < NW; c++) {
+                        const T v = f * Rv[c];
+                        accp[c] += v * yp;
+                        accm[c] += v * ym;
+                    }
+                } else {
+                    const T df = base[lay.scal + 1], f_r = base[lay.scal + 2];
+                    const T dx = base[lay.dir + x];
+                    const T ypd = yp * dx, ymd = ym * dx;
+                    T dR[NW];
+#pragma unroll
+                    for (int c = 0; c < NW; c += 4) load4<T>(base + lay.rd + l * NC + h * NW + c, dR + c);
+#pragma unroll
+                    for (int c = 0; c < NW; c++) {
+                        const T ra = df * Rv[c] + f * dR[c];
+                        const T rb = f_r * Rv[c];
+                        accp[c] -= ra * ypd + rb * gp;
+                        accm[c] -= ra * ymd + rb * gm;
+                    }
+                }
+            }
+            if (cur_t >= 0) flush();
+        }
+    }
+    __syncthreads();
+    // write out, dropping the padding; the self contribution goes to the l = 0
+    // values of the center's own type (spherical_expansion_pair.rs:222-252)
+    const int ct = a.species[i];
+    const int slots = L2 * n;
+    for (int k = tid; k < S * slots; k += TILE_THREADS) {
+        const int t = k / slots;
+        const int s = k - t * slots;
+        const int lm = s / n, c = s - lm * n;
+        T v = acc[((size_t)t * 4) * CS + lm * NC + c];
+        if (t == ct && s < n) v += a.self_contribution[s];
+        out_coef[(long long)i * S * slots + k] = v;
+    }
+    if (grads) {
+        for (int k = tid; k < S * 3 * slots; k += TILE_THREADS) {
+            const int t = k / (3 * slots);
+            const int r = k - t * 3 * slots;
+            const int x = r / slots;
+            const int s = r - x * slots;
+            const int lm = s / n, c = s - lm * n;
+            out_grad[(long long)i * S * 3 * slots + k] = acc[((size_t)t * 4 + 1 + x) * CS + lm * NC + c];
         }
     }
 }
@@ -1774,9 +1981,25 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
 }
 
 template <typename T>
+size_t entry_prep_size(const SoapLaunch<T>& p, long long n_entries) {
+    const int NC = (p.n + 3) / 4 * 4;
+    return (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * (size_t)n_entries;
+}
+
+template <typename T>
+void launch_entry_prep(const SoapLaunch<T>& p, long long n_entries, T* data) {
+    if (n_entries == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const int NC = (p.n + 3) / 4 * 4;
+    const long long total = n_entries * 2 * p.L;
+    entry_prep_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, p.stream>>>(a, n_entries, NC, data);
+}
+
+template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
-                              const int* windows, const int* n_windows, const int* self_rows, T* out) {
+                              const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
+                              T* out) {
     if (p.n_atoms == 0) return;
     SoapArgs<T> a = make_args(p);
     const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass);
@@ -1784,7 +2007,7 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     const int4* ws = reinterpret_cast<const int4*>(windows);
 #define RA_LAUNCH_TILED(NCV, RPV)                                                                                      \
     ps_gradient_tiled_kernel<T, NCV, RPV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                              \
-        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, out)
+        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
 #define RA_LAUNCH_TILED_RP(NCV)       \
     if (rows_per_pass == 4) {         \
         RA_LAUNCH_TILED(NCV, 4);      \
@@ -1804,6 +2027,32 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     }
 #undef RA_LAUNCH_TILED_RP
 #undef RA_LAUNCH_TILED
+}
+
+template <typename T>
+size_t expansion_tiled_shmem(const SoapLaunch<T>& p, int max_entries) {
+    const int NC = (p.n + 3) / 4 * 4;
+    const size_t cs = (size_t)p.L * p.L * NC;
+    return ((size_t)p.n_species * 4 * cs + (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries) * sizeof(T);
+}
+
+template <typename T>
+void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* entry_data, T* coef, T* dself) {
+    if (p.n_atoms == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = expansion_tiled_shmem<T>(p, max_entries);
+#define RA_LAUNCH_EXP(NCV, NHV) \
+    expansion_tiled_kernel<T, NCV, NHV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(a, max_entries, entry_data, coef, dself)
+    if (p.n <= 4) {
+        RA_LAUNCH_EXP(4, 1);
+    } else if (p.n <= 8) {
+        RA_LAUNCH_EXP(8, 2);
+    } else if (p.n <= 12) {
+        RA_LAUNCH_EXP(12, 1);
+    } else {
+        RA_LAUNCH_EXP(16, 2);
+    }
+#undef RA_LAUNCH_EXP
 }
 
 template <typename T>
@@ -1836,7 +2085,11 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
                                              const int*, T*, int*);                                                    \
     template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int, int);                                   \
     template void launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, int, const T*, const T*, const unsigned*, \
-                                              const int*, const int*, const int*, const int*, const int*, T*);         \
+                                              const int*, const int*, const int*, const int*, const int*, const T*, T*); \
+    template size_t entry_prep_size<T>(const SoapLaunch<T>&, long long);                                               \
+    template size_t expansion_tiled_shmem<T>(const SoapLaunch<T>&, int);                                               \
+    template void launch_expansion_tiled<T>(const SoapLaunch<T>&, int, const T*, T*, T*);                              \
+    template void launch_entry_prep<T>(const SoapLaunch<T>&, long long, T*);                                            \
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
                                           const long long*, const long long*, const long long*, const long long*,      \
                                           const long long*, const int*, const int*, T*, int*);                         \
