@@ -243,6 +243,11 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
 #ifndef RA_TILED_LDS_BUDGET
 #define RA_TILED_LDS_BUDGET 53248
 #endif
+// (neighbour runs, entries, staged tile rows) per window of the tiled gradient
+// kernel, tried in order until one fits the LDS budget
+#ifndef RA_TILED_WINDOWS
+#define RA_TILED_WINDOWS {{4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}}
+#endif
 
 namespace {
 
@@ -542,9 +547,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         bool done = false;
         if (ps && total_rows < (1ll << 31)) {
             // planned, windowed tiled kernel: the first (runs, entries, staged rows)
-            // configuration that keeps the workgroup within the LDS budget (3
+            // configuration that keeps the workgroup within the LDS budget (4
             // workgroups per CU in FP32, 2 in FP64)
-            static const int windows[][3] = {{4, 32, 4}, {4, 32, 2}, {4, 16, 2}, {4, 16, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}};
+            static const int windows[][3] = RA_TILED_WINDOWS;
             const size_t cap = sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920;
             const size_t budget = std::min<size_t>(lds > 1024 ? lds - 1024 : 0, cap);
             for (const auto& w : windows) {

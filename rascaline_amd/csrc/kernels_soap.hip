@@ -849,6 +849,14 @@ struct TileEntryLayout {
     }
 };
 
+// Workgroup barrier that orders LDS only: the kernels using it exchange data
+// between waves through LDS alone, so global stores keep draining across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Offsets in the per-l / per-(l, m) coefficient table of the spherical harmonics
 // (built on the host, every value rounded to T as the serial evaluation rounds it).
 struct ShTable {
@@ -930,12 +938,29 @@ __device__ __forceinline__ void small_divmod(int u, int d, float inv, int& q, in
     }
 }
 
-// acc[r][c] += Σ_m R[m][4q + r] * Cm[m][c] over m in [l², (l+1)²); R and Cm are [L2][NC].
-template <typename T, int NC>
+// Rows of the 4 x NC register tile of lane-quad q, for RP rows staged per pass:
+// accumulator row a = h RP + rr (h = a / RP, rr = a % RP) holds tile row
+// RP q + rr + h RP nq, so that in every pass the nq lanes of one tile hold
+// RP nq adjacent rows.
+template <int NC, int RP>
+__device__ __forceinline__ int tile_row(int q, int a) {
+    constexpr int nq = NC / 4;
+    return RP * q + (a % RP) + (a / RP) * RP * nq;
+}
+
+// acc[a][c] += Σ_m R[m][tile_row(q, a)] * Cm[m][c] over m in [l², (l+1)²); R and
+// Cm are [L2][NC].
+template <typename T, int NC, int RP>
 __device__ __forceinline__ void tile_fma(int l, int q, const T* __restrict__ R, const T* __restrict__ Cm, T (&acc)[4][NC]) {
     for (int m = l * l; m < (l + 1) * (l + 1); m++) {
         T rv[4];
-        load4<T>(R + m * NC + 4 * q, rv);
+        const T* rrow = R + m * NC;
+        if constexpr (RP == 4) {
+            load4<T>(rrow + 4 * q, rv);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 4; a++) rv[a] = rrow[tile_row<NC, RP>(q, a)];
+        }
         T cv[NC];
 #pragma unroll
         for (int c = 0; c < NC; c += 4) load4<T>(Cm + m * NC + c, cv + c);
@@ -946,67 +971,68 @@ __device__ __forceinline__ void tile_fma(int l, int q, const T* __restrict__ R, 
     }
 }
 
-// Write rows 4q..4q+3 (those < n) of an [n][n] tile at `o`, scaled by s.
-template <typename T, int NC>
+// Write the tile rows of lane-quad q (those < n) of an [n][n] tile at `o`, scaled by s.
+template <typename T, int NC, int RP>
 __device__ __forceinline__ void tile_store(T* __restrict__ o, int q, int n, T s, const T (&acc)[4][NC]) {
-    if (n == NC) {
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+    for (int a = 0; a < 4; a++) {
+        const int row = tile_row<NC, RP>(q, a);
+        if (row < n) {
 #pragma unroll
-            for (int c = 0; c < NC; c += 4)
-                store4<T>(o + (4 * q + r) * NC + c, acc[r][c] * s, acc[r][c + 1] * s, acc[r][c + 2] * s, acc[r][c + 3] * s);
-    } else {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (4 * q + r < n) {
-#pragma unroll
-                for (int c = 0; c < NC; c++)
-                    if (c < n) o[(4 * q + r) * n + c] = acc[r][c] * s;
-            }
+            for (int c = 0; c < NC; c++)
+                if (c < n) o[row * n + c] = acc[a][c] * s;
         }
     }
 }
 
-// Per-wave staged store: every lane holds rows 4q..4q+3 of an [n][n] tile (n ==
-// NC) bound for `tile` (or nothing, tile == nullptr). The rows go through the
-// wave's LDS staging area, RP rows per pass, and are written back so that
-// consecutive lanes store consecutive 16-byte pieces of each lane's rows: a store
-// instruction then covers a few contiguous runs instead of 64 separate lines.
+// Per-wave staged store: every lane holds the tile rows of lane-quad q of an
+// [n][n] tile (n == NC) bound for `tile` (or nothing, tile == nullptr). The rows go
+// through the wave's LDS staging area, RP per pass (the nq lanes of a tile then
+// hold RP nq adjacent rows), and are written back so that consecutive lanes store
+// consecutive 16-byte pieces: a store instruction covers whole contiguous runs.
 template <typename T, int NC, int RP>
-__device__ __forceinline__ void staged_tile_store(T* tile, int q, T s, const T (&acc)[4][NC], T* stage, T** stage_ptr) {
+__device__ __forceinline__ void staged_tile_store(T* __restrict__ out, long long tile, int q, T s, const T (&acc)[4][NC],
+                                                  T* stage, long long* stage_off) {
+    // `tile` is an element offset into `out` (a kernel argument, so the stores are
+    // global stores), or -1 for no tile
     constexpr int VEC = 16 / sizeof(T);           // values per 16-byte piece
     constexpr int PER_OWNER = RP * NC / VEC;      // pieces per lane and pass
+    constexpr int nq = NC / 4;
     const int lane = threadIdx.x & (WAVE - 1);
+    typedef typename Vec4Of<T>::type V;
 #pragma unroll
-    for (int h = 0; h < 4; h += RP) {
-        if (tile != nullptr) {
+    for (int h = 0; h < 4 / RP; h++) {
+        if (tile >= 0) {
 #pragma unroll
             for (int r = 0; r < RP; r++)
 #pragma unroll
                 for (int c = 0; c < NC; c += 4)
-                    store4<T>(stage + lane * RP * NC + r * NC + c, acc[h + r][c] * s, acc[h + r][c + 1] * s,
-                              acc[h + r][c + 2] * s, acc[h + r][c + 3] * s);
-            stage_ptr[lane] = tile + (4 * q + h) * NC;
+                    store4<T>(stage + lane * RP * NC + r * NC + c, acc[h * RP + r][c] * s, acc[h * RP + r][c + 1] * s,
+                              acc[h * RP + r][c + 2] * s, acc[h * RP + r][c + 3] * s);
+            stage_off[lane] = tile + (RP * q + h * RP * nq) * NC;
         } else {
-            stage_ptr[lane] = nullptr;
+            stage_off[lane] = -1;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        long long off[PER_OWNER];
+        V val[PER_OWNER];
 #pragma unroll
         for (int s2 = 0; s2 < PER_OWNER; s2++) {
             const int p = s2 * WAVE + lane;
             const int owner = p / PER_OWNER, sub = p - owner * PER_OWNER;
-            T* dst = stage_ptr[owner];
-            if (dst != nullptr) {
-                typedef typename Vec4Of<T>::type V;
-                *reinterpret_cast<V*>(dst + sub * VEC) =
-                    *reinterpret_cast<const V*>(stage + owner * RP * NC + sub * VEC);
-            }
+            off[s2] = stage_off[owner];
+            if (off[s2] >= 0) off[s2] += sub * VEC;
+            val[s2] = *reinterpret_cast<const V*>(stage + owner * RP * NC + sub * VEC);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+        for (int s2 = 0; s2 < PER_OWNER; s2++) {
+            if (off[s2] >= 0) *reinterpret_cast<V*>(out + off[s2]) = val[s2];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
     }
 }
 
@@ -1374,7 +1400,7 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     // staging of the stores overlays the entry data (free while rows are emitted)
     const int wave_id = tid / WAVE;
     T* stage = ent + (size_t)wave_id * WAVE * RP * NC;
-    T** stage_ptrs = reinterpret_cast<T**>(ent + (size_t)(TILE_THREADS / WAVE) * WAVE * RP * NC) + wave_id * WAVE;
+    long long* stage_offs = reinterpret_cast<long long*>(ent + (size_t)(TILE_THREADS / WAVE) * WAVE * RP * NC) + wave_id * WAVE;
     const bool staged = n == NC;
 
     const long long e0 = a.offsets[i];
@@ -1427,9 +1453,9 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             const int count = ec * lay.per_entry / VEC;
             for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
         }
-        __syncthreads();
+        lds_barrier();
         plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
-        __syncthreads();
+        lds_barrier();
 
         // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
         // so that the nq lanes of one (run, type, x, l) tile are adjacent
@@ -1441,7 +1467,7 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         const float inv_np = 1.0f / (float)(n_present > 0 ? n_present : 1);
         for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
             const int u = ubase + (tid & (WAVE - 1));
-            T* tile = nullptr;
+            long long tile = -1;
             int q = 0;
             T s = 0;
             T acc[4][NC];
@@ -1461,18 +1487,18 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
                 const int t = present[k], tj = run_tj[r];
                 const T* D = DC + ((size_t)r * 3 + x) * CS;
                 const T* Ct = C + (size_t)t * CS;
-                if (t >= tj) tile_fma<T, NC>(l, q, D, Ct, acc);
-                if (t <= tj) tile_fma<T, NC>(l, q, Ct, D, acc);
+                if (t >= tj) tile_fma<T, NC, RP>(l, q, D, Ct, acc);
+                if (t <= tj) tile_fma<T, NC, RP>(l, q, Ct, D, acc);
                 s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-                tile = out + (long long)run_row[r][k] * 3 * P + (long long)x * P + l * n * n;
+                tile = (long long)run_row[r][k] * 3 * P + (long long)x * P + l * n * n;
             }
             if (staged) {
-                staged_tile_store<T, NC, RP>(tile, q, s, acc, stage, stage_ptrs);
-            } else if (tile != nullptr) {
-                tile_store<T, NC>(tile, q, n, s, acc);
+                staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
+            } else if (tile >= 0) {
+                tile_store<T, NC, RP>(out + tile, q, n, s, acc);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // the center's own rows: d c^{t} / d r_i are the self gradients
@@ -1485,14 +1511,14 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         const int lm = r2 / NC, c = r2 - lm * NC;
         DS[k] = c < n ? dself[(((long long)i * S + present[kt]) * 3 + x) * L2 * n + lm * n + c] : (T)0;
     }
-    __syncthreads();
+    lds_barrier();
     const int npp = n_present * (n_present + 1) / 2;
     const int per_lx = npp * nq;
     const int total = per_lx * 3 * L;
     const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
     for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
         const int u = ubase + (tid & (WAVE - 1));
-        T* tile = nullptr;
+        long long tile = -1;
         int q = 0;
         T s = 0;
         T acc[4][NC];
@@ -1518,15 +1544,15 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             const int t1 = present[k1], t2 = present[k2];
             const int local = pair_index(t1, t2, S);
             const long long row = self_rows[(long long)i * npairs + local];
-            tile_fma<T, NC>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
-            tile_fma<T, NC>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
+            tile_fma<T, NC, RP>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
+            tile_fma<T, NC, RP>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
             s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-            tile = out + row * 3 * P + (long long)x * P + l * n * n;
+            tile = row * 3 * P + (long long)x * P + l * n * n;
         }
         if (staged) {
-            staged_tile_store<T, NC, RP>(tile, q, s, acc, stage, stage_ptrs);
-        } else if (tile != nullptr) {
-            tile_store<T, NC>(tile, q, n, s, acc);
+            staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
+        } else if (tile >= 0) {
+            tile_store<T, NC, RP>(out + tile, q, n, s, acc);
         }
     }
 }
@@ -1575,7 +1601,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
     }
     for (int cb = 0; cb < len; cb += max_entries) {
         const int cnt = min(max_entries, len - cb);
-        __syncthreads();
+        lds_barrier();
         {
             typedef typename Vec4Of<T>::type V;
             constexpr int VEC = 16 / sizeof(T);
@@ -1589,7 +1615,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
             chunk_t[c] = a.species[j];
             chunk_self[c] = j == i ? 1 : 0;
         }
-        __syncthreads();
+        lds_barrier();
         for (int item = tid; item < ncomp * leg * NH; item += TILE_THREADS) {
             const int comp = item / (leg * NH);
             const int rest = item - comp * leg * NH;
@@ -1704,7 +1730,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
             if (cur_t >= 0) flush();
         }
     }
-    __syncthreads();
+    lds_barrier();
     // write out, dropping the padding; the self contribution goes to the l = 0
     // values of the center's own type (spherical_expansion_pair.rs:222-252)
     const int ct = a.species[i];
