@@ -19,6 +19,7 @@ SOURCES = [
     ("kernels_nl.hip", ["-ffp-contract=off"]),
     ("neighbors.cpp", ["-ffp-contract=off"]),
     ("kernels_soap.hip", []),
+    ("kernels_lode.hip", []),
     ("scan.hip", []),
     ("calculator.cpp", []),
     ("lode_host.cpp", []),

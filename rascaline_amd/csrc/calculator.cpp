@@ -51,17 +51,6 @@ int device_lds_limit() {
     return limit;
 }
 
-template <typename T>
-void upload(T* dst, const std::vector<T>& src, hipStream_t stream) {
-    if (!src.empty()) RA_HIP_CHECK(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, stream));
-}
-
-template <typename T>
-void download(std::vector<T>& dst, const T* src, size_t count, hipStream_t stream) {
-    dst.resize(count);
-    if (count) RA_HIP_CHECK(hipMemcpyAsync(dst.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost, stream));
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------- construction
@@ -135,28 +124,6 @@ const char* Calculator::name() const {
 
 // ---------------------------------------------------------------------------- device tables
 
-template <typename T>
-struct Calculator::DeviceTables {
-    DeviceRadial<T> radial;
-    T* sh_a = nullptr;
-    T* sh_b = nullptr;
-    T* sh_c = nullptr;
-    T* self_c = nullptr;
-    std::vector<void*> owned;
-
-    ~DeviceTables() {
-        for (void* p : owned) (void)hipFree(p);
-    }
-
-    template <typename U>
-    U* put(const std::vector<U>& v) {
-        void* p = nullptr;
-        RA_HIP_CHECK(hipMalloc(&p, sizeof(U) * std::max<size_t>(v.size(), 1)));
-        if (!v.empty()) RA_HIP_CHECK(hipMemcpy(p, v.data(), sizeof(U) * v.size(), hipMemcpyHostToDevice));
-        owned.push_back(p);
-        return static_cast<U*>(p);
-    }
-};
 
 template <typename T>
 const Calculator::DeviceTables<T>& Calculator::device_tables() {
@@ -248,47 +215,6 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
 #ifndef RA_TILED_WINDOWS
 #define RA_TILED_WINDOWS {{4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}}
 #endif
-
-namespace {
-
-struct Species {
-    std::vector<int32_t> types;  // sorted unique atomic types
-    std::vector<int32_t> index;  // per atom
-};
-
-Species species_of(const HostBatch& batch) {
-    Species s;
-    s.types = batch.types;
-    std::sort(s.types.begin(), s.types.end());
-    s.types.erase(std::unique(s.types.begin(), s.types.end()), s.types.end());
-    if (s.types.size() > 32) throw Error::invalid_parameter("at most 32 different atomic types are supported");
-    s.index.resize(batch.types.size());
-    for (size_t i = 0; i < batch.types.size(); i++) {
-        s.index[i] = static_cast<int32_t>(std::lower_bound(s.types.begin(), s.types.end(), batch.types[i]) - s.types.begin());
-    }
-    return s;
-}
-
-std::unique_ptr<mts_block_t> make_block(LabelsData samples, std::vector<LabelsData> components, LabelsData properties,
-                                        std::vector<uintptr_t> shape) {
-    auto b = std::make_unique<mts_block_t>();
-    b->samples = std::move(samples);
-    b->components = std::move(components);
-    b->properties = std::move(properties);
-    b->array.shape = std::move(shape);
-    size_t total = 1;
-    for (auto s : b->array.shape) total *= s;
-    b->array.data.assign(total, 0.0);
-    return b;
-}
-
-LabelsData mu_labels(int l) {
-    std::vector<int32_t> v;
-    for (int m = -l; m <= l; m++) v.push_back(m);
-    return LabelsData({"o3_mu"}, v);
-}
-
-}  // namespace
 
 template <typename T>
 mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& grads, hipStream_t stream, bool to_host,

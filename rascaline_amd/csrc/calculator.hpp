@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <string>
@@ -11,6 +12,7 @@
 
 #include "../../include/featomic.h"
 #include "../../include/featomic_amd.h"
+#include "device.hpp"
 #include "hypers.hpp"
 #include "radial.hpp"
 #include "tensormap.hpp"
@@ -133,5 +135,81 @@ struct DeviceNeighbors {
 // or the caller's half list when provided).
 DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cutoff, const double* d_positions,
                                 const int* d_atom_system, hipStream_t stream);
+
+
+// Device copies of the radial and spherical-harmonics tables (per precision).
+template <typename T>
+struct Calculator::DeviceTables {
+    DeviceRadial<T> radial;
+    T* sh_a = nullptr;
+    T* sh_b = nullptr;
+    T* sh_c = nullptr;
+    T* self_c = nullptr;
+    std::vector<void*> owned;
+
+    ~DeviceTables() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+
+    template <typename U>
+    U* put(const std::vector<U>& v) {
+        void* p = nullptr;
+        RA_HIP_CHECK(hipMalloc(&p, sizeof(U) * std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) RA_HIP_CHECK(hipMemcpy(p, v.data(), sizeof(U) * v.size(), hipMemcpyHostToDevice));
+        owned.push_back(p);
+        return static_cast<U*>(p);
+    }
+};
+
+// ---- helpers shared by the SOAP and LODE host code
+
+template <typename T>
+inline void upload(T* dst, const std::vector<T>& src, hipStream_t stream) {
+    if (!src.empty()) RA_HIP_CHECK(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, stream));
+}
+
+template <typename T>
+inline void download(std::vector<T>& dst, const T* src, size_t count, hipStream_t stream) {
+    dst.resize(count);
+    if (count) RA_HIP_CHECK(hipMemcpyAsync(dst.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost, stream));
+}
+
+struct Species {
+    std::vector<int32_t> types;  // sorted unique atomic types
+    std::vector<int32_t> index;  // per atom
+};
+
+inline Species species_of(const HostBatch& batch) {
+    Species s;
+    s.types = batch.types;
+    std::sort(s.types.begin(), s.types.end());
+    s.types.erase(std::unique(s.types.begin(), s.types.end()), s.types.end());
+    if (s.types.size() > 32) throw Error::invalid_parameter("at most 32 different atomic types are supported");
+    s.index.resize(batch.types.size());
+    for (size_t i = 0; i < batch.types.size(); i++) {
+        s.index[i] = static_cast<int32_t>(std::lower_bound(s.types.begin(), s.types.end(), batch.types[i]) - s.types.begin());
+    }
+    return s;
+}
+
+inline std::unique_ptr<mts_block_t> make_block(LabelsData samples, std::vector<LabelsData> components, LabelsData properties,
+                                        std::vector<uintptr_t> shape) {
+    auto b = std::make_unique<mts_block_t>();
+    b->samples = std::move(samples);
+    b->components = std::move(components);
+    b->properties = std::move(properties);
+    b->array.shape = std::move(shape);
+    size_t total = 1;
+    for (auto s : b->array.shape) total *= s;
+    b->array.data.assign(total, 0.0);
+    return b;
+}
+
+inline LabelsData mu_labels(int l) {
+    std::vector<int32_t> v;
+    for (int m = -l; m <= l; m++) v.push_back(m);
+    return LabelsData({"o3_mu"}, v);
+}
+
 
 }  // namespace rascaline_amd
