@@ -200,7 +200,7 @@ def main():
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": None,
-        "kernel": "gradient_rows_kernel<float, PS=true>",
+        "kernel": "ps_gradient_tiled_kernel (power-spectrum position-gradient rows)",
         "kernel_ms_per_step": kernel_ms,
         "device_ms_per_step": device_ms,
         "algorithmic_bytes_per_step": total_bytes,

@@ -19,7 +19,11 @@ void* Workspace::get(const std::string& name, size_t bytes) {
     Slot& slot = slots_[name];
     if (bytes == 0) bytes = 16;
     if (slot.bytes < bytes) {
-        if (slot.ptr) RA_HIP_CHECK(hipFree(slot.ptr));
+        // kernels queued by an earlier call may still use the old buffer
+        if (slot.ptr) {
+            RA_HIP_CHECK(hipDeviceSynchronize());
+            RA_HIP_CHECK(hipFree(slot.ptr));
+        }
         slot.ptr = nullptr;
         // grow with some slack so that repeated calls with similar sizes reuse the buffer
         const size_t target = bytes + bytes / 8;

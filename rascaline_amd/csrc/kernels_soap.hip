@@ -1427,6 +1427,9 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         for (int t = 0; t < S; t++)
             if ((mask >> t) & 1u) present[k++] = t;
     }
+    // the tables above are read by every thread, also by the self rows of a center
+    // without any window
+    lds_barrier();
 
     const int nw = n_windows[i];
     for (int w = 0; w < nw; w++) {
