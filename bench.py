@@ -86,6 +86,23 @@ def run_chunk(calc, batch, stream_ptr, grads):
     return out
 
 
+def measured_traffic(precision, gradient_rows):
+    """HBM bytes per step of the gradient kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) for this exact workload,
+    from the committed profile summary; None for any other workload."""
+    path = os.path.join(ROOT, "profiles", "soap_ps_gradients_4096_structures_fp32.md")
+    try:
+        with open(path) as fd:
+            for line in fd:
+                if line.startswith("traffic_per_step:"):
+                    fields = dict(item.split("=") for item in line.split()[1:])
+                    if int(fields["precision"]) == precision and int(fields["gradient_rows"]) == gradient_rows:
+                        return float(fields["fetch_bytes"]) + float(fields["write_bytes"])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(systems, target_seconds=15.0):
     """The FP64 C restatement of the reference algorithm (oracle) on the host cores,
     over a bounded sample of the same structures."""
@@ -199,7 +216,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(args.precision, rows_total),
         "kernel": "ps_gradient_tiled_kernel (power-spectrum position-gradient rows)",
         "kernel_ms_per_step": kernel_ms,
         "device_ms_per_step": device_ms,
