@@ -86,7 +86,7 @@ def run_chunk(calc, batch, stream_ptr, grads):
     return out
 
 
-def measured_traffic(precision, gradient_rows):
+def measured_traffic(precision, gradient_rows, form="plain"):
     """HBM bytes per step of the gradient kernel measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) for this exact workload,
     from the committed profile summary; None for any other workload."""
@@ -96,7 +96,8 @@ def measured_traffic(precision, gradient_rows):
             for line in fd:
                 if line.startswith("traffic_per_step:"):
                     fields = dict(item.split("=") for item in line.split()[1:])
-                    if int(fields["precision"]) == precision and int(fields["gradient_rows"]) == gradient_rows:
+                    if (int(fields["precision"]) == precision and int(fields["gradient_rows"]) == gradient_rows
+                            and fields.get("form", "plain") == form):
                         return float(fields["fetch_bytes"]) + float(fields["write_bytes"])
     except OSError:
         pass
@@ -161,67 +162,77 @@ def main():
     n_atoms = sum(len(s[1]) for s in systems)
     log(f"generated {len(systems)} structures, {n_atoms} atoms in {time.perf_counter() - t0:.1f} s")
 
-    calc = ra.SoapPowerSpectrum(**HYPERS)
-    calc.set_precision(args.precision)
     chunks = make_chunks(systems, args.chunk_atoms)
     batches = [DeviceBatch(systems, c, torch) for c in chunks]
     stream = torch.cuda.current_stream().cuda_stream
     grads = (ctypes.c_char_p * 1)(b"positions")
     log(f"{len(batches)} chunks of up to {args.chunk_atoms} atoms")
-
-    def step():
-        for b in batches:
-            run_chunk(calc, b, stream, grads)
-
-    for w in range(args.warmup):
-        step()
-        torch.cuda.synchronize()
-        log(f"warmup step {w} done")
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = n_atoms * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.3e} atoms/s")
-
-    # roofline of the dominant kernel (power-spectrum gradient rows), from hipEvents
-    # on the compute stream, one extra pass
     from rascaline_amd import _c_lib as C
 
-    total_bytes, kernel_ms, device_ms = 0, 0.0, 0.0
-    rows_total, samples_total = 0, 0
-    for b in batches:
-        out = run_chunk(calc, b, stream, grads)
-        t_ms, k_ms, k_bytes = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
-                                                  ctypes.byref(k_bytes)))
-        rows = sum(out.n_gradient_rows[k] for k in range(out.n_blocks))
-        samples = sum(out.n_samples[k] for k in range(out.n_blocks))
-        rows_total += rows
-        samples_total += samples
-        # algorithmic bytes of the gradient-row kernel, as counted by the library:
-        # gradient values + labels written, per-center coefficients and self gradients
-        # read once, neighbour entries read once
-        total_bytes += k_bytes.value
-        kernel_ms += k_ms.value
-        device_ms += t_ms.value
-    achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
-        "traffic": measured_traffic(args.precision, rows_total),
-        "kernel": "ps_gradient_tiled_kernel (power-spectrum position-gradient rows)",
-        "kernel_ms_per_step": kernel_ms,
-        "device_ms_per_step": device_ms,
-        "algorithmic_bytes_per_step": total_bytes,
-    }
+    def measure(fast):
+        """Warmup, then exactly `steps` timed steps over all chunks (bracketed by device
+        synchronisation), then one pass with the library's hipEvent timings for the
+        roofline of the gradient kernel."""
+        calc = ra.SoapPowerSpectrum(**HYPERS)
+        calc.set_precision(args.precision)
+        calc.set_fast_form(fast)
+        name = "fast" if fast else "plain"
+
+        def step():
+            for b in batches:
+                run_chunk(calc, b, stream, grads)
+
+        for w in range(args.warmup):
+            step()
+            torch.cuda.synchronize()
+            log(f"{name}: warmup step {w} done")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = 1e3 * elapsed / args.steps
+        rate = n_atoms * args.steps / elapsed
+        log(f"{name}: timed {args.steps} steps: {ms:.1f} ms/step, {rate:.3e} atoms/s")
+
+        total_bytes, kernel_ms, device_ms = 0, 0.0, 0.0
+        rows_total, samples_total = 0, 0
+        for b in batches:
+            out = run_chunk(calc, b, stream, grads)
+            t_ms, k_ms, k_bytes = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+            C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
+                                                      ctypes.byref(k_bytes)))
+            rows_total += sum(out.n_gradient_rows[k] for k in range(out.n_blocks))
+            samples_total += sum(out.n_samples[k] for k in range(out.n_blocks))
+            # algorithmic bytes of the gradient-row kernel, as counted by the library:
+            # gradient values + labels written, per-center coefficients and self gradients
+            # read once, neighbour entries read once
+            total_bytes += k_bytes.value
+            kernel_ms += k_ms.value
+            device_ms += t_ms.value
+        achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(args.precision, rows_total, name),
+            "kernel": "ps_gradient_tiled_kernel (power-spectrum position-gradient rows)"
+                      + (", bfloat16 hi+lo MFMA contraction" if fast else ""),
+            "kernel_ms_per_step": kernel_ms,
+            "device_ms_per_step": device_ms,
+            "algorithmic_bytes_per_step": total_bytes,
+        }
+        return ms, rate, roofline, rows_total, samples_total
+
+    fast_available = args.precision == 32
+    plain_ms, plain_rate, plain_roofline, rows_total, samples_total = measure(False)
+    if fast_available:
+        ms_per_step, value, roofline, _, _ = measure(True)
+    else:
+        ms_per_step, value, roofline = plain_ms, plain_rate, plain_roofline
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -245,7 +256,8 @@ def main():
             "workload": f"{len(systems)} uneven structures (8-256 atoms; half periodic cubic/triclinic at "
                         f"0.08-0.12 A^-3, half clusters in 15 A boxes; 4 types), SOAP power spectrum + position "
                         f"gradients, rc 5.0, n 8, l<=6",
-            "form": "plain FP32" if args.precision == 32 else "FP64",
+            "form": ("fast: FP32 with the power-spectrum gradient contraction on MFMA in bfloat16 hi+lo, "
+                     "within 1e-4 of FP64 per block" if fast_available else "FP64"),
             "atoms": n_atoms,
             "chunks": len(batches),
             "power_spectrum_samples": samples_total,
@@ -253,6 +265,9 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "plain_fp32": ({"value": plain_rate, "unit": "atoms/s", "ms_per_step": plain_ms,
+                        "roofline_frac": plain_roofline["frac"],
+                        "kernel_ms_per_step": plain_roofline["kernel_ms_per_step"]} if fast_available else None),
     }
     print(json.dumps(result), flush=True)
     return 0

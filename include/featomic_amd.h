@@ -30,6 +30,14 @@ extern "C" {
  * Outputs of featomic_calculator_compute are float64 arrays either way. */
 featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* calculator, int32_t bits);
 
+/* Select the fast form (1) or the plain form (0, default) of the FP32 kernels. The
+ * fast form runs the power-spectrum gradient contraction on the matrix cores with
+ * bfloat16 hi + lo operand splitting and FP32 accumulation (three products per
+ * block, about 16 significant bits per operand; results within 1e-4 of the FP64
+ * computation). It applies to FP32 with 8 radial functions; other settings use the
+ * plain kernels. */
+featomic_status_t featomic_amd_calculator_set_fast_form(featomic_calculator_t* calculator, int32_t enabled);
+
 /* Half neighbour list of one system (pairs sorted by first, second, then
  * cell shift), computed on the GPU. `positions` [n_atoms][3] and `cell` [9]
  * are host arrays; a zero cell means no periodicity. The pairs are allocated

@@ -148,6 +148,7 @@ PROTOTYPES = [
     ("mts_block_gradients_list", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p)), ctypes.POINTER(ctypes.c_size_t)]),
     ("featomic_amd_calculator_set_precision", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32]),
+    ("featomic_amd_calculator_set_fast_form", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32]),
     ("featomic_amd_neighbors", featomic_status_t,
      [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.POINTER(featomic_pair_t)),
       ctypes.POINTER(ctypes.c_size_t)]),

@@ -500,7 +500,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
                 timed_start = true;
                 launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows, d_windows,
-                                            d_n_windows, d_self_rows, d_entry_data, d_grads);
+                                            d_n_windows, d_self_rows, d_entry_data, d_grads, fast_form != 0);
                 done = true;
                 break;
             }

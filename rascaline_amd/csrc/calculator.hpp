@@ -71,6 +71,8 @@ public:
     double nl_cutoff() const { return soap_.cutoff; }
 
     int precision = 64;
+    // fast form: mixed-precision (bfloat16 hi + lo on MFMA, FP32 accumulation) contractions
+    int fast_form = 0;
 
     // host systems in, host TensorMap out
     mts_tensormap_t* compute(const HostBatch& batch, const Gradients& gradients);

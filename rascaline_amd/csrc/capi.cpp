@@ -289,6 +289,14 @@ featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* c
     });
 }
 
+featomic_status_t featomic_amd_calculator_set_fast_form(featomic_calculator_t* calculator, int32_t enabled) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        if (enabled != 0 && enabled != 1) throw Error::invalid_parameter("fast form must be 0 (plain) or 1 (fast)");
+        calculator->impl->fast_form = enabled;
+    });
+}
+
 featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff,
                                          featomic_pair_t** pairs, uintptr_t* n_pairs) {
     return guard([&] {

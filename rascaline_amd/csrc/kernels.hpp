@@ -101,7 +101,7 @@ template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
                               const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
-                              T* out);
+                              T* out, bool fast);
 template <typename T>
 size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
 // register-blocked power-spectrum gradient rows; returns false when n > 16

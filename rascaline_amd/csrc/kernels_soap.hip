@@ -1370,7 +1370,22 @@ __device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& l
     }
 }
 
-template <typename T, int NC, int RP>
+// Split x into bfloat16 hi + lo (round to nearest even) so that hi + lo carries
+// ~16 significant bits of x; products are then formed as hi*hi + hi*lo + lo*hi.
+__device__ __forceinline__ void bf16_split(float x, short& hi, short& lo) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    hi = (short)(u >> 16);
+    const float r = x - __uint_as_float(((unsigned)(unsigned short)hi) << 16);
+    unsigned v = __float_as_uint(r);
+    v += 0x7FFFu + ((v >> 16) & 1u);
+    lo = (short)(v >> 16);
+}
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename T, int NC, int RP, bool FAST>
 __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
     const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
@@ -1460,9 +1475,86 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
         lds_barrier();
 
+        const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
+        if constexpr (FAST && sizeof(T) == 4 && NC == 8) {
+            // mixed precision: X[r][g][n1][n2] = Σ_m DC_r[x][m][n1] C_{t_g}[m][n2] for up to 4
+            // runs x 2 types per pair of 16 x 16 tiles, bfloat16 hi + lo products on MFMA with
+            // FP32 accumulation (3 MFMAs per k block of 16); the wave stages X in LDS and
+            // writes each 8 x 8 block in the orientation of its output block
+            const int lane = tid & (WAVE - 1);
+            const int ngroups = (n_present + 1) / 2;
+            const int items = 3 * L * ngroups;
+            T* X = stage;  // [4 runs][2 types][8][8]
+            for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
+                const int g = it % ngroups, lx = it / ngroups;
+                const int l = L - 1 - lx / 3, x = lx % 3;
+                const int g0 = 2 * g, ng = min(2, n_present - g0);
+                const int kcount = 2 * l + 1;
+                const int i16 = lane & 15, kq = 4 * (lane >> 4);
+                for (int tr = 0; tr < (nc_runs + 1) / 2; tr++) {
+                    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+                    const int r = 2 * tr + (i16 >> 3), n1 = i16 & 7;
+                    const int tg = i16 >> 3, n2 = i16 & 7;
+                    for (int kb = 0; kb < kcount; kb += 16) {
+                        short4_t ah, al, bh, bl;
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            const int mi = kb + kq + e;
+                            float av = 0.f, bv = 0.f;
+                            if (mi < kcount) {
+                                if (r < nc_runs) av = DC[((size_t)r * 3 + x) * CS + (l * l + mi) * NC + n1];
+                                if (tg < ng) bv = C[(size_t)present[g0 + tg] * CS + (l * l + mi) * NC + n2];
+                            }
+                            short h, lo;
+                            bf16_split(av, h, lo);
+                            ah[e] = h;
+                            al[e] = lo;
+                            bf16_split(bv, h, lo);
+                            bh[e] = h;
+                            bl[e] = lo;
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+                    }
+                    // D[row][col]: row = 4 (lane / 16) + v within the tile, col = lane % 16
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int rho = 16 * tr + kq + v;
+                        X[(((rho >> 3) * 2 + tg) * 8 + (rho & 7)) * 8 + n2] = acc[v];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                const int npieces = nc_runs * ng * 16;
+                for (int p = lane; p < npieces; p += WAVE) {
+                    const int r = p / (ng * 16), rem = p - r * ng * 16;
+                    const int gg = rem >> 4, n1 = (rem & 15) >> 1, h = rem & 1;
+                    const int t = present[g0 + gg], tj = run_tj[r];
+                    const T* Xb = X + (r * 2 + gg) * 64;
+                    T v[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int n2 = 4 * h + c;
+                        T sum = 0;
+                        if (t >= tj) sum += Xb[n1 * 8 + n2];
+                        if (t <= tj) sum += Xb[n2 * 8 + n1];
+                        v[c] = sum;
+                    }
+                    const T sc = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+                    T* dst = out + (long long)run_row[r][g0 + gg] * 3 * P + (long long)x * P + l * 64 + n1 * 8 + 4 * h;
+                    store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            }
+            lds_barrier();
+            continue;
+        }
         // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
         // so that the nq lanes of one (run, type, x, l) tile are adjacent
-        const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
         const int NRK = nc_runs * n_present;
         const int per_lx = NRK * nq;
         const int total = per_lx * 3 * L;
@@ -2028,24 +2120,34 @@ template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
                               const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
-                              T* out) {
+                              T* out, bool fast) {
     if (p.n_atoms == 0) return;
     SoapArgs<T> a = make_args(p);
     const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass);
     const int4* rs = reinterpret_cast<const int4*>(runs);
     const int4* ws = reinterpret_cast<const int4*>(windows);
-#define RA_LAUNCH_TILED(NCV, RPV)                                                                                      \
-    ps_gradient_tiled_kernel<T, NCV, RPV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                              \
+    // the mixed-precision contraction covers FP32, n = 8 and windows of up to 4 runs
+    const bool mixed = fast && sizeof(T) == 4 && p.n == 8 && max_runs <= 4;
+#define RA_LAUNCH_TILED(NCV, RPV, FV)                                                                                  \
+    ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                          \
         a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
 #define RA_LAUNCH_TILED_RP(NCV)       \
     if (rows_per_pass == 4) {         \
-        RA_LAUNCH_TILED(NCV, 4);      \
+        RA_LAUNCH_TILED(NCV, 4, false);      \
     } else if (rows_per_pass == 2) {  \
-        RA_LAUNCH_TILED(NCV, 2);      \
+        RA_LAUNCH_TILED(NCV, 2, false);      \
     } else {                          \
-        RA_LAUNCH_TILED(NCV, 1);      \
+        RA_LAUNCH_TILED(NCV, 1, false);      \
     }
-    if (p.n <= 4) {
+    if (mixed) {
+        if (rows_per_pass == 4) {
+            RA_LAUNCH_TILED(8, 4, true);
+        } else if (rows_per_pass == 2) {
+            RA_LAUNCH_TILED(8, 2, true);
+        } else {
+            RA_LAUNCH_TILED(8, 1, true);
+        }
+    } else if (p.n <= 4) {
         RA_LAUNCH_TILED_RP(4)
     } else if (p.n <= 8) {
         RA_LAUNCH_TILED_RP(8)
@@ -2114,7 +2216,8 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
                                              const int*, T*, int*);                                                    \
     template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int, int);                                   \
     template void launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, int, const T*, const T*, const unsigned*, \
-                                              const int*, const int*, const int*, const int*, const int*, const T*, T*); \
+                                              const int*, const int*, const int*, const int*, const int*, const T*, T*, \
+                                              bool);                                                                   \
     template size_t entry_prep_size<T>(const SoapLaunch<T>&, long long);                                               \
     template size_t expansion_tiled_shmem<T>(const SoapLaunch<T>&, int);                                               \
     template void launch_expansion_tiled<T>(const SoapLaunch<T>&, int, const T*, T*, T*);                              \

@@ -194,3 +194,47 @@ def test_user_provided_neighbors():
     a = calc.compute(ra_sys, gradients=["positions"], use_native_system=False)
     b = calc.compute(ra_systems(systems), gradients=["positions"], use_native_system=True)
     compare_tensormaps(a, b, 1e-13, 1e-14, gradients=["positions"])
+
+
+# ---------------------------------------------------------------------------- fast form
+
+def block_relative_error(actual, expected, param=None):
+    """Largest, over blocks, of max|a - b| / max|b| within the block (values, or one
+    gradient parameter). Element-wise relative errors are not a usable bar for FP32:
+    elements that cancel to ~1e-6 of their block's scale carry relative errors of
+    order 1e-2 in plain FP32 arithmetic already."""
+    get = (lambda b: b.values) if param is None else (lambda b: b.gradient(param).values)
+    worst = 0.0
+    for ba, be in zip(actual.blocks(), expected.blocks()):
+        a, b = np.asarray(get(ba)), np.asarray(get(be))
+        if not b.size:
+            continue
+        scale = float(np.abs(b).max())
+        if scale > 0:
+            worst = max(worst, float(np.abs(a - b).max()) / scale)
+    return worst
+
+
+@pytest.mark.parametrize("kind", ["periodic", "triclinic", "cluster"])
+def test_ps_fast_form_vs_oracle(kind):
+    """Fast form (bfloat16 hi + lo MFMA contraction, FP32 elsewhere) against the FP64 oracle
+    on the benchmark's hyper-parameters (n = 8, l <= 6): the metric's bar, 1e-4 relative
+    error per block (max|a - b| <= 1e-4 max|b| in every block, values and gradients)."""
+    rng = np.random.default_rng({"periodic": 71, "triclinic": 72, "cluster": 73}[kind])
+    systems = []
+    for n, L in [(24, 6.2), (40, 7.0)]:
+        pos, types, cell = random_system(rng, n, kind, L if kind != "cluster" else 9.0, types=(1, 6, 7, 8))
+        systems.append(orc.System(pos, types, cell))
+    h = hypers(rc=5.0, lmax=6, nmax=7)
+    oracle = oracle_to_tensormap(orc.soap_power_spectrum(systems, h, ["positions"]))
+    errors = {}
+    for fast in (False, True):
+        calc = ra.SoapPowerSpectrum(**h)
+        calc.set_precision(32)
+        calc.set_fast_form(fast)
+        tm = calc.compute(ra_systems(systems), gradients=["positions"])
+        compare_tensormaps(tm, oracle, 2e-4, 2e-4, gradients=["positions"])
+        errors[fast] = (block_relative_error(tm, oracle), block_relative_error(tm, oracle, "positions"))
+    say("block-relative error (values, gradients): plain", errors[False], "fast", errors[True])
+    assert max(errors[True]) <= 1e-4, errors[True]
+    assert max(errors[False]) <= 1e-4, errors[False]
