@@ -227,12 +227,16 @@ def main():
         }
         return ms, rate, roofline, rows_total, samples_total
 
+    # both forms meet the metric's 1e-4 bar (tests/test_gpu_soap.py::test_ps_fast_form_vs_oracle);
+    # the line reports the faster one and lists both
     fast_available = args.precision == 32
     plain_ms, plain_rate, plain_roofline, rows_total, samples_total = measure(False)
+    forms = {"plain": (plain_ms, plain_rate, plain_roofline)}
     if fast_available:
-        ms_per_step, value, roofline, _, _ = measure(True)
-    else:
-        ms_per_step, value, roofline = plain_ms, plain_rate, plain_roofline
+        fast_ms, fast_rate, fast_roofline, _, _ = measure(True)
+        forms["fast"] = (fast_ms, fast_rate, fast_roofline)
+    best = max(forms, key=lambda f: forms[f][1])
+    ms_per_step, value, roofline = forms[best]
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -256,8 +260,9 @@ def main():
             "workload": f"{len(systems)} uneven structures (8-256 atoms; half periodic cubic/triclinic at "
                         f"0.08-0.12 A^-3, half clusters in 15 A boxes; 4 types), SOAP power spectrum + position "
                         f"gradients, rc 5.0, n 8, l<=6",
-            "form": ("fast: FP32 with the power-spectrum gradient contraction on MFMA in bfloat16 hi+lo, "
-                     "within 1e-4 of FP64 per block" if fast_available else "FP64"),
+            "form": (("plain FP32" if best == "plain" else
+                      "fast: FP32 with the power-spectrum gradient contraction on MFMA in bfloat16 hi+lo")
+                     if fast_available else "FP64"),
             "atoms": n_atoms,
             "chunks": len(batches),
             "power_spectrum_samples": samples_total,
@@ -265,9 +270,9 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
-        "plain_fp32": ({"value": plain_rate, "unit": "atoms/s", "ms_per_step": plain_ms,
-                        "roofline_frac": plain_roofline["frac"],
-                        "kernel_ms_per_step": plain_roofline["kernel_ms_per_step"]} if fast_available else None),
+        "forms": {name: {"value": rate, "unit": "atoms/s", "ms_per_step": ms, "roofline_frac": rl["frac"],
+                         "kernel_ms_per_step": rl["kernel_ms_per_step"]}
+                  for name, (ms, rate, rl) in forms.items()},
     }
     print(json.dumps(result), flush=True)
     return 0
