@@ -97,6 +97,74 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
 featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
                                             int64_t* main_kernel_bytes);
 
+/* ---- learned radial integral
+ *
+ * A network 1 -> 64 -> 64 -> 2 (max_angular + 1) n_radial with SiLU activations
+ * maps a pair distance to every R_nl(r) and dR_nl/dr. It takes the place of the
+ * reference's SoapRadialIntegral implementations
+ * (featomic/src/calculators/soap/radial_integral/mod.rs:19-58: the exact GTO
+ * integral of gto.rs:115-205 or its spline of spline.rs:28-70) inside the SOAP
+ * calculators. Training runs on the GPU: exact GTO targets in FP64, Adam with
+ * FP32 forward / backward kernels, then the output layer solved by ridge least
+ * squares in FP64. Inference multiplies FP16 hi + lo operands on the matrix
+ * cores with FP32 accumulation. The calculator's spline (or exact integral when
+ * `spline_accuracy` is null) judges the network on a uniform grid; the network
+ * is used only when every per-(l, n) error is within the tolerance, and the
+ * spline stays in use otherwise. */
+typedef struct featomic_amd_surrogate_options_t {
+    int64_t n_samples;       /* training distances, uniform on [0, cutoff) (default 1000000) */
+    int64_t steps;           /* Adam steps (default 6000) */
+    int32_t batch;           /* distances per Adam step (default 2048) */
+    double learning_rate;    /* peak Adam rate, cosine decay to 1e-3 of it (default 3e-3) */
+    double ridge;            /* output-layer ridge, relative to trace(G) / 65 (default 1e-12) */
+    double tolerance;        /* acceptance: max |net - judge| / max |judge| per (l, n) (default 1e-3) */
+    uint64_t seed;           /* default 1 */
+    int64_t n_validation;    /* judge grid points (default 20001) */
+} featomic_amd_surrogate_options_t;
+
+typedef struct featomic_amd_surrogate_report_t {
+    int32_t accepted;              /* 1 when the network is now in use */
+    double adam_loss;              /* mean squared error of the last Adam step (normalised outputs) */
+    double max_value_error;        /* over (l, n): max |net - judge| / max |judge| of R_nl */
+    double max_derivative_error;   /* same for dR_nl/dr */
+    double train_ms;               /* device time of sampling, training, solve and judge */
+    double* value_errors;          /* caller buffer [max_angular + 1][n_radial], or NULL */
+    double* derivative_errors;     /* caller buffer [max_angular + 1][n_radial], or NULL */
+    int64_t errors_size;           /* entries in each caller buffer */
+} featomic_amd_surrogate_report_t;
+
+/* Fill `options` with the defaults above. */
+featomic_status_t featomic_amd_surrogate_options_default(featomic_amd_surrogate_options_t* options);
+
+/* Train the radial surrogate of a SOAP calculator (spherical_expansion or
+ * soap_power_spectrum) and use it for later computations when it is accepted.
+ * `report` may be NULL. */
+featomic_status_t featomic_amd_calculator_train_radial_surrogate(featomic_calculator_t* calculator,
+                                                                 const featomic_amd_surrogate_options_t* options,
+                                                                 featomic_amd_surrogate_report_t* report);
+
+/* Use (1) the accepted surrogate or (0) the spline / exact radial integral. */
+featomic_status_t featomic_amd_calculator_use_radial_surrogate(featomic_calculator_t* calculator, int32_t enabled);
+
+/* The trained network's FP32 parameters, flat: W1[64], b1[64], W2[64][64],
+ * b2[64], W3[n_out][64], b3[n_out], then the n_out output scales; `count` must
+ * be at least *required (set on return; call with params NULL to query). */
+featomic_status_t featomic_amd_calculator_radial_surrogate_parameters(featomic_calculator_t* calculator, float* params,
+                                                                      int64_t count, int64_t* required);
+
+/* One forward + backward pass of the device training kernels (FP32) over
+ * host arrays: inputs [n_samples], targets [n_samples][n_out]; writes the mean
+ * squared error and its gradient with respect to the flat parameters
+ * (layout above, without the scales). */
+featomic_status_t featomic_amd_mlp_loss_gradient(int32_t n_out, int64_t n_samples, const float* params,
+                                                 const float* inputs, const float* targets, double* loss,
+                                                 float* gradient);
+
+/* The device inference kernel (FP16 hi + lo on the matrix cores, FP32
+ * accumulation) over host inputs: outputs [n_samples][n_out]. */
+featomic_status_t featomic_amd_mlp_forward(int32_t n_out, int64_t n_samples, const float* params, const float* inputs,
+                                           float* outputs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,11 @@ def lib():
         L.orc_k_vectors.argtypes = [vp, d, vp]
         L.orc_lode_system.restype = ctypes.c_int
         L.orc_lode_system.argtypes = [vp, sz, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_mlp_n_params.restype = ctypes.c_int64
+        L.orc_mlp_n_params.argtypes = [ctypes.c_int]
+        L.orc_mlp_forward.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp]
+        L.orc_mlp_loss_gradient.restype = d
+        L.orc_mlp_loss_gradient.argtypes = [ctypes.c_int, ctypes.c_int64, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -144,6 +149,50 @@ def lib():
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
+
+
+# ----------------------------------------------------------------------------
+# radial-integral surrogate network (orc_mlp.c)
+
+MLP_HIDDEN = 64
+
+
+def mlp_n_params(n_out):
+    return int(lib().orc_mlp_n_params(n_out))
+
+
+def mlp_forward(params, inputs, n_out):
+    """FP64 forward of the 1 -> 64 -> 64 -> n_out SiLU network."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+    assert params.size == mlp_n_params(n_out)
+    out = np.zeros((inputs.size, n_out))
+    lib().orc_mlp_forward(n_out, inputs.size, _ptr(params), _ptr(inputs), _ptr(out))
+    return out
+
+
+def mlp_loss_gradient(params, inputs, targets, n_out):
+    """Mean squared error over samples and outputs, and its gradient with
+    respect to the flat parameter vector (FP64)."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+    targets = np.ascontiguousarray(targets, dtype=np.float64)
+    assert params.size == mlp_n_params(n_out) and targets.shape == (inputs.size, n_out)
+    grad = np.zeros(params.size)
+    loss = lib().orc_mlp_loss_gradient(n_out, inputs.size, _ptr(params), _ptr(inputs), _ptr(targets), _ptr(grad))
+    return loss, grad
+
+
+def radial_table(radial, r):
+    """Exact (or splined) radial integral at every r: values and derivatives,
+    each [len(r)][L][n]."""
+    L = radial.max_angular + 1
+    v = np.zeros((len(r), L, radial.n))
+    g = np.zeros((len(r), L, radial.n))
+    for i, x in enumerate(r):
+        for l in range(L):
+            v[i, l], g[i, l] = radial.compute(l, x)
+    return v, g
 
 # ----------------------------------------------------------------------------
 # systems and hyper-parameters

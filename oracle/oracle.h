@@ -143,6 +143,14 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
                     const int32_t* types, const double* cell, int n_species,
                     const int32_t* species, double* values, double* grads, double* self_grads);
 
+/* radial-integral surrogate network (orc_mlp.c): 1 -> 64 -> 64 -> n_out, SiLU */
+#define ORC_MLP_HIDDEN 64
+int64_t orc_mlp_n_params(int n_out);
+void orc_mlp_forward(int n_out, int64_t n_samples, const double* params, const double* inputs, double* outputs);
+/* returns the mean squared error; gradient has orc_mlp_n_params(n_out) entries */
+double orc_mlp_loss_gradient(int n_out, int64_t n_samples, const double* params, const double* inputs,
+                             const double* targets, double* gradient);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,32 @@ class featomic_amd_device_output_t(ctypes.Structure):
     ]
 
 
+class featomic_amd_surrogate_options_t(ctypes.Structure):
+    _fields_ = [
+        ("n_samples", ctypes.c_int64),
+        ("steps", ctypes.c_int64),
+        ("batch", ctypes.c_int32),
+        ("learning_rate", ctypes.c_double),
+        ("ridge", ctypes.c_double),
+        ("tolerance", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("n_validation", ctypes.c_int64),
+    ]
+
+
+class featomic_amd_surrogate_report_t(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_int32),
+        ("adam_loss", ctypes.c_double),
+        ("max_value_error", ctypes.c_double),
+        ("max_derivative_error", ctypes.c_double),
+        ("train_ms", ctypes.c_double),
+        ("value_errors", ctypes.POINTER(ctypes.c_double)),
+        ("derivative_errors", ctypes.POINTER(ctypes.c_double)),
+        ("errors_size", ctypes.c_int64),
+    ]
+
+
 # every function the C headers declare: (name, restype, argtypes)
 PROTOTYPES = [
     ("featomic_last_error", ctypes.c_char_p, []),
@@ -158,6 +184,17 @@ PROTOTYPES = [
       ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_output_t)]),
     ("featomic_amd_last_timings", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+    ("featomic_amd_surrogate_options_default", featomic_status_t, [ctypes.POINTER(featomic_amd_surrogate_options_t)]),
+    ("featomic_amd_calculator_train_radial_surrogate", featomic_status_t,
+     [ctypes.c_void_p, ctypes.POINTER(featomic_amd_surrogate_options_t), ctypes.POINTER(featomic_amd_surrogate_report_t)]),
+    ("featomic_amd_calculator_use_radial_surrogate", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32]),
+    ("featomic_amd_calculator_radial_surrogate_parameters", featomic_status_t,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    ("featomic_amd_mlp_loss_gradient", featomic_status_t,
+     [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+      ctypes.c_void_p]),
+    ("featomic_amd_mlp_forward", featomic_status_t,
+     [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 ]
 
 _LIB = None

@@ -20,6 +20,8 @@ SOURCES = [
     ("neighbors.cpp", ["-ffp-contract=off"]),
     ("kernels_soap.hip", []),
     ("kernels_lode.hip", []),
+    ("kernels_mlp.hip", []),
+    ("surrogate.cpp", []),
     ("scan.hip", []),
     ("calculator.cpp", []),
     ("lode_host.cpp", []),

@@ -3,6 +3,7 @@
 librascaline_amd.so."""
 
 import ctypes
+import json
 from typing import List, Optional
 
 import numpy as np
@@ -122,6 +123,53 @@ class CalculatorBase:
         """FP32 only: mixed-precision (bfloat16 hi + lo on MFMA) power-spectrum gradient
         contraction, within 1e-4 of the FP64 result."""
         C.check(self._lib.featomic_amd_calculator_set_fast_form(self._as_parameter_, 1 if enabled else 0))
+
+    def train_radial_surrogate(self, **options) -> dict:
+        """Train the learned radial integral on the GPU (SOAP calculators) and use it
+        for later computations when the calculator's spline judges every (l, n)
+        channel within ``tolerance``. Options: n_samples, steps, batch,
+        learning_rate, ridge, tolerance, seed, n_validation (defaults from the
+        library). Returns the report with per-(l, n) errors as [L][n] arrays."""
+        opts = C.featomic_amd_surrogate_options_t()
+        C.check(self._lib.featomic_amd_surrogate_options_default(ctypes.byref(opts)))
+        for key, value in options.items():
+            if not hasattr(opts, key):
+                raise ValueError(f"unknown radial surrogate option '{key}'")
+            setattr(opts, key, value)
+        params = json.loads(self.parameters)
+        L = params["basis"]["max_angular"] + 1
+        n = params["basis"]["radial"]["max_radial"] + 1
+        value_errors = np.zeros((L, n))
+        derivative_errors = np.zeros((L, n))
+        report = C.featomic_amd_surrogate_report_t()
+        report.value_errors = value_errors.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        report.derivative_errors = derivative_errors.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        report.errors_size = L * n
+        C.check(self._lib.featomic_amd_calculator_train_radial_surrogate(self._as_parameter_, ctypes.byref(opts),
+                                                                           ctypes.byref(report)))
+        return {
+            "accepted": bool(report.accepted),
+            "adam_loss": report.adam_loss,
+            "max_value_error": report.max_value_error,
+            "max_derivative_error": report.max_derivative_error,
+            "train_ms": report.train_ms,
+            "value_errors": value_errors,
+            "derivative_errors": derivative_errors,
+        }
+
+    def use_radial_surrogate(self, enabled: bool):
+        """Use the accepted learned radial integral (True) or the spline / exact one."""
+        C.check(self._lib.featomic_amd_calculator_use_radial_surrogate(self._as_parameter_, 1 if enabled else 0))
+
+    def radial_surrogate_parameters(self) -> np.ndarray:
+        """Flat FP32 parameters of the trained network, then its output scales."""
+        required = ctypes.c_int64()
+        C.check(self._lib.featomic_amd_calculator_radial_surrogate_parameters(self._as_parameter_, None, 0,
+                                                                                ctypes.byref(required)))
+        out = np.zeros(required.value, dtype=np.float32)
+        C.check(self._lib.featomic_amd_calculator_radial_surrogate_parameters(
+            self._as_parameter_, out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(required)))
+        return out
 
     def compute(self, systems, *, gradients: Optional[List[str]] = None, use_native_system: bool = True,
                 selected_samples=None, selected_properties=None, selected_keys=None) -> TensorMap:

@@ -209,6 +209,29 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
     return *static_cast<DeviceTables<T>*>(slot.get());
 }
 
+// ---------------------------------------------------------------------------- radial surrogate
+
+SurrogateReport Calculator::train_radial_surrogate(const SurrogateOptions& options) {
+    if (kind_ == Kind::LodeSphericalExpansion) {
+        throw Error::invalid_parameter("the radial surrogate is only available for the SOAP calculators");
+    }
+    check_surrogate_options(options);
+    auto surrogate = std::make_unique<RadialSurrogate>(soap_.max_angular, soap_.n_radial, soap_.cutoff);
+    const DeviceTables<float>& tab = device_tables<float>();
+    SurrogateReport report = surrogate->train(tables_, tab.radial.gto, tab.radial.ortho,
+                                              tables_.use_spline ? &tab.radial : nullptr, nullptr, options);
+    surrogate_ = std::move(surrogate);
+    use_surrogate_ = report.accepted;
+    return report;
+}
+
+void Calculator::use_radial_surrogate(bool enabled) {
+    if (enabled && !(surrogate_ && surrogate_->accepted())) {
+        throw Error::invalid_parameter("no accepted radial surrogate: train one within its tolerance first");
+    }
+    use_surrogate_ = enabled;
+}
+
 // ---------------------------------------------------------------------------- SOAP pipeline
 
 #ifndef RA_TILED_LDS_BUDGET
@@ -338,7 +361,15 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     lp.entry_dist = nl.entry_dist;
     lp.species = d_species;
     lp.radial = tab.radial;
-    if (!tab.radial.use_spline) {
+    if (radial_surrogate_active()) {
+        // learned radial integral: every (l, n) value and derivative per neighbour entry
+        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
+        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
+        surrogate_->evaluate<T>(stream, nl.n_entries, nl.entry_dist, ev, ed);
+        lp.radial.use_spline = 0;
+        lp.radial.exact_val = ev;
+        lp.radial.exact_der = ed;
+    } else if (!tab.radial.use_spline) {
         // exact radial integrals: evaluated once per neighbour entry in FP64
         T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
         T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);

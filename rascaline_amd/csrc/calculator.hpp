@@ -15,6 +15,7 @@
 #include "device.hpp"
 #include "hypers.hpp"
 #include "radial.hpp"
+#include "surrogate.hpp"
 #include "tensormap.hpp"
 
 namespace rascaline_amd {
@@ -82,6 +83,15 @@ public:
                         featomic_amd_device_output_t* output);
     void last_timings(double* total_ms, double* main_ms, int64_t* main_bytes);
 
+    // learned radial integral (SOAP calculators): trains a network on the exact
+    // GTO integral, judges it against this calculator's radial integral and
+    // uses it for the following computations when it is accepted
+    SurrogateReport train_radial_surrogate(const SurrogateOptions& options);
+    // switches between the accepted network and the spline / exact integral
+    void use_radial_surrogate(bool enabled);
+    bool radial_surrogate_active() const { return use_surrogate_ && surrogate_ && surrogate_->accepted(); }
+    const RadialSurrogate* radial_surrogate() const { return surrogate_.get(); }
+
 private:
     std::string registry_name_;
     std::string parameters_;
@@ -103,6 +113,9 @@ private:
         const int32_t* sample_labels = nullptr;
         const int32_t* gradient_labels = nullptr;
     } device_result_;
+
+    std::unique_ptr<RadialSurrogate> surrogate_;
+    bool use_surrogate_ = false;
 
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr, ev_main_start_ = nullptr, ev_main_stop_ = nullptr;
     int64_t main_bytes_ = 0;

@@ -297,6 +297,112 @@ featomic_status_t featomic_amd_calculator_set_fast_form(featomic_calculator_t* c
     });
 }
 
+featomic_status_t featomic_amd_surrogate_options_default(featomic_amd_surrogate_options_t* options) {
+    return guard([&] {
+        check_pointer(options, "options");
+        const rascaline_amd::SurrogateOptions d;
+        options->n_samples = d.n_samples;
+        options->steps = d.steps;
+        options->batch = d.batch;
+        options->learning_rate = d.learning_rate;
+        options->ridge = d.ridge;
+        options->tolerance = d.tolerance;
+        options->seed = d.seed;
+        options->n_validation = d.n_validation;
+    });
+}
+
+featomic_status_t featomic_amd_calculator_train_radial_surrogate(featomic_calculator_t* calculator,
+                                                                 const featomic_amd_surrogate_options_t* options,
+                                                                 featomic_amd_surrogate_report_t* report) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(options, "options");
+        rascaline_amd::SurrogateOptions o;
+        o.n_samples = options->n_samples;
+        o.steps = options->steps;
+        o.batch = options->batch;
+        o.learning_rate = options->learning_rate;
+        o.ridge = options->ridge;
+        o.tolerance = options->tolerance;
+        o.seed = options->seed;
+        o.n_validation = options->n_validation;
+        if (report && (report->value_errors || report->derivative_errors) && report->errors_size < 0) {
+            throw Error::invalid_parameter("errors_size must not be negative");
+        }
+        const rascaline_amd::SurrogateReport r = calculator->impl->train_radial_surrogate(o);
+        if (report) {
+            report->accepted = r.accepted ? 1 : 0;
+            report->adam_loss = r.adam_loss;
+            report->max_value_error = r.max_value_error;
+            report->max_derivative_error = r.max_derivative_error;
+            report->train_ms = r.train_ms;
+            const int64_t count = static_cast<int64_t>(r.value_errors.size());
+            if (report->value_errors || report->derivative_errors) {
+                if (report->errors_size < count) {
+                    throw Error::buffer_size("error buffers hold " + std::to_string(report->errors_size) +
+                                             " entries, " + std::to_string(count) + " are needed");
+                }
+                for (int64_t k = 0; k < count; k++) {
+                    if (report->value_errors) report->value_errors[k] = r.value_errors[k];
+                    if (report->derivative_errors) report->derivative_errors[k] = r.derivative_errors[k];
+                }
+            }
+        }
+    });
+}
+
+featomic_status_t featomic_amd_calculator_use_radial_surrogate(featomic_calculator_t* calculator, int32_t enabled) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        if (enabled != 0 && enabled != 1) throw Error::invalid_parameter("enabled must be 0 or 1");
+        calculator->impl->use_radial_surrogate(enabled == 1);
+    });
+}
+
+featomic_status_t featomic_amd_calculator_radial_surrogate_parameters(featomic_calculator_t* calculator, float* params,
+                                                                      int64_t count, int64_t* required) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(required, "required");
+        const rascaline_amd::RadialSurrogate* s = calculator->impl->radial_surrogate();
+        if (!s) throw Error::invalid_parameter("this calculator has no trained radial surrogate");
+        const std::vector<float>& p = s->parameters();
+        const std::vector<float>& sc = s->output_scale();
+        *required = static_cast<int64_t>(p.size() + sc.size());
+        if (!params) return;
+        if (count < *required) {
+            throw Error::buffer_size("parameter buffer holds " + std::to_string(count) + " values, " +
+                                     std::to_string(*required) + " are needed");
+        }
+        std::copy(p.begin(), p.end(), params);
+        std::copy(sc.begin(), sc.end(), params + p.size());
+    });
+}
+
+featomic_status_t featomic_amd_mlp_loss_gradient(int32_t n_out, int64_t n_samples, const float* params,
+                                                 const float* inputs, const float* targets, double* loss,
+                                                 float* gradient) {
+    return guard([&] {
+        check_pointer(params, "params");
+        check_pointer(inputs, "inputs");
+        check_pointer(targets, "targets");
+        check_pointer(loss, "loss");
+        check_pointer(gradient, "gradient");
+        *loss = rascaline_amd::mlp_loss_gradient_device(n_out, n_samples, params, inputs, targets, gradient);
+    });
+}
+
+featomic_status_t featomic_amd_mlp_forward(int32_t n_out, int64_t n_samples, const float* params, const float* inputs,
+                                           float* outputs) {
+    return guard([&] {
+        check_pointer(params, "params");
+        check_pointer(inputs, "inputs");
+        check_pointer(outputs, "outputs");
+        rascaline_amd::mlp_forward_device(n_out, n_samples, params, inputs, outputs);
+    });
+}
+
 featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff,
                                          featomic_pair_t** pairs, uintptr_t* n_pairs) {
     return guard([&] {

@@ -63,6 +63,11 @@ size_t soap_chunk_bytes(int L, int n, int chunk);
 template <typename T>
 void launch_radial_exact(hipStream_t stream, long long n_entries, int L, int n, const double* entry_dist,
                          double sigma2_atomic, const double* gto, const double* ortho, T* out_val, T* out_der);
+// radial integral at `count` distances, [i][L][n] (spline tables, or the
+// precomputed per-entry arrays with entry index i)
+template <typename T>
+void launch_radial_eval(hipStream_t stream, long long count, const DeviceRadial<T>& radial, const double* r,
+                        T* out_val, T* out_der);
 template <typename T>
 void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad);
 template <typename T>

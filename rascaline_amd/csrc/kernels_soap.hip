@@ -1967,7 +1967,32 @@ __global__ void radial_exact_kernel(long long n_entries, int L, int n, const dou
     }
 }
 
+// Radial integral (spline or precomputed values) at arbitrary distances: one
+// thread per (distance, l), output [i][L][n].
+template <typename T>
+__global__ void radial_eval_kernel(long long count, DeviceRadial<T> rad, const double* __restrict__ r,
+                                   T* __restrict__ out_val, T* __restrict__ out_der) {
+    const int L = rad.max_angular + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count * L) return;
+    const long long i = idx / L;
+    const int l = (int)(idx - i * L);
+    radial_integral<T>(rad, l, i, r[i], out_val + idx * rad.n, out_der + idx * rad.n);
+}
+
 }  // namespace
+
+template <typename T>
+void launch_radial_eval(hipStream_t stream, long long count, const DeviceRadial<T>& radial, const double* r,
+                        T* out_val, T* out_der) {
+    const long long total = count * (radial.max_angular + 1);
+    if (total == 0) return;
+    radial_eval_kernel<T><<<(unsigned)((total + 127) / 128), 128, 0, stream>>>(count, radial, r, out_val, out_der);
+}
+template void launch_radial_eval<float>(hipStream_t, long long, const DeviceRadial<float>&, const double*, float*,
+                                        float*);
+template void launch_radial_eval<double>(hipStream_t, long long, const DeviceRadial<double>&, const double*, double*,
+                                         double*);
 
 template <typename T>
 void launch_radial_exact(hipStream_t stream, long long n_entries, int L, int n, const double* entry_dist,

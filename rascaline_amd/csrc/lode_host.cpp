@@ -77,7 +77,7 @@ double density_fourier(const LodeParams& p, double k) {
     }
     const double pe = static_cast<double>(p.exponent);
     const double p_eff = 3.0 - pe;
-    const double factor = std::pow(RA_PI, 1.5) * std::pow(2.0 * s2, 0.5 * p_eff) / std::tgamma(0.5 * pe);
+    const double factor = std::pow(RA_PI, 1.5) * std::pow(2.0 * s2, 0.5 * p_eff) / gamma_fn(0.5 * pe);
     const double x = 0.5 * k * k * s2;
     double v = 0.0;
     switch (p.exponent) {
@@ -105,7 +105,7 @@ double k0_factor(const LodeParams& p) {
     }
     const double pe = static_cast<double>(p.exponent);
     const double p_eff = 3.0 - pe;
-    return 0.5 * RA_PI * std::pow(2.0, p_eff) / std::tgamma(0.5 * pe) * std::pow(2.0, (pe - 1.0) / 2.0) / -p_eff *
+    return 0.5 * RA_PI * std::pow(2.0, p_eff) / gamma_fn(0.5 * pe) * std::pow(2.0, (pe - 1.0) / 2.0) / -p_eff *
            std::pow(s, -p_eff) / std::pow(s, 2.0 * pe - 6.0);
 }
 

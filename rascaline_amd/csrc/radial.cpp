@@ -237,12 +237,12 @@ std::vector<double> lode_center_contribution(const LodeParams& p, const RadialTa
         if (p.exponent == 0) {
             const double factor = std::pow(RA_PI, -0.25) / std::pow(s * s, 0.75);
             const double alpha = 0.5 * (1.0 / (s * s) + 1.0 / (w * w));
-            contrib[i] = factor * std::tgamma(n_eff) / std::pow(alpha, n_eff);
+            contrib[i] = factor * gamma_fn(n_eff) / std::pow(alpha, n_eff);
         } else {
-            const double factor = 2.0 * std::sqrt(4.0 * RA_PI) / std::tgamma(e / 2.0) / e;
+            const double factor = 2.0 * std::sqrt(4.0 * RA_PI) / gamma_fn(e / 2.0) / e;
             const double r = s / w;
             const double arg = 1.0 / (1.0 + r * r);
-            contrib[i] = factor * std::pow(2.0, (1.0 + i - e) / 2.0) * std::pow(s, 3.0 + i - e) * std::tgamma(n_eff) *
+            contrib[i] = factor * std::pow(2.0, (1.0 + i - e) / 2.0) * std::pow(s, 3.0 + i - e) * gamma_fn(n_eff) *
                          hyp2f1_series(1.0, n_eff, (e + 2.0) / 2.0, arg) * std::pow(arg, n_eff);
         }
     }

@@ -175,3 +175,20 @@ def _close(a, b, rtol, atol):
         raise AssertionError(f"{int(bad.sum())}/{bad.size} elements differ (atol {atol:.3e}): " +
                              ", ".join(f"{tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}" for i in idx))
     return float(np.max(diff / np.maximum(np.abs(b) + atol / max(rtol, 1e-300), 1e-300)))
+
+
+def block_relative_error(actual, expected, param=None):
+    """Largest, over blocks, of max|a - b| / max|b| within the block (values, or one
+    gradient parameter). Element-wise relative errors are not a usable bar for FP32:
+    elements that cancel to ~1e-6 of their block's scale carry relative errors of
+    order 1e-2 in plain FP32 arithmetic already."""
+    get = (lambda b: b.values) if param is None else (lambda b: b.gradient(param).values)
+    worst = 0.0
+    for ba, be in zip(actual.blocks(), expected.blocks()):
+        a, b = np.asarray(get(ba)), np.asarray(get(be))
+        if not b.size:
+            continue
+        scale = float(np.abs(b).max())
+        if scale > 0:
+            worst = max(worst, float(np.abs(a - b).max()) / scale)
+    return worst

@@ -10,6 +10,7 @@ import json
 import os
 import re
 
+import numpy as np
 import pytest
 
 from rascaline_amd import _c_lib as C
@@ -158,3 +159,44 @@ def test_profiling_and_logging_entry_points():
     assert C.lib().featomic_profiling_get(b"json", buffer, 4096) == FEATOMIC_SUCCESS
     json.loads(buffer.value.decode())
     assert C.lib().featomic_profiling_get(b"no-such-format", buffer, 4096) == FEATOMIC_INVALID_PARAMETER_ERROR
+
+
+def test_radial_surrogate_host_checks():
+    opts = C.featomic_amd_surrogate_options_t()
+    assert C.lib().featomic_amd_surrogate_options_default(ctypes.byref(opts)) == FEATOMIC_SUCCESS
+    assert (opts.n_samples, opts.steps, opts.batch, opts.n_validation) == (1000000, 6000, 2048, 20001)
+    assert (opts.learning_rate, opts.ridge, opts.tolerance, opts.seed) == (3e-3, 1e-12, 1e-3, 1)
+    calc = make("soap_power_spectrum", SOAP)
+    try:
+        # nothing trained yet: the spline stays in use
+        assert C.lib().featomic_amd_calculator_use_radial_surrogate(calc, 1) == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "no accepted radial surrogate" in last_error()
+        assert C.lib().featomic_amd_calculator_use_radial_surrogate(calc, 0) == FEATOMIC_SUCCESS
+        assert C.lib().featomic_amd_calculator_use_radial_surrogate(calc, 2) == FEATOMIC_INVALID_PARAMETER_ERROR
+        required = ctypes.c_int64()
+        assert C.lib().featomic_amd_calculator_radial_surrogate_parameters(calc, None, 0, ctypes.byref(required)) \
+            == FEATOMIC_INVALID_PARAMETER_ERROR
+        bad = C.featomic_amd_surrogate_options_t()
+        C.lib().featomic_amd_surrogate_options_default(ctypes.byref(bad))
+        bad.n_samples = 0
+        assert C.lib().featomic_amd_calculator_train_radial_surrogate(calc, ctypes.byref(bad), None) \
+            == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "invalid radial surrogate options" in last_error()
+    finally:
+        C.lib().featomic_calculator_free(calc)
+    lode = make("lode_spherical_expansion", json.dumps({
+        "density": {"type": "SmearedPowerLaw", "smearing": 1.0, "exponent": 1},
+        "basis": {"type": "TensorProduct", "max_angular": 2, "radial": {"type": "Gto", "max_radial": 2, "radius": 3.0}}}))
+    try:
+        assert C.lib().featomic_amd_calculator_train_radial_surrogate(lode, ctypes.byref(opts), None) \
+            == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "only available for the SOAP calculators" in last_error()
+    finally:
+        C.lib().featomic_calculator_free(lode)
+    # network kernels reject sizes before touching the GPU
+    loss = ctypes.c_double()
+    buf = np.zeros(4, dtype=np.float32)
+    ptr = buf.ctypes.data_as(ctypes.c_void_p)
+    assert C.lib().featomic_amd_mlp_loss_gradient(0, 1, ptr, ptr, ptr, ctypes.byref(loss), ptr) \
+        == FEATOMIC_INVALID_PARAMETER_ERROR
+    assert C.lib().featomic_amd_mlp_forward(513, 1, ptr, ptr, ptr) == FEATOMIC_INVALID_PARAMETER_ERROR

@@ -14,8 +14,8 @@ import pytest
 
 import featomic_oracle as orc
 import rascaline_amd as ra
-from helpers import (assert_relative_eq, compare_tensormaps, flatten_ps, flatten_se, golden, load_npy_gz,
-                     oracle_to_tensormap, random_system, sum_gradients)
+from helpers import (assert_relative_eq, block_relative_error, compare_tensormaps, flatten_ps, flatten_se, golden,
+                     load_npy_gz, oracle_to_tensormap, random_system, sum_gradients)
 
 pytestmark = pytest.mark.gpu
 
@@ -197,23 +197,6 @@ def test_user_provided_neighbors():
 
 
 # ---------------------------------------------------------------------------- fast form
-
-def block_relative_error(actual, expected, param=None):
-    """Largest, over blocks, of max|a - b| / max|b| within the block (values, or one
-    gradient parameter). Element-wise relative errors are not a usable bar for FP32:
-    elements that cancel to ~1e-6 of their block's scale carry relative errors of
-    order 1e-2 in plain FP32 arithmetic already."""
-    get = (lambda b: b.values) if param is None else (lambda b: b.gradient(param).values)
-    worst = 0.0
-    for ba, be in zip(actual.blocks(), expected.blocks()):
-        a, b = np.asarray(get(ba)), np.asarray(get(be))
-        if not b.size:
-            continue
-        scale = float(np.abs(b).max())
-        if scale > 0:
-            worst = max(worst, float(np.abs(a - b).max()) / scale)
-    return worst
-
 
 @pytest.mark.parametrize("kind", ["periodic", "triclinic", "cluster"])
 def test_ps_fast_form_vs_oracle(kind):
