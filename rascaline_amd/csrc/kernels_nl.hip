@@ -110,16 +110,24 @@ __global__ void nl_fill_bins(int n_atoms, const long long* __restrict__ atom_bin
     bin_atoms[bin_start[bin] + slot] = i;
 }
 
-// One thread per center. `emit` = false counts, true writes packed keys.
+// One wave per center; the lanes take a bin's atoms 64 at a time, so the
+// candidate positions are read coalesced and the lanes never diverge over
+// loop lengths. `emit` = false counts, true writes packed keys: each hit goes
+// to the center's running offset plus its rank among the wave's hits (the
+// per-center order is fixed later by the segment sort).
+constexpr int NL_WAVES = 4;
+
 template <bool EMIT>
-__global__ void nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __restrict__ atom_system,
-                              const NLSystem* __restrict__ systems, const long long* __restrict__ atom_bin,
-                              const int* __restrict__ atom_wrap, const long long* __restrict__ bin_start,
-                              const int* __restrict__ bin_len, const int* __restrict__ bin_atoms, double cutoff2,
-                              long long* __restrict__ counts, const long long* __restrict__ offsets,
-                              unsigned long long* __restrict__ keys, int* __restrict__ error_flag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_atoms) return;
+__global__ void __launch_bounds__(64 * NL_WAVES)
+nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __restrict__ atom_system,
+              const NLSystem* __restrict__ systems, const long long* __restrict__ atom_bin,
+              const int* __restrict__ atom_wrap, const long long* __restrict__ bin_start,
+              const int* __restrict__ bin_len, const int* __restrict__ bin_atoms, double cutoff2,
+              long long* __restrict__ counts, const long long* __restrict__ offsets,
+              unsigned long long* __restrict__ keys, int* __restrict__ error_flag) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
+    if (i >= n_atoms) return;  // uniform over the wave
     const NLSystem& sys = systems[atom_system[i]];
     const long long local_bin = atom_bin[i] - sys.bin_offset;
     const int nc0 = sys.n_cells[0], nc1 = sys.n_cells[1], nc2 = sys.n_cells[2];
@@ -128,6 +136,7 @@ __global__ void nl_candidates(int n_atoms, const double* __restrict__ positions,
     const int b2 = (int)(local_bin % nc2);
     const int wi0 = atom_wrap[3 * i], wi1 = atom_wrap[3 * i + 1], wi2 = atom_wrap[3 * i + 2];
     const double* pi = positions + 3 * (long long)i;
+    const unsigned long long below = (1ull << lane) - 1ull;
     long long count = 0;
     long long out = EMIT ? offsets[i] : 0;
     for (int d0 = -sys.n_search[0]; d0 <= sys.n_search[0]; d0++) {
@@ -145,35 +154,45 @@ __global__ void nl_candidates(int n_atoms, const double* __restrict__ positions,
                 const long long nb = sys.bin_offset + ((long long)r0 * nc1 + r1) * nc2 + r2;
                 const long long start = bin_start[nb];
                 const int len = bin_len[nb];
-                for (int a = 0; a < len; a++) {
-                    const int j = bin_atoms[start + a];
-                    int s[3];
-                    if (sys.periodic) {
-                        s[0] = q0 + wi0 - atom_wrap[3 * j];
-                        s[1] = q1 + wi1 - atom_wrap[3 * j + 1];
-                        s[2] = q2 + wi2 - atom_wrap[3 * j + 2];
-                    } else {
-                        s[0] = s[1] = s[2] = 0;
-                    }
-                    if (j == i && s[0] == 0 && s[1] == 0 && s[2] == 0) continue;
-                    double v[3];
-                    pair_vector(pi, positions + 3 * (long long)j, sys.cell, s, v);
-                    const double d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-                    if (d2v < cutoff2) {
-                        if (EMIT) {
-                            if (s[0] < -SHIFT_BIAS || s[0] >= SHIFT_BIAS || s[1] < -SHIFT_BIAS || s[1] >= SHIFT_BIAS ||
-                                s[2] < -SHIFT_BIAS || s[2] >= SHIFT_BIAS) {
-                                atomicOr(error_flag, 1);
-                            }
-                            keys[out++] = pack_key(j, s[0], s[1], s[2]);
+                for (int base = 0; base < len; base += 64) {
+                    const int a = base + lane;
+                    bool hit = false, overflow = false;
+                    unsigned long long key = 0;
+                    if (a < len) {
+                        const int j = bin_atoms[start + a];
+                        int s[3];
+                        if (sys.periodic) {
+                            s[0] = q0 + wi0 - atom_wrap[3 * j];
+                            s[1] = q1 + wi1 - atom_wrap[3 * j + 1];
+                            s[2] = q2 + wi2 - atom_wrap[3 * j + 2];
+                        } else {
+                            s[0] = s[1] = s[2] = 0;
                         }
-                        count++;
+                        if (!(j == i && s[0] == 0 && s[1] == 0 && s[2] == 0)) {
+                            double v[3];
+                            pair_vector(pi, positions + 3 * (long long)j, sys.cell, s, v);
+                            const double d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                            hit = d2v < cutoff2;
+                            if (EMIT && hit) {
+                                overflow = s[0] < -SHIFT_BIAS || s[0] >= SHIFT_BIAS || s[1] < -SHIFT_BIAS ||
+                                           s[1] >= SHIFT_BIAS || s[2] < -SHIFT_BIAS || s[2] >= SHIFT_BIAS;
+                                key = pack_key(j, s[0], s[1], s[2]);
+                            }
+                        }
                     }
+                    const unsigned long long mask = __ballot(hit);
+                    if (EMIT && hit) {
+                        keys[out + __popcll(mask & below)] = key;
+                        if (overflow) atomicOr(error_flag, 1);
+                    }
+                    const int hits = __popcll(mask);
+                    out += hits;
+                    count += hits;
                 }
             }
         }
     }
-    if (!EMIT) counts[i] = count;
+    if (!EMIT && lane == 0) counts[i] = count;
 }
 
 // Bitonic sort of each center's key segment in LDS (one workgroup per center).
@@ -338,8 +357,8 @@ void launch_nl_candidates(hipStream_t stream, bool emit, int n_atoms, const doub
                           const long long* bin_start, const int* bin_len, const int* bin_atoms, double cutoff2,
                           long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag) {
     if (n_atoms == 0) return;
-    const int threads = 128;
-    const int blocks = (n_atoms + threads - 1) / threads;
+    const int threads = 64 * NL_WAVES;
+    const int blocks = (n_atoms + NL_WAVES - 1) / NL_WAVES;
     if (emit) {
         nl_candidates<true><<<blocks, threads, 0, stream>>>(n_atoms, positions, atom_system, systems, atom_bin, atom_wrap,
                                                             bin_start, bin_len, bin_atoms, cutoff2, counts, offsets,
