@@ -237,6 +237,14 @@ void Calculator::use_radial_surrogate(bool enabled) {
 #ifndef RA_TILED_LDS_BUDGET
 #define RA_TILED_LDS_BUDGET 53248
 #endif
+// neighbour entries per LDS chunk of the expansion kernel: small chunks leave
+// room for more resident workgroups, which hides the latency of the per-entry
+// LDS reads (8 entries: 27 ms per step on the 4096-structure workload, against
+// 48 ms with 64)
+#ifndef RA_EXPANSION_ENTRIES
+#define RA_EXPANSION_ENTRIES 8
+#endif
+static_assert(RA_EXPANSION_ENTRIES >= 1 && RA_EXPANSION_ENTRIES <= 64, "expansion chunks hold at most one wave of entries");
 // (neighbour runs, entries, staged tile rows) per window of the tiled gradient
 // kernel, tried in order until one fits the LDS budget
 #ifndef RA_TILED_WINDOWS
@@ -422,7 +430,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     if (tiled) {
         entry_data = ws_.get<T>("entry_data", std::max<size_t>(entry_prep_size<T>(lp, nl.n_entries), 1));
         launch_entry_prep<T>(lp, nl.n_entries, entry_data);
-        int max_entries = 64;
+        int max_entries = RA_EXPANSION_ENTRIES;
         while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(std::min(lds, 65536))) {
             max_entries /= 2;
         }
