@@ -145,12 +145,18 @@ def main():
     parser.add_argument("--chunk-atoms", type=int, default=48000)
     parser.add_argument("--precision", type=int, default=32)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--workload", choices=["soap", "lode"], default="soap",
+                        help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
+                             "expansion + gradients on periodic cells of 500-1500 atoms (its own JSON line)")
     args = parser.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
               "described in DESIGN.md but not built", file=sys.stderr)
         return 2
+
+    if args.workload == "lode":
+        return lode_main(args)
 
     import torch
 
@@ -273,6 +279,132 @@ def main():
         "forms": {name: {"value": rate, "unit": "atoms/s", "ms_per_step": ms, "roofline_frac": rl["frac"],
                          "kernel_ms_per_step": rl["kernel_ms_per_step"]}
                   for name, (ms, rate, rl) in forms.items()},
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+LODE_HYPERS = {
+    "density": {"type": "SmearedPowerLaw", "smearing": 1.0, "exponent": 1},
+    "basis": {"type": "TensorProduct", "max_angular": 4, "radial": {"type": "Gto", "max_radial": 5, "radius": 5.0}},
+}
+FP32_PEAK_TFLOPS = 157.3
+
+
+def lode_cpu_baseline(target_seconds=15.0):
+    """The FP64 oracle's LODE values + position gradients on the host cores, on
+    small cells of the same density (the cost grows as N^2 n_k, so the sample
+    states its size)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import featomic_oracle as orc
+    from rascaline_amd.datasets import lode_cells
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    done, atoms, t_total = 0, 0, 0.0
+    cells = lode_cells(seed=7, n_structures=64, n_min=96, n_max=128)
+    for pos, t, cell in cells:
+        t0 = time.perf_counter()
+        orc.lode_spherical_expansion([orc.System(pos, t, cell)], LODE_HYPERS, ["positions"])
+        t_total += time.perf_counter() - t0
+        done += 1
+        atoms += len(t)
+        if t_total > target_seconds:
+            break
+    return {
+        "value": atoms / t_total,
+        "unit": "atoms/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} periodic triclinic cells of 96-128 atoms at 0.1 A^-3 ({atoms} atoms), FP64 oracle, "
+                  f"one thread, {t_total:.1f} s (the GPU cells hold 500-1500 atoms; cost grows as N^2 n_k)",
+    }
+
+
+def lode_main(args):
+    """LODE spherical expansion + position gradients, FP32 with FP64 structure-factor
+    phases, one cell per device call (outputs of up to 4 GB per cell stay on the GPU)."""
+    import torch
+
+    import rascaline_amd as ra
+    from rascaline_amd import _c_lib as C
+    from rascaline_amd.datasets import lode_cells
+
+    n = args.structures if args.structures != 4096 else 16
+    t0 = time.perf_counter()
+    systems = lode_cells(seed=0, n_structures=n)
+    n_atoms = sum(len(s[1]) for s in systems)
+    log(f"generated {len(systems)} LODE cells, {n_atoms} atoms in {time.perf_counter() - t0:.1f} s")
+    batches = [DeviceBatch(systems, [k], torch) for k in range(len(systems))]
+    stream = torch.cuda.current_stream().cuda_stream
+    grads = (ctypes.c_char_p * 1)(b"positions")
+    calc = ra.LodeSphericalExpansion(**LODE_HYPERS)
+    calc.set_precision(args.precision)
+
+    def step():
+        for b in batches:
+            run_chunk(calc, b, stream, grads)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"lode: warmup step {w} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / args.steps
+    rate = n_atoms * args.steps / elapsed
+    log(f"lode: timed {args.steps} steps: {ms:.1f} ms/step, {rate:.3e} atoms/s")
+
+    flops, kernel_ms, device_ms, rows = 0, 0.0, 0.0, 0
+    for b in batches:
+        out = run_chunk(calc, b, stream, grads)
+        t_ms, k_ms, k_flops = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
+                                                  ctypes.byref(k_flops)))
+        flops += k_flops.value
+        kernel_ms += k_ms.value
+        device_ms += t_ms.value
+        rows += sum(out.n_gradient_rows[k] for k in range(out.n_blocks))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    baseline = None if args.no_cpu_baseline else lode_cpu_baseline()
+    result = {
+        "metric": "atoms/s LODE spherical expansion + position gradients",
+        "value": rate,
+        "unit": "atoms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (structure-factor phases f64)" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(systems)} periodic triclinic cells of 500-1500 atoms at 0.1 A^-3, 3 types, LODE p=1, "
+                        "smearing 1.0, 6 radial functions, l<=4, radius 5.0, k_cutoff 1.2 pi",
+            "atoms": n_atoms,
+            "gradient_rows": rows,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "lode_gemm_kernel<float, 2> (pair gradient rows: A from the FP64 phases, K = k-vectors)",
+            "kernel_ms_per_step": kernel_ms,
+            "device_ms_per_step": device_ms,
+            "algorithmic_flops_per_step": flops,
+        },
+        "cpu_baseline": baseline,
     }
     print(json.dumps(result), flush=True)
     return 0

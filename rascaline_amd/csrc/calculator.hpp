@@ -124,7 +124,10 @@ private:
     mts_tensormap_t* run_soap(const HostBatch& batch, const Gradients& gradients, hipStream_t stream, bool to_host,
                               featomic_amd_device_output_t* output);
     template <typename T>
-    mts_tensormap_t* run_lode(const HostBatch& batch, const Gradients& gradients, hipStream_t stream);
+    mts_tensormap_t* run_lode(const HostBatch& batch, const Gradients& gradients, hipStream_t stream,
+                              featomic_amd_device_output_t* output);
+    // LODE: device time of the pair-row GEMMs summed over systems (negative when unused)
+    double main_ms_sum_ = -1.0;
 
     template <typename T>
     struct DeviceTables;

@@ -30,7 +30,8 @@ constexpr int LT = 256;  // threads per workgroup
 
 // cos / sin (k . r_i) for all (i, k) of one system, FP64
 __global__ void lode_phase_kernel(int n_atoms, int n_k, const double* __restrict__ positions, const double* __restrict__ kvec,
-                                  double* __restrict__ cosv, double* __restrict__ sinv) {
+                                  double* __restrict__ cosv, double* __restrict__ sinv, float* __restrict__ cosf,
+                                  float* __restrict__ sinf) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n_atoms * n_k) return;
     const int i = (int)(idx / n_k), k = (int)(idx - (long long)i * n_k);
@@ -42,6 +43,10 @@ __global__ void lode_phase_kernel(int n_atoms, int n_k, const double* __restrict
     sincos(arg, &s, &c);
     cosv[idx] = c;
     sinv[idx] = s;
+    if (cosf) {
+        cosf[idx] = (float)c;
+        sinf[idx] = (float)s;
+    }
 }
 
 // per-type sums over atoms, FP64: sumc[t][k], sums[t][k]
@@ -113,9 +118,94 @@ __global__ void lode_projection_kernel(LodeProjectionArgs<T> p) {
     }
 }
 
+// A(row, k) of the three LODE GEMMs, from the FP64 phases (values: the structure
+// factor; self rows: -Im / Re minus the centre's own term; pair rows:
+// 2 (s_i c_j - c_i s_j) / -2 (c_i c_j + s_i s_j)), rounded once to the GEMM type.
+template <typename T, int MODE>
+__device__ __forceinline__ double lode_a_value(const LodeGemmArgs<T>& g, long long row, int k) {
+    const int N = g.n_atoms, nk = g.n_k;
+    const int par = g.parity;
+    if (MODE == 2) {
+        const int i = (int)(row / N), j = (int)(row - (long long)i * N);
+        const double ci = g.cosv[(long long)i * nk + k], si = g.sinv[(long long)i * nk + k];
+        const double cj = g.cosv[(long long)j * nk + k], sj = g.sinv[(long long)j * nk + k];
+        return par == 0 ? 2.0 * (si * cj - ci * sj) : -2.0 * (ci * cj + si * sj);
+    }
+    const int i = (int)(row / g.n_types), tl = (int)(row - (long long)i * g.n_types);
+    const int t = g.types_present[tl];
+    const double ci = g.cosv[(long long)i * nk + k], si = g.sinv[(long long)i * nk + k];
+    const double sc = g.sumc[(long long)t * nk + k], ss = g.sums[(long long)t * nk + k];
+    const double re = 2.0 * (ci * sc + si * ss), im = 2.0 * (si * sc - ci * ss);
+    if (MODE == 0) return par == 0 ? re : im;
+    return par == 0 ? -im : re - (t == g.type_index[i] ? 2.0 : 0.0);
+}
+
+// Writes GEMM element (row, col) into its output block (values, self rows or pair
+// rows; pair rows with j == i are skipped).
+template <typename T, int MODE>
+__device__ __forceinline__ void lode_store(const LodeGemmArgs<T>& g, long long row, int col, T v) {
+    const int N = g.n_atoms;
+    int i, t, j = -1;
+    if (MODE == 2) {
+        i = (int)(row / N);
+        j = (int)(row - (long long)i * N);
+        if (j == i) return;
+        t = g.type_index[j];
+    } else {
+        i = (int)(row / g.n_types);
+        t = g.types_present[(int)(row - (long long)i * g.n_types)];
+    }
+    const int ct = g.type_index[i];
+    const int S = g.n_species;
+    const long long pair = (long long)ct * S + t;
+    const int cv = MODE == 0 ? col : col % g.ncol_v;
+    const int xyz = MODE == 0 ? 0 : col / g.ncol_v;
+    const int l = g.col_l[cv];
+    const int in_l = g.col_in_l[cv];  // m * n + q within the l segment
+    const int width = (2 * l + 1) * g.n;
+    const long long block = (long long)l * S * S + pair;
+    if (MODE == 0) {
+        const long long srow = g.value_row0[pair] + g.type_rank[i];
+        if (l == 0) v += g.value_shift[pair * g.n + in_l];
+        g.values[g.value_base[block] + srow * width + in_l] = v;
+    } else {
+        // gradient row: the sample's first row plus the position of atom j (or i)
+        const long long first = g.grad_row0[pair] + (long long)g.type_rank[i] * g.rows_per_sample[pair];
+        const long long rank = MODE == 1 ? g.below[(long long)i * S + t]
+                                         : g.type_rank[j] + ((ct != t && i < j) ? 1 : 0);
+        g.grads[g.grad_base[block] + (first + rank) * 3 * width + (long long)xyz * width + in_l] = v;
+    }
+}
+
 // Tiled GEMM out[row][col] = Σ_k A(row, k) B[k][col] for one parity, 64 x 64 tiles,
-// 16 k per step, 4 x 4 outputs per thread.
+// 16 k per step. VALU version (FP64 arithmetic): 4 x 4 outputs per thread.
 constexpr int GT = 64, GK = 16;
+
+template <typename T, int MODE>
+__device__ __forceinline__ void lode_load_tiles(const LodeGemmArgs<T>& g, long long row0, int col0, int k0,
+                                                long long n_rows, int ncol, const T* B, T (*As)[GT + 4],
+                                                T (*Bs)[GT + 4]) {
+    const int tid = threadIdx.x;
+    const int nk = g.n_k;
+    // A tile: 64 rows x 16 k, 4 per thread
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int idx = tid + e * LT;
+        const int r = idx / GK, kk = idx % GK;
+        const long long row = row0 + r;
+        const int k = k0 + kk;
+        const double v = (row < n_rows && k < nk) ? lode_a_value<T, MODE>(g, row, k) : 0.0;
+        As[kk][r] = (T)v;
+    }
+    // B tile: 16 k x 64 cols
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int idx = tid + e * LT;
+        const int kk = idx / GT, c = idx % GT;
+        const int k = k0 + kk, col = col0 + c;
+        Bs[kk][c] = (k < nk && col < ncol) ? B[(long long)k * ncol + col] : (T)0;
+    }
+}
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(LT) lode_gemm_kernel(LodeGemmArgs<T> g) {
@@ -126,7 +216,6 @@ __global__ void __launch_bounds__(LT) lode_gemm_kernel(LodeGemmArgs<T> g) {
     const long long row0 = (long long)blockIdx.x * GT;
     const int col0 = blockIdx.y * GT;
     const int N = g.n_atoms, nk = g.n_k;
-    const int par = g.parity;
     const long long n_rows = MODE == 2 ? (long long)N * N : (long long)N * g.n_types;
     const int ncol = MODE == 0 ? g.ncol_v : 3 * g.ncol_v;
     const T* B = MODE == 0 ? g.bv : g.bg;
@@ -137,43 +226,7 @@ __global__ void __launch_bounds__(LT) lode_gemm_kernel(LodeGemmArgs<T> g) {
         for (int b = 0; b < 4; b++) acc[a][b] = 0;
 
     for (int k0 = 0; k0 < nk; k0 += GK) {
-        // A tile: 64 rows x 16 k, 4 per thread
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + e * LT;
-            const int r = idx / GK, kk = idx % GK;
-            const long long row = row0 + r;
-            const int k = k0 + kk;
-            double v = 0.0;
-            if (row < n_rows && k < nk) {
-                if (MODE == 2) {
-                    const int i = (int)(row / N), j = (int)(row - (long long)i * N);
-                    const double ci = g.cosv[(long long)i * nk + k], si = g.sinv[(long long)i * nk + k];
-                    const double cj = g.cosv[(long long)j * nk + k], sj = g.sinv[(long long)j * nk + k];
-                    v = par == 0 ? 2.0 * (si * cj - ci * sj) : -2.0 * (ci * cj + si * sj);
-                } else {
-                    const int i = (int)(row / g.n_types), tl = (int)(row - (long long)i * g.n_types);
-                    const int t = g.types_present[tl];
-                    const double ci = g.cosv[(long long)i * nk + k], si = g.sinv[(long long)i * nk + k];
-                    const double sc = g.sumc[(long long)t * nk + k], ss = g.sums[(long long)t * nk + k];
-                    const double re = 2.0 * (ci * sc + si * ss), im = 2.0 * (si * sc - ci * ss);
-                    if (MODE == 0) {
-                        v = par == 0 ? re : im;
-                    } else {
-                        v = par == 0 ? -im : re - (t == g.type_index[i] ? 2.0 : 0.0);
-                    }
-                }
-            }
-            As[kk][r] = (T)v;
-        }
-        // B tile: 16 k x 64 cols
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + e * LT;
-            const int kk = idx / GT, c = idx % GT;
-            const int k = k0 + kk, col = col0 + c;
-            Bs[kk][c] = (k < nk && col < ncol) ? B[(long long)k * ncol + col] : (T)0;
-        }
+        lode_load_tiles<T, MODE>(g, row0, col0, k0, n_rows, ncol, B, As, Bs);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < GK; kk++) {
@@ -190,65 +243,175 @@ __global__ void __launch_bounds__(LT) lode_gemm_kernel(LodeGemmArgs<T> g) {
         }
         __syncthreads();
     }
-
-    // epilogue: scatter into the blocks
 #pragma unroll
     for (int x = 0; x < 4; x++) {
         const long long row = row0 + ty + 16 * x;
         if (row >= n_rows) continue;
-        int i, t, j = -1;
-        if (MODE == 2) {
-            i = (int)(row / N);
-            j = (int)(row - (long long)i * N);
-            if (j == i) continue;
-            t = g.type_index[j];
-        } else {
-            i = (int)(row / g.n_types);
-            t = g.types_present[(int)(row - (long long)i * g.n_types)];
-        }
-        const int ct = g.type_index[i];
-        const int S = g.n_species;
-        const long long pair = (long long)ct * S + t;
 #pragma unroll
         for (int y = 0; y < 4; y++) {
             const int col = col0 + tx + 16 * y;
-            if (col >= ncol) continue;
-            const int cv = MODE == 0 ? col : col % g.ncol_v;
-            const int xyz = MODE == 0 ? 0 : col / g.ncol_v;
-            const int l = g.col_l[cv];
-            const int in_l = g.col_in_l[cv];  // m * n + q within the l segment
-            const int width = (2 * l + 1) * g.n;
-            const long long block = (long long)l * S * S + pair;
-            T v = acc[x][y];
-            long long dst;
-            if (MODE == 0) {
-                const long long srow = g.value_row0[pair] + g.type_rank[i];
-                if (l == 0) v += g.value_shift[pair * g.n + in_l];
-                dst = g.value_base[block] + srow * width + in_l;
-                g.values[dst] = v;
-            } else {
-                // gradient row: the sample's first row plus the position of atom j (or i)
-                const long long first = g.grad_row0[pair] + (long long)g.type_rank[i] * g.rows_per_sample[pair];
-                long long rank;
-                if (MODE == 1) {
-                    rank = g.below[(long long)i * S + t];
-                } else {
-                    rank = g.type_rank[j] + ((ct != t && i < j) ? 1 : 0);
-                }
-                dst = g.grad_base[block] + (first + rank) * 3 * width + (long long)xyz * width + in_l;
-                g.grads[dst] = v;
-            }
+            if (col < ncol) lode_store<T, MODE>(g, row, col, acc[x][y]);
         }
     }
+}
+
+// FP32 version on the matrix cores: each wave computes a 32 x 32 quadrant of the
+// 64 x 64 tile as 2 x 2 blocks of v_mfma_f32_16x16x4f32 (A[i][k] at lane i + 16 k,
+// B[k][j] at lane j + 16 k, D[i][j] at lane j + 16 (i / 4), register i % 4).
+typedef float lode_float4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__global__ void __launch_bounds__(LT) lode_gemm_mfma_kernel(LodeGemmArgs<float> g) {
+    __shared__ float As[GK][GT + 4];
+    __shared__ float Bs[GK][GT + 4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const long long row0 = (long long)blockIdx.x * GT;
+    const int col0 = blockIdx.y * GT;
+    const int N = g.n_atoms, nk = g.n_k;
+    const long long n_rows = MODE == 2 ? (long long)N * N : (long long)N * g.n_types;
+    const int ncol = MODE == 0 ? g.ncol_v : 3 * g.ncol_v;
+    const float* B = MODE == 0 ? g.bv : g.bg;
+    const int rq = (wave >> 1) * 32, cq = (wave & 1) * 32;
+    const int i16 = lane & 15, kq = lane >> 4;
+    lode_float4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = {0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < nk; k0 += GK) {
+        lode_load_tiles<float, MODE>(g, row0, col0, k0, n_rows, ncol, B, As, Bs);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 4) {
+            const float a0 = As[ks + kq][rq + i16], a1 = As[ks + kq][rq + 16 + i16];
+            const float b0 = Bs[ks + kq][cq + i16], b1 = Bs[ks + kq][cq + 16 + i16];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const long long row = row0 + rq + 16 * rb + 4 * kq + v;
+                const int col = col0 + cq + 16 * cb + i16;
+                if (row < n_rows && col < ncol) lode_store<float, MODE>(g, row, col, acc[rb][cb][v]);
+            }
+}
+
+// Pair rows (i, j != i) in FP32 on the matrix cores, tiled for reuse of the
+// neighbours' phases: a workgroup takes 4 centers x 64 neighbours x 64 columns,
+// with the k-vectors outermost in chunks of 32. Per chunk the neighbours' phases
+// (rounded once from FP64) and B go into LDS and serve all four centers; wave w
+// keeps center i0 + w's 64 x 64 block in 16 accumulators and forms its A
+// fragments A[j][k] = 2 (s_i c_j - c_i s_j) (even l) or -2 (c_i c_j + s_i s_j)
+// (odd l) from them.
+constexpr int PJ = 64, PC = 64, PK = 32, PI = 4;
+
+__global__ void __launch_bounds__(LT) lode_pair_mfma_kernel(LodeGemmArgs<float> g) {
+    __shared__ float cj[PK][PJ + 4], sj[PK][PJ + 4], Bs[PK][PC + 4];
+    __shared__ float ci[PI][PK], si[PI][PK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int N = g.n_atoms, nk = g.n_k;
+    const int par = g.parity;
+    const int ncol = 3 * g.ncol_v;
+    const int i0 = blockIdx.x * PI, j0 = blockIdx.y * PJ, c0 = blockIdx.z * PC;
+    const int i16 = lane & 15, kq = lane >> 4;
+    lode_float4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = {0.f, 0.f, 0.f, 0.f};
+
+    // the next chunk's operands in registers: 8 neighbour cosines, sines and B
+    // values per thread, and one center phase pair for the first 128 threads
+    constexpr int PER = PK * PJ / LT;
+    float rc[PER], rs[PER], rb[PER], rci = 0.f, rsi = 0.f;
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int idx = tid + e * LT;
+            const int kk = idx / PJ, jj = idx - kk * PJ;
+            const int k = k0 + kk, j = j0 + jj, col = c0 + jj;
+            const bool in = k < nk && j < N;
+            rc[e] = in ? g.cosf[(long long)j * nk + k] : 0.f;
+            rs[e] = in ? g.sinf[(long long)j * nk + k] : 0.f;
+            rb[e] = (k < nk && col < ncol) ? g.bg[(long long)k * ncol + col] : 0.f;
+        }
+        if (tid < PI * PK) {
+            const int ii = tid / PK, kk = tid - ii * PK;
+            const int i = i0 + ii, k = k0 + kk;
+            const bool in = k < nk && i < N;
+            rci = in ? g.cosf[(long long)i * nk + k] : 0.f;
+            rsi = in ? g.sinf[(long long)i * nk + k] : 0.f;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < nk; k0 += PK) {
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int idx = tid + e * LT;
+            const int kk = idx / PJ, jj = idx - kk * PJ;
+            cj[kk][jj] = rc[e];
+            sj[kk][jj] = rs[e];
+            Bs[kk][jj] = rb[e];
+        }
+        if (tid < PI * PK) {
+            const int ii = tid / PK, kk = tid - ii * PK;
+            ci[ii][kk] = rci;
+            si[ii][kk] = rsi;
+        }
+        __syncthreads();
+        // the following chunk's loads fly while this one is multiplied
+        if (k0 + PK < nk) fetch(k0 + PK);
+#pragma unroll
+        for (int ks = 0; ks < PK; ks += 4) {
+            const int k = ks + kq;
+            const float c_i = ci[wave][k], s_i = si[wave][k];
+            float b[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) b[cb] = Bs[k][cb * 16 + i16];
+#pragma unroll
+            for (int jb = 0; jb < 4; jb++) {
+                const float c_j = cj[k][jb * 16 + i16], s_j = sj[k][jb * 16 + i16];
+                const float a = par == 0 ? 2.0f * (s_i * c_j - c_i * s_j) : -2.0f * (c_i * c_j + s_i * s_j);
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+                    acc[jb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[cb], acc[jb][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const int i = i0 + wave;
+    if (i >= N) return;
+#pragma unroll
+    for (int jb = 0; jb < 4; jb++)
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + jb * 16 + 4 * kq + v;
+                const int col = c0 + cb * 16 + i16;
+                if (j < N && col < ncol) lode_store<float, 2>(g, (long long)i * N + j, col, acc[jb][cb][v]);
+            }
 }
 
 }  // namespace
 
 void launch_lode_phases(hipStream_t stream, int n_atoms, int n_k, const double* positions, const double* kvec, double* cosv,
-                        double* sinv) {
+                        double* sinv, float* cosf, float* sinf) {
     const long long total = (long long)n_atoms * n_k;
     if (total == 0) return;
-    lode_phase_kernel<<<(unsigned)((total + LT - 1) / LT), LT, 0, stream>>>(n_atoms, n_k, positions, kvec, cosv, sinv);
+    lode_phase_kernel<<<(unsigned)((total + LT - 1) / LT), LT, 0, stream>>>(n_atoms, n_k, positions, kvec, cosv, sinv,
+                                                                          cosf, sinf);
 }
 
 void launch_lode_type_sums(hipStream_t stream, int n_atoms, int n_k, int S, const int* type_index, const double* cosv,
@@ -269,6 +432,19 @@ void launch_lode_gemm(hipStream_t stream, int mode, const LodeGemmArgs<T>& g) {
     const int ncol = mode == 0 ? g.ncol_v : 3 * g.ncol_v;
     if (n_rows == 0 || ncol == 0) return;
     dim3 grid((unsigned)((n_rows + GT - 1) / GT), (unsigned)((ncol + GT - 1) / GT));
+    if constexpr (sizeof(T) == 4) {
+        // FP32: products and sums on the matrix cores, FP32 throughout
+        if (mode == 0) {
+            lode_gemm_mfma_kernel<0><<<grid, LT, 0, stream>>>(g);
+        } else if (mode == 1) {
+            lode_gemm_mfma_kernel<1><<<grid, LT, 0, stream>>>(g);
+        } else {
+            const long long n = g.n_atoms;
+            dim3 pgrid((unsigned)((n + PI - 1) / PI), (unsigned)((n + PJ - 1) / PJ), (unsigned)((ncol + PC - 1) / PC));
+            lode_pair_mfma_kernel<<<pgrid, LT, 0, stream>>>(g);
+        }
+        return;
+    }
     if (mode == 0) {
         lode_gemm_kernel<T, 0><<<grid, LT, 0, stream>>>(g);
     } else if (mode == 1) {

@@ -33,6 +33,8 @@ struct LodeGemmArgs {
     const int* below;              // [n_atoms][n_species] atoms of each type before this one
     const double* cosv;            // [n_atoms][n_k]
     const double* sinv;
+    const float* cosf;             // the same phases rounded to FP32 (FP32 pair rows), or null
+    const float* sinf;
     const double* sumc;            // [n_species][n_k]
     const double* sums;
     const T* bv;                   // [n_k][ncol_v]
@@ -49,8 +51,9 @@ struct LodeGemmArgs {
     const long long* rows_per_sample;  // [S][S]
 };
 
+// cos / sin (k . r_i) in FP64, plus FP32 copies when cosf / sinf are not null
 void launch_lode_phases(hipStream_t stream, int n_atoms, int n_k, const double* positions, const double* kvec, double* cosv,
-                        double* sinv);
+                        double* sinv, float* cosf, float* sinf);
 void launch_lode_type_sums(hipStream_t stream, int n_atoms, int n_k, int S, const int* type_index, const double* cosv,
                            const double* sinv, double* sumc, double* sums);
 template <typename T>

@@ -112,8 +112,18 @@ double k0_factor(const LodeParams& p) {
 }  // namespace
 
 template <typename T>
-mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& grads, hipStream_t stream) {
+mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
+                                      featomic_amd_device_output_t* output) {
     const LodeParams& p = lode_;
+    if (!ev_start_) {
+        RA_HIP_CHECK(hipEventCreate(&ev_start_));
+        RA_HIP_CHECK(hipEventCreate(&ev_stop_));
+        RA_HIP_CHECK(hipEventCreate(&ev_main_start_));
+        RA_HIP_CHECK(hipEventCreate(&ev_main_stop_));
+    }
+    RA_HIP_CHECK(hipEventRecord(ev_start_, stream));
+    main_ms_sum_ = 0.0;
+    main_bytes_ = 0;
     const int L = p.L(), n = p.n_radial;
     if (L > LODE_MAX_L || n > LODE_MAX_N) {
         throw Error::invalid_parameter("max_angular must be < " + std::to_string(LODE_MAX_L) + " and max_radial < " +
@@ -122,7 +132,9 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
     const Species sp = species_of(batch);
     const int S = static_cast<int>(sp.types.size());
     const size_t n_sys = batch.n_systems();
-    if (batch.positions.empty() && batch.n_atoms() > 0) throw Error::internal("LODE needs host positions");
+    if (batch.positions.empty() && !batch.device_positions && batch.n_atoms() > 0) {
+        throw Error::internal("LODE needs the positions");
+    }
 
     // types present per system; keys are the type pairs found together in a system
     std::vector<std::vector<int>> count(n_sys, std::vector<int>(S, 0));
@@ -233,8 +245,13 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
                     shift[(static_cast<size_t>(ct) * S + nt) * n + q] = static_cast<T>(v);
                 }
 
-        double* d_pos = ws_.get<double>("lode_pos", 3 * static_cast<size_t>(N));
-        RA_HIP_CHECK(hipMemcpyAsync(d_pos, batch.positions.data() + 3 * a0, sizeof(double) * 3 * N, hipMemcpyHostToDevice, stream));
+        const double* d_pos = batch.device_positions ? batch.device_positions + 3 * a0 : nullptr;
+        if (!d_pos) {
+            double* staged = ws_.get<double>("lode_pos", 3 * static_cast<size_t>(N));
+            RA_HIP_CHECK(hipMemcpyAsync(staged, batch.positions.data() + 3 * a0, sizeof(double) * 3 * N,
+                                        hipMemcpyHostToDevice, stream));
+            d_pos = staged;
+        }
         double* d_kv = ws_.get<double>("lode_kv", kv.size());
         double* d_df = ws_.get<double>("lode_df", df.size());
         int* d_tidx = ws_.get<int>("lode_tidx", N);
@@ -260,7 +277,13 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         double* d_sin = ws_.get<double>("lode_sin", static_cast<size_t>(N) * nk);
         double* d_sumc = ws_.get<double>("lode_sumc", static_cast<size_t>(S) * nk);
         double* d_sums = ws_.get<double>("lode_sums", static_cast<size_t>(S) * nk);
-        launch_lode_phases(stream, N, nk, d_pos, d_kv, d_cos, d_sin);
+        float* d_cosf = nullptr;
+        float* d_sinf = nullptr;
+        if (sizeof(T) == 4) {
+            d_cosf = ws_.get<float>("lode_cosf", static_cast<size_t>(N) * nk);
+            d_sinf = ws_.get<float>("lode_sinf", static_cast<size_t>(N) * nk);
+        }
+        launch_lode_phases(stream, N, nk, d_pos, d_kv, d_cos, d_sin, d_cosf, d_sinf);
         launch_lode_type_sums(stream, N, nk, S, d_tidx, d_cos, d_sin, d_sumc, d_sums);
 
         LodeProjectionArgs<T> pa;
@@ -280,8 +303,7 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         }
         launch_lode_projection<T>(stream, pa);
 
-        for (int par = 0; par < 2; par++) {
-            if (ncol_v[par] == 0) continue;
+        auto gemm_args = [&](int par) {
             LodeGemmArgs<T> g;
             g.n_atoms = N;
             g.n_k = nk;
@@ -296,6 +318,8 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
             g.below = d_below;
             g.cosv = d_cos;
             g.sinv = d_sin;
+            g.cosf = d_cosf;
+            g.sinf = d_sinf;
             g.sumc = d_sumc;
             g.sums = d_sums;
             g.bv = pa.bv[par];
@@ -310,15 +334,68 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
             g.grad_base = d_grad_base;
             g.grad_row0 = d_grow0;
             g.rows_per_sample = d_rps;
+            return g;
+        };
+        for (int par = 0; par < 2; par++) {
+            if (ncol_v[par] == 0) continue;
+            LodeGemmArgs<T> g = gemm_args(par);
             launch_lode_gemm<T>(stream, 0, g);
-            if (grads.positions) {
-                launch_lode_gemm<T>(stream, 1, g);
+            if (grads.positions) launch_lode_gemm<T>(stream, 1, g);
+        }
+        if (grads.positions) {
+            // the pair rows (i, j != i): the dominant GEMM, timed on the compute stream
+            RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+            for (int par = 0; par < 2; par++) {
+                if (ncol_v[par] == 0) continue;
+                LodeGemmArgs<T> g = gemm_args(par);
                 launch_lode_gemm<T>(stream, 2, g);
             }
+            RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+            main_bytes_ += 2LL * nk * 3 * (ncol_v[0] + ncol_v[1]) * static_cast<long long>(N) * (N - 1);
         }
         RA_HIP_CHECK(hipGetLastError());
         // the per-system scratch buffers are reused by the next system
         RA_HIP_CHECK(hipStreamSynchronize(stream));
+        if (grads.positions) {
+            float ms = 0.0f;
+            RA_HIP_CHECK(hipEventElapsedTime(&ms, ev_main_start_, ev_main_stop_));
+            main_ms_sum_ += ms;
+        }
+    }
+    RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+
+    if (output) {
+        // outputs stay in the device buffers; blocks in (center type, neighbour type)-major,
+        // l-minor order, like the TensorMap's keys
+        DeviceResult r;
+        r.key_size = 4;
+        for (int pr = 0; pr < S * S; pr++) {
+            if (!pair_present[pr]) continue;
+            const int ct = pr / S, nt = pr % S;
+            for (int l = 0; l < L; l++) {
+                r.keys.insert(r.keys.end(), {l, 1, sp.types[ct], sp.types[nt]});
+                r.n_samples.push_back(n_samples[pr]);
+                r.n_rows.push_back(grads.positions ? n_rows[pr] : 0);
+                r.value_offsets.push_back(value_base[static_cast<size_t>(l) * S * S + pr]);
+                r.gradient_offsets.push_back(grads.positions ? grad_base[static_cast<size_t>(l) * S * S + pr] : 0);
+                r.block_properties.push_back((2LL * l + 1) * n);
+            }
+        }
+        device_result_ = std::move(r);
+        output->n_blocks = device_result_.n_samples.size();
+        output->keys = device_result_.keys.data();
+        output->key_size = device_result_.key_size;
+        output->n_samples = device_result_.n_samples.data();
+        output->n_gradient_rows = device_result_.n_rows.data();
+        output->value_offsets = device_result_.value_offsets.data();
+        output->gradient_offsets = device_result_.gradient_offsets.data();
+        output->block_properties = device_result_.block_properties.data();
+        output->values = d_values;
+        output->gradients = d_grads;
+        output->sample_labels = nullptr;
+        output->gradient_labels = nullptr;
+        output->precision = sizeof(T) == 4 ? 32 : 64;
+        return nullptr;
     }
 
     // ---- download and assemble the TensorMap
@@ -374,7 +451,9 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
     return tm.release();
 }
 
-template mts_tensormap_t* Calculator::run_lode<float>(const HostBatch&, const Gradients&, hipStream_t);
-template mts_tensormap_t* Calculator::run_lode<double>(const HostBatch&, const Gradients&, hipStream_t);
+template mts_tensormap_t* Calculator::run_lode<float>(const HostBatch&, const Gradients&, hipStream_t,
+                                                     featomic_amd_device_output_t*);
+template mts_tensormap_t* Calculator::run_lode<double>(const HostBatch&, const Gradients&, hipStream_t,
+                                                      featomic_amd_device_output_t*);
 
 }  // namespace rascaline_amd

@@ -132,3 +132,24 @@ def bulk_cell(seed=0, n_atoms=20000, density=0.1, types=(1, 6, 7, 8, 16), min_di
             count += 1
     t = rng.choice(np.array(types, dtype=np.int32), size=n_atoms)
     return [(pos, t, cell)]
+
+
+def lode_cells(seed=0, n_structures=256, n_min=500, n_max=1500, density=0.1, types=(1, 6, 8), min_dist=0.8):
+    """Periodic triclinic cells (angles U[60, 120] degrees, volume N / density) with
+    U{n_min..n_max} atoms and types uniform: the LODE workload."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    for _ in range(n_structures):
+        n = int(rng.integers(n_min, n_max + 1))
+        t = rng.choice(np.array(types, dtype=np.int32), size=n)
+        volume = n / density
+        while True:
+            angles = rng.uniform(60.0, 120.0, 3)
+            unit = triclinic_cell(1.0, 1.0, 1.0, *angles)
+            if unit is None or abs(np.linalg.det(unit)) < 0.3:
+                continue
+            cell = unit * (volume / abs(np.linalg.det(unit))) ** (1.0 / 3.0)
+            break
+        pos = _reject_place(rng, n, lambda: rng.uniform(0, 1, 3) @ cell, min_dist, cell)
+        systems.append((pos, t, cell))
+    return systems
