@@ -399,7 +399,7 @@ def lode_main(args):
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "lode_gemm_kernel<float, 2> (pair gradient rows: A from the FP64 phases, K = k-vectors)",
+            "kernel": "lode_pair_mfma_kernel (pair gradient rows on FP32 MFMA: A from the phases, K = k-vectors)",
             "kernel_ms_per_step": kernel_ms,
             "device_ms_per_step": device_ms,
             "algorithmic_flops_per_step": flops,
