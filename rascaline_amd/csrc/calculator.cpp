@@ -503,7 +503,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     const size_t slots_bytes = slots * sizeof(T);
     if (ps) {
         if (static_cast<long long>(S * slots_bytes) > lds) throw Error::invalid_parameter("too many atomic types for the GPU kernels' local memory");
-        launch_ps_values<T>(lp, coef, type_mask, sample_flag, sample_scan, d_values);
+        launch_ps_values<T>(lp, coef, type_mask, sample_flag, sample_scan, d_values, fast_form != 0, lds);
     } else {
         launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
     }

@@ -70,9 +70,11 @@ void launch_radial_eval(hipStream_t stream, long long count, const DeviceRadial<
                         T* out_val, T* out_der);
 template <typename T>
 void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad);
+// fast (FP32 only): the m-contraction in bfloat16 hi + lo on MFMA when n is a
+// multiple of 4, at most 16 types and the per-center Gram matrix fits in lds_limit
 template <typename T>
 void launch_ps_values(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
-                      const long long* sample_scan, T* out);
+                      const long long* sample_scan, T* out, bool fast, int lds_limit);
 template <typename T>
 size_t gradient_rows_shmem(const SoapLaunch<T>& p, bool ps);
 template <typename T>

@@ -1552,6 +1552,93 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             }
             lds_barrier();
             continue;
+        } else if constexpr (FAST && sizeof(T) == 4 && (NC == 12 || NC == 16)) {
+            // mixed precision for n = 12 or 16: one 16 x 16 tile per (run, type),
+            // X[n1][n2] = Σ_m DC_r[x][m][n1] C_t[m][n2] in bfloat16 hi + lo on MFMA. The
+            // type's operand (B) is split once per (l, x, type) item; the wave stages up
+            // to RP * NC / 4 tiles, then writes their n x n blocks as 16-byte pieces
+            constexpr int STAGE_TILES = RP * NC / 4;
+            constexpr int KB_MAX = (2 * TILE_MAX_L - 1 + 15) / 16;
+            constexpr int nq4 = NC / 4;
+            const int lane = tid & (WAVE - 1);
+            const int i16 = lane & 15, kq = 4 * (lane >> 4);
+            const int items = 3 * L * n_present;
+            T* X = stage;  // [STAGE_TILES][16][16]
+            for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
+                const int g = it % n_present, lx = it / n_present;
+                const int l = L - 1 - lx / 3, x = lx % 3;
+                const int kcount = 2 * l + 1;
+                const int t = present[g];
+                const T* Cb = C + (size_t)t * CS + l * l * NC;
+                short4_t bh[KB_MAX], bl[KB_MAX];
+#pragma unroll
+                for (int kb = 0; kb < KB_MAX; kb++) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int mi = 16 * kb + kq + e;
+                        const float bv = (mi < kcount && i16 < NC) ? Cb[mi * NC + i16] : 0.f;
+                        short h, lo;
+                        bf16_split(bv, h, lo);
+                        bh[kb][e] = h;
+                        bl[kb][e] = lo;
+                    }
+                }
+                const T sc_same = scale_l[l], sc_diff = scale_l[l] * (T)1.4142135623730951;
+                for (int r0 = 0; r0 < nc_runs; r0 += STAGE_TILES) {
+                    const int nr = min(STAGE_TILES, nc_runs - r0);
+                    for (int rr = 0; rr < nr; rr++) {
+                        const T* D = DC + ((size_t)(r0 + rr) * 3 + x) * CS + l * l * NC;
+                        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int kb = 0; kb < KB_MAX; kb++) {
+                            if (16 * kb >= kcount) break;
+                            short4_t ah, al;
+#pragma unroll
+                            for (int e = 0; e < 4; e++) {
+                                const int mi = 16 * kb + kq + e;
+                                const float av = (mi < kcount && i16 < NC) ? D[mi * NC + i16] : 0.f;
+                                short h, lo;
+                                bf16_split(av, h, lo);
+                                ah[e] = h;
+                                al[e] = lo;
+                            }
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[kb], acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[kb], acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[kb], acc, 0, 0, 0);
+                        }
+                        // D[row][col]: row = 4 (lane / 16) + v, col = lane % 16
+#pragma unroll
+                        for (int v = 0; v < 4; v++) X[rr * 256 + (kq + v) * 16 + i16] = acc[v];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                    const int npieces = nr * NC * nq4;
+                    for (int p = lane; p < npieces; p += WAVE) {
+                        const int rr = p / (NC * nq4), rem = p - rr * (NC * nq4);
+                        const int n1 = rem / nq4, h = rem - n1 * nq4;
+                        const int r = r0 + rr, tj = run_tj[r];
+                        const T* Xb = X + rr * 256;
+                        T v[4];
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const int n2 = 4 * h + c;
+                            T sum = 0;
+                            if (t >= tj) sum += Xb[n1 * 16 + n2];
+                            if (t <= tj) sum += Xb[n2 * 16 + n1];
+                            v[c] = sum;
+                        }
+                        const T sc = t != tj ? sc_diff : sc_same;
+                        T* dst = out + (long long)run_row[r][g] * 3 * P + (long long)x * P + l * NC * NC + n1 * NC + 4 * h;
+                        store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                }
+            }
+            lds_barrier();
+            continue;
         }
         // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
         // so that the nq lanes of one (run, type, x, l) tile are adjacent
@@ -1649,6 +1736,116 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         } else if (tile >= 0) {
             tile_store<T, NC, RP>(out + tile, q, n, s, acc);
         }
+    }
+}
+
+// Power-spectrum values with the m-contraction on MFMA (FP32 in and out, bfloat16
+// hi + lo products, FP32 accumulation), one center per workgroup
+// (soap/power_spectrum.rs:529-618). Per l, the Gram matrix over all (type, n)
+// columns, G_l[(t1, n1)][(t2, n2)] = Σ_m c^{t1}[l, m, n1] c^{t2}[l, m, n2], is
+// formed as the 16 x 16 tiles of its upper triangle; every block (t1 <= t2) with
+// a sample at this center then takes its n x n piece, times κ / ((-1)^l √(2l+1)).
+// Columns of types absent around the center are zero.
+constexpr int PSV_THREADS = 256;
+constexpr int PSV_MAX_TYPES = 16;
+
+__global__ void __launch_bounds__(PSV_THREADS) ps_values_mfma_kernel(
+    int n_atoms, int S, int L, int n, int W16, const float* __restrict__ coef, const int* __restrict__ species,
+    const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_flag,
+    const long long* __restrict__ sample_scan, float* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ long long pair_row[PSV_MAX_TYPES * (PSV_MAX_TYPES + 1) / 2];
+    __shared__ int pair_t1[PSV_MAX_TYPES * (PSV_MAX_TYPES + 1) / 2], pair_t2[PSV_MAX_TYPES * (PSV_MAX_TYPES + 1) / 2];
+    const int Wp = 16 * W16;
+    const int L2 = L * L;
+    float* Cm = reinterpret_cast<float*>(smem);  // [l m][t n], zero-padded to Wp columns
+    float* G = Cm + (size_t)L2 * Wp;             // [Wp][Wp], upper tiles
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const unsigned mask = type_mask[i];
+    const int ct = species[i];
+    const int npairs = S * (S + 1) / 2;
+    for (int k = tid; k < L2 * Wp; k += PSV_THREADS) {
+        const int lm = k / Wp, col = k - lm * Wp;
+        const int t = col / n, c = col - t * n;
+        float v = 0.f;
+        if (t < S && ((mask >> t) & 1u)) v = coef[((long long)i * S + t) * L2 * n + lm * n + c];
+        Cm[k] = v;
+    }
+    for (int p = tid; p < npairs; p += PSV_THREADS) {
+        int t1 = 0, rem = p;
+        while (rem >= S - t1) {
+            rem -= S - t1;
+            t1++;
+        }
+        const int t2 = t1 + rem;
+        const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
+        const bool on = ((mask >> t1) & 1u) && ((mask >> t2) & 1u) && sample_flag[idx] != 0;
+        pair_t1[p] = t1;
+        pair_t2[p] = t2;
+        pair_row[p] = on ? sample_scan[idx] : -1;
+    }
+    __syncthreads();
+    const int P = L * n * n;
+    const int nq4 = n / 4;
+    const int ntiles = W16 * (W16 + 1) / 2;
+    const int i16 = lane & 15, kq = 4 * (lane >> 4);
+    for (int l = 0; l < L; l++) {
+        const int kcount = 2 * l + 1;
+        const float* Cl = Cm + (size_t)l * l * Wp;
+        for (int tile = wave; tile < ntiles; tile += PSV_THREADS / WAVE) {
+            int bi = 0, rem = tile;
+            while (rem >= W16 - bi) {
+                rem -= W16 - bi;
+                bi++;
+            }
+            const int bj = bi + rem;
+            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < kcount; kb += 16) {
+                short4_t ah, al, bh, bl;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int mi = kb + kq + e;
+                    const float av = mi < kcount ? Cl[mi * Wp + 16 * bi + i16] : 0.f;
+                    const float bv = mi < kcount ? Cl[mi * Wp + 16 * bj + i16] : 0.f;
+                    short h, lo;
+                    bf16_split(av, h, lo);
+                    ah[e] = h;
+                    al[e] = lo;
+                    bf16_split(bv, h, lo);
+                    bh[e] = h;
+                    bl[e] = lo;
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) G[(size_t)(16 * bi + kq + v) * Wp + 16 * bj + i16] = acc[v];
+        }
+        __syncthreads();
+        const float scale = (float)((l % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * l + 1.0));
+        const int per_pair = n * nq4;
+        for (int k = tid; k < npairs * per_pair; k += PSV_THREADS) {
+            const int p = k / per_pair, rem = k - p * per_pair;
+            const long long s = pair_row[p];
+            if (s < 0) continue;
+            const int n1 = rem / nq4, h = rem - n1 * nq4;
+            const int t1 = pair_t1[p], t2 = pair_t2[p];
+            const int rho = t1 * n + n1, kap = t2 * n + 4 * h;
+            const float sc = t1 != t2 ? scale * 1.4142135623730951f : scale;
+            float v[4];
+            if ((rho >> 4) > (kap >> 4)) {
+                // lower tile of a diagonal (t1 == t2) block: G is symmetric there
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[c] = G[(size_t)(kap + c) * Wp + rho];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[c] = G[(size_t)rho * Wp + kap + c];
+            }
+            store4<float>(out + s * P + l * n * n + n1 * n + 4 * h, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+        }
+        __syncthreads();
     }
 }
 
@@ -2031,8 +2228,18 @@ void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad) {
 
 template <typename T>
 void launch_ps_values(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
-                      const long long* sample_scan, T* out) {
+                      const long long* sample_scan, T* out, bool fast, int lds_limit) {
     if (p.n_atoms == 0) return;
+    if constexpr (sizeof(T) == 4) {
+        // mixed precision on MFMA: n a multiple of 4, at most 16 types, Gram matrix in LDS
+        const int W16 = (p.n_species * p.n + 15) / 16;
+        const size_t mshm = ((size_t)p.L * p.L * 16 * W16 + (size_t)256 * W16 * W16) * sizeof(float);
+        if (fast && p.n % 4 == 0 && p.n_species <= PSV_MAX_TYPES && mshm + 8192 <= (size_t)lds_limit) {
+            ps_values_mfma_kernel<<<p.n_atoms, PSV_THREADS, mshm, p.stream>>>(
+                p.n_atoms, p.n_species, p.L, p.n, W16, coef, p.species, type_mask, sample_flag, sample_scan, out);
+            return;
+        }
+    }
     const size_t shmem = (size_t)p.n_species * p.L * p.L * p.n * sizeof(T);
     ps_values_kernel<T><<<p.n_atoms, WAVE, shmem, p.stream>>>(p.n_atoms, p.n_species, p.L, p.n, coef, p.species,
                                                               type_mask, sample_flag, sample_scan, out);
@@ -2151,8 +2358,9 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass);
     const int4* rs = reinterpret_cast<const int4*>(runs);
     const int4* ws = reinterpret_cast<const int4*>(windows);
-    // the mixed-precision contraction covers FP32, n = 8 and windows of up to 4 runs
-    const bool mixed = fast && sizeof(T) == 4 && p.n == 8 && max_runs <= 4;
+    // the mixed-precision contraction covers FP32 with n = 8 (windows of up to 4 runs),
+    // n = 12 and n = 16
+    const bool mixed = fast && sizeof(T) == 4 && ((p.n == 8 && max_runs <= 4) || p.n == 12 || p.n == 16);
 #define RA_LAUNCH_TILED(NCV, RPV, FV)                                                                                  \
     ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                          \
         a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
@@ -2164,13 +2372,21 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     } else {                          \
         RA_LAUNCH_TILED(NCV, 1, false);      \
     }
+#define RA_LAUNCH_MIXED(NCV)         \
+    if (rows_per_pass == 4) {        \
+        RA_LAUNCH_TILED(NCV, 4, true);      \
+    } else if (rows_per_pass == 2) { \
+        RA_LAUNCH_TILED(NCV, 2, true);      \
+    } else {                         \
+        RA_LAUNCH_TILED(NCV, 1, true);      \
+    }
     if (mixed) {
-        if (rows_per_pass == 4) {
-            RA_LAUNCH_TILED(8, 4, true);
-        } else if (rows_per_pass == 2) {
-            RA_LAUNCH_TILED(8, 2, true);
+        if (p.n == 8) {
+            RA_LAUNCH_MIXED(8)
+        } else if (p.n == 12) {
+            RA_LAUNCH_MIXED(12)
         } else {
-            RA_LAUNCH_TILED(8, 1, true);
+            RA_LAUNCH_MIXED(16)
         }
     } else if (p.n <= 4) {
         RA_LAUNCH_TILED_RP(4)
@@ -2181,6 +2397,7 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     } else {
         RA_LAUNCH_TILED_RP(16)
     }
+#undef RA_LAUNCH_MIXED
 #undef RA_LAUNCH_TILED_RP
 #undef RA_LAUNCH_TILED
 }
@@ -2233,7 +2450,7 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
 #define RA_INSTANTIATE(T)                                                                                              \
     template void launch_soap_center<T>(const SoapLaunch<T>&, int, T*, T*);                                            \
     template void launch_ps_values<T>(const SoapLaunch<T>&, const T*, const unsigned*, const long long*,              \
-                                      const long long*, T*);                                                           \
+                                      const long long*, T*, bool, int);                                                \
     template size_t gradient_rows_shmem<T>(const SoapLaunch<T>&, bool);                                                \
     template size_t ps_gradient_shmem<T>(const SoapLaunch<T>&, int);                                                   \
     template bool launch_ps_gradient_fast<T>(const SoapLaunch<T>&, int, const T*, const T*, const unsigned*,          \

@@ -221,3 +221,50 @@ def test_ps_fast_form_vs_oracle(kind):
     say("block-relative error (values, gradients): plain", errors[False], "fast", errors[True])
     assert max(errors[True]) <= 1e-4, errors[True]
     assert max(errors[False]) <= 1e-4, errors[False]
+
+
+def per_l_pair_errors(actual, expected, L, n, param=None):
+    """max|a - b| / max|b| for every (l, block) slice: the error per (l, type pair)."""
+    get = (lambda b: b.values) if param is None else (lambda b: b.gradient(param).values)
+    out = {}
+    for key, ba, be in zip(expected.keys.values, actual.blocks(), expected.blocks()):
+        a = np.asarray(get(ba)).reshape(-1, L, n * n)
+        b = np.asarray(get(be)).reshape(-1, L, n * n)
+        if not b.size:
+            continue
+        for l in range(L):
+            scale = float(np.abs(b[:, l]).max())
+            if scale > 0:
+                out[(l, tuple(int(k) for k in key))] = float(np.abs(a[:, l] - b[:, l]).max()) / scale
+    return out
+
+
+@pytest.mark.parametrize("nmax,lmax,n_atoms", [(11, 8, 36), (15, 4, 30)])
+def test_ps_fast_form_bulk_per_l_and_type_pair(nmax, lmax, n_atoms):
+    """Mixed-precision power spectrum of the bulk hyper-parameters (rc 6 A, 12 radial
+    functions, l <= 8, 5 types uniform, 0.1 atoms/A^3; and n = 16 at l <= 4) against
+    the FP64 oracle: the error of every (l, center type, neighbour type pair) slice,
+    max|a - b| / max|b|, is within 1e-4 for values and position gradients."""
+    say("bulk fast form, n", nmax + 1, "l <=", lmax, "atoms", n_atoms)
+    rng = np.random.default_rng(97 + nmax)
+    side = (n_atoms / 0.1) ** (1.0 / 3.0)
+    pos, types, cell = random_system(rng, n_atoms, "periodic", side, types=(1, 6, 7, 8, 9))
+    systems = [orc.System(pos, types, cell)]
+    h = hypers(rc=6.0, lmax=lmax, nmax=nmax)
+    oracle = oracle_to_tensormap(orc.soap_power_spectrum(systems, h, ["positions"]))
+    L, n = lmax + 1, nmax + 1
+    worst = {}
+    for fast in (False, True):
+        calc = ra.SoapPowerSpectrum(**h)
+        calc.set_precision(32)
+        calc.set_fast_form(fast)
+        tm = calc.compute(ra_systems(systems), gradients=["positions"])
+        compare_tensormaps(tm, oracle, 2e-4, 2e-4, gradients=["positions"])
+        ev = per_l_pair_errors(tm, oracle, L, n)
+        eg = per_l_pair_errors(tm, oracle, L, n, "positions")
+        worst[fast] = (max(ev.values()), max(eg.values()))
+        say("  fast" if fast else "  plain", "worst (l, type pair) error: values %.2e at %s, gradients %.2e at %s"
+            % (worst[fast][0], max(ev, key=ev.get), worst[fast][1], max(eg, key=eg.get)))
+        assert len(ev) == L * len(oracle.keys.values)
+    assert max(worst[True]) <= 1e-4, worst[True]
+    assert max(worst[False]) <= 1e-4, worst[False]
