@@ -18,6 +18,7 @@ Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
 
 import argparse
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -55,26 +56,11 @@ def make_chunks(systems, max_atoms):
     return chunks
 
 
-class DeviceBatch:
-    """Positions of one chunk resident in device memory (torch allocation)."""
+def device_batch(systems, indices):
+    """Positions of one chunk resident in device memory."""
+    from rascaline_amd.device import DeviceSystems
 
-    def __init__(self, systems, indices, torch):
-        from rascaline_amd import _c_lib as C
-
-        self.indices = indices
-        pos = np.concatenate([systems[k][0] for k in indices]).astype(np.float64)
-        self.n_atoms = len(pos)
-        self.positions = torch.from_numpy(pos).to("cuda:0")
-        self.types = np.ascontiguousarray(np.concatenate([systems[k][1] for k in indices]).astype(np.int32))
-        self.cells = np.ascontiguousarray(np.stack([systems[k][2].reshape(9) for k in indices]).astype(np.float64))
-        self.offsets = np.zeros(len(indices) + 1, dtype=np.int64)
-        self.offsets[1:] = np.cumsum([len(systems[k][1]) for k in indices])
-        self.struct = C.featomic_amd_device_systems_t()
-        self.struct.n_systems = len(indices)
-        self.struct.atom_offsets = self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-        self.struct.positions = ctypes.c_void_p(self.positions.data_ptr())
-        self.struct.types = self.types.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        self.struct.cells = self.cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return DeviceSystems([systems[k] for k in indices])
 
 
 def run_chunk(calc, batch, stream_ptr, grads):
@@ -145,9 +131,12 @@ def main():
     parser.add_argument("--chunk-atoms", type=int, default=48000)
     parser.add_argument("--precision", type=int, default=32)
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--workload", choices=["soap", "lode"], default="soap",
+    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate"], default="soap",
                         help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
-                             "expansion + gradients on periodic cells of 500-1500 atoms (its own JSON line)")
+                             "expansion + gradients on periodic cells of 500-1500 atoms; bulk: power spectrum + "
+                             "gradients of one 20 000-atom cell (12 radial functions, l <= 8, 5 types); "
+                             "surrogate: spherical expansion + gradients of one 100 000-atom cell with the "
+                             "spline and with the learned radial integral (each its own JSON line)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -157,6 +146,10 @@ def main():
 
     if args.workload == "lode":
         return lode_main(args)
+    if args.workload == "bulk":
+        return bulk_main(args)
+    if args.workload == "surrogate":
+        return surrogate_main(args)
 
     import torch
 
@@ -169,7 +162,7 @@ def main():
     log(f"generated {len(systems)} structures, {n_atoms} atoms in {time.perf_counter() - t0:.1f} s")
 
     chunks = make_chunks(systems, args.chunk_atoms)
-    batches = [DeviceBatch(systems, c, torch) for c in chunks]
+    batches = [device_batch(systems, c) for c in chunks]
     stream = torch.cuda.current_stream().cuda_stream
     grads = (ctypes.c_char_p * 1)(b"positions")
     log(f"{len(batches)} chunks of up to {args.chunk_atoms} atoms")
@@ -183,6 +176,9 @@ def main():
         calc.set_precision(args.precision)
         calc.set_fast_form(fast)
         name = "fast" if fast else "plain"
+        # launch shapes chosen by the autotuner on the first chunk (outside the timed steps)
+        tuning_report = calc.autotune(batches[0], repeats=2)
+        log(f"{name}: autotuned {tuning_report['chosen']}")
 
         def step():
             for b in batches:
@@ -230,6 +226,7 @@ def main():
             "kernel_ms_per_step": kernel_ms,
             "device_ms_per_step": device_ms,
             "algorithmic_bytes_per_step": total_bytes,
+            "tuning": tuning_report["chosen"],
         }
         return ms, rate, roofline, rows_total, samples_total
 
@@ -277,7 +274,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": baseline,
         "forms": {name: {"value": rate, "unit": "atoms/s", "ms_per_step": ms, "roofline_frac": rl["frac"],
-                         "kernel_ms_per_step": rl["kernel_ms_per_step"]}
+                         "kernel_ms_per_step": rl["kernel_ms_per_step"], "tuning": rl["tuning"]}
                   for name, (ms, rate, rl) in forms.items()},
     }
     print(json.dumps(result), flush=True)
@@ -337,7 +334,7 @@ def lode_main(args):
     systems = lode_cells(seed=0, n_structures=n)
     n_atoms = sum(len(s[1]) for s in systems)
     log(f"generated {len(systems)} LODE cells, {n_atoms} atoms in {time.perf_counter() - t0:.1f} s")
-    batches = [DeviceBatch(systems, [k], torch) for k in range(len(systems))]
+    batches = [device_batch(systems, [k]) for k in range(len(systems))]
     stream = torch.cuda.current_stream().cuda_stream
     grads = (ctypes.c_char_p * 1)(b"positions")
     calc = ra.LodeSphericalExpansion(**LODE_HYPERS)
@@ -405,6 +402,160 @@ def lode_main(args):
             "algorithmic_flops_per_step": flops,
         },
         "cpu_baseline": baseline,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+BULK_HYPERS = {
+    "cutoff": {"radius": 6.0, "smoothing": {"type": "ShiftedCosine", "width": 0.5}},
+    "density": {"type": "Gaussian", "width": 0.3},
+    "basis": {"type": "TensorProduct", "max_angular": 8, "radial": {"type": "Gto", "max_radial": 11}},
+}
+
+
+def time_forms(forms, batch, n_atoms, args, grads, label):
+    """For each (name, calculator): warmup, `steps` timed device computations bracketed
+    by synchronisation, then one pass with the library's hipEvent timings of the
+    dominant output kernel. Returns {name: (ms, atoms/s, kernel ms, kernel bytes, device ms)}."""
+    import torch
+
+    from rascaline_amd import _c_lib as C
+
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, calc in forms:
+        for w in range(args.warmup):
+            run_chunk(calc, batch, stream, grads)
+            torch.cuda.synchronize()
+            log(f"{label} {name}: warmup step {w} done")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run_chunk(calc, batch, stream, grads)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        run_chunk(calc, batch, stream, grads)
+        t_ms, k_ms, k_bytes = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
+                                                  ctypes.byref(k_bytes)))
+        out[name] = (1e3 * elapsed / args.steps, n_atoms * args.steps / elapsed, k_ms.value, k_bytes.value, t_ms.value)
+        log(f"{label} {name}: {out[name][0]:.1f} ms/step, {out[name][1]:.3e} atoms/s, output kernel "
+            f"{k_ms.value:.1f} ms")
+    return out
+
+
+def hbm_roofline(kernel_ms, kernel_bytes, device_ms, kernel):
+    achieved = kernel_bytes / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms_per_step": kernel_ms,
+            "device_ms_per_step": device_ms, "algorithmic_bytes_per_step": kernel_bytes}
+
+
+def cores_available():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def bulk_cpu_baseline(target_seconds=15.0):
+    """The oracle's threaded power spectrum + gradients on small cells of the bulk
+    density and hyper-parameters (one system per thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import featomic_oracle as orc
+    from rascaline_amd.datasets import bulk_cell
+
+    cores = cores_available()
+    cells = [orc.System(*bulk_cell(seed=100 + k, n_atoms=200, density=0.1, types=(1, 6, 7, 8, 9))[0])
+             for k in range(2 * cores)]
+    done, atoms, t_total = 0, 0, 0.0
+    while t_total < target_seconds:
+        t0 = time.perf_counter()
+        orc.soap_ps_baseline(cells, BULK_HYPERS, n_threads=cores)
+        t_total += time.perf_counter() - t0
+        done += len(cells)
+        atoms += sum(len(c.types) for c in cells)
+    return {"value": atoms / t_total, "unit": "atoms/s", "cores": cores, "kind": "port",
+            "sample": f"{done} periodic cells of 200 atoms at 0.1 A^-3 (same hyper-parameters, 5 types), FP64, "
+                      f"{cores} threads, {t_total:.1f} s"}
+
+
+def bulk_main(args):
+    """Power spectrum + position gradients of one 20 000-atom periodic cell (5 types,
+    rc 6, 12 radial functions, l <= 8), plain and fast FP32 forms, output (about
+    145 GB) kept in device memory."""
+    import rascaline_amd as ra
+    from rascaline_amd.datasets import bulk_cell
+
+    systems = bulk_cell(seed=0, n_atoms=20000, density=0.1, types=(1, 6, 7, 8, 9))
+    n_atoms = len(systems[0][1])
+    batch = device_batch(systems, [0])
+    grads = (ctypes.c_char_p * 1)(b"positions")
+    res, tuned = {}, {}
+    for fast in (False, True):
+        # one calculator (and its ~145 GB of outputs) at a time in device memory
+        calc = ra.SoapPowerSpectrum(**BULK_HYPERS)
+        calc.set_precision(32)
+        calc.set_fast_form(fast)
+        tuned[("fast" if fast else "plain")] = calc.autotune(batch, repeats=1)
+        log(f"bulk: autotuned {tuned['fast' if fast else 'plain']}")
+        res.update(time_forms([("fast" if fast else "plain", calc)], batch, n_atoms, args, grads, "bulk"))
+        del calc
+        gc.collect()
+    best = max(res, key=lambda k: res[k][1])
+    ms, rate, k_ms, k_bytes, d_ms = res[best]
+    baseline = None if args.no_cpu_baseline else bulk_cpu_baseline()
+    result = {
+        "metric": "atoms/s SOAP power spectrum + position gradients, bulk cell",
+        "value": rate, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "one periodic cubic cell of 20000 atoms at 0.1 A^-3, 5 types, SOAP power spectrum + "
+                               "position gradients, rc 6.0, n 12, l<=8", "form": best, "atoms": n_atoms},
+        "roofline": hbm_roofline(k_ms, k_bytes, d_ms, "power-spectrum position-gradient rows"),
+        "cpu_baseline": baseline,
+        "forms": {k: {"value": v[1], "ms_per_step": v[0], "kernel_ms_per_step": v[2], "autotune": tuned[k]}
+                  for k, v in res.items()},
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def surrogate_main(args):
+    """Spherical expansion + position gradients of one 100 000-atom periodic cell
+    (4 types, rc 6, 12 radial functions, l <= 8, FP32) with the spline radial
+    integral and with the learned one (trained here on 1e6 distances)."""
+    import rascaline_amd as ra
+    from rascaline_amd.datasets import bulk_cell
+
+    systems = bulk_cell(seed=0, n_atoms=100000, density=0.1, types=(1, 6, 7, 8))
+    n_atoms = len(systems[0][1])
+    batch = device_batch(systems, [0])
+    grads = (ctypes.c_char_p * 1)(b"positions")
+    calc = ra.SphericalExpansion(**BULK_HYPERS)
+    calc.set_precision(32)
+    report = calc.train_radial_surrogate()
+    log(f"surrogate: accepted {report['accepted']}, value error {report['max_value_error']:.2e}, derivative error "
+        f"{report['max_derivative_error']:.2e}, {report['train_ms']:.0f} ms")
+    if not report["accepted"]:
+        raise RuntimeError("the radial network was not accepted by the spline judge")
+    res = {}
+    for name, use in (("spline", False), ("surrogate", True)):
+        calc.use_radial_surrogate(use)
+        res.update(time_forms([(name, calc)], batch, n_atoms, args, grads, "surrogate"))
+    ms, rate, k_ms, k_bytes, d_ms = res["surrogate"]
+    result = {
+        "metric": "atoms/s SOAP spherical expansion + position gradients, learned radial integral",
+        "value": rate, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (network fp16 hi+lo, fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": "one periodic cubic cell of 100000 atoms at 0.1 A^-3, 4 types, SOAP spherical "
+                               "expansion + position gradients, rc 6.0, n 12, l<=8, MLP 1-64-64-216",
+                   "atoms": n_atoms, "train_ms": report["train_ms"],
+                   "judged_errors": [report["max_value_error"], report["max_derivative_error"]]},
+        "roofline": hbm_roofline(k_ms, k_bytes, d_ms, "spherical-expansion gradient rows"),
+        "cpu_baseline": None,
+        "forms": {k: {"value": v[1], "ms_per_step": v[0], "kernel_ms_per_step": v[2]} for k, v in res.items()},
     }
     print(json.dumps(result), flush=True)
     return 0

@@ -63,7 +63,11 @@ typedef struct featomic_amd_device_systems_t {
  * 3 * properties values starting at gradients + gradient_offsets[b]. For the
  * power spectrum, `properties` is the same for every block; for the spherical
  * expansion one block per (o3_lambda, center_type, neighbor_type) key is
- * reported and `properties` is (2 o3_lambda + 1) * n for that block. */
+ * reported and `properties` is (2 o3_lambda + 1) * n for that block. Block b's
+ * labels are the n_samples[b] sample labels from sample_label_offsets[b] and the
+ * n_gradient_rows[b] gradient labels from gradient_label_offsets[b]; the blocks
+ * of one spherical-expansion type pair share their labels. The LODE calculator
+ * leaves the labels NULL. */
 typedef struct featomic_amd_device_output_t {
     uintptr_t n_blocks;
     const int32_t* keys;               /* host, [n_blocks][key size] */
@@ -75,9 +79,11 @@ typedef struct featomic_amd_device_output_t {
     const int64_t* block_properties;   /* host, values per sample of each block */
     const void* values;                /* device, float32 or float64 */
     const void* gradients;             /* device, or NULL */
-    const int32_t* sample_labels;      /* device, [total samples][2] (system, atom) */
-    const int32_t* gradient_labels;    /* device, [total rows][3] (sample, system, atom) or NULL */
+    const int32_t* sample_labels;      /* device, [label samples][2] (system, atom), or NULL */
+    const int32_t* gradient_labels;    /* device, [label rows][3] (sample in block, system, atom), or NULL */
     int32_t precision;                 /* FEATOMIC_AMD_FLOAT32 or FEATOMIC_AMD_FLOAT64 */
+    const int64_t* sample_label_offsets;    /* host, first sample label of each block, or NULL */
+    const int64_t* gradient_label_offsets;  /* host, first gradient label of each block, or NULL */
 } featomic_amd_device_output_t;
 
 /* Compute the calculator on a device-resident batch. `gradients` may hold
@@ -97,6 +103,29 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
  * Requires a stream synchronisation, which this call performs. */
 featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
                                             int64_t* main_kernel_bytes);
+
+/* Autotuner of the SOAP launch shapes (no counterpart in the reference, whose CPU
+ * kernels have no launch shapes). On a device-resident batch, times every
+ * candidate window shape (runs, entries, rows per pass) of the tiled
+ * power-spectrum gradient kernel within the device's whole LDS, the
+ * register-blocked gradient kernel and the built-in choice, then 4 to 64
+ * neighbour entries per LDS chunk of the expansion kernel, each the best device
+ * time of `repeats` runs after a warm-up run. The fastest choices stay in the
+ * calculator for its following computations (also when `report` is too small);
+ * `report` (may be NULL) receives a JSON summary of every candidate, with
+ * FEATOMIC_BUFFER_SIZE_ERROR when `report_size` is too small. */
+featomic_status_t featomic_amd_calculator_autotune(featomic_calculator_t* calculator,
+                                                  const featomic_amd_device_systems_t* systems,
+                                                  const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                                  int32_t repeats, char* report, uintptr_t report_size);
+
+/* Launch shapes in use: `expansion_entries` 0 (built-in) or a power of two up to
+ * 64; `gradient_window` -1 (built-in), -2 (register-blocked kernel) or the index
+ * of a window shape of the autotuner's list (as its report numbers them). */
+featomic_status_t featomic_amd_calculator_set_tuning(featomic_calculator_t* calculator, int32_t expansion_entries,
+                                                     int32_t gradient_window);
+featomic_status_t featomic_amd_calculator_get_tuning(featomic_calculator_t* calculator, int32_t* expansion_entries,
+                                                     int32_t* gradient_window);
 
 /* ---- learned radial integral
  *

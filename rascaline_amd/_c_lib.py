@@ -6,6 +6,7 @@ with ``python -m rascaline_amd.build``.
 
 import ctypes
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(HERE, "librascaline_amd.so")
@@ -118,6 +119,8 @@ class featomic_amd_device_output_t(ctypes.Structure):
         ("sample_labels", ctypes.c_void_p),
         ("gradient_labels", ctypes.c_void_p),
         ("precision", ctypes.c_int32),
+        ("sample_label_offsets", ctypes.POINTER(ctypes.c_int64)),
+        ("gradient_label_offsets", ctypes.POINTER(ctypes.c_int64)),
     ]
 
 
@@ -179,6 +182,12 @@ PROTOTYPES = [
      [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.POINTER(featomic_pair_t)),
       ctypes.POINTER(ctypes.c_size_t)]),
     ("featomic_amd_free", None, [ctypes.c_void_p]),
+    ("featomic_amd_calculator_autotune", featomic_status_t,
+     [ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.POINTER(ctypes.c_char_p), ctypes.c_size_t,
+      ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t]),
+    ("featomic_amd_calculator_set_tuning", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("featomic_amd_calculator_get_tuning", featomic_status_t,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     ("featomic_amd_compute_device", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.POINTER(ctypes.c_char_p), ctypes.c_size_t,
       ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_output_t)]),
@@ -214,6 +223,12 @@ def lib():
             raise RuntimeError(
                 f"{LIBRARY_PATH} is missing: build it with `python -m rascaline_amd.build` (HIP, gfx950)"
             )
+        # torch ships its own HIP runtime, which finds no GPU when this library's
+        # runtime has initialised the device first: an imported torch sets up its
+        # GPU context before the library is loaded
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         L = ctypes.CDLL(LIBRARY_PATH)
         for name, restype, argtypes in PROTOTYPES:
             fn = getattr(L, name)

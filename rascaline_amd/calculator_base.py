@@ -161,6 +161,33 @@ class CalculatorBase:
         """Use the accepted learned radial integral (True) or the spline / exact one."""
         C.check(self._lib.featomic_amd_calculator_use_radial_surrogate(self._as_parameter_, 1 if enabled else 0))
 
+    def autotune(self, systems, gradients=("positions",), repeats=3, stream=None) -> dict:
+        """Time the candidate launch shapes of the SOAP kernels on device-resident
+        systems (a ``rascaline_amd.device.DeviceSystems``), keep the fastest for the
+        following computations and return the timings of every candidate."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        names = [g.encode("utf8") for g in gradients]
+        arr = (ctypes.c_char_p * max(len(names), 1))(*names)
+        buf = ctypes.create_string_buffer(1 << 16)
+        C.check(self._lib.featomic_amd_calculator_autotune(self._as_parameter_, ctypes.byref(systems.struct), arr,
+                                                           len(names), ctypes.c_void_p(s.cuda_stream), int(repeats),
+                                                           buf, len(buf)))
+        return json.loads(buf.value.decode("utf8"))
+
+    @property
+    def tuning(self) -> dict:
+        """Launch shapes in use: {"expansion_entries": 0 or 2^k, "gradient_window": -2, -1 or k}."""
+        e, w = ctypes.c_int32(), ctypes.c_int32()
+        C.check(self._lib.featomic_amd_calculator_get_tuning(self._as_parameter_, ctypes.byref(e), ctypes.byref(w)))
+        return {"expansion_entries": e.value, "gradient_window": w.value}
+
+    @tuning.setter
+    def tuning(self, value: dict):
+        C.check(self._lib.featomic_amd_calculator_set_tuning(self._as_parameter_, int(value.get("expansion_entries", 0)),
+                                                             int(value.get("gradient_window", -1))))
+
     def radial_surrogate_parameters(self) -> np.ndarray:
         """Flat FP32 parameters of the trained network, then its output scales."""
         required = ctypes.c_int64()

@@ -403,6 +403,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
     const int lds = device_lds_limit();
     const size_t per_entry = soap_chunk_bytes<T>(L, n, 1);
+    const SoapTuning tune = tuning;
     auto chunk_for = [&](size_t fixed) -> int {
         const long long budget = static_cast<long long>(lds) - static_cast<long long>(fixed) - 1024;
         const long long c = budget / static_cast<long long>(per_entry);
@@ -430,7 +431,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     if (tiled) {
         entry_data = ws_.get<T>("entry_data", std::max<size_t>(entry_prep_size<T>(lp, nl.n_entries), 1));
         launch_entry_prep<T>(lp, nl.n_entries, entry_data);
-        int max_entries = RA_EXPANSION_ENTRIES;
+        int max_entries = tune.expansion_entries > 0 ? tune.expansion_entries : RA_EXPANSION_ENTRIES;
         while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(std::min(lds, 65536))) {
             max_entries /= 2;
         }
@@ -514,14 +515,23 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     if (grads.positions) {
         bool done = false;
-        if (ps && total_rows < (1ll << 31)) {
+        if (ps && total_rows < (1ll << 31) && tune.gradient_window != -2) {
             // planned, windowed tiled kernel: the first (runs, entries, staged rows)
             // configuration that keeps the workgroup within the LDS budget (4
-            // workgroups per CU in FP32, 2 in FP64)
-            static const int windows[][3] = RA_TILED_WINDOWS;
-            const size_t cap = sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920;
-            const size_t budget = std::min<size_t>(lds > 1024 ? lds - 1024 : 0, cap);
-            for (const auto& w : windows) {
+            // workgroups per CU in FP32, 2 in FP64), else the first within the whole
+            // LDS (large bases: one or two workgroups per CU, still well ahead of the
+            // register-blocked kernel), or the tuned shape within the whole LDS
+            static const int default_windows[][3] = RA_TILED_WINDOWS;
+            const size_t whole = lds > 1024 ? static_cast<size_t>(lds) - 1024 : 0;
+            std::vector<std::pair<std::array<int, 3>, size_t>> windows;
+            if (tune.gradient_window >= 0 && tune.gradient_window < static_cast<int>(soap_window_shapes().size())) {
+                windows.push_back({soap_window_shapes()[tune.gradient_window], whole});
+            } else {
+                const size_t cap = std::min<size_t>(whole, sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920);
+                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, cap});
+                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, whole});
+            }
+            for (const auto& [w, budget] : windows) {
                 if (!ps_gradient_tiled_supported(L, n, S, w[0], w[1], w[2])) continue;
                 if (ps_gradient_tiled_shmem<T>(lp, w[0], w[1], w[2]) > budget) continue;
                 const long long ne = std::max<long long>(nl.n_entries, 1);
@@ -590,7 +600,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
     // ---- describe the blocks
     std::vector<int32_t> keys;
-    std::vector<int64_t> out_ns, out_nr, out_voff, out_goff, out_props;
+    std::vector<int64_t> out_ns, out_nr, out_voff, out_goff, out_props, out_slab, out_glab;
     struct BlockRef {
         int b, l;
     };
@@ -607,6 +617,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             out_voff.push_back(block_sample_base[b] * P);
             out_goff.push_back(block_row_base[b] * 3 * P);
             out_props.push_back(P);
+            out_slab.push_back(block_sample_base[b]);
+            out_glab.push_back(block_row_base[b]);
             refs.push_back({b, 0});
         } else {
             for (int l = 0; l < L; l++) {
@@ -616,6 +628,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 out_voff.push_back(value_base[b * L + l]);
                 out_goff.push_back(grad_base[b * L + l]);
                 out_props.push_back((2LL * l + 1) * n);
+                // the L blocks of one (center type, neighbour type) pair share its labels
+                out_slab.push_back(block_sample_base[b]);
+                out_glab.push_back(block_row_base[b]);
                 refs.push_back({b, l});
             }
         }
@@ -629,6 +644,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         device_result_.value_offsets = out_voff;
         device_result_.gradient_offsets = out_goff;
         device_result_.block_properties = out_props;
+        device_result_.sample_label_offsets = out_slab;
+        device_result_.gradient_label_offsets = out_glab;
         output->n_blocks = refs.size();
         output->keys = device_result_.keys.data();
         output->key_size = device_result_.key_size;
@@ -641,6 +658,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         output->gradients = d_grads;
         output->sample_labels = sample_labels;
         output->gradient_labels = grad_labels;
+        output->sample_label_offsets = device_result_.sample_label_offsets.data();
+        output->gradient_label_offsets = grad_labels ? device_result_.gradient_label_offsets.data() : nullptr;
         output->precision = sizeof(T) == 4 ? 32 : 64;
     }
     if (!to_host) return nullptr;
@@ -755,6 +774,106 @@ void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, 
     } else {
         run_soap<double>(batch, grads, stream, false, output);
     }
+}
+
+const std::vector<std::array<int, 3>>& soap_window_shapes() {
+    static const std::vector<std::array<int, 3>> shapes = {
+        {4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1},
+        {8, 16, 1}, {4, 32, 1}, {2, 16, 1}, {4, 16, 2}, {4, 4, 1}, {8, 32, 1},
+    };
+    return shapes;
+}
+
+std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads, hipStream_t stream, int repeats) {
+    if (kind_ == Kind::LodeSphericalExpansion) throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
+    if (repeats < 1 || repeats > 100) throw Error::invalid_parameter("autotune repeats must be in [1, 100]");
+    featomic_amd_device_output_t out{};
+    // best device time of `repeats` runs after one warm-up run (which also sizes the workspace)
+    auto measure = [&]() -> double {
+        double best = 1e300;
+        for (int r = 0; r <= repeats; r++) {
+            compute_device(batch, grads, stream, &out);
+            double total = 0.0;
+            last_timings(&total, nullptr, nullptr);
+            if (r > 0) best = std::min(best, total);
+        }
+        return best;
+    };
+    const SoapTuning saved = tuning;
+    SoapTuning best = saved;
+    std::string report = "{\"lds_bytes\": " + std::to_string(device_lds_limit());
+    auto fmt = [](double v) {
+        char buf[32];
+        std::snprintf(buf, sizeof(buf), "%.4f", v);
+        return std::string(buf);
+    };
+    try {
+        // 1. power-spectrum gradient rows: the built-in choice, every window shape within
+        //    the whole LDS, the register-blocked kernel
+        if (kind_ == Kind::PowerSpectrum && grads.positions) {
+            // the fast form keeps its mixed-precision contraction: only the shapes the
+            // mixed kernel covers (n = 8: windows of up to 4 runs; n = 12, 16: all)
+            const int n = soap_.n_radial;
+            const bool mixed = fast_form != 0 && precision == 32 && (n == 8 || n == 12 || n == 16);
+            std::vector<int> candidates = {-1};
+            for (int k = 0; k < static_cast<int>(soap_window_shapes().size()); k++) {
+                if (mixed && n == 8 && soap_window_shapes()[k][0] > 4) continue;
+                candidates.push_back(k);
+            }
+            if (!mixed) candidates.push_back(-2);
+            double best_ms = 1e300;
+            report += ", \"gradient_window\": [";
+            bool first = true;
+            for (int c : candidates) {
+                tuning = saved;
+                tuning.gradient_window = c;
+                std::string entry = "{\"choice\": " + std::to_string(c);
+                if (c >= 0) {
+                    const auto& w = soap_window_shapes()[c];
+                    entry += ", \"shape\": [" + std::to_string(w[0]) + ", " + std::to_string(w[1]) + ", " +
+                             std::to_string(w[2]) + "]";
+                }
+                try {
+                    const double ms = measure();
+                    entry += ", \"ms\": " + fmt(ms);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.gradient_window = c;
+                    }
+                } catch (const std::exception& e) {
+                    // a shape the device cannot launch (e.g. beyond its LDS) is skipped
+                    (void)hipGetLastError();
+                    entry += ", \"error\": \"launch failed\"";
+                }
+                report += (first ? "" : ", ") + entry + "}";
+                first = false;
+            }
+            report += "]";
+        }
+        // 2. neighbour entries per LDS chunk of the expansion kernel
+        report += ", \"expansion_entries\": [";
+        double best_ms = 1e300;
+        bool first = true;
+        for (int e : {4, 8, 16, 32, 64}) {
+            tuning = best;
+            tuning.expansion_entries = e;
+            const double ms = measure();
+            report += (first ? "" : ", ") + std::string("{\"entries\": ") + std::to_string(e) + ", \"ms\": " + fmt(ms) + "}";
+            first = false;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.expansion_entries = e;
+            }
+        }
+        report += "]";
+    } catch (...) {
+        tuning = saved;
+        throw;
+    }
+    tuning = best;
+    report += ", \"chosen\": {\"gradient_window\": " + std::to_string(best.gradient_window) +
+              ", \"expansion_entries\": " + std::to_string(best.expansion_entries) + "}}";
+    return report;
 }
 
 void Calculator::last_timings(double* total_ms, double* main_ms, int64_t* main_bytes) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
@@ -56,6 +57,21 @@ struct HostBatch {
 
 enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion };
 
+// Launch shapes of the SOAP kernels that the autotuner chooses (0 / -1: the
+// built-in choice).
+struct SoapTuning {
+    // neighbour entries per LDS chunk of the expansion kernel (power of two <= 64)
+    int expansion_entries = 0;
+    // power-spectrum gradient rows: -1 the first window shape of the built-in table
+    // that fits the default LDS budget; k >= 0 window shape k of
+    // soap_window_shapes() within the whole LDS; -2 the register-blocked kernel
+    int gradient_window = -1;
+};
+
+// (runs, entries, rows per pass) window shapes of the tiled power-spectrum
+// gradient kernel that the autotuner tries
+const std::vector<std::array<int, 3>>& soap_window_shapes();
+
 struct Gradients {
     bool positions = false, strain = false, cell = false;
 };
@@ -83,6 +99,12 @@ public:
                         featomic_amd_device_output_t* output);
     void last_timings(double* total_ms, double* main_ms, int64_t* main_bytes);
 
+    SoapTuning tuning;
+    // times the candidate launch shapes on a device-resident batch (the best of
+    // `repeats` runs each, device time), keeps the fastest in `tuning` and
+    // returns a JSON report of every candidate
+    std::string autotune(const HostBatch& batch, const Gradients& gradients, hipStream_t stream, int repeats);
+
     // learned radial integral (SOAP calculators): trains a network on the exact
     // GTO integral, judges it against this calculator's radial integral and
     // uses it for the following computations when it is accepted
@@ -108,6 +130,7 @@ private:
         std::vector<int32_t> keys;
         size_t key_size = 0;
         std::vector<int64_t> n_samples, n_rows, value_offsets, gradient_offsets, block_properties;
+        std::vector<int64_t> sample_label_offsets, gradient_label_offsets;
         const void* values = nullptr;
         const void* gradients = nullptr;
         const int32_t* sample_labels = nullptr;

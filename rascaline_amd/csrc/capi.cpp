@@ -501,6 +501,58 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
     });
 }
 
+featomic_status_t featomic_amd_calculator_autotune(featomic_calculator_t* calculator,
+                                                  const featomic_amd_device_systems_t* systems,
+                                                  const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                                  int32_t repeats, char* report, uintptr_t report_size) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(systems, "systems");
+        check_pointer(systems->atom_offsets, "systems.atom_offsets");
+        check_pointer(systems->cells, "systems.cells");
+        auto& impl = *calculator->impl;
+        const auto grads = parse_gradients(gradients, gradients_count, impl.kind());
+        if (grads.strain || grads.cell) throw Error::invalid_parameter("the autotuner supports positions gradients only");
+        rascaline_amd::HostBatch batch;
+        batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
+        const int64_t n_atoms = batch.n_atoms();
+        if (n_atoms > 0) {
+            check_pointer(systems->positions, "systems.positions");
+            check_pointer(systems->types, "systems.types");
+        }
+        batch.device_positions = systems->positions;
+        batch.types.assign(systems->types, systems->types + n_atoms);
+        batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
+        const std::string text = impl.autotune(batch, grads, static_cast<hipStream_t>(stream), repeats);
+        if (report) copy_str(text, report, report_size);
+    });
+}
+
+featomic_status_t featomic_amd_calculator_set_tuning(featomic_calculator_t* calculator, int32_t expansion_entries,
+                                                     int32_t gradient_window) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        if (expansion_entries < 0 || expansion_entries > 64 || (expansion_entries & (expansion_entries - 1)) != 0) {
+            throw Error::invalid_parameter("expansion_entries must be 0 (built-in) or a power of two up to 64");
+        }
+        if (gradient_window < -2 || gradient_window >= static_cast<int32_t>(rascaline_amd::soap_window_shapes().size())) {
+            throw Error::invalid_parameter("gradient_window must be -2, -1 or a window shape index below " +
+                                           std::to_string(rascaline_amd::soap_window_shapes().size()));
+        }
+        calculator->impl->tuning.expansion_entries = expansion_entries;
+        calculator->impl->tuning.gradient_window = gradient_window;
+    });
+}
+
+featomic_status_t featomic_amd_calculator_get_tuning(featomic_calculator_t* calculator, int32_t* expansion_entries,
+                                                     int32_t* gradient_window) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        if (expansion_entries) *expansion_entries = calculator->impl->tuning.expansion_entries;
+        if (gradient_window) *gradient_window = calculator->impl->tuning.gradient_window;
+    });
+}
+
 featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
                                             int64_t* main_kernel_bytes) {
     return guard([&] {

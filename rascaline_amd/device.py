@@ -1,0 +1,126 @@
+"""Device-resident inputs and outputs of ``featomic_amd_compute_device``.
+
+`DeviceSystems` holds a batch of systems whose positions live in GPU memory;
+`compute_device` runs a calculator on it and returns a `DeviceResult` whose
+blocks are torch tensors viewing the calculator's device buffers (no copy).
+The views stay valid until the next computation with the same calculator or its
+release; clone them to keep them longer.
+
+The layout is the one of the host TensorMap (featomic's output contract): one
+block per key, values [samples][properties], position gradients
+[rows][3][properties], sample labels (system, atom) and gradient labels
+(sample within the block, system, atom).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _c_lib as C
+
+
+class DeviceSystems:
+    """Systems given as (positions [n][3], types [n], cell [3][3]) tuples, with the
+    positions copied once to device memory (float64, torch allocation)."""
+
+    def __init__(self, systems, device="cuda:0"):
+        import torch
+
+        pos = np.concatenate([np.asarray(s[0], dtype=np.float64).reshape(-1, 3) for s in systems])
+        self.n_atoms = len(pos)
+        self.positions = torch.from_numpy(pos).to(device)
+        self.types = np.ascontiguousarray(np.concatenate([np.asarray(s[1]) for s in systems]).astype(np.int32))
+        self.cells = np.ascontiguousarray(np.stack([np.asarray(s[2], dtype=np.float64).reshape(9) for s in systems]))
+        self.offsets = np.zeros(len(systems) + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum([len(s[1]) for s in systems])
+        self.struct = C.featomic_amd_device_systems_t()
+        self.struct.n_systems = len(systems)
+        self.struct.atom_offsets = self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        self.struct.positions = ctypes.c_void_p(self.positions.data_ptr())
+        self.struct.types = self.types.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self.struct.cells = self.cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class _DeviceArray:
+    """A device pointer described through the CUDA array interface (version 3),
+    which torch.as_tensor accepts without copying."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": tuple(int(s) for s in shape), "typestr": typestr,
+                                         "data": (int(ptr), False), "version": 3, "strides": None}
+
+
+def _view(ptr, shape, typestr, device):
+    import torch
+
+    if ptr is None or ptr == 0 or int(np.prod(shape)) == 0:
+        dtype = {"<f4": torch.float32, "<f8": torch.float64, "<i4": torch.int32}[typestr]
+        return torch.empty(shape, dtype=dtype, device=device)
+    return torch.as_tensor(_DeviceArray(ptr, shape, typestr), device=device)
+
+
+class DeviceResult:
+    """Blocks of one device computation as torch views (see the module docstring)."""
+
+    def __init__(self, out, device="cuda:0"):
+        self.n_blocks = int(out.n_blocks)
+        ks = int(out.key_size)
+        self.keys = np.array([[out.keys[b * ks + k] for k in range(ks)] for b in range(self.n_blocks)], dtype=np.int32)
+        self.n_samples = [int(out.n_samples[b]) for b in range(self.n_blocks)]
+        self.n_gradient_rows = [int(out.n_gradient_rows[b]) for b in range(self.n_blocks)]
+        self.properties = [int(out.block_properties[b]) for b in range(self.n_blocks)]
+        self.precision = int(out.precision)
+        self._typestr = "<f4" if self.precision == 32 else "<f8"
+        self._item = 4 if self.precision == 32 else 8
+        self._out = out
+        self._device = device
+        self._slab = [int(out.sample_label_offsets[b]) for b in range(self.n_blocks)] \
+            if out.sample_label_offsets else None
+        self._glab = [int(out.gradient_label_offsets[b]) for b in range(self.n_blocks)] \
+            if out.gradient_label_offsets else None
+
+    def values(self, b):
+        off = int(self._out.value_offsets[b])
+        ptr = self._out.values + off * self._item if self._out.values else 0
+        return _view(ptr, (self.n_samples[b], self.properties[b]), self._typestr, self._device)
+
+    def gradients(self, b):
+        if not self._out.gradients:
+            raise ValueError("no position gradients were computed")
+        off = int(self._out.gradient_offsets[b])
+        return _view(self._out.gradients + off * self._item, (self.n_gradient_rows[b], 3, self.properties[b]),
+                     self._typestr, self._device)
+
+    def sample_labels(self, b):
+        """[samples][2] (system, atom) of block b, or None when the calculator does
+        not produce labels."""
+        if not self._out.sample_labels or self._slab is None:
+            return None
+        return _view(self._out.sample_labels + 8 * self._slab[b], (self.n_samples[b], 2), "<i4", self._device)
+
+    def gradient_labels(self, b):
+        """[rows][3] (sample within the block, system, atom) of block b, or None when
+        the calculator does not produce labels."""
+        if not self._out.gradient_labels or self._glab is None:
+            return None
+        return _view(self._out.gradient_labels + 12 * self._glab[b], (self.n_gradient_rows[b], 3), "<i4",
+                     self._device)
+
+    @property
+    def total_rows(self):
+        return sum(self.n_gradient_rows)
+
+
+def compute_device(calculator, systems, gradients=(), stream=None):
+    """Run `calculator` on the device-resident `systems` (a DeviceSystems); the
+    work is queued on `stream` (a torch.cuda.Stream, or the current stream) and
+    the result's tensors are ready once that stream has reached this point."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    names = [g.encode("utf8") for g in gradients]
+    arr = (ctypes.c_char_p * max(len(names), 1))(*names)
+    out = C.featomic_amd_device_output_t()
+    C.check(C.lib().featomic_amd_compute_device(calculator._as_parameter_, ctypes.byref(systems.struct), arr,
+                                                len(names), ctypes.c_void_p(s.cuda_stream), ctypes.byref(out)))
+    return DeviceResult(out)

@@ -20,3 +20,16 @@ def pytest_collection_modifyitems(config, items):
     for item in items:
         if "gpu" in item.keywords:
             item.add_marker(skip)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_gpu_context_first(request):
+    """torch ships its own HIP runtime; when the library's runtime initialises the GPU
+    first, torch's later initialisation finds no GPU. Tests that use torch device
+    tensors therefore get torch's GPU context set up before any library call."""
+    if os.path.exists("/dev/kfd") and any("gpu" in item.keywords for item in request.session.items):
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield

@@ -200,3 +200,23 @@ def test_radial_surrogate_host_checks():
     assert C.lib().featomic_amd_mlp_loss_gradient(0, 1, ptr, ptr, ptr, ctypes.byref(loss), ptr) \
         == FEATOMIC_INVALID_PARAMETER_ERROR
     assert C.lib().featomic_amd_mlp_forward(513, 1, ptr, ptr, ptr) == FEATOMIC_INVALID_PARAMETER_ERROR
+
+
+def test_tuning_host_checks():
+    calc = make("soap_power_spectrum", SOAP)
+    try:
+        e, w = ctypes.c_int32(-5), ctypes.c_int32(-5)
+        assert C.lib().featomic_amd_calculator_get_tuning(calc, ctypes.byref(e), ctypes.byref(w)) == FEATOMIC_SUCCESS
+        assert (e.value, w.value) == (0, -1)
+        assert C.lib().featomic_amd_calculator_set_tuning(calc, 16, 3) == FEATOMIC_SUCCESS
+        C.lib().featomic_amd_calculator_get_tuning(calc, ctypes.byref(e), ctypes.byref(w))
+        assert (e.value, w.value) == (16, 3)
+        for entries, window in [(12, -1), (128, -1), (-1, -1), (8, -3), (8, 1000)]:
+            assert C.lib().featomic_amd_calculator_set_tuning(calc, entries, window) == FEATOMIC_INVALID_PARAMETER_ERROR
+        # the rejected values left the last accepted ones in place
+        C.lib().featomic_amd_calculator_get_tuning(calc, ctypes.byref(e), ctypes.byref(w))
+        assert (e.value, w.value) == (16, 3)
+        assert C.lib().featomic_amd_calculator_autotune(calc, None, None, 0, None, 3, None, 0) \
+            == FEATOMIC_INVALID_PARAMETER_ERROR
+    finally:
+        C.lib().featomic_calculator_free(calc)
