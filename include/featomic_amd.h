@@ -107,8 +107,8 @@ featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, d
 /* Autotuner of the SOAP launch shapes (no counterpart in the reference, whose CPU
  * kernels have no launch shapes). On a device-resident batch, times every
  * candidate window shape (runs, entries, rows per pass) of the tiled
- * power-spectrum gradient kernel within the device's whole LDS, the
- * register-blocked gradient kernel and the built-in choice, then 4 to 64
+ * gradient-row kernel within the device's whole LDS, the untiled gradient
+ * kernel and the built-in choice, then 4 to 64
  * neighbour entries per LDS chunk of the expansion kernel, each the best device
  * time of `repeats` runs after a warm-up run. The fastest choices stay in the
  * calculator for its following computations (also when `report` is too small);
@@ -120,7 +120,7 @@ featomic_status_t featomic_amd_calculator_autotune(featomic_calculator_t* calcul
                                                   int32_t repeats, char* report, uintptr_t report_size);
 
 /* Launch shapes in use: `expansion_entries` 0 (built-in) or a power of two up to
- * 64; `gradient_window` -1 (built-in), -2 (register-blocked kernel) or the index
+ * 64; `gradient_window` -1 (built-in), -2 (untiled gradient kernel) or the index
  * of a window shape of the autotuner's list (as its report numbers them). */
 featomic_status_t featomic_amd_calculator_set_tuning(featomic_calculator_t* calculator, int32_t expansion_entries,
                                                      int32_t gradient_window);

@@ -515,7 +515,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     if (grads.positions) {
         bool done = false;
-        if (ps && total_rows < (1ll << 31) && tune.gradient_window != -2) {
+        if (total_rows < (1ll << 31) && tune.gradient_window != -2 && entry_data) {
             // planned, windowed tiled kernel: the first (runs, entries, staged rows)
             // configuration that keeps the workgroup within the LDS budget (4
             // workgroups per CU in FP32, 2 in FP64), else the first within the whole
@@ -533,23 +533,31 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             }
             for (const auto& [w, budget] : windows) {
                 if (!ps_gradient_tiled_supported(L, n, S, w[0], w[1], w[2])) continue;
-                if (ps_gradient_tiled_shmem<T>(lp, w[0], w[1], w[2]) > budget) continue;
+                // the spherical expansion's rows are the stage-2 sums themselves: no
+                // coefficients or store staging in LDS, one staged row per pass
+                if (!ps && w[2] != 1) continue;
+                const size_t shmem = ps ? ps_gradient_tiled_shmem<T>(lp, w[0], w[1], w[2])
+                                        : se_gradient_tiled_shmem<T>(lp, w[0], w[1]);
+                if (shmem > budget) continue;
                 const long long ne = std::max<long long>(nl.n_entries, 1);
-                const int npairs = S * (S + 1) / 2;
+                const int nloc = ps ? S * (S + 1) / 2 : S;
                 int* d_runs = ws_.get<int>("plan_runs", ne * 4);
                 int* d_run_rows = ws_.get<int>("plan_run_rows", ne * S);
                 int* d_windows = ws_.get<int>("plan_windows", ne * 4);
                 int* d_n_windows = ws_.get<int>("plan_n_windows", std::max<long long>(n_atoms, 1));
-                int* d_self_rows = ws_.get<int>("plan_self_rows", std::max<long long>(n_atoms, 1) * npairs);
-                if (!entry_data) continue;
-                T* d_entry_data = entry_data;
+                int* d_self_rows = ws_.get<int>("plan_self_rows", std::max<long long>(n_atoms, 1) * nloc);
                 launch_ps_grad_plan(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, sample_scan,
                                     row_scan, d_block_sample_base, d_atom_system, d_atom_local, w[0], w[1], d_runs,
-                                    d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels);
+                                    d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels, ps);
                 RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
                 timed_start = true;
-                launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows, d_windows,
-                                            d_n_windows, d_self_rows, d_entry_data, d_grads, fast_form != 0);
+                if (ps) {
+                    launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows,
+                                                d_windows, d_n_windows, d_self_rows, entry_data, d_grads, fast_form != 0);
+                } else {
+                    launch_se_gradient_tiled<T>(lp, w[0], w[1], dself, type_mask, d_runs, d_run_rows, d_windows,
+                                                d_n_windows, d_self_rows, d_se_grad, entry_data, d_grads);
+                }
                 done = true;
                 break;
             }
@@ -808,16 +816,19 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
         return std::string(buf);
     };
     try {
-        // 1. power-spectrum gradient rows: the built-in choice, every window shape within
-        //    the whole LDS, the register-blocked kernel
-        if (kind_ == Kind::PowerSpectrum && grads.positions) {
+        // 1. gradient rows: the built-in choice, every window shape of the tiled kernels
+        //    within the whole LDS, the untiled kernel
+        if (grads.positions) {
             // the fast form keeps its mixed-precision contraction: only the shapes the
             // mixed kernel covers (n = 8: windows of up to 4 runs; n = 12, 16: all)
             const int n = soap_.n_radial;
-            const bool mixed = fast_form != 0 && precision == 32 && (n == 8 || n == 12 || n == 16);
+            const bool mixed = kind_ == Kind::PowerSpectrum && fast_form != 0 && precision == 32 &&
+                               (n == 8 || n == 12 || n == 16);
             std::vector<int> candidates = {-1};
             for (int k = 0; k < static_cast<int>(soap_window_shapes().size()); k++) {
                 if (mixed && n == 8 && soap_window_shapes()[k][0] > 4) continue;
+                // the spherical-expansion rows leave without store staging
+                if (kind_ == Kind::SphericalExpansion && soap_window_shapes()[k][2] != 1) continue;
                 candidates.push_back(k);
             }
             if (!mixed) candidates.push_back(-2);

@@ -62,9 +62,11 @@ enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion };
 struct SoapTuning {
     // neighbour entries per LDS chunk of the expansion kernel (power of two <= 64)
     int expansion_entries = 0;
-    // power-spectrum gradient rows: -1 the first window shape of the built-in table
-    // that fits the default LDS budget; k >= 0 window shape k of
-    // soap_window_shapes() within the whole LDS; -2 the register-blocked kernel
+    // gradient rows: -1 the first window shape of the built-in table that fits the
+    // default LDS budget (else the whole LDS); k >= 0 window shape k of
+    // soap_window_shapes() within the whole LDS; -2 the untiled kernel
+    // (register-blocked for the power spectrum, one wave per center for the
+    // spherical expansion)
     int gradient_window = -1;
 };
 

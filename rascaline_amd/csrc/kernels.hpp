@@ -92,7 +92,16 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
                          const int* species, const unsigned* type_mask, const long long* sample_scan,
                          const long long* row_scan, const long long* block_sample_base, const int* atom_system,
                          const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
-                         int* n_windows, int* self_rows, int* labels);
+                         int* n_windows, int* self_rows, int* labels, bool power_spectrum);
+// spherical-expansion gradient rows from the same plan (power_spectrum = false
+// above: one row per run in run_rows[run * S], self rows at self_rows[i * S + t])
+template <typename T>
+size_t se_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries);
+template <typename T>
+void launch_se_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* dself,
+                              const unsigned* type_mask, const int* runs, const int* run_rows, const int* windows,
+                              const int* n_windows, const int* self_rows, const long long* se_grad_base,
+                              const T* entry_data, T* out);
 // Per-entry data (Legendre functions, direction, trigonometric and cutoff factors,
 // radial values) of the tiled gradient kernel: entry_prep_size values.
 template <typename T>

@@ -1042,6 +1042,7 @@ __device__ __forceinline__ void staged_tile_store(T* __restrict__ out, long long
 // = {first entry, entry count, first run, run count}, the center's own rows to
 // self_rows[i * npairs + block], and every row's (sample, system, atom) label to
 // labels. Row indices are global over the batch (from row_scan).
+template <bool PS>
 __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     int n_atoms, int S, const long long* __restrict__ offsets, const int* __restrict__ entry_j,
     const int* __restrict__ species, const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_scan,
@@ -1051,11 +1052,14 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     int* __restrict__ self_rows, int* __restrict__ labels) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int present[TILE_MAX_TYPES];
+    // local blocks of a center: (t1 <= t2) type pairs for the power spectrum,
+    // neighbour types for the spherical expansion
     const int npairs = S * (S + 1) / 2;
+    const int nloc = PS ? npairs : S;
     long long* rowbase = reinterpret_cast<long long*>(smem);
-    int* sample_in_block = reinterpret_cast<int*>(rowbase + npairs);
-    int* counters = sample_in_block + npairs;
-    int* snapshot = counters + npairs;
+    int* sample_in_block = reinterpret_cast<int*>(rowbase + nloc);
+    int* counters = sample_in_block + nloc;
+    int* snapshot = counters + nloc;
 
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
@@ -1065,10 +1069,10 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     const unsigned mask = type_mask[i];
     const int n_present = __popc(mask);
     const int sys = atom_system[i];
-    for (int k = lane; k < npairs; k += WAVE) {
-        const long long idx = (long long)(ct * npairs + k) * n_atoms + i;
+    for (int k = lane; k < nloc; k += WAVE) {
+        const long long idx = (long long)(ct * nloc + k) * n_atoms + i;
         rowbase[k] = row_scan[idx];
-        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * npairs + k]);
+        sample_in_block[k] = (int)(sample_scan[idx] - block_sample_base[ct * nloc + k]);
         counters[k] = 0;
     }
     if (lane == 0) {
@@ -1098,11 +1102,21 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
         const int tj = species[j];
         if (lane == 0) runs[e0 + r] = make_int4(first, count, tj, 0);
         if (!snapped && j > i) {
-            for (int k = lane; k < npairs; k += WAVE) snapshot[k] = counters[k];
+            for (int k = lane; k < nloc; k += WAVE) snapshot[k] = counters[k];
             snapped = true;
         }
         __builtin_amdgcn_wave_barrier();
-        if (lane < n_present) {
+        if (!PS) {
+            // one row, in block (center type, tj)
+            if (lane == 0) {
+                const long long row = rowbase[tj] + counters[tj] + (i < j ? 1 : 0);
+                counters[tj] += 1;
+                run_rows[(e0 + r) * S] = (int)row;
+                labels[3 * row] = sample_in_block[tj];
+                labels[3 * row + 1] = sys;
+                labels[3 * row + 2] = atom_local[j];
+            }
+        } else if (lane < n_present) {
             const int t = present[lane];
             const int lo = t < tj ? t : tj, hi = t < tj ? tj : t;
             const int local = pair_index(lo, hi, S);
@@ -1163,9 +1177,20 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
 
     // the center's own rows: before its first neighbour j > i in every block
     if (!snapped) {
-        for (int k = lane; k < npairs; k += WAVE) snapshot[k] = counters[k];
+        for (int k = lane; k < nloc; k += WAVE) snapshot[k] = counters[k];
     }
     __builtin_amdgcn_wave_barrier();
+    if (!PS) {
+        for (int p = lane; p < n_present; p += WAVE) {
+            const int nt = present[p];
+            const long long row = rowbase[nt] + snapshot[nt];
+            self_rows[(long long)i * S + nt] = (int)row;
+            labels[3 * row] = sample_in_block[nt];
+            labels[3 * row + 1] = sys;
+            labels[3 * row + 2] = atom_local[i];
+        }
+        return;
+    }
     for (int p = lane; p < n_present * n_present; p += WAVE) {
         const int k1 = p / n_present, k2 = p - k1 * n_present;
         if (k2 < k1) continue;
@@ -1735,6 +1760,141 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
         } else if (tile >= 0) {
             tile_store<T, NC, RP>(out + tile, q, n, s, acc);
+        }
+    }
+}
+
+// Spherical-expansion position gradients from the planned windows: the same
+// per-center workgroup, window staging and stage-2 sums as the power-spectrum
+// kernel, DC_run = d c_i / d r_j for each neighbour atom j (images summed), whose
+// rows are the output itself: in block (l, center type, type of j), row (i, j)
+// holds [3][2l+1][n] (soap/spherical_expansion.rs:426-525). A run's rows for one
+// l are 3 (2l+1) n contiguous values, written in 4-value pieces by consecutive
+// lanes. The center's own rows copy its self gradients.
+template <typename T, int NC>
+__global__ void __launch_bounds__(TILE_THREADS) se_gradient_tiled_kernel(
+    SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ dself, const unsigned* __restrict__ type_mask,
+    const int4* __restrict__ runs, const int* __restrict__ run_rows, const int4* __restrict__ windows,
+    const int* __restrict__ n_windows, const int* __restrict__ self_rows, const long long* __restrict__ se_grad_base,
+    const T* __restrict__ entry_data, T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int run_lo[TILE_MAX_RUNS], run_hi[TILE_MAX_RUNS], run_complete[TILE_MAX_RUNS];
+    __shared__ long long run_base[TILE_MAX_RUNS];
+    __shared__ int run_type[TILE_MAX_RUNS];
+    __shared__ int carry_flag;
+    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L;
+    const int CS = L2 * NC;
+    const TileEntryLayout lay(L, n, NC);
+    T* DC = reinterpret_cast<T*>(smem);
+    T* ent = DC + (size_t)max_runs * 3 * CS;
+    const long long e0 = a.offsets[i];
+    const int ct = a.species[i];
+    const unsigned mask = type_mask[i];
+    // values per run: 3 L² n, as 4-value pieces when n fills the padded width
+    const bool quads = n == NC;
+    const int per_run = quads ? 3 * CS / 4 : 3 * L2 * n;
+    const int unit = quads ? 3 * n / 4 : 3 * n;  // pieces per (l, m) over the 3 directions
+
+    for (int k = tid; k < lay.leg; k += TILE_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        leg_l[k] = l;
+        leg_m[k] = k - l * (l + 1) / 2;
+    }
+    lds_barrier();
+
+    // piece k of a run's (or the center's) rows -> (l, x, offset within the (2l+1) n values)
+    auto decode = [&](int k, int& l, int& x, int& off) {
+        const int u = k / unit;  // in [l², (l + 1)²)
+        l = (int)sqrtf((float)u);
+        while (l * l > u) l--;
+        while ((l + 1) * (l + 1) <= u) l++;
+        const int q = (2 * l + 1) * (quads ? n / 4 : n);  // pieces per direction
+        const int rem = k - l * l * unit;
+        x = rem / q;
+        off = (rem - x * q) * (quads ? 4 : 1);
+    };
+
+    const int nw = n_windows[i];
+    for (int w = 0; w < nw; w++) {
+        const int4 wd = windows[e0 + w];
+        const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
+        if (tid < rc) {
+            const int4 ri = runs[e0 + rs + tid];
+            run_lo[tid] = max(ri.x, es) - es;
+            run_hi[tid] = min(ri.x + ri.y, es + ec) - es;
+            run_complete[tid] = ri.x + ri.y <= es + ec ? 1 : 0;
+            run_type[tid] = ri.z;
+            run_base[tid] = run_rows[(e0 + rs + tid) * S];
+            if (tid == 0) carry_flag = ri.x < es ? 1 : 0;
+        }
+        {
+            typedef typename Vec4Of<T>::type V;
+            constexpr int VEC = 16 / sizeof(T);
+            const V* src = reinterpret_cast<const V*>(entry_data + (e0 + es) * lay.per_entry);
+            V* dst = reinterpret_cast<V*>(ent);
+            const int count = ec * lay.per_entry / VEC;
+            for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
+        }
+        lds_barrier();
+        plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
+        lds_barrier();
+
+        const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
+        for (int t = tid; t < nc_runs * per_run; t += TILE_THREADS) {
+            const int r = t / per_run, k = t - r * per_run;
+            int l, x, off;
+            decode(k, l, x, off);
+            const int wl = (2 * l + 1) * n;
+            const long long tb = (long long)ct * S + run_type[r];
+            T* dst = out + se_grad_base[tb * L + l] + run_base[r] * 3 * wl + (long long)x * wl + off;
+            const T* src = DC + ((size_t)r * 3 + x) * CS + l * l * NC;
+            if (quads) {
+                T v[4];
+                load4<T>(src + off, v);
+                store4<T>(dst, v[0], v[1], v[2], v[3]);
+            } else {
+                const int lm = off / n, c = off - lm * n;
+                *dst = src[lm * NC + c];
+            }
+        }
+        lds_barrier();
+    }
+
+    // the center's own rows: its self gradients d c_i^{nt} / d r_i
+    const int n_present = __popc(mask);
+    const int per_self = quads ? 3 * L2 * n / 4 : 3 * L2 * n;
+    for (int t = tid; t < n_present * per_self; t += TILE_THREADS) {
+        const int p = t / per_self, k = t - p * per_self;
+        int nt = -1;
+        for (int s = 0, c = 0; s < S; s++) {
+            if ((mask >> s) & 1u) {
+                if (c == p) {
+                    nt = s;
+                    break;
+                }
+                c++;
+            }
+        }
+        int l, x, off;
+        decode(k, l, x, off);
+        const int wl = (2 * l + 1) * n;
+        const long long tb = (long long)ct * S + nt;
+        const long long row = self_rows[(long long)i * S + nt];
+        T* dst = out + se_grad_base[tb * L + l] + row * 3 * wl + (long long)x * wl + off;
+        const T* src = dself + (((long long)i * S + nt) * 3 + x) * L2 * n + l * l * n + off;
+        if (quads) {
+            T v[4];
+            load4<T>(src, v);
+            store4<T>(dst, v[0], v[1], v[2], v[3]);
+        } else {
+            *dst = *src;
         }
     }
 }
@@ -2323,14 +2483,21 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
                          const int* species, const unsigned* type_mask, const long long* sample_scan,
                          const long long* row_scan, const long long* block_sample_base, const int* atom_system,
                          const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
-                         int* n_windows, int* self_rows, int* labels) {
+                         int* n_windows, int* self_rows, int* labels, bool power_spectrum) {
     if (n_atoms == 0) return;
     const size_t npairs = (size_t)n_species * (n_species + 1) / 2;
     const size_t shmem = npairs * (8 + 4 * 3);
-    ps_grad_plan_kernel<<<n_atoms, WAVE, shmem, stream>>>(
-        n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base, atom_system,
-        atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows, reinterpret_cast<int4*>(windows),
-        n_windows, self_rows, labels);
+    if (power_spectrum) {
+        ps_grad_plan_kernel<true><<<n_atoms, WAVE, shmem, stream>>>(
+            n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base,
+            atom_system, atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows,
+            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels);
+    } else {
+        ps_grad_plan_kernel<false><<<n_atoms, WAVE, shmem, stream>>>(
+            n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base,
+            atom_system, atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows,
+            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels);
+    }
 }
 
 template <typename T>
@@ -2403,6 +2570,38 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
 }
 
 template <typename T>
+size_t se_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries) {
+    const int NC = (p.n + 3) / 4 * 4;
+    const size_t cs = (size_t)p.L * p.L * NC;
+    return ((size_t)max_runs * 3 * cs + (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries) * sizeof(T);
+}
+
+template <typename T>
+void launch_se_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, const T* dself,
+                              const unsigned* type_mask, const int* runs, const int* run_rows, const int* windows,
+                              const int* n_windows, const int* self_rows, const long long* se_grad_base,
+                              const T* entry_data, T* out) {
+    if (p.n_atoms == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = se_gradient_tiled_shmem<T>(p, max_runs, max_entries);
+    const int4* rs = reinterpret_cast<const int4*>(runs);
+    const int4* ws = reinterpret_cast<const int4*>(windows);
+#define RA_LAUNCH_SE(NCV)                                                                                             \
+    se_gradient_tiled_kernel<T, NCV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                                  \
+        a, max_runs, max_entries, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, se_grad_base, entry_data, out)
+    if (p.n <= 4) {
+        RA_LAUNCH_SE(4);
+    } else if (p.n <= 8) {
+        RA_LAUNCH_SE(8);
+    } else if (p.n <= 12) {
+        RA_LAUNCH_SE(12);
+    } else {
+        RA_LAUNCH_SE(16);
+    }
+#undef RA_LAUNCH_SE
+}
+
+template <typename T>
 size_t expansion_tiled_shmem(const SoapLaunch<T>& p, int max_entries) {
     const int NC = (p.n + 3) / 4 * 4;
     const size_t cs = (size_t)p.L * p.L * NC;
@@ -2462,6 +2661,10 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
                                               bool);                                                                   \
     template size_t entry_prep_size<T>(const SoapLaunch<T>&, long long);                                               \
     template size_t expansion_tiled_shmem<T>(const SoapLaunch<T>&, int);                                               \
+    template size_t se_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int);                                        \
+    template void launch_se_gradient_tiled<T>(const SoapLaunch<T>&, int, int, const T*, const unsigned*, const int*,    \
+                                              const int*, const int*, const int*, const int*, const long long*,        \
+                                              const T*, T*);                                                           \
     template void launch_expansion_tiled<T>(const SoapLaunch<T>&, int, const T*, T*, T*);                              \
     template void launch_entry_prep<T>(const SoapLaunch<T>&, long long, T*);                                            \
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
