@@ -170,3 +170,31 @@ def test_every_tuned_launch_shape_and_autotune(precision, fast):
         say(f"  expansion entries {e}: {err:.1e}")
         assert err <= bar, e
     torch.cuda.synchronize()
+
+
+def test_calculators_on_concurrent_streams():
+    """Chunks alternating between two calculators on their own streams (as the benchmark
+    pipelines them) give the results each chunk has alone."""
+    import torch
+
+    chunks = [DeviceSystems(batch(20 + k)) for k in range(4)]
+    calcs = []
+    for _ in range(2):
+        c = ra.SoapPowerSpectrum(**SOAP)
+        c.set_precision(32)
+        calcs.append(c)
+    alone = []
+    for ds in chunks:
+        alone.append(block_arrays(compute_device(calcs[0], ds, ["positions"])))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for rep in range(2):
+        results = [None] * len(chunks)
+        # chunks 0 and 1 in flight together, then 2 and 3
+        for pair in ((0, 1), (2, 3)):
+            devs = [compute_device(calcs[k % 2], chunks[k], ["positions"], stream=streams[k % 2]) for k in pair]
+            torch.cuda.synchronize()
+            for k, dev in zip(pair, devs):
+                results[k] = block_arrays(dev)
+        for k in range(len(chunks)):
+            assert max_block_error(results[k], alone[k]) == 0.0, (rep, k)
