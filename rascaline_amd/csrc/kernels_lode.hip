@@ -315,7 +315,9 @@ __global__ void __launch_bounds__(LT) lode_gemm_mfma_kernel(LodeGemmArgs<float> 
 // (odd l) from them.
 constexpr int PJ = 64, PC = 64, PK = 32, PI = 4;
 
-__global__ void __launch_bounds__(LT) lode_pair_mfma_kernel(LodeGemmArgs<float> g) {
+// three waves per SIMD: at most ~168 registers per lane, which the k loop fits when
+// unrolled by 4 (fully unrolled, it hoists all LDS reads and needs 212)
+__global__ void __launch_bounds__(LT, 3) lode_pair_mfma_kernel(LodeGemmArgs<float> g) {
     __shared__ float cj[PK][PJ + 4], sj[PK][PJ + 4], Bs[PK][PC + 4];
     __shared__ float ci[PI][PK], si[PI][PK];
     const int tid = threadIdx.x;
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(LT) lode_pair_mfma_kernel(LodeGemmArgs<float> 
         __syncthreads();
         // the following chunk's loads fly while this one is multiplied
         if (k0 + PK < nk) fetch(k0 + PK);
-#pragma unroll
+#pragma unroll 4
         for (int ks = 0; ks < PK; ks += 4) {
             const int k = ks + kq;
             const float c_i = ci[wave][k], s_i = si[wave][k];
