@@ -2012,12 +2012,14 @@ __global__ void __launch_bounds__(PSV_THREADS) ps_values_mfma_kernel(
 // Spherical expansion c_i and self gradients d c_i / d r_i of one center per
 // 256-thread workgroup, from the per-entry data of entry_prep_kernel
 // (soap/spherical_expansion.rs:101-358, spherical_expansion_pair.rs:323-422).
-// The center's entries are read in contiguous chunks; one item per
-// (component, l, m >= 0, half of n) forms Y_l(±m) (component 0) or r dY_l(±m)/dx
-// (components 1..3) and accumulates f R Y (values) or -(ra Y d_x + rb dY/dx)
-// (self gradients, not for periodic images of the center itself) for the
-// neighbour type of each entry. Per-type sums live in LDS; each item keeps its
-// running sums in registers while the neighbour type stays the same.
+// The center's entries are read in contiguous chunks. Per chunk:
+//   A. one item per (entry, l, m >= 0) forms Y_l(±m) (spherical_harmonics.rs:
+//      304-424) and the three r dY_l(±m)/dx once, into LDS;
+//   B. one item per (component, l m, NC / NH radial values) accumulates
+//      f R Y (values) or -((df R + f dR) Y d_x + (f / r) R r dY/dx) (self
+//      gradients; not for periodic images of the center itself) for the
+//      neighbour type of each entry, keeping its sums in registers while the type
+//      stays the same and adding them to the per-type sums in LDS when it changes.
 template <typename T, int NC, int NH>
 __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<T> a, int max_entries,
                                                                        const T* __restrict__ entry_data,
@@ -2027,7 +2029,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
     __shared__ int chunk_self[WAVE];
     __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
     __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
-    constexpr int NW = NC / NH;  // n values per item
+    constexpr int NW = NC / NH;  // radial values per item
     const int i = blockIdx.x;
     const int tid = threadIdx.x;
     const int S = a.n_species, L = a.L, n = a.n;
@@ -2039,6 +2041,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
     const int ncomp = grads ? 4 : 1;
     T* acc = reinterpret_cast<T*>(smem);                   // [S][4][L2][NC]
     T* ent = acc + (size_t)S * 4 * CS;                      // [max_entries][per_entry]
+    T* ang = ent + (size_t)max_entries * lay.per_entry;    // [max_entries][4][L2]: Y, r dY/dx, r dY/dy, r dY/dz
     const long long e0 = a.offsets[i];
     const int len = (int)(a.offsets[i + 1] - e0);
     const ShTable tb(L);
@@ -2068,32 +2071,76 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
             chunk_self[c] = j == i ? 1 : 0;
         }
         lds_barrier();
-        for (int item = tid; item < ncomp * leg * NH; item += TILE_THREADS) {
-            const int comp = item / (leg * NH);
-            const int rest = item - comp * leg * NH;
-            const int k = rest / NH, h = rest - k * NH;
+        // A. angular factors of every (entry, l, m >= 0)
+        for (int item = tid; item < cnt * leg; item += TILE_THREADS) {
+            const int e = item / leg, k = item - e * leg;
             const int l = leg_l[k], m = leg_m[k];
             const int lm0 = l * l + l;
+            const T* base = ent + e * lay.per_entry;
+            const T* P = base + lay.p;
+            const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+            const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+            T* A = ang + (size_t)e * 4 * L2;
+            if (m == 0) {
+                A[lm0] = P[k] / SQRT_2;
+                if (grads) {
+                    T g[3] = {0, 0, 0};
+                    if (l > 0) {
+                        const T lf = a.sh_c[tb.gl + l] * P[leg_index(l, 1)];
+                        g[0] = cos_phi * cos_theta * lf;
+                        g[1] = sin_phi * cos_theta * lf;
+                        g[2] = -sin_theta * lf;
+                    }
+                    for (int x = 0; x < 3; x++) A[(1 + x) * L2 + lm0] = g[x];
+                }
+            } else {
+                T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+                const T minus_two_cos = -2 * cos_phi;
+                T sin_m = 0, cos_m = 1;
+                for (int mm = 1; mm <= m; mm++) {
+                    sin_m = minus_two_cos * sin_1 - sin_2;
+                    cos_m = minus_two_cos * cos_1 - cos_2;
+                    sin_2 = sin_1;
+                    sin_1 = sin_m;
+                    cos_2 = cos_1;
+                    cos_1 = cos_m;
+                }
+                const T p = P[k];
+                A[lm0 + m] = p * cos_m;
+                A[lm0 - m] = p * sin_m;
+                if (grads) {
+                    const T c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
+                    const T pm1 = P[leg_index(l, m - 1)];
+                    const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
+                    const T dp = c_dpa * pm1 - c_dpb * pp1;
+                    const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
+                    const T s_dp = sin_m * dp, c_dp = cos_m * dp;
+                    A[L2 + lm0 + m] = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
+                    A[L2 + lm0 - m] = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
+                    A[2 * L2 + lm0 + m] = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
+                    A[2 * L2 + lm0 - m] = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
+                    A[3 * L2 + lm0 + m] = (T)0.5 * sin_theta * c_dp;
+                    A[3 * L2 + lm0 - m] = (T)0.5 * sin_theta * s_dp;
+                }
+            }
+        }
+        lds_barrier();
+        // B. accumulation per (component, l m, part of n)
+        for (int item = tid; item < ncomp * L2 * NH; item += TILE_THREADS) {
+            const int comp = item / (L2 * NH);
+            const int rest = item - comp * L2 * NH;
+            const int lm = rest / NH, h = rest - lm * NH;
+            const int l = l_of_lm(lm);
             const int x = comp - 1;
-            const T c_gl = a.sh_c[tb.gl + l], c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
-            T accp[NW], accm[NW];
+            T sum[NW];
             int cur_t = -1;
             auto flush = [&]() {
-                T* dp_ = acc + ((size_t)cur_t * 4 + comp) * CS + (lm0 + m) * NC + h * NW;
-                T* dm_ = acc + ((size_t)cur_t * 4 + comp) * CS + (lm0 - m) * NC + h * NW;
+                T* d = acc + ((size_t)cur_t * 4 + comp) * CS + lm * NC + h * NW;
 #pragma unroll
                 for (int c = 0; c < NW; c += 4) {
                     T o[4];
-                    load4<T>(dp_ + c, o);
-                    store4<T>(dp_ + c, o[0] + accp[c], o[1] + accp[c + 1], o[2] + accp[c + 2], o[3] + accp[c + 3]);
-                }
-                if (m > 0) {
-#pragma unroll
-                    for (int c = 0; c < NW; c += 4) {
-                        T o[4];
-                        load4<T>(dm_ + c, o);
-                        store4<T>(dm_ + c, o[0] + accm[c], o[1] + accm[c + 1], o[2] + accm[c + 2], o[3] + accm[c + 3]);
-                    }
+                    load4<T>(d + c, o);
+                    store4<T>(d + c, o[0] + sum[c], o[1] + sum[c + 1], o[2] + sum[c + 2], o[3] + sum[c + 3]);
                 }
             };
             for (int e = 0; e < cnt; e++) {
@@ -2103,56 +2150,12 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
                     if (cur_t >= 0) flush();
                     cur_t = t;
 #pragma unroll
-                    for (int c = 0; c < NW; c++) {
-                        accp[c] = 0;
-                        accm[c] = 0;
-                    }
+                    for (int c = 0; c < NW; c++) sum[c] = 0;
                 }
                 const T* base = ent + e * lay.per_entry;
-                const T* P = base + lay.p;
-                const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
-                const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
-                T yp, ym = 0, gp = 0, gm = 0;
-                if (m == 0) {
-                    yp = P[k] / SQRT_2;
-                    if (comp > 0 && l > 0) {
-                        const T lf = c_gl * P[leg_index(l, 1)];
-                        gp = x == 0 ? cos_phi * cos_theta * lf : (x == 1 ? sin_phi * cos_theta * lf : -sin_theta * lf);
-                    }
-                } else {
-                    T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
-                    const T minus_two_cos = -2 * cos_phi;
-                    T sin_m = 0, cos_m = 1;
-                    for (int mm = 1; mm <= m; mm++) {
-                        sin_m = minus_two_cos * sin_1 - sin_2;
-                        cos_m = minus_two_cos * cos_1 - cos_2;
-                        sin_2 = sin_1;
-                        sin_1 = sin_m;
-                        cos_2 = cos_1;
-                        cos_1 = cos_m;
-                    }
-                    const T p = P[k];
-                    yp = p * cos_m;
-                    ym = p * sin_m;
-                    if (comp > 0) {
-                        const T pm1 = P[leg_index(l, m - 1)];
-                        const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
-                        const T dp = c_dpa * pm1 - c_dpb * pp1;
-                        const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
-                        const T s_dp = sin_m * dp, c_dp = cos_m * dp;
-                        if (x == 0) {
-                            gp = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
-                            gm = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
-                        } else if (x == 1) {
-                            gp = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
-                            gm = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
-                        } else {
-                            gp = (T)0.5 * sin_theta * c_dp;
-                            gm = (T)0.5 * sin_theta * s_dp;
-                        }
-                    }
-                }
+                const T* A = ang + (size_t)e * 4 * L2;
                 const T f = base[lay.scal + 0];
+                const T y = A[lm];
                 T Rv[NW];
 #pragma unroll
                 for (int c = 0; c < NW; c += 4) load4<T>(base + lay.rv + l * NC + h * NW + c, Rv + c);
@@ -2160,13 +2163,12 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
 #pragma unroll
                     for (int c = 0; c < NW; c++) {
                         const T v = f * Rv[c];
-                        accp[c] += v * yp;
-                        accm[c] += v * ym;
+                        sum[c] += v * y;
                     }
                 } else {
                     const T df = base[lay.scal + 1], f_r = base[lay.scal + 2];
-                    const T dx = base[lay.dir + x];
-                    const T ypd = yp * dx, ymd = ym * dx;
+                    const T yd = y * base[lay.dir + x];
+                    const T g = A[comp * L2 + lm];
                     T dR[NW];
 #pragma unroll
                     for (int c = 0; c < NW; c += 4) load4<T>(base + lay.rd + l * NC + h * NW + c, dR + c);
@@ -2174,8 +2176,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
                     for (int c = 0; c < NW; c++) {
                         const T ra = df * Rv[c] + f * dR[c];
                         const T rb = f_r * Rv[c];
-                        accp[c] -= ra * ypd + rb * gp;
-                        accm[c] -= ra * ymd + rb * gm;
+                        sum[c] -= ra * yd + rb * g;
                     }
                 }
             }
@@ -2605,7 +2606,8 @@ template <typename T>
 size_t expansion_tiled_shmem(const SoapLaunch<T>& p, int max_entries) {
     const int NC = (p.n + 3) / 4 * 4;
     const size_t cs = (size_t)p.L * p.L * NC;
-    return ((size_t)p.n_species * 4 * cs + (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries) * sizeof(T);
+    const size_t per_entry = (size_t)TileEntryLayout(p.L, p.n, NC).per_entry + 4 * (size_t)p.L * p.L;
+    return ((size_t)p.n_species * 4 * cs + per_entry * max_entries) * sizeof(T);
 }
 
 template <typename T>
@@ -2618,7 +2620,7 @@ void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* en
     if (p.n <= 4) {
         RA_LAUNCH_EXP(4, 1);
     } else if (p.n <= 8) {
-        RA_LAUNCH_EXP(8, 2);
+        RA_LAUNCH_EXP(8, 1);
     } else if (p.n <= 12) {
         RA_LAUNCH_EXP(12, 1);
     } else {
