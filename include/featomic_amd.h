@@ -104,6 +104,21 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
 featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
                                             int64_t* main_kernel_bytes);
 
+/* Vector-Jacobian product of the position gradients of a SOAP calculator
+ * (spherical expansion or power spectrum) on a device-resident batch:
+ * positions_grad[atom][3] = Σ_blocks Σ_rows (dX/dr)[row][xyz][:] · grad_values[sample of row][:],
+ * the contraction featomic-torch's backward performs with the gradient rows
+ * (featomic-torch/src/autograd.cpp:237-300: PositionsGrad::forward), computed
+ * here without forming them. `grad_values` (device) holds dA/dX in the layout of
+ * the values that featomic_amd_compute_device returns for the same batch and
+ * calculator (per-block value_offsets, the calculator's precision);
+ * `positions_grad` (device, [n_atoms][3], the calculator's precision) is
+ * overwritten. Work is queued on `stream`; the call synchronises it once after the
+ * metadata pass, like featomic_amd_compute_device. An atom whose neighbour list is
+ * not symmetric (which the library's own lists always are) gets NaN. */
+featomic_status_t featomic_amd_positions_vjp(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
+                                             const void* grad_values, void* stream, void* positions_grad);
+
 /* Autotuner of the SOAP launch shapes (no counterpart in the reference, whose CPU
  * kernels have no launch shapes). On a device-resident batch, times every
  * candidate window shape (runs, entries, rows per pass) of the tiled

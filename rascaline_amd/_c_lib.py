@@ -188,6 +188,8 @@ PROTOTYPES = [
     ("featomic_amd_calculator_set_tuning", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("featomic_amd_calculator_get_tuning", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    ("featomic_amd_positions_vjp", featomic_status_t,
+     [ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("featomic_amd_compute_device", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.POINTER(ctypes.c_char_p), ctypes.c_size_t,
       ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_output_t)]),

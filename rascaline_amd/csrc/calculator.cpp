@@ -455,6 +455,49 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     RA_HIP_CHECK(hipGetLastError());
 
+    if (vjp_grad_) {
+        // vector-Jacobian product of the position gradients: dA/dc, then per-entry
+        // contributions and their per-atom sums; no gradient rows
+        if (!entry_data) throw Error::invalid_parameter("the positions vector-Jacobian product needs max_radial < 16, max_angular < 16 and at most 32 atomic types");
+        const T* W = static_cast<const T*>(vjp_grad_);
+        T* H = ws_.get<T>("vjp_h", static_cast<size_t>(n_atoms) * S * slots);
+        if (ps) {
+            if (ps_vjp_coef_shmem<T>(lp) + 1024 > static_cast<size_t>(lds)) {
+                throw Error::invalid_parameter("too many atomic types or radial / angular functions for the GPU kernels' local memory");
+            }
+            launch_ps_vjp_coef<T>(lp, coef, type_mask, sample_flag, sample_scan, W, H);
+        } else {
+            std::vector<long long> vb(static_cast<size_t>(NB) * L);
+            long long size = 0;
+            for (int tb = 0; tb < NB; tb++) {
+                const long long ns = block_sample_base[tb + 1] - block_sample_base[tb];
+                for (int l = 0; l < L; l++) {
+                    vb[tb * L + l] = size;
+                    size += ns * (2LL * l + 1) * n;
+                }
+            }
+            long long* d_vb = ws_.get<long long>("vjp_value_base", vb.size());
+            upload(d_vb, vb, stream);
+            launch_se_vjp_coef<T>(lp, sample_flag, sample_scan, d_block_sample_base, d_vb, W, H);
+        }
+        int vjp_entries = 8;
+        while (vjp_entries > 1 && vjp_pair_shmem<T>(lp, vjp_entries) + 1024 > static_cast<size_t>(std::min(lds, 65536))) vjp_entries /= 2;
+        if (vjp_pair_shmem<T>(lp, vjp_entries) + 1024 > static_cast<size_t>(lds)) {
+            throw Error::invalid_parameter("too many atomic types or radial / angular functions for the GPU kernels' local memory");
+        }
+        T* g = ws_.get<T>("vjp_g", 3 * static_cast<size_t>(std::max<long long>(nl.n_entries, 1)));
+        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        launch_vjp_pair<T>(lp, vjp_entries, entry_data, H, g);
+        launch_vjp_gather<T>(stream, n_atoms, nl.offsets, nl.entry_j, nl.entry_shift, g, static_cast<T*>(vjp_out_));
+        RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+        RA_HIP_CHECK(hipGetLastError());
+        RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+        // algorithmic bytes: dA/dX read once, dA/dr written once, neighbour entries read once
+        main_bytes_ = (ps ? total_samples * P : total_samples * static_cast<long long>(L2) * n) * static_cast<long long>(sizeof(T)) +
+                      3LL * n_atoms * static_cast<long long>(sizeof(T)) + nl.n_entries * 48;
+        return nullptr;
+    }
+
     // ---- outputs
     int* sample_labels = ws_.get<int>("sample_labels", 2 * static_cast<size_t>(std::max<long long>(total_samples, 1)));
     launch_sample_labels(stream, NB, n_atoms, sample_flag, sample_scan, d_atom_system, d_atom_local, sample_labels);
@@ -885,6 +928,27 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
     report += ", \"chosen\": {\"gradient_window\": " + std::to_string(best.gradient_window) +
               ", \"expansion_entries\": " + std::to_string(best.expansion_entries) + "}}";
     return report;
+}
+
+void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
+    if (kind_ == Kind::LodeSphericalExpansion) throw Error::invalid_parameter("the positions vector-Jacobian product covers the SOAP calculators");
+    if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
+    vjp_grad_ = grad_values;
+    vjp_out_ = out;
+    main_ms_sum_ = -1.0;
+    try {
+        if (precision == 32) {
+            run_soap<float>(batch, Gradients{}, stream, false, nullptr);
+        } else {
+            run_soap<double>(batch, Gradients{}, stream, false, nullptr);
+        }
+    } catch (...) {
+        vjp_grad_ = nullptr;
+        vjp_out_ = nullptr;
+        throw;
+    }
+    vjp_grad_ = nullptr;
+    vjp_out_ = nullptr;
 }
 
 void Calculator::last_timings(double* total_ms, double* main_ms, int64_t* main_bytes) {

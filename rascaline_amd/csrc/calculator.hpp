@@ -101,6 +101,12 @@ public:
                         featomic_amd_device_output_t* output);
     void last_timings(double* total_ms, double* main_ms, int64_t* main_bytes);
 
+    // vector-Jacobian product of the position gradients on a device-resident batch:
+    // out[atom][3] = Σ_blocks Σ_rows (dX/dr)[row] · grad_values[sample of row], with
+    // grad_values in the device layout of this calculator's values for the batch
+    // (the precision of the calculator); computed without the gradient rows
+    void positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out);
+
     SoapTuning tuning;
     // times the candidate launch shapes on a device-resident batch (the best of
     // `repeats` runs each, device time), keeps the fastest in `tuning` and
@@ -141,6 +147,8 @@ private:
 
     std::unique_ptr<RadialSurrogate> surrogate_;
     bool use_surrogate_ = false;
+    const void* vjp_grad_ = nullptr;
+    void* vjp_out_ = nullptr;
 
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr, ev_main_start_ = nullptr, ev_main_stop_ = nullptr;
     int64_t main_bytes_ = 0;

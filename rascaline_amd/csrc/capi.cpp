@@ -501,6 +501,29 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
     });
 }
 
+featomic_status_t featomic_amd_positions_vjp(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
+                                             const void* grad_values, void* stream, void* positions_grad) {
+    return guard([&] {
+        check_pointer(calculator, "calculator");
+        check_pointer(systems, "systems");
+        check_pointer(systems->atom_offsets, "systems.atom_offsets");
+        check_pointer(systems->cells, "systems.cells");
+        check_pointer(grad_values, "grad_values");
+        check_pointer(positions_grad, "positions_grad");
+        rascaline_amd::HostBatch batch;
+        batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
+        const int64_t n_atoms = batch.n_atoms();
+        if (n_atoms > 0) {
+            check_pointer(systems->positions, "systems.positions");
+            check_pointer(systems->types, "systems.types");
+        }
+        batch.device_positions = systems->positions;
+        batch.types.assign(systems->types, systems->types + n_atoms);
+        batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
+        calculator->impl->positions_vjp(batch, static_cast<hipStream_t>(stream), grad_values, positions_grad);
+    });
+}
+
 featomic_status_t featomic_amd_calculator_autotune(featomic_calculator_t* calculator,
                                                   const featomic_amd_device_systems_t* systems,
                                                   const char* const* gradients, uintptr_t gradients_count, void* stream,

@@ -70,6 +70,24 @@ void launch_radial_eval(hipStream_t stream, long long count, const DeviceRadial<
                         T* out_val, T* out_der);
 template <typename T>
 void launch_soap_center(const SoapLaunch<T>& p, int mode, T* coef, T* grad);
+// vector-Jacobian product of the position gradients (no gradient rows): H = dA/dc
+// from dA/dPS (power spectrum) or from the per-block dA/dc (spherical expansion),
+// g[entry][3] per neighbour entry, then dA/dr per atom
+template <typename T>
+size_t ps_vjp_coef_shmem(const SoapLaunch<T>& p);
+template <typename T>
+void launch_ps_vjp_coef(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
+                        const long long* sample_scan, const T* grad_values, T* H);
+template <typename T>
+void launch_se_vjp_coef(const SoapLaunch<T>& p, const long long* sample_flag, const long long* sample_scan,
+                        const long long* block_sample_base, const long long* value_base, const T* grad_values, T* H);
+template <typename T>
+size_t vjp_pair_shmem(const SoapLaunch<T>& p, int max_entries);
+template <typename T>
+void launch_vjp_pair(const SoapLaunch<T>& p, int max_entries, const T* entry_data, const T* H, T* g);
+template <typename T>
+void launch_vjp_gather(hipStream_t stream, int n_atoms, const long long* offsets, const int* entry_j,
+                       const int* entry_shift, const T* g, T* out);
 // fast (FP32 only): the m-contraction in bfloat16 hi + lo on MFMA when n is a
 // multiple of 4, at most 16 types and the per-center Gram matrix fits in lds_limit
 template <typename T>

@@ -2009,6 +2009,70 @@ __global__ void __launch_bounds__(PSV_THREADS) ps_values_mfma_kernel(
     }
 }
 
+// Angular factors of a chunk of neighbour entries (per-entry data in LDS), one
+// item per (entry, l, m >= 0): ang[e][0][lm] = Y_lm and, with `grads`,
+// ang[e][1 + x][lm] = r dY_lm/dx (spherical_harmonics.rs:256-298, :304-424).
+template <typename T>
+__device__ void chunk_angular_factors(const SoapArgs<T>& a, const TileEntryLayout& lay, int cnt, const int* leg_l,
+                                      const int* leg_m, const T* ent, T* ang, bool grads) {
+    const int leg = lay.leg, L2 = lay.L2;
+    const ShTable tb(lay.L);
+    const T SQRT_2 = (T)1.4142135623730951;
+    const int tid = threadIdx.x;
+    for (int item = tid; item < cnt * leg; item += TILE_THREADS) {
+        const int e = item / leg, k = item - e * leg;
+        const int l = leg_l[k], m = leg_m[k];
+        const int lm0 = l * l + l;
+        const T* base = ent + e * lay.per_entry;
+        const T* P = base + lay.p;
+        const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+        const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+        T* A = ang + (size_t)e * 4 * L2;
+        if (m == 0) {
+            A[lm0] = P[k] / SQRT_2;
+            if (grads) {
+                T g[3] = {0, 0, 0};
+                if (l > 0) {
+                    const T lf = a.sh_c[tb.gl + l] * P[leg_index(l, 1)];
+                    g[0] = cos_phi * cos_theta * lf;
+                    g[1] = sin_phi * cos_theta * lf;
+                    g[2] = -sin_theta * lf;
+                }
+                for (int x = 0; x < 3; x++) A[(1 + x) * L2 + lm0] = g[x];
+            }
+        } else {
+            T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+            const T minus_two_cos = -2 * cos_phi;
+            T sin_m = 0, cos_m = 1;
+            for (int mm = 1; mm <= m; mm++) {
+                sin_m = minus_two_cos * sin_1 - sin_2;
+                cos_m = minus_two_cos * cos_1 - cos_2;
+                sin_2 = sin_1;
+                sin_1 = sin_m;
+                cos_2 = cos_1;
+                cos_1 = cos_m;
+            }
+            const T p = P[k];
+            A[lm0 + m] = p * cos_m;
+            A[lm0 - m] = p * sin_m;
+            if (grads) {
+                const T c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
+                const T pm1 = P[leg_index(l, m - 1)];
+                const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
+                const T dp = c_dpa * pm1 - c_dpb * pp1;
+                const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
+                const T s_dp = sin_m * dp, c_dp = cos_m * dp;
+                A[L2 + lm0 + m] = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
+                A[L2 + lm0 - m] = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
+                A[2 * L2 + lm0 + m] = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
+                A[2 * L2 + lm0 - m] = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
+                A[3 * L2 + lm0 + m] = (T)0.5 * sin_theta * c_dp;
+                A[3 * L2 + lm0 - m] = (T)0.5 * sin_theta * s_dp;
+            }
+        }
+    }
+}
+
 // Spherical expansion c_i and self gradients d c_i / d r_i of one center per
 // 256-thread workgroup, from the per-entry data of entry_prep_kernel
 // (soap/spherical_expansion.rs:101-358, spherical_expansion_pair.rs:323-422).
@@ -2072,58 +2136,7 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
         }
         lds_barrier();
         // A. angular factors of every (entry, l, m >= 0)
-        for (int item = tid; item < cnt * leg; item += TILE_THREADS) {
-            const int e = item / leg, k = item - e * leg;
-            const int l = leg_l[k], m = leg_m[k];
-            const int lm0 = l * l + l;
-            const T* base = ent + e * lay.per_entry;
-            const T* P = base + lay.p;
-            const T cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
-            const T cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
-            T* A = ang + (size_t)e * 4 * L2;
-            if (m == 0) {
-                A[lm0] = P[k] / SQRT_2;
-                if (grads) {
-                    T g[3] = {0, 0, 0};
-                    if (l > 0) {
-                        const T lf = a.sh_c[tb.gl + l] * P[leg_index(l, 1)];
-                        g[0] = cos_phi * cos_theta * lf;
-                        g[1] = sin_phi * cos_theta * lf;
-                        g[2] = -sin_theta * lf;
-                    }
-                    for (int x = 0; x < 3; x++) A[(1 + x) * L2 + lm0] = g[x];
-                }
-            } else {
-                T cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
-                const T minus_two_cos = -2 * cos_phi;
-                T sin_m = 0, cos_m = 1;
-                for (int mm = 1; mm <= m; mm++) {
-                    sin_m = minus_two_cos * sin_1 - sin_2;
-                    cos_m = minus_two_cos * cos_1 - cos_2;
-                    sin_2 = sin_1;
-                    sin_1 = sin_m;
-                    cos_2 = cos_1;
-                    cos_1 = cos_m;
-                }
-                const T p = P[k];
-                A[lm0 + m] = p * cos_m;
-                A[lm0 - m] = p * sin_m;
-                if (grads) {
-                    const T c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
-                    const T pm1 = P[leg_index(l, m - 1)];
-                    const T pp1 = m < l ? P[leg_index(l, m + 1)] : (T)0;
-                    const T dp = c_dpa * pm1 - c_dpb * pp1;
-                    const T pt = (sin_theta > (T)0.1) ? (T)m / sin_theta * p : -(T)0.5 / cos_theta * dp;
-                    const T s_dp = sin_m * dp, c_dp = cos_m * dp;
-                    A[L2 + lm0 + m] = sin_phi * pt * sin_m - (T)0.5 * cos_theta * cos_phi * c_dp;
-                    A[L2 + lm0 - m] = -sin_phi * pt * cos_m - (T)0.5 * cos_theta * cos_phi * s_dp;
-                    A[2 * L2 + lm0 + m] = -cos_phi * pt * sin_m - (T)0.5 * cos_theta * sin_phi * c_dp;
-                    A[2 * L2 + lm0 - m] = cos_phi * pt * cos_m - (T)0.5 * cos_theta * sin_phi * s_dp;
-                    A[3 * L2 + lm0 + m] = (T)0.5 * sin_theta * c_dp;
-                    A[3 * L2 + lm0 - m] = (T)0.5 * sin_theta * s_dp;
-                }
-            }
-        }
+        chunk_angular_factors<T>(a, lay, cnt, leg_l, leg_m, ent, ang, grads);
         lds_barrier();
         // B. accumulation per (component, l m, part of n)
         for (int item = tid; item < ncomp * L2 * NH; item += TILE_THREADS) {
@@ -2206,6 +2219,226 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
             out_grad[(long long)i * S * 3 * slots + k] = acc[((size_t)t * 4 + 1 + x) * CS + lm * NC + c];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Vector-Jacobian product of the position gradients without the gradient rows:
+// dA/dr_atom = Σ_blocks Σ_rows (dX/dr)[row] · (dA/dX)[sample of row], the
+// contraction of featomic-torch's PositionsGrad (featomic-torch/src/
+// autograd.cpp:237-300). With H_i^t = dA/dc_i^t (the loss gradient with respect to
+// the expansion coefficients), each neighbour entry (i, j) contributes
+// g_ij = Σ_{lm, n} (d c_i^{t_j} / d r_j)[x][lm][n] H_i^{t_j}[lm][n] to dA/dr_j and
+// -g_ij to dA/dr_i (translation invariance of c_i; soap/spherical_expansion.rs:
+// 276-354 for the per-pair derivative and its self counterpart).
+
+// H from dA/dPS: for each block (t1 <= t2) with a sample at this center,
+// dPS[l n1 n2] = κ / ((-1)^l √(2l+1)) Σ_m c^{t1}[lm n1] c^{t2}[lm n2] gives
+// H^{t1}[lm n1] += Σ_{n2} s W[l n1 n2] c^{t2}[lm n2] and
+// H^{t2}[lm n2] += Σ_{n1} s W[l n1 n2] c^{t1}[lm n1] (soap/power_spectrum.rs:529-618).
+template <typename T>
+__global__ void __launch_bounds__(TILE_THREADS) ps_vjp_coef_kernel(
+    int n_atoms, int S, int L, int n, const T* __restrict__ coef, const int* __restrict__ species,
+    const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_flag,
+    const long long* __restrict__ sample_scan, const T* __restrict__ grad_values, T* __restrict__ H) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const int L2 = L * L, slots = L2 * n, P = L * n * n;
+    T* C = reinterpret_cast<T*>(smem);  // [S][L2][n]
+    T* Hs = C + (size_t)S * slots;      // [S][L2][n]
+    T* W = Hs + (size_t)S * slots;      // [P]
+    for (int k = tid; k < S * slots; k += TILE_THREADS) {
+        C[k] = coef[(long long)i * S * slots + k];
+        Hs[k] = 0;
+    }
+    const int ct = species[i];
+    const unsigned mask = type_mask[i];
+    const int npairs = S * (S + 1) / 2;
+    for (int t1 = 0; t1 < S; t1++) {
+        if (!((mask >> t1) & 1u)) continue;
+        for (int t2 = t1; t2 < S; t2++) {
+            if (!((mask >> t2) & 1u)) continue;
+            const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
+            if (!sample_flag[idx]) continue;
+            const long long smp = sample_scan[idx];
+            __syncthreads();
+            for (int k = tid; k < P; k += TILE_THREADS) W[k] = grad_values[smp * P + k];
+            __syncthreads();
+            const T kappa = t1 != t2 ? (T)1.4142135623730951 : (T)1;
+            for (int item = tid; item < slots; item += TILE_THREADS) {
+                const int lm = item / n, c = item - lm * n;
+                const int l = l_of_lm(lm);
+                const T sc = kappa * (T)((l % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * l + 1.0));
+                const T* Wl = W + l * n * n;
+                T h1 = 0, h2 = 0;
+                for (int o = 0; o < n; o++) {
+                    h1 += Wl[c * n + o] * C[t2 * slots + lm * n + o];
+                    h2 += Wl[o * n + c] * C[t1 * slots + lm * n + o];
+                }
+                // the same thread owns (lm, c) of both types: no race when t1 == t2
+                Hs[t1 * slots + item] += sc * h1;
+                Hs[t2 * slots + item] += sc * h2;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < S * slots; k += TILE_THREADS) H[(long long)i * S * slots + k] = Hs[k];
+}
+
+// H from dA/dc given per spherical-expansion block (l, center type, neighbour type):
+// a gather from the blocks' [samples][2l+1][n] layout into [atom][type][L2][n].
+template <typename T>
+__global__ void se_vjp_coef_kernel(int n_atoms, int S, int L, int n, const int* __restrict__ species,
+                                   const long long* __restrict__ sample_flag, const long long* __restrict__ sample_scan,
+                                   const long long* __restrict__ block_sample_base,
+                                   const long long* __restrict__ value_base, const T* __restrict__ grad_values,
+                                   T* __restrict__ H) {
+    const int L2 = L * L, slots = L2 * n;
+    const long long total = (long long)n_atoms * S * slots;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ((long long)S * slots));
+        const int r = (int)(k - (long long)i * S * slots);
+        const int nt = r / slots, s = r - nt * slots;
+        const int lm = s / n, c = s - lm * n;
+        const int l = l_of_lm(lm);
+        const long long tb = (long long)species[i] * S + nt;
+        const long long idx = tb * n_atoms + i;
+        T v = 0;
+        if (sample_flag[idx]) {
+            const long long local = sample_scan[idx] - block_sample_base[tb];
+            v = grad_values[value_base[tb * L + l] + local * (2 * l + 1) * n + (lm - l * l) * n + c];
+        }
+        H[k] = v;
+    }
+}
+
+// g_e[x] for every neighbour entry e = (i, j): one workgroup per center, the
+// entries in chunks; angular factors as in the expansion, then per (entry, l m)
+// A1 = Σ_n (df R + f dR)[l n] H[lm n] and A2 = Σ_n (f / r) R[l n] H[lm n], and per
+// (entry, x) g = d_x Σ_lm Y A1 + Σ_lm (r dY/dx) A2 in a fixed order. Periodic
+// images of the center itself move with it: g = 0.
+template <typename T, int NC>
+__global__ void __launch_bounds__(TILE_THREADS) vjp_pair_kernel(SoapArgs<T> a, int max_entries,
+                                                                const T* __restrict__ entry_data, const T* __restrict__ H,
+                                                                T* __restrict__ g_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int chunk_t[WAVE];
+    __shared__ int chunk_self[WAVE];
+    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L;
+    const TileEntryLayout lay(L, n, NC);
+    T* Hc = reinterpret_cast<T*>(smem);                 // [S][L2][NC]
+    T* ent = Hc + (size_t)S * L2 * NC;                   // [max_entries][per_entry]
+    T* ang = ent + (size_t)max_entries * lay.per_entry;  // [max_entries][4][L2]
+    T* part = ang + (size_t)max_entries * 4 * L2;        // [max_entries][L2][4]
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    for (int k = tid; k < S * L2 * NC; k += TILE_THREADS) {
+        const int t = k / (L2 * NC), r = k - t * L2 * NC;
+        const int lm = r / NC, c = r - lm * NC;
+        Hc[k] = c < n ? H[((long long)i * S + t) * L2 * n + lm * n + c] : (T)0;
+    }
+    for (int k = tid; k < lay.leg; k += TILE_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        leg_l[k] = l;
+        leg_m[k] = k - l * (l + 1) / 2;
+    }
+    for (int cb = 0; cb < len; cb += max_entries) {
+        const int cnt = min(max_entries, len - cb);
+        lds_barrier();
+        {
+            typedef typename Vec4Of<T>::type V;
+            constexpr int VEC = 16 / sizeof(T);
+            const V* src = reinterpret_cast<const V*>(entry_data + (e0 + cb) * lay.per_entry);
+            V* dst = reinterpret_cast<V*>(ent);
+            const int count = cnt * lay.per_entry / VEC;
+            for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
+        }
+        for (int c = tid; c < cnt; c += TILE_THREADS) {
+            const int j = a.entry_j[e0 + cb + c];
+            chunk_t[c] = a.species[j];
+            chunk_self[c] = j == i ? 1 : 0;
+        }
+        lds_barrier();
+        chunk_angular_factors<T>(a, lay, cnt, leg_l, leg_m, ent, ang, true);
+        lds_barrier();
+        for (int item = tid; item < cnt * L2; item += TILE_THREADS) {
+            const int e = item / L2, lm = item - e * L2;
+            const int l = l_of_lm(lm);
+            const T* base = ent + e * lay.per_entry;
+            const T* Ht = Hc + ((size_t)chunk_t[e] * L2 + lm) * NC;
+            const T f = base[lay.scal + 0], df = base[lay.scal + 1], f_r = base[lay.scal + 2];
+            const T* Rv = base + lay.rv + l * NC;
+            const T* dR = base + lay.rd + l * NC;
+            T a1 = 0, a2 = 0;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                a1 += (df * Rv[c] + f * dR[c]) * Ht[c];
+                a2 += (f_r * Rv[c]) * Ht[c];
+            }
+            const T* A = ang + (size_t)e * 4 * L2;
+            T* pp = part + ((size_t)e * L2 + lm) * 4;
+            pp[0] = A[lm] * a1;
+            pp[1] = A[L2 + lm] * a2;
+            pp[2] = A[2 * L2 + lm] * a2;
+            pp[3] = A[3 * L2 + lm] * a2;
+        }
+        lds_barrier();
+        for (int item = tid; item < cnt * 3; item += TILE_THREADS) {
+            const int e = item / 3, x = item - e * 3;
+            T g = 0;
+            if (!chunk_self[e]) {
+                const T* base = ent + e * lay.per_entry;
+                const T* pe = part + (size_t)e * L2 * 4;
+                T sy = 0, sg = 0;
+                for (int lm = 0; lm < L2; lm++) {
+                    sy += pe[lm * 4];
+                    sg += pe[lm * 4 + 1 + x];
+                }
+                g = base[lay.dir + x] * sy + sg;
+            }
+            g_out[(e0 + cb + e) * 3 + x] = g;
+        }
+    }
+}
+
+// dA/dr_j = Σ over j's entries e = (j, k, s) of g[mirror of e] - g[e], where the
+// mirror (k, j, -s) is found in k's entries (sorted by neighbour); fixed order,
+// FP64 sums. An entry without a mirror (a neighbour list that is not symmetric)
+// makes the atom's result NaN.
+template <typename T>
+__global__ void vjp_gather_kernel(int n_atoms, const long long* __restrict__ offsets, const int* __restrict__ entry_j,
+                                  const int* __restrict__ entry_shift, const T* __restrict__ g, T* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_atoms) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long long e = offsets[j]; e < offsets[j + 1]; e++) {
+        const int k = entry_j[e];
+        if (k == j) continue;
+        const int s0 = entry_shift[3 * e], s1 = entry_shift[3 * e + 1], s2 = entry_shift[3 * e + 2];
+        // first entry of k's list with neighbour j
+        long long lo = offsets[k], hi = offsets[k + 1];
+        while (lo < hi) {
+            const long long mid = (lo + hi) / 2;
+            if (entry_j[mid] < j) lo = mid + 1; else hi = mid;
+        }
+        long long m = -1;
+        for (long long q = lo; q < offsets[k + 1] && entry_j[q] == j; q++) {
+            if (entry_shift[3 * q] == -s0 && entry_shift[3 * q + 1] == -s1 && entry_shift[3 * q + 2] == -s2) {
+                m = q;
+                break;
+            }
+        }
+        if (m < 0) {
+            acc[0] = acc[1] = acc[2] = __builtin_nan("");
+            break;
+        }
+        for (int x = 0; x < 3; x++) acc[x] += (double)g[3 * m + x] - (double)g[3 * e + x];
+    }
+    for (int x = 0; x < 3; x++) out[3 * (long long)j + x] = (T)acc[x];
 }
 
 // Power spectrum strain / cell gradients: one row of [3][3][P] per sample.
@@ -2630,6 +2863,65 @@ void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* en
 }
 
 template <typename T>
+size_t ps_vjp_coef_shmem(const SoapLaunch<T>& p) {
+    const size_t slots = (size_t)p.L * p.L * p.n;
+    return (2 * (size_t)p.n_species * slots + (size_t)p.L * p.n * p.n) * sizeof(T);
+}
+
+template <typename T>
+void launch_ps_vjp_coef(const SoapLaunch<T>& p, const T* coef, const unsigned* type_mask, const long long* sample_flag,
+                        const long long* sample_scan, const T* grad_values, T* H) {
+    if (p.n_atoms == 0) return;
+    ps_vjp_coef_kernel<T><<<p.n_atoms, TILE_THREADS, ps_vjp_coef_shmem<T>(p), p.stream>>>(
+        p.n_atoms, p.n_species, p.L, p.n, coef, p.species, type_mask, sample_flag, sample_scan, grad_values, H);
+}
+
+template <typename T>
+void launch_se_vjp_coef(const SoapLaunch<T>& p, const long long* sample_flag, const long long* sample_scan,
+                        const long long* block_sample_base, const long long* value_base, const T* grad_values, T* H) {
+    const long long total = (long long)p.n_atoms * p.n_species * p.L * p.L * p.n;
+    if (total == 0) return;
+    const long long blocks = std::min<long long>((total + 255) / 256, 65536);
+    se_vjp_coef_kernel<T><<<(unsigned)blocks, 256, 0, p.stream>>>(p.n_atoms, p.n_species, p.L, p.n, p.species,
+                                                                  sample_flag, sample_scan, block_sample_base,
+                                                                  value_base, grad_values, H);
+}
+
+template <typename T>
+size_t vjp_pair_shmem(const SoapLaunch<T>& p, int max_entries) {
+    const int NC = (p.n + 3) / 4 * 4;
+    const size_t L2 = (size_t)p.L * p.L;
+    const size_t per_entry = (size_t)TileEntryLayout(p.L, p.n, NC).per_entry + 8 * L2;
+    return ((size_t)p.n_species * L2 * NC + per_entry * max_entries) * sizeof(T);
+}
+
+template <typename T>
+void launch_vjp_pair(const SoapLaunch<T>& p, int max_entries, const T* entry_data, const T* H, T* g) {
+    if (p.n_atoms == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const size_t shmem = vjp_pair_shmem<T>(p, max_entries);
+#define RA_LAUNCH_VJP(NCV) \
+    vjp_pair_kernel<T, NCV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(a, max_entries, entry_data, H, g)
+    if (p.n <= 4) {
+        RA_LAUNCH_VJP(4);
+    } else if (p.n <= 8) {
+        RA_LAUNCH_VJP(8);
+    } else if (p.n <= 12) {
+        RA_LAUNCH_VJP(12);
+    } else {
+        RA_LAUNCH_VJP(16);
+    }
+#undef RA_LAUNCH_VJP
+}
+
+template <typename T>
+void launch_vjp_gather(hipStream_t stream, int n_atoms, const long long* offsets, const int* entry_j,
+                       const int* entry_shift, const T* g, T* out) {
+    if (n_atoms == 0) return;
+    vjp_gather_kernel<T><<<(n_atoms + 255) / 256, 256, 0, stream>>>(n_atoms, offsets, entry_j, entry_shift, g, out);
+}
+
+template <typename T>
 void launch_ps_cell_strain(const SoapLaunch<T>& p, const T* coef, const T* dcs, const unsigned* type_mask,
                            const long long* sample_flag, const long long* sample_scan, T* out) {
     if (p.n_atoms == 0) return;
@@ -2672,6 +2964,14 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
     template void launch_gradient_rows<T>(const SoapLaunch<T>&, bool, const T*, const T*, const unsigned*,             \
                                           const long long*, const long long*, const long long*, const long long*,      \
                                           const long long*, const int*, const int*, T*, int*);                         \
+    template size_t ps_vjp_coef_shmem<T>(const SoapLaunch<T>&);                                                        \
+    template void launch_ps_vjp_coef<T>(const SoapLaunch<T>&, const T*, const unsigned*, const long long*,            \
+                                        const long long*, const T*, T*);                                               \
+    template void launch_se_vjp_coef<T>(const SoapLaunch<T>&, const long long*, const long long*, const long long*,   \
+                                        const long long*, const T*, T*);                                               \
+    template size_t vjp_pair_shmem<T>(const SoapLaunch<T>&, int);                                                      \
+    template void launch_vjp_pair<T>(const SoapLaunch<T>&, int, const T*, const T*, T*);                               \
+    template void launch_vjp_gather<T>(hipStream_t, int, const long long*, const int*, const int*, const T*, T*);      \
     template void launch_ps_cell_strain<T>(const SoapLaunch<T>&, const T*, const T*, const unsigned*,                 \
                                            const long long*, const long long*, T*);                                    \
     template void launch_se_gather<T>(const SoapLaunch<T>&, int, const T*, const long long*, const long long*,        \

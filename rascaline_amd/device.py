@@ -79,6 +79,17 @@ class DeviceResult:
         self._glab = [int(out.gradient_label_offsets[b]) for b in range(self.n_blocks)] \
             if out.gradient_label_offsets else None
 
+    def value_offset(self, b):
+        """Element offset of block b's values in the result's value buffer (the layout
+        positions_vjp expects for dA/dX)."""
+        return int(self._out.value_offsets[b])
+
+    @property
+    def value_size(self):
+        """Elements of the value buffer: the end of the last block."""
+        return max((self.value_offset(b) + self.n_samples[b] * self.properties[b] for b in range(self.n_blocks)),
+                   default=0)
+
     def values(self, b):
         off = int(self._out.value_offsets[b])
         ptr = self._out.values + off * self._item if self._out.values else 0
@@ -124,3 +135,22 @@ def compute_device(calculator, systems, gradients=(), stream=None):
     C.check(C.lib().featomic_amd_compute_device(calculator._as_parameter_, ctypes.byref(systems.struct), arr,
                                                 len(names), ctypes.c_void_p(s.cuda_stream), ctypes.byref(out)))
     return DeviceResult(out)
+
+
+def positions_vjp(calculator, systems, grad_values, stream=None):
+    """dA/dr [n_atoms, 3] = Σ_blocks Σ_rows (dX/dr)[row] · dA/dX[sample of row] for a SOAP
+    calculator on device-resident `systems`, without forming the gradient rows.
+    `grad_values` is a contiguous device tensor in the layout of the values of
+    `compute_device(calculator, systems)` (all blocks, at their value offsets) and
+    the calculator's precision."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    dtype = grad_values.dtype
+    if not grad_values.is_cuda or not grad_values.is_contiguous():
+        raise ValueError("grad_values must be a contiguous GPU tensor")
+    out = torch.empty((systems.n_atoms, 3), dtype=dtype, device=grad_values.device)
+    C.check(C.lib().featomic_amd_positions_vjp(calculator._as_parameter_, ctypes.byref(systems.struct),
+                                               ctypes.c_void_p(grad_values.data_ptr()), ctypes.c_void_p(s.cuda_stream),
+                                               ctypes.c_void_p(out.data_ptr())))
+    return out
