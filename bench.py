@@ -133,12 +133,14 @@ def main():
     parser.add_argument("--streams", type=int, default=4,
                         help="calculators on their own streams that consecutive chunks alternate between")
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate"], default="soap",
+    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp"], default="soap",
                         help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
                              "expansion + gradients on periodic cells of 500-1500 atoms; bulk: power spectrum + "
                              "gradients of one 20 000-atom cell (12 radial functions, l <= 8, 5 types); "
                              "surrogate: spherical expansion + gradients of one 100 000-atom cell with the "
-                             "spline and with the learned radial integral (each its own JSON line)")
+                             "spline and with the learned radial integral; vjp: the 4096 structures' power "
+                             "spectrum values followed by the vector-Jacobian product of their position gradients "
+                             "(each its own JSON line)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -152,6 +154,8 @@ def main():
         return bulk_main(args)
     if args.workload == "surrogate":
         return surrogate_main(args)
+    if args.workload == "vjp":
+        return vjp_main(args)
 
     import torch
 
@@ -575,6 +579,85 @@ def surrogate_main(args):
     print(json.dumps(result), flush=True)
     return 0
 
+
+
+def vjp_main(args):
+    """Power spectrum values of the 4096 structures, then the vector-Jacobian product
+    of their position gradients with a synthetic dA/dX (the backward pass a model
+    trained on forces runs), without the gradient rows: a forward and a backward
+    call per chunk, chunks alternating over `streams` calculators."""
+    import torch
+
+    import rascaline_amd as ra
+    from rascaline_amd.datasets import batched_uneven
+    from rascaline_amd.device import compute_device, positions_vjp
+
+    systems = batched_uneven(seed=0, n_structures=args.structures)
+    n_atoms = sum(len(s[1]) for s in systems)
+    chunks = make_chunks(systems, args.chunk_atoms)
+    batches = [device_batch(systems, c) for c in chunks]
+    dtype = torch.float32 if args.precision == 32 else torch.float64
+    lanes = []
+    for _ in range(args.streams):
+        calc = ra.SoapPowerSpectrum(**HYPERS)
+        calc.set_precision(args.precision)
+        lanes.append((calc, torch.cuda.Stream()))
+    # dA/dX per chunk in the layout of its values (synthetic, fixed seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    weights = []
+    for b in batches:
+        dev = compute_device(lanes[0][0], b)
+        weights.append(torch.randn(dev.value_size, dtype=dtype, device="cuda", generator=gen))
+    torch.cuda.synchronize()
+    outs = [None] * len(batches)
+
+    def step():
+        for k, b in enumerate(batches):
+            calc, s = lanes[k % len(lanes)]
+            compute_device(calc, b, stream=s)
+            outs[k] = positions_vjp(calc, b, weights[k], stream=s)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"vjp: warmup step {w} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / args.steps
+    rate = n_atoms * args.steps / elapsed
+    log(f"vjp: timed {args.steps} steps: {ms:.1f} ms/step, {rate:.3e} atoms/s")
+    # one calculator, chunk after chunk: the library's hipEvent timing of the VJP kernels
+    from rascaline_amd import _c_lib as C
+
+    calc = lanes[0][0]
+    stream = torch.cuda.current_stream()
+    k_ms, k_bytes, d_ms = 0.0, 0, 0.0
+    for k, b in enumerate(batches):
+        positions_vjp(calc, b, weights[k], stream=stream)
+        t, km, kb = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t), ctypes.byref(km),
+                                                  ctypes.byref(kb)))
+        k_ms += km.value
+        k_bytes += kb.value
+        d_ms += t.value
+    result = {
+        "metric": "atoms/s SOAP power spectrum + vector-Jacobian product of the position gradients",
+        "value": rate, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64", "data": "synthetic",
+        "config": {"workload": f"{len(systems)} uneven structures as the default workload, power spectrum values "
+                               "(forward) + dA/dr from a synthetic dA/dX (backward), no gradient rows",
+                   "atoms": n_atoms, "chunks": len(batches), "streams": args.streams},
+        "roofline": hbm_roofline(k_ms, k_bytes, d_ms, "vjp_pair_kernel + vjp_gather_kernel (dA/dr per entry and atom)"),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
 
 if __name__ == "__main__":
     sys.exit(main())

@@ -21,14 +21,23 @@ from . import _c_lib as C
 
 class DeviceSystems:
     """Systems given as (positions [n][3], types [n], cell [3][3]) tuples, with the
-    positions copied once to device memory (float64, torch allocation)."""
+    positions copied once to device memory (float64, torch allocation). With
+    `positions` (a contiguous float64 GPU tensor [total atoms, 3]), that tensor is
+    used in place of the tuples' positions, without a copy."""
 
-    def __init__(self, systems, device="cuda:0"):
+    def __init__(self, systems, device="cuda:0", positions=None):
         import torch
 
-        pos = np.concatenate([np.asarray(s[0], dtype=np.float64).reshape(-1, 3) for s in systems])
-        self.n_atoms = len(pos)
-        self.positions = torch.from_numpy(pos).to(device)
+        if positions is None:
+            pos = np.concatenate([np.asarray(s[0], dtype=np.float64).reshape(-1, 3) for s in systems])
+            self.positions = torch.from_numpy(pos).to(device)
+        else:
+            if positions.dtype != torch.float64 or not positions.is_cuda or not positions.is_contiguous():
+                raise ValueError("positions must be a contiguous float64 GPU tensor")
+            if positions.shape != (sum(len(s[1]) for s in systems), 3):
+                raise ValueError("positions must hold [total atoms, 3] values")
+            self.positions = positions
+        self.n_atoms = int(self.positions.shape[0])
         self.types = np.ascontiguousarray(np.concatenate([np.asarray(s[1]) for s in systems]).astype(np.int32))
         self.cells = np.ascontiguousarray(np.stack([np.asarray(s[2], dtype=np.float64).reshape(9) for s in systems]))
         self.offsets = np.zeros(len(systems) + 1, dtype=np.int64)
