@@ -220,3 +220,21 @@ def test_tuning_host_checks():
             == FEATOMIC_INVALID_PARAMETER_ERROR
     finally:
         C.lib().featomic_calculator_free(calc)
+
+
+def test_positions_vjp_host_checks():
+    calc = make("soap_power_spectrum", SOAP)
+    try:
+        assert C.lib().featomic_amd_positions_vjp(calc, None, None, None, None) == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "systems" in last_error()
+        systems = C.featomic_amd_device_systems_t()
+        offsets = np.zeros(1, dtype=np.int64)
+        cells = np.zeros(9)
+        systems.n_systems = 0
+        systems.atom_offsets = offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        systems.cells = cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        assert C.lib().featomic_amd_positions_vjp(calc, ctypes.byref(systems), None, None, None) \
+            == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "grad_values" in last_error()
+    finally:
+        C.lib().featomic_calculator_free(calc)
