@@ -558,9 +558,11 @@ def surrogate_main(args):
         f"{report['max_derivative_error']:.2e}, {report['train_ms']:.0f} ms")
     if not report["accepted"]:
         raise RuntimeError("the radial network was not accepted by the spline judge")
-    res = {}
+    res, tuned = {}, {}
     for name, use in (("spline", False), ("surrogate", True)):
         calc.use_radial_surrogate(use)
+        tuned[name] = calc.autotune(batch, repeats=1)["chosen"]
+        log(f"surrogate {name}: autotuned {tuned[name]}")
         res.update(time_forms([(name, calc)], batch, n_atoms, args, grads, "surrogate"))
     ms, rate, k_ms, k_bytes, d_ms = res["surrogate"]
     result = {

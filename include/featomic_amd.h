@@ -124,8 +124,9 @@ featomic_status_t featomic_amd_positions_vjp(featomic_calculator_t* calculator, 
  * candidate window shape (runs, entries, rows per pass) of the tiled
  * gradient-row kernel within the device's whole LDS, the untiled gradient
  * kernel and the built-in choice, then 4 to 64
- * neighbour entries per LDS chunk of the expansion kernel, each the best device
- * time of `repeats` runs after a warm-up run. The fastest choices stay in the
+ * neighbour entries per LDS chunk of the expansion kernel, each by the best time,
+ * over `repeats` runs after a warm-up run, of the phase it changes (the gradient
+ * rows with their planning pass; the expansion kernel), measured with HIP events. The fastest choices stay in the
  * calculator for its following computations (also when `report` is too small);
  * `report` (may be NULL) receives a JSON summary of every candidate, with
  * FEATOMIC_BUFFER_SIZE_ERROR when `report_size` is too small. */

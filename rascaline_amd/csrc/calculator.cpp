@@ -115,6 +115,9 @@ Calculator::~Calculator() {
     if (ev_stop_) (void)hipEventDestroy(ev_stop_);
     if (ev_main_start_) (void)hipEventDestroy(ev_main_start_);
     if (ev_main_stop_) (void)hipEventDestroy(ev_main_stop_);
+    if (ev_exp_start_) (void)hipEventDestroy(ev_exp_start_);
+    if (ev_exp_stop_) (void)hipEventDestroy(ev_exp_stop_);
+    if (ev_grad_start_) (void)hipEventDestroy(ev_grad_start_);
 }
 
 const char* Calculator::name() const {
@@ -267,6 +270,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         RA_HIP_CHECK(hipEventCreate(&ev_stop_));
         RA_HIP_CHECK(hipEventCreate(&ev_main_start_));
         RA_HIP_CHECK(hipEventCreate(&ev_main_stop_));
+        RA_HIP_CHECK(hipEventCreate(&ev_exp_start_));
+        RA_HIP_CHECK(hipEventCreate(&ev_exp_stop_));
+        RA_HIP_CHECK(hipEventCreate(&ev_grad_start_));
     }
     RA_HIP_CHECK(hipEventRecord(ev_start_, stream));
 
@@ -276,6 +282,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                       : LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, {});
     }
     if (n_atoms == 0) {
+        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_exp_stop_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_grad_start_, stream));
         RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
         RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
         RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
@@ -431,18 +440,23 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     if (tiled) {
         entry_data = ws_.get<T>("entry_data", std::max<size_t>(entry_prep_size<T>(lp, nl.n_entries), 1));
         launch_entry_prep<T>(lp, nl.n_entries, entry_data);
+        // entries per chunk: the tuned or built-in count, halved until two workgroups fit
+        // per CU (large bases: the per-type sums alone take ~62 KB at n = 12, l <= 8)
         int max_entries = tune.expansion_entries > 0 ? tune.expansion_entries : RA_EXPANSION_ENTRIES;
-        while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(std::min(lds, 65536))) {
+        while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(lds) / 2) {
             max_entries /= 2;
         }
+        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
         if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
             launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);
         } else {
             launch_soap_center<T>(lp, 0, coef, dself);
         }
     } else {
+        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
         launch_soap_center<T>(lp, 0, coef, dself);
     }
+    RA_HIP_CHECK(hipEventRecord(ev_exp_stop_, stream));
     T* dstrain = nullptr;
     T* dcell = nullptr;
     if (grads.strain) {
@@ -552,6 +566,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
     }
     bool timed_start = false;
+    RA_HIP_CHECK(hipEventRecord(ev_grad_start_, stream));
     if (!grads.positions) {
         RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
         timed_start = true;
@@ -839,14 +854,21 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
     if (kind_ == Kind::LodeSphericalExpansion) throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
     if (repeats < 1 || repeats > 100) throw Error::invalid_parameter("autotune repeats must be in [1, 100]");
     featomic_amd_device_output_t out{};
-    // best device time of `repeats` runs after one warm-up run (which also sizes the workspace)
-    auto measure = [&]() -> double {
+    // best time of one phase over `repeats` runs after a warm-up run (which also sizes
+    // the workspace): 0 the expansion kernel, 1 the gradient rows with their planning
+    // pass (device time between events, free of the host's share of the call)
+    auto measure = [&](int phase) -> double {
         double best = 1e300;
         for (int r = 0; r <= repeats; r++) {
             compute_device(batch, grads, stream, &out);
-            double total = 0.0;
-            last_timings(&total, nullptr, nullptr);
-            if (r > 0) best = std::min(best, total);
+            RA_HIP_CHECK(hipEventSynchronize(ev_stop_));
+            float ms = 0.f;
+            if (phase == 0) {
+                RA_HIP_CHECK(hipEventElapsedTime(&ms, ev_exp_start_, ev_exp_stop_));
+            } else {
+                RA_HIP_CHECK(hipEventElapsedTime(&ms, ev_grad_start_, ev_main_stop_));
+            }
+            if (r > 0) best = std::min(best, static_cast<double>(ms));
         }
         return best;
     };
@@ -888,7 +910,7 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
                              std::to_string(w[2]) + "]";
                 }
                 try {
-                    const double ms = measure();
+                    const double ms = measure(1);
                     entry += ", \"ms\": " + fmt(ms);
                     if (ms < best_ms) {
                         best_ms = ms;
@@ -911,7 +933,7 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
         for (int e : {4, 8, 16, 32, 64}) {
             tuning = best;
             tuning.expansion_entries = e;
-            const double ms = measure();
+            const double ms = measure(0);
             report += (first ? "" : ", ") + std::string("{\"entries\": ") + std::to_string(e) + ", \"ms\": " + fmt(ms) + "}";
             first = false;
             if (ms < best_ms) {

@@ -151,6 +151,9 @@ private:
     void* vjp_out_ = nullptr;
 
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr, ev_main_start_ = nullptr, ev_main_stop_ = nullptr;
+    // SOAP phases the autotuner times: the expansion kernel, and the gradient rows
+    // from their planning pass to the end of ev_main_stop_
+    hipEvent_t ev_exp_start_ = nullptr, ev_exp_stop_ = nullptr, ev_grad_start_ = nullptr;
     int64_t main_bytes_ = 0;
 
     template <typename T>
