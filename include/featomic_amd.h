@@ -104,8 +104,9 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
 featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, double* total_ms, double* main_kernel_ms,
                                             int64_t* main_kernel_bytes);
 
-/* Vector-Jacobian product of the position gradients of a SOAP calculator
- * (spherical expansion or power spectrum) on a device-resident batch:
+/* Vector-Jacobian product of the position gradients of a SOAP or LODE calculator
+ * (spherical expansion, power spectrum, LODE spherical expansion) on a
+ * device-resident batch:
  * positions_grad[atom][3] = Σ_blocks Σ_rows (dX/dr)[row][xyz][:] · grad_values[sample of row][:],
  * the contraction featomic-torch's backward performs with the gradient rows
  * (featomic-torch/src/autograd.cpp:237-300: PositionsGrad::forward), computed
@@ -114,8 +115,10 @@ featomic_status_t featomic_amd_last_timings(featomic_calculator_t* calculator, d
  * calculator (per-block value_offsets, the calculator's precision);
  * `positions_grad` (device, [n_atoms][3], the calculator's precision) is
  * overwritten. Work is queued on `stream`; the call synchronises it once after the
- * metadata pass, like featomic_amd_compute_device. An atom whose neighbour list is
- * not symmetric (which the library's own lists always are) gets NaN. */
+ * metadata pass (SOAP) or once per system (LODE), like featomic_amd_compute_device.
+ * For SOAP, an atom whose neighbour list is not symmetric (which the library's own
+ * lists always are) gets NaN. For LODE the N^2 pair rows reduce to O(N n_k) sums
+ * over the structure factors. */
 featomic_status_t featomic_amd_positions_vjp(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
                                              const void* grad_values, void* stream, void* positions_grad);
 

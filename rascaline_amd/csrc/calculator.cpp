@@ -953,13 +953,18 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
 }
 
 void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
-    if (kind_ == Kind::LodeSphericalExpansion) throw Error::invalid_parameter("the positions vector-Jacobian product covers the SOAP calculators");
     if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
     vjp_grad_ = grad_values;
     vjp_out_ = out;
     main_ms_sum_ = -1.0;
     try {
-        if (precision == 32) {
+        if (kind_ == Kind::LodeSphericalExpansion) {
+            if (precision == 32) {
+                run_lode<float>(batch, Gradients{}, stream, nullptr);
+            } else {
+                run_lode<double>(batch, Gradients{}, stream, nullptr);
+            }
+        } else if (precision == 32) {
             run_soap<float>(batch, Gradients{}, stream, false, nullptr);
         } else {
             run_soap<double>(batch, Gradients{}, stream, false, nullptr);

@@ -101,7 +101,8 @@ public:
                         featomic_amd_device_output_t* output);
     void last_timings(double* total_ms, double* main_ms, int64_t* main_bytes);
 
-    // vector-Jacobian product of the position gradients on a device-resident batch:
+    // vector-Jacobian product of the position gradients on a device-resident batch
+    // (SOAP and LODE calculators):
     // out[atom][3] = Σ_blocks Σ_rows (dX/dr)[row] · grad_values[sample of row], with
     // grad_values in the device layout of this calculator's values for the batch
     // (the precision of the calculator); computed without the gradient rows

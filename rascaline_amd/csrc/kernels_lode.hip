@@ -18,6 +18,8 @@
 //   A_pair  = 2 (s_i c_j - c_i s_j)  /  -2 (c_i c_j + s_i s_j).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels_lode.hpp"
 #include "soap_common.hpp"
 #include "special.hpp"
@@ -406,6 +408,128 @@ __global__ void __launch_bounds__(LT, 3) lode_pair_mfma_kernel(LodeGemmArgs<floa
             }
 }
 
+// ---- vector-Jacobian product of the position gradients (see kernels_lode.hpp)
+constexpr int VJP_AT = 8;  // atoms per workgroup of lode_vjp_v_kernel
+
+// V_p[t][x][i][k] for a tile of VJP_AT atoms, one thread per k-vector: each
+// thread walks its row of Bg once for all the tile's atoms and directions.
+template <typename T>
+__global__ void __launch_bounds__(LT) lode_vjp_v_kernel(LodeVjpArgs<T> v, int par, int ncol_max) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* W = reinterpret_cast<T*>(smem);  // [VJP_AT][ncol]
+    const int N = v.n_atoms, nk = v.n_k, S = v.n_species;
+    const int tl = blockIdx.z, t = v.types_present[tl];
+    const int i0 = blockIdx.y * VJP_AT;
+    const int ncol = v.ncol_v[par];
+    for (int idx = threadIdx.x; idx < VJP_AT * ncol; idx += LT) {
+        const int a = idx / ncol, cv = idx - a * ncol;
+        const int i = i0 + a;
+        T w = 0;
+        if (i < N) {
+            const int ct = v.type_index[i];
+            const long long pair = (long long)ct * S + t;
+            const int l = v.col_l[par][cv];
+            const int width = (2 * l + 1) * v.n;
+            const long long block = (long long)l * S * S + pair;
+            w = v.grad_values[v.value_base[block] + (v.value_row0[pair] + v.type_rank[i]) * width + v.col_in_l[par][cv]];
+        }
+        W[idx] = w;
+    }
+    __syncthreads();
+    const int k = blockIdx.x * LT + threadIdx.x;
+    if (k >= nk) return;
+    T acc[VJP_AT][3];
+#pragma unroll
+    for (int a = 0; a < VJP_AT; a++) acc[a][0] = acc[a][1] = acc[a][2] = 0;
+    const T* B = v.bg[par] + (long long)k * 3 * ncol;
+    for (int cv = 0; cv < ncol; cv++) {
+        const T b0 = B[cv], b1 = B[ncol + cv], b2 = B[2 * ncol + cv];
+#pragma unroll
+        for (int a = 0; a < VJP_AT; a++) {
+            const T w = W[a * ncol + cv];
+            acc[a][0] += b0 * w;
+            acc[a][1] += b1 * w;
+            acc[a][2] += b2 * w;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < VJP_AT; a++) {
+        const int i = i0 + a;
+        if (i >= N) break;
+#pragma unroll
+        for (int x = 0; x < 3; x++)
+            v.V[((((long long)par * v.n_types + tl) * 3 + x) * N + i) * nk + k] = acc[a][x];
+    }
+}
+
+// U_p[t][x][cos / sin][k] = Σ_i (cos_i, sin_i)(k) V_p[t][x][i][k], in FP64 and atom order
+template <typename T>
+__global__ void lode_vjp_u_kernel(LodeVjpArgs<T> v) {
+    const int N = v.n_atoms, nk = v.n_k;
+    const long long total = 2LL * v.n_types * 3 * nk;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int k = (int)(idx % nk);
+    const long long ptx = idx / nk;  // (par, tl, x)
+    const T* Vr = v.V + ptx * N * nk + k;
+    double uc = 0.0, us = 0.0;
+    for (int i = 0; i < N; i++) {
+        const double w = (double)Vr[(long long)i * nk];
+        uc += v.cosv[(long long)i * nk + k] * w;
+        us += v.sinv[(long long)i * nk + k] * w;
+    }
+    v.U[(ptx * 2 + 0) * nk + k] = uc;
+    v.U[(ptx * 2 + 1) * nk + k] = us;
+}
+
+// dA/dr_j: one wave per atom, lanes over k, FP64 sums with a fixed reduction order.
+// Pair rows (i != j), even l: A = 2 (s_i c_j - c_i s_j) -> 2 c_j U_s - 2 s_j U_c; odd l:
+// A = -2 (c_i c_j + s_i s_j) -> -2 c_j U_c - 2 s_j U_s, plus 2 V_j for the excluded
+// i = j term. Own rows, per type t: -SF_im (even) and SF_re - 2 [t = type j] (odd)
+// times V_p[t][x][j].
+template <typename T>
+__global__ void __launch_bounds__(LT) lode_vjp_atom_kernel(LodeVjpArgs<T> v) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (LT / 64) + threadIdx.x / 64;
+    if (j >= v.n_atoms) return;
+    const int N = v.n_atoms, nk = v.n_k, NT = v.n_types;
+    const int tj = v.type_index[j];
+    int tlj = 0;
+    for (int q = 0; q < NT; q++)
+        if (v.types_present[q] == tj) tlj = q;
+    double acc[3] = {0.0, 0.0, 0.0};
+    auto Vp = [&](int par, int tl, int x, int k) -> double {
+        return (double)v.V[((((long long)par * NT + tl) * 3 + x) * N + j) * nk + k];
+    };
+    auto Up = [&](int par, int tl, int x, int cs, int k) -> double {
+        return v.U[((((long long)par * NT + tl) * 3 + x) * 2 + cs) * nk + k];
+    };
+    for (int k = lane; k < nk; k += 64) {
+        const double cj = v.cosv[(long long)j * nk + k], sj = v.sinv[(long long)j * nk + k];
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            double g = 2.0 * cj * Up(0, tlj, x, 1, k) - 2.0 * sj * Up(0, tlj, x, 0, k);
+            g += -2.0 * cj * Up(1, tlj, x, 0, k) - 2.0 * sj * Up(1, tlj, x, 1, k) + 2.0 * Vp(1, tlj, x, k);
+            for (int tl = 0; tl < NT; tl++) {
+                const int t = v.types_present[tl];
+                const double sc = v.sumc[(long long)t * nk + k], ss = v.sums[(long long)t * nk + k];
+                const double im = 2.0 * (sj * sc - cj * ss), re = 2.0 * (cj * sc + sj * ss);
+                g += -im * Vp(0, tl, x, k) + (re - (t == tj ? 2.0 : 0.0)) * Vp(1, tl, x, k);
+            }
+            acc[x] += g;
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        double s = acc[x];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        acc[x] = s;
+    }
+    if (lane == 0) {
+        for (int x = 0; x < 3; x++) v.out[3LL * j + x] = (T)acc[x];
+    }
+}
+
 }  // namespace
 
 void launch_lode_phases(hipStream_t stream, int n_atoms, int n_k, const double* positions, const double* kvec, double* cosv,
@@ -456,9 +580,25 @@ void launch_lode_gemm(hipStream_t stream, int mode, const LodeGemmArgs<T>& g) {
     }
 }
 
+template <typename T>
+void launch_lode_vjp(hipStream_t stream, const LodeVjpArgs<T>& v) {
+    const int N = v.n_atoms, nk = v.n_k;
+    if (N == 0 || nk == 0 || v.n_types == 0) return;
+    for (int par = 0; par < 2; par++) {
+        const size_t shmem = (size_t)VJP_AT * std::max(v.ncol_v[par], 1) * sizeof(T);
+        dim3 grid((unsigned)((nk + LT - 1) / LT), (unsigned)((N + VJP_AT - 1) / VJP_AT), (unsigned)v.n_types);
+        lode_vjp_v_kernel<T><<<grid, LT, shmem, stream>>>(v, par, v.ncol_v[par]);
+    }
+    const long long total = 2LL * v.n_types * 3 * nk;
+    lode_vjp_u_kernel<T><<<(unsigned)((total + LT - 1) / LT), LT, 0, stream>>>(v);
+    lode_vjp_atom_kernel<T><<<(unsigned)((N + LT / 64 - 1) / (LT / 64)), LT, 0, stream>>>(v);
+}
+
 template void launch_lode_projection<float>(hipStream_t, const LodeProjectionArgs<float>&);
 template void launch_lode_projection<double>(hipStream_t, const LodeProjectionArgs<double>&);
 template void launch_lode_gemm<float>(hipStream_t, int, const LodeGemmArgs<float>&);
 template void launch_lode_gemm<double>(hipStream_t, int, const LodeGemmArgs<double>&);
+template void launch_lode_vjp<float>(hipStream_t, const LodeVjpArgs<float>&);
+template void launch_lode_vjp<double>(hipStream_t, const LodeVjpArgs<double>&);
 
 }  // namespace rascaline_amd

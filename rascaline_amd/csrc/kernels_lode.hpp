@@ -51,6 +51,36 @@ struct LodeGemmArgs {
     const long long* rows_per_sample;  // [S][S]
 };
 
+// Vector-Jacobian product of the position gradients of one system, without the
+// N^2 gradient rows: W = dA/dc in the values layout, then
+//   V_p[t][x][i](k) = Σ_col Bg_p[k][x][col] W(i, t, col)            (per parity p)
+//   U_p[t][x](k)    = Σ_i (cos_i, sin_i)(k) V_p[t][x][i](k)
+//   dA/dr_j = Σ_k of the pair terms (from U, minus the j = i term) and the self
+//   terms (from the per-type sums and V_p[.][x][j]).
+template <typename T>
+struct LodeVjpArgs {
+    int n_atoms, n_k, n_species, n_types, n;
+    const int* types_present;      // [n_types]
+    const int* type_index;         // [n_atoms]
+    const int* type_rank;          // [n_atoms]
+    const double* cosv;            // [n_atoms][n_k]
+    const double* sinv;
+    const double* sumc;            // [n_species][n_k]
+    const double* sums;
+    const T* bg[2];                // [n_k][3][ncol_v]
+    int ncol_v[2];
+    const int* col_l[2];
+    const int* col_in_l[2];
+    const T* grad_values;          // dA/dc, values layout
+    const long long* value_base;   // [L][S][S]
+    const long long* value_row0;   // [S][S] this system's first sample row
+    T* V;                          // [2][n_types][3][n_atoms][n_k]
+    double* U;                     // [2][n_types][3][2][n_k] (cos, sin)
+    T* out;                        // [n_atoms][3], this system's atoms
+};
+template <typename T>
+void launch_lode_vjp(hipStream_t stream, const LodeVjpArgs<T>& v);
+
 // cos / sin (k . r_i) in FP64, plus FP32 copies when cosf / sinf are not null
 void launch_lode_phases(hipStream_t stream, int n_atoms, int n_k, const double* positions, const double* kvec, double* cosv,
                         double* sinv, float* cosf, float* sinf);

@@ -303,6 +303,49 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         }
         launch_lode_projection<T>(stream, pa);
 
+        if (vjp_grad_) {
+            // vector-Jacobian product of the position gradients: O(N n_k) sums instead
+            // of the N^2 gradient rows
+            const int NT = static_cast<int>(present.size());
+            LodeVjpArgs<T> v;
+            v.n_atoms = N;
+            v.n_k = nk;
+            v.n_species = S;
+            v.n_types = NT;
+            v.n = n;
+            v.types_present = d_present;
+            v.type_index = d_tidx;
+            v.type_rank = d_trank;
+            v.cosv = d_cos;
+            v.sinv = d_sin;
+            v.sumc = d_sumc;
+            v.sums = d_sums;
+            for (int par = 0; par < 2; par++) {
+                v.bg[par] = pa.bg[par];
+                v.ncol_v[par] = ncol_v[par];
+                v.col_l[par] = d_col_l[par];
+                v.col_in_l[par] = d_col_in_l[par];
+            }
+            v.grad_values = static_cast<const T*>(vjp_grad_);
+            v.value_base = d_value_base;
+            v.value_row0 = d_vrow0;
+            v.V = ws_.get<T>("lode_vjp_v", std::max<size_t>(2ULL * NT * 3 * static_cast<size_t>(N) * nk, 1));
+            v.U = ws_.get<double>("lode_vjp_u", std::max<size_t>(2ULL * NT * 3 * 2 * static_cast<size_t>(nk), 1));
+            v.out = static_cast<T*>(vjp_out_) + 3 * a0;
+            RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+            launch_lode_vjp<T>(stream, v);
+            RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+            RA_HIP_CHECK(hipGetLastError());
+            // the per-system scratch buffers are reused by the next system
+            RA_HIP_CHECK(hipStreamSynchronize(stream));
+            float ms = 0.0f;
+            RA_HIP_CHECK(hipEventElapsedTime(&ms, ev_main_start_, ev_main_stop_));
+            main_ms_sum_ += ms;
+            // dA/dc read once (every atom, every type present, L² n values), dA/dr written once
+            main_bytes_ += (static_cast<long long>(N) * NT * L * L * n + 3LL * N) * static_cast<long long>(sizeof(T));
+            continue;
+        }
+
         auto gemm_args = [&](int par) {
             LodeGemmArgs<T> g;
             g.n_atoms = N;
@@ -363,6 +406,7 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         }
     }
     RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+    if (vjp_grad_) return nullptr;
 
     if (output) {
         // outputs stay in the device buffers; blocks in (center type, neighbour type)-major,

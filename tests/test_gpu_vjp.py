@@ -127,3 +127,40 @@ def test_positions_vjp_matches_materialised_rows():
     got = positions_vjp(calc, ds, w).cpu().numpy()
     scale = np.abs(expected).max()
     assert np.all(np.abs(got - expected) <= 1e-12 * (np.abs(expected) + scale))
+
+
+LODE = {
+    "density": {"type": "SmearedPowerLaw", "smearing": 1.0, "exponent": 1},
+    "basis": {"type": "TensorProduct", "max_angular": 3, "radial": {"type": "Gto", "max_radial": 3, "radius": 4.0}},
+}
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_lode_positions_vjp_vs_oracle(precision):
+    """LODE: the VJP from the structure factors against featomic-torch's contraction of
+    the oracle's N^2 gradient rows (two periodic systems, three types)."""
+    import torch
+
+    say("LODE positions VJP, precision", precision)
+    rng = np.random.default_rng(44)
+    systems = [random_system(rng, 14, "triclinic", 6.5, types=(1, 6, 8)),
+               random_system(rng, 10, "periodic", 6.0, types=(1, 8))]
+    ref = orc.lode_spherical_expansion([orc.System(p, t, c) for p, t, c in systems], LODE, ["positions"])
+    calc = ra.LodeSphericalExpansion(**LODE)
+    calc.set_precision(precision)
+    weights = [rng.normal(size=(len(b["samples"]), int(np.prod(b["values"].shape[1:])))) for b in ref["blocks"]]
+    offsets = np.concatenate([[0], np.cumsum([len(t) for _, t, _ in systems])])
+    expected = reference_vjp(ref, weights, offsets)
+    ds = DeviceSystems(systems)
+    dev = compute_device(calc, ds)
+    assert dev.n_blocks == len(ref["blocks"])
+    assert np.array_equal(dev.keys, np.asarray(ref["keys"]))
+    dtype = torch.float64 if precision == 64 else torch.float32
+    got = positions_vjp(calc, ds, device_weights(dev, weights, torch, dtype)).double().cpu().numpy()
+    scale = np.abs(expected).max()
+    err = np.abs(got - expected).max() / scale
+    say(f"  max |a - b| / max|b| = {err:.2e}")
+    if precision == 64:
+        assert np.all(np.abs(got - expected) <= 1e-10 * (np.abs(expected) + scale))
+    else:
+        assert err <= 1e-4
