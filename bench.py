@@ -133,14 +133,15 @@ def main():
     parser.add_argument("--streams", type=int, default=4,
                         help="calculators on their own streams that consecutive chunks alternate between")
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp"], default="soap",
+    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp", "lode-vjp"], default="soap",
                         help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
                              "expansion + gradients on periodic cells of 500-1500 atoms; bulk: power spectrum + "
                              "gradients of one 20 000-atom cell (12 radial functions, l <= 8, 5 types); "
                              "surrogate: spherical expansion + gradients of one 100 000-atom cell with the "
                              "spline and with the learned radial integral; vjp: the 4096 structures' power "
                              "spectrum values followed by the vector-Jacobian product of their position gradients "
-                             "(each its own JSON line)")
+                             "; lode-vjp: the LODE cells' values followed by the vector-Jacobian product of "
+                             "their position gradients (each its own JSON line)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +157,8 @@ def main():
         return surrogate_main(args)
     if args.workload == "vjp":
         return vjp_main(args)
+    if args.workload == "lode-vjp":
+        return lode_vjp_main(args)
 
     import torch
 
@@ -656,6 +659,77 @@ def vjp_main(args):
                                "(forward) + dA/dr from a synthetic dA/dX (backward), no gradient rows",
                    "atoms": n_atoms, "chunks": len(batches), "streams": args.streams},
         "roofline": hbm_roofline(k_ms, k_bytes, d_ms, "vjp_pair_kernel + vjp_gather_kernel (dA/dr per entry and atom)"),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def lode_vjp_main(args):
+    """LODE values of the 16 cells of the LODE workload, then the vector-Jacobian
+    product of their position gradients with a synthetic dA/dX, without the N^2
+    gradient rows (a forward and a backward call per cell)."""
+    import torch
+
+    import rascaline_amd as ra
+    from rascaline_amd import _c_lib as C
+    from rascaline_amd.datasets import lode_cells
+    from rascaline_amd.device import compute_device, positions_vjp
+
+    n = args.structures if args.structures != 4096 else 16
+    systems = lode_cells(seed=0, n_structures=n)
+    n_atoms = sum(len(s[1]) for s in systems)
+    batches = [device_batch(systems, [k]) for k in range(len(systems))]
+    calc = ra.LodeSphericalExpansion(**LODE_HYPERS)
+    calc.set_precision(args.precision)
+    dtype = torch.float32 if args.precision == 32 else torch.float64
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    weights = []
+    for b in batches:
+        dev = compute_device(calc, b)
+        weights.append(torch.randn(dev.value_size, dtype=dtype, device="cuda", generator=gen))
+    torch.cuda.synchronize()
+
+    def step():
+        for k, b in enumerate(batches):
+            compute_device(calc, b)
+            positions_vjp(calc, b, weights[k])
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"lode-vjp: warmup step {w} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / args.steps
+    rate = n_atoms * args.steps / elapsed
+    log(f"lode-vjp: timed {args.steps} steps: {ms:.1f} ms/step, {rate:.3e} atoms/s")
+    k_ms, k_bytes, d_ms = 0.0, 0, 0.0
+    for k, b in enumerate(batches):
+        positions_vjp(calc, b, weights[k])
+        t, km, kb = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t), ctypes.byref(km),
+                                                  ctypes.byref(kb)))
+        k_ms += km.value
+        k_bytes += kb.value
+        d_ms += t.value
+    result = {
+        "metric": "atoms/s LODE spherical expansion + vector-Jacobian product of the position gradients",
+        "value": rate, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (structure-factor phases f64)" if args.precision == 32 else "f64", "data": "synthetic",
+        "config": {"workload": f"{len(systems)} periodic triclinic cells of 500-1500 atoms as the LODE workload, "
+                               "values (forward) + dA/dr from a synthetic dA/dX (backward), no gradient rows",
+                   "atoms": n_atoms},
+        "roofline": {"bound": "fp32", "achieved": k_bytes / (k_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": k_bytes / (k_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "lode_vjp_v/u/atom kernels (V = Bg W products counted)", "kernel_ms_per_step": k_ms,
+                     "device_ms_per_step": d_ms, "algorithmic_flops_per_step": k_bytes},
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)

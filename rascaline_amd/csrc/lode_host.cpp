@@ -341,8 +341,9 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
             float ms = 0.0f;
             RA_HIP_CHECK(hipEventElapsedTime(&ms, ev_main_start_, ev_main_stop_));
             main_ms_sum_ += ms;
-            // dA/dc read once (every atom, every type present, L² n values), dA/dr written once
-            main_bytes_ += (static_cast<long long>(N) * NT * L * L * n + 3LL * N) * static_cast<long long>(sizeof(T));
+            // algorithmic flops of the V products (the dominant work; the reported count
+            // of the LODE main kernels is in flops): 2 NT 3 N n_k (columns of both parities)
+            main_bytes_ += 2LL * NT * 3 * static_cast<long long>(N) * nk * (ncol_v[0] + ncol_v[1]);
             continue;
         }
 
