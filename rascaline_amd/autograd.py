@@ -60,8 +60,9 @@ class _SoapValues(torch.autograd.Function):
 
 
 def soap_values(calculator, systems, positions):
-    """Values of a SOAP calculator (spherical expansion or power spectrum) as one flat
-    tensor plus its BlockLayout; differentiable with respect to `positions`."""
+    """Values of a calculator (SOAP spherical expansion or power spectrum, LODE
+    spherical expansion) as one flat tensor plus its BlockLayout; differentiable with
+    respect to `positions`."""
     values = _SoapValues.apply(positions, calculator, systems)
     # the layout of this computation (the same batch gives the same layout)
     layout = values.grad_fn.layout if values.grad_fn is not None else None

@@ -3,7 +3,7 @@ is the vector-Jacobian product of the position gradients.
 
 * torch.autograd.gradcheck (fast mode: a random direction, central finite
   differences of the forward in FP64) on small periodic and cluster systems, for
-  the power spectrum and the spherical expansion;
+  the power spectrum, the spherical expansion and LODE (periodic only);
 * loss.backward() on Σ W · values against featomic-torch's contraction of the FP64
   oracle's gradient rows with W (1e-10).
 """
@@ -39,13 +39,24 @@ def small_systems(seed):
     return [a, b]
 
 
-@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion"])
+LODE = {
+    "density": {"type": "SmearedPowerLaw", "smearing": 1.0, "exponent": 1},
+    "basis": {"type": "TensorProduct", "max_angular": 2, "radial": {"type": "Gto", "max_radial": 2, "radius": 3.0}},
+}
+
+
+@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion", "lode"])
 def test_gradcheck(kind):
     import torch
 
     say("gradcheck", kind)
     systems = small_systems(3)
-    calc = ra.SoapPowerSpectrum(**SOAP) if kind == "power_spectrum" else ra.SphericalExpansion(**SOAP)
+    if kind == "lode":
+        rng = np.random.default_rng(4)
+        systems = [random_system(rng, 6, "triclinic", 5.0, types=(1, 8)), random_system(rng, 5, "periodic", 4.5, types=(1, 6))]
+        calc = ra.LodeSphericalExpansion(**LODE)
+    else:
+        calc = ra.SoapPowerSpectrum(**SOAP) if kind == "power_spectrum" else ra.SphericalExpansion(**SOAP)
     calc.set_precision(64)
     pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda",
                        requires_grad=True)
