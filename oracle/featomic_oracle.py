@@ -525,6 +525,34 @@ def soap_spherical_expansion(systems, hypers, gradients=()):
 # SOAP power spectrum
 
 
+def soap_radial_spectrum(systems, hypers, gradients=()):
+    """SoapRadialSpectrum (featomic/src/calculators/soap/radial_spectrum.rs:75-135,
+    :230-330): the spherical expansion with one angular channel (l = 0), keys
+    (center_type, neighbor_type), no components, properties n; gradients keep the
+    expansion's rows with the o3_mu component dropped."""
+    basis = dict(hypers["basis"])
+    se_hypers = {"cutoff": hypers["cutoff"], "density": hypers["density"],
+                 "basis": {"type": "TensorProduct", "max_angular": 0, "radial": basis["radial"]}}
+    if "spline_accuracy" in basis:
+        se_hypers["basis"]["spline_accuracy"] = basis["spline_accuracy"]
+    se = soap_spherical_expansion(systems, se_hypers, gradients)
+    keys = np.asarray(se["keys"])[:, 2:4]
+    blocks = []
+    for blk in se["blocks"]:
+        ns = len(blk["samples"])
+        out = _block(blk["values"].reshape(ns, -1), blk["samples_names"], blk["samples"], [],
+                     blk["properties_names"], blk["properties"])
+        for param, g in blk["gradients"].items():
+            rows = len(g["samples"])
+            shape = g["values"].shape  # [rows][gradient components...][1][n]
+            values = g["values"].reshape(shape[:-2] + (shape[-1],))
+            comps = g["components"][:-1]
+            gb = _block(values, g["samples_names"], g["samples"], comps, g["properties_names"], g["properties"])
+            out["gradients"][param] = gb
+        blocks.append(out)
+    return {"keys_names": ["center_type", "neighbor_type"], "keys": keys, "blocks": blocks}
+
+
 def soap_power_spectrum(systems, hypers, gradients=()):
     comp = SoapComputation(systems, hypers, gradients)
     lmax, N = comp.lmax, comp.N

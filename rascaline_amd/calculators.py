@@ -32,6 +32,15 @@ class SoapPowerSpectrum(CalculatorBase):
         super().__init__("soap_power_spectrum", json.dumps(_to_json(parameters)))
 
 
+class SoapRadialSpectrum(CalculatorBase):
+    """SOAP radial spectrum <n | rho_i^{a}> (the l = 0 spherical expansion), with the
+    reference's parameters: cutoff, density and basis = {radial, spline_accuracy}."""
+
+    def __init__(self, *, cutoff, density, basis):
+        parameters = {"cutoff": cutoff, "density": density, "basis": basis}
+        super().__init__("soap_radial_spectrum", json.dumps(_to_json(parameters)))
+
+
 class LodeSphericalExpansion(CalculatorBase):
     """Long-distance equivariant (LODE) spherical expansion, computed in k-space."""
 

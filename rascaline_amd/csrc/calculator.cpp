@@ -70,9 +70,13 @@ Calculator::Calculator(const std::string& registry_name, const std::string& para
     } else if (registry_name == "lode_spherical_expansion") {
         kind_ = Kind::LodeSphericalExpansion;
         lode_ = parse_lode_params(parameters);
+    } else if (registry_name == "soap_radial_spectrum") {
+        // the l = 0 spherical expansion, relabelled (soap/radial_spectrum.rs:75-135)
+        kind_ = Kind::RadialSpectrum;
+        soap_ = parse_soap_params(parameters, true, true);
     } else if (registry_name == "atomic_composition" || registry_name == "dummy_calculator" ||
                registry_name == "neighbor_list" || registry_name == "sorted_distances" ||
-               registry_name == "spherical_expansion_by_pair" || registry_name == "soap_radial_spectrum") {
+               registry_name == "spherical_expansion_by_pair") {
         throw Error::invalid_parameter("the '" + registry_name + "' calculator is not provided by this GPU implementation");
     } else {
         throw Error::invalid_parameter("unknown calculator with name '" + registry_name + "'");
@@ -125,6 +129,7 @@ const char* Calculator::name() const {
     case Kind::SphericalExpansion: return "spherical expansion";
     case Kind::PowerSpectrum: return "SOAP power spectrum";
     case Kind::LodeSphericalExpansion: return "lode spherical expansion";
+    case Kind::RadialSpectrum: return "radial spectrum";
     }
     return "";
 }
@@ -258,6 +263,8 @@ template <typename T>
 mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& grads, hipStream_t stream, bool to_host,
                                       featomic_amd_device_output_t* output) {
     const bool ps = kind_ == Kind::PowerSpectrum;
+    // the radial spectrum runs as the l = 0 spherical expansion, with its own labels
+    const bool rs = kind_ == Kind::RadialSpectrum;
     const int n_atoms = static_cast<int>(batch.n_atoms());
     const Species sp = species_of(batch);
     const int S = static_cast<int>(sp.types.size());
@@ -278,8 +285,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
     auto tm = to_host ? std::make_unique<mts_tensormap_t>() : nullptr;
     if (tm) {
-        tm->keys = ps ? LabelsData({"center_type", "neighbor_1_type", "neighbor_2_type"}, {})
-                      : LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, {});
+        tm->keys = ps   ? LabelsData({"center_type", "neighbor_1_type", "neighbor_2_type"}, {})
+                   : rs ? LabelsData({"center_type", "neighbor_type"}, {})
+                        : LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, {});
     }
     if (n_atoms == 0) {
         RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
@@ -292,7 +300,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         if (output) {
             device_result_ = DeviceResult();
             std::memset(output, 0, sizeof(*output));
-            output->key_size = ps ? 3 : 4;
+            output->key_size = ps ? 3 : (rs ? 2 : 4);
             output->precision = sizeof(T) == 4 ? 32 : 64;
         }
         return tm.release();
@@ -688,7 +696,11 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             refs.push_back({b, 0});
         } else {
             for (int l = 0; l < L; l++) {
-                keys.insert(keys.end(), {l, 1, sp.types[ct], sp.types[t1]});
+                if (rs) {
+                    keys.insert(keys.end(), {sp.types[ct], sp.types[t1]});
+                } else {
+                    keys.insert(keys.end(), {l, 1, sp.types[ct], sp.types[t1]});
+                }
                 out_ns.push_back(ns);
                 out_nr.push_back(nr);
                 out_voff.push_back(value_base[b * L + l]);
@@ -704,7 +716,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
     if (output) {
         device_result_.keys = keys;
-        device_result_.key_size = ps ? 3 : 4;
+        device_result_.key_size = ps ? 3 : (rs ? 2 : 4);
         device_result_.n_samples = out_ns;
         device_result_.n_rows = out_nr;
         device_result_.value_offsets = out_voff;
@@ -744,7 +756,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     RA_HIP_CHECK(hipStreamSynchronize(stream));
     if (flag) throw Error::invalid_parameter("an atom has more than 2048 neighbours within the cutoff");
 
-    const size_t key_size = ps ? 3 : 4;
+    const size_t key_size = ps ? 3 : (rs ? 2 : 4);
     tm->keys.values = keys;
     std::vector<int32_t> prop_values;
     if (ps) {
@@ -767,6 +779,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         std::vector<uintptr_t> shape;
         if (ps) {
             shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(P)};
+        } else if (rs) {
+            shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(n)};
         } else {
             comps.push_back(mu_labels(l));
             shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(2 * l + 1), static_cast<uintptr_t>(n)};
@@ -893,7 +907,7 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
             for (int k = 0; k < static_cast<int>(soap_window_shapes().size()); k++) {
                 if (mixed && n == 8 && soap_window_shapes()[k][0] > 4) continue;
                 // the spherical-expansion rows leave without store staging
-                if (kind_ == Kind::SphericalExpansion && soap_window_shapes()[k][2] != 1) continue;
+                if (kind_ != Kind::PowerSpectrum && soap_window_shapes()[k][2] != 1) continue;
                 candidates.push_back(k);
             }
             if (!mixed) candidates.push_back(-2);

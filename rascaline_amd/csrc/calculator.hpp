@@ -55,7 +55,7 @@ struct HostBatch {
     int64_t n_atoms() const { return offsets.empty() ? 0 : offsets.back(); }
 };
 
-enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion };
+enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion, RadialSpectrum };
 
 // Launch shapes of the SOAP kernels that the autotuner chooses (0 / -1: the
 // built-in choice).

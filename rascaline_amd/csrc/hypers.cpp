@@ -52,15 +52,22 @@ std::string tag(const Json& obj, const char* context) {
 
 // SphericalExpansionBasis<RadialBasis> with a GTO radial basis
 // (featomic/src/calculators/shared/basis/mod.rs:9-116, basis/radial/mod.rs:10-88)
+// With `radial_only`, the RadialSpectrumBasis of the radial spectrum
+// (soap/radial_spectrum.rs:44-58): {radial, spline_accuracy}, l = 0 only.
 void parse_basis(const Json& basis, int* max_angular, int* n_radial, double* gto_radius, bool* has_radius,
-                 bool* use_spline, double* accuracy) {
-    std::string type = tag(basis, "basis");
-    if (type == "Explicit") {
-        throw Error::invalid_parameter("the Explicit basis is not supported by this implementation, use TensorProduct");
+                 bool* use_spline, double* accuracy, bool radial_only = false) {
+    if (radial_only) {
+        deny_unknown(basis, {"radial", "spline_accuracy"});
+        *max_angular = 0;
+    } else {
+        std::string type = tag(basis, "basis");
+        if (type == "Explicit") {
+            throw Error::invalid_parameter("the Explicit basis is not supported by this implementation, use TensorProduct");
+        }
+        if (type != "TensorProduct") throw Error::json("unknown variant `" + type + "` for basis");
+        deny_unknown(basis, {"type", "max_angular", "radial", "spline_accuracy"});
+        *max_angular = unsigned_int(basis, "max_angular", "basis");
     }
-    if (type != "TensorProduct") throw Error::json("unknown variant `" + type + "` for basis");
-    deny_unknown(basis, {"type", "max_angular", "radial", "spline_accuracy"});
-    *max_angular = unsigned_int(basis, "max_angular", "basis");
     const Json& radial = require(basis, "radial", "basis");
     std::string rtype = tag(radial, "radial basis");
     if (rtype == "Tabulated") {
@@ -96,7 +103,7 @@ void parse_basis(const Json& basis, int* max_angular, int* n_radial, double* gto
 
 }  // namespace
 
-SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields) {
+SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields, bool radial_spectrum) {
     Json root = parse_json(text);
     if (!root.is_object()) throw Error::json("invalid type, expected a map");
     if (deny_unknown_fields) deny_unknown(root, {"cutoff", "density", "basis"});
@@ -162,7 +169,8 @@ SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields) 
     const Json& basis = require(root, "basis", "parameters");
     double gto_radius = 0.0;
     bool has_radius = false;
-    parse_basis(basis, &p.max_angular, &p.n_radial, &gto_radius, &has_radius, &p.use_spline, &p.spline_accuracy);
+    parse_basis(basis, &p.max_angular, &p.n_radial, &gto_radius, &has_radius, &p.use_spline, &p.spline_accuracy,
+                radial_spectrum);
     if (has_radius && gto_radius != p.cutoff) {
         // featomic/src/calculators/soap/radial_integral/gto.rs:39-47
         throw Error::internal("GTO radius must be the same as the cutoff radius in SOAP, or should not provided");

@@ -74,7 +74,9 @@ struct LodeParams {
 // `deny_unknown_fields = true` mirrors serde's attribute on the SE parameters;
 // the power spectrum parameters do not deny unknown top-level fields
 // (featomic/src/calculators/soap/power_spectrum.rs:33-35).
-SoapParams parse_soap_params(const std::string& json, bool deny_unknown_fields);
+// radial_spectrum: the radial spectrum's parameters (basis {radial, spline_accuracy}),
+// parsed as a spherical expansion with max_angular = 0
+SoapParams parse_soap_params(const std::string& json, bool deny_unknown_fields, bool radial_spectrum = false);
 LodeParams parse_lode_params(const std::string& json);
 
 }  // namespace rascaline_amd

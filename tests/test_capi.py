@@ -238,3 +238,20 @@ def test_positions_vjp_host_checks():
         assert "grad_values" in last_error()
     finally:
         C.lib().featomic_calculator_free(calc)
+
+
+def test_radial_spectrum_parameters():
+    radial = {"cutoff": {"radius": 4.0, "smoothing": {"type": "ShiftedCosine", "width": 0.5}},
+              "density": {"type": "Gaussian", "width": 0.3},
+              "basis": {"radial": {"type": "Gto", "max_radial": 3}}}
+    calc = make("soap_radial_spectrum", json.dumps(radial))
+    try:
+        buf = ctypes.create_string_buffer(64)
+        assert C.lib().featomic_calculator_name(calc, buf, 64) == FEATOMIC_SUCCESS
+        assert buf.value == b"radial spectrum"
+    finally:
+        C.lib().featomic_calculator_free(calc)
+    # the basis of the radial spectrum has no type or max_angular (deny_unknown_fields)
+    bad = dict(radial, basis={"type": "TensorProduct", "max_angular": 2, "radial": {"type": "Gto", "max_radial": 3}})
+    assert not C.lib().featomic_calculator(b"soap_radial_spectrum", json.dumps(bad).encode())
+    assert "unknown field" in last_error()

@@ -268,3 +268,34 @@ def test_ps_fast_form_bulk_per_l_and_type_pair(nmax, lmax, n_atoms):
         assert len(ev) == L * len(oracle.keys.values)
     assert max(worst[True]) <= 1e-4, worst[True]
     assert max(worst[False]) <= 1e-4, worst[False]
+
+
+# ---------------------------------------------------------------------------- radial spectrum
+RADIAL_SPECTRUM = {
+    "cutoff": {"radius": 4.5, "smoothing": {"type": "ShiftedCosine", "width": 0.5}},
+    "density": {"type": "Gaussian", "width": 0.3},
+    "basis": {"radial": {"type": "Gto", "max_radial": 7}, "spline_accuracy": 1e-8},
+}
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_radial_spectrum_vs_oracle(precision):
+    """SoapRadialSpectrum (the l = 0 expansion, relabelled) against the oracle: keys
+    (center_type, neighbor_type), no components, values and position / strain / cell
+    gradients; FP64 within 1e-10, FP32 within 2e-4 (the comparator of this file)."""
+    rng = np.random.default_rng(61)
+    systems = []
+    for kind, n, L in [("periodic", 20, 6.5), ("triclinic", 16, 6.0), ("cluster", 12, 8.0)]:
+        pos, types, cell = random_system(rng, n, kind, L, types=(1, 6, 8))
+        systems.append(orc.System(pos, types, cell))
+    grads = ["positions", "strain", "cell"]
+    expected = oracle_to_tensormap(orc.soap_radial_spectrum(systems, RADIAL_SPECTRUM, grads))
+    calc = ra.SoapRadialSpectrum(**RADIAL_SPECTRUM)
+    calc.set_precision(precision)
+    got = calc.compute(ra_systems(systems), gradients=grads)
+    assert got.keys.names == ["center_type", "neighbor_type"]
+    for block in got.blocks():
+        assert len(block.components) == 0
+        assert block.properties.names == ["n"]
+    tol = 1e-10 if precision == 64 else 2e-4
+    compare_tensormaps(got, expected, tol, tol, gradients=grads)

@@ -104,3 +104,35 @@ def test_lode_gradients(p):
     expected = load_npy_gz(f"lode-spherical-expansion/exponent-{p}/positions-gradient.npy.gz")
     assert_relative_eq(sum_gradients(n_atoms, tm.block().gradient("positions")), expected, 1e-8,
                        1e-9 * np.abs(expected).max())
+
+
+def test_oracle_radial_spectrum_is_the_l0_expansion():
+    """The radial spectrum (soap/radial_spectrum.rs:75-135) is the l = 0 block of the
+    spherical expansion: the oracle's relabelled expansion with max_angular = 0 equals
+    the l = 0 blocks of an expansion with max_angular = 3 (the l = 0 radial functions
+    and spline do not depend on max_angular), values and all gradients."""
+    import featomic_oracle as orc
+
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 6.0, size=(10, 3))
+    types = np.array([1, 6, 8, 1, 1, 6, 8, 1, 6, 1], dtype=np.int32)
+    system = orc.System(pos, types, np.eye(3) * 6.0)
+    radial = {"type": "Gto", "max_radial": 4}
+    cutoff = {"radius": 4.0, "smoothing": {"type": "ShiftedCosine", "width": 0.5}}
+    density = {"type": "Gaussian", "width": 0.4}
+    grads = ["positions", "strain", "cell"]
+    rs = orc.soap_radial_spectrum([system], {"cutoff": cutoff, "density": density,
+                                             "basis": {"radial": radial}}, grads)
+    se = orc.soap_spherical_expansion([system], {"cutoff": cutoff, "density": density,
+                                                 "basis": {"type": "TensorProduct", "max_angular": 3, "radial": radial}},
+                                      grads)
+    l0 = [k for k, key in enumerate(se["keys"]) if key[0] == 0]
+    assert [tuple(k) for k in rs["keys"]] == [tuple(se["keys"][k][2:]) for k in l0]
+    for blk, k in zip(rs["blocks"], l0):
+        ref = se["blocks"][k]
+        assert np.array_equal(blk["samples"], ref["samples"])
+        assert np.allclose(blk["values"], ref["values"][:, 0, :], rtol=1e-12, atol=1e-14)
+        for param in grads:
+            g, gr = blk["gradients"][param], ref["gradients"][param]
+            assert np.array_equal(g["samples"], gr["samples"])
+            assert np.allclose(g["values"], gr["values"][..., 0, :], rtol=1e-12, atol=1e-14)
