@@ -15,7 +15,8 @@ the reference's label builders:
   ``featomic/src/labels/samples/atom_centered.rs:31-150``;
 * gradient row values: ``soap/spherical_expansion.rs:426-604``;
 * SOAP power spectrum: ``soap/power_spectrum.rs:82-767`` and ``labels/keys.rs:101-170``;
-* LODE: ``lode/spherical_expansion.rs:463-880``, ``labels/samples/long_range.rs:25-78``.
+* LODE: ``lode/spherical_expansion.rs:463-880``, ``labels/samples/long_range.rs:25-78``;
+* NeighborList: ``calculators/neighbor_list.rs:108-620``.
 
 Outputs are plain dictionaries::
 
@@ -290,6 +291,80 @@ def neighbors(positions, cell, cutoff):
         arr = np.zeros(0, dtype=dtype)
     L.orc_free(ctypes.cast(pp, ctypes.c_void_p))
     return arr
+
+
+def neighbor_list(systems, params, gradients=()):
+    """NeighborList calculator (featomic/src/calculators/neighbor_list.rs): the pair
+    vectors of the half list of each system, relabelled per (first, second) atom-type
+    key. Half list: one sample per pair, stored from the lower type to the higher one
+    (atoms swapped, shift and vector negated when the pair is inverted,
+    neighbor_list.rs:202-365); full list: each pair in both directions
+    (neighbor_list.rs:416-620); self pairs (i, i, 0, 0, 0) with a zero vector after
+    each system's pairs. Position gradients: -I for the first atom, +I for the second,
+    none for a true self pair (neighbor_list.rs:108-127); a pair between an atom and
+    its own periodic image gets one zero row."""
+    cutoff = float(params["cutoff"])
+    full = bool(params["full_neighbor_list"])
+    self_pairs = bool(params.get("self_pairs", False))
+    per_system = [neighbors(s.positions, s.cell, cutoff) for s in systems]
+    keys = set()
+    for s, pairs in zip(systems, per_system):
+        for p in pairs:
+            a, b = int(s.types[p["first"]]), int(s.types[p["second"]])
+            if full:
+                keys.update({(a, b), (b, a)})
+            else:
+                keys.add((min(a, b), max(a, b)))
+        if self_pairs:
+            keys.update((int(t), int(t)) for t in s.types)
+    keys = sorted(keys)
+    samples = {k: [] for k in keys}
+    vectors = {k: [] for k in keys}
+
+    def add(key, system, first, second, shift, vector):
+        samples[key].append([system, int(first), int(second)] + [int(x) for x in shift])
+        vectors[key].append([float(x) for x in vector])
+
+    for si, (s, pairs) in enumerate(zip(systems, per_system)):
+        for p in pairs:
+            i, j = int(p["first"]), int(p["second"])
+            a, b = int(s.types[i]), int(s.types[j])
+            shift, vec = p["shift"], p["vector"]
+            if full:
+                add((a, b), si, i, j, shift, vec)
+                add((b, a), si, j, i, -shift, -vec)
+            elif a <= b:
+                add((a, b), si, i, j, shift, vec)
+            else:
+                add((b, a), si, j, i, -shift, -vec)
+        if self_pairs:
+            for i, t in enumerate(s.types):
+                add((int(t), int(t)), si, i, i, (0, 0, 0), (0.0, 0.0, 0.0))
+
+    names = ["system", "first_atom", "second_atom", "cell_shift_a", "cell_shift_b", "cell_shift_c"]
+    pair_xyz = ("pair_xyz", np.arange(3, dtype=np.int32).reshape(3, 1))
+    blocks = []
+    for k in keys:
+        smp = samples[k]
+        values = np.asarray(vectors[k], dtype=np.float64).reshape(len(smp), 3, 1)
+        blk = _block(values, names, smp, [pair_xyz], ["distance"], [[1]])
+        if "positions" in gradients:
+            rows, gvals = [], []
+            for r, (system, first, second, sa, sb, sc) in enumerate(smp):
+                if first == second and (sa, sb, sc) == (0, 0, 0):
+                    continue
+                if first == second:
+                    rows.append([r, system, first])
+                    gvals.append(np.zeros((3, 3)))
+                    continue
+                rows += [[r, system, first], [r, system, second]]
+                gvals += [-np.eye(3), np.eye(3)]
+            gv = np.asarray(gvals, dtype=np.float64).reshape(len(rows), 3, 3, 1)
+            blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], rows, [_xyz(), pair_xyz],
+                                                   ["distance"], [[1]])
+        blocks.append(blk)
+    return {"keys_names": ["first_atom_type", "second_atom_type"],
+            "keys": np.asarray(keys, dtype=np.int32).reshape(len(keys), 2), "blocks": blocks}
 
 
 def spherical_harmonics(lmax, direction, gradients=True):

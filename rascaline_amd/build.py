@@ -25,6 +25,7 @@ SOURCES = [
     ("scan.hip", []),
     ("calculator.cpp", []),
     ("lode_host.cpp", []),
+    ("neighbor_list_host.cpp", []),
     ("hypers.cpp", []),
     ("radial.cpp", []),
     ("tensormap.cpp", []),

@@ -1,4 +1,4 @@
-"""The SOAP and LODE calculators (python/featomic/featomic/calculators.py:120-307)."""
+"""The SOAP, LODE and neighbour-list calculators (python/featomic/featomic/calculators.py:44-307)."""
 
 import json
 
@@ -47,3 +47,17 @@ class LodeSphericalExpansion(CalculatorBase):
     def __init__(self, *, density, basis, k_cutoff=None):
         parameters = {"k_cutoff": k_cutoff, "density": density, "basis": basis}
         super().__init__("lode_spherical_expansion", json.dumps(_to_json(parameters)))
+
+
+class NeighborList(CalculatorBase):
+    """Pair vectors of the neighbour list within ``cutoff``, one block per (first,
+    second) atom-type pair (featomic/src/calculators/neighbor_list.rs; the Python
+    class of python/featomic/featomic/calculators.py:44-74). A half list stores each
+    pair once, from the lower type to the higher; a full list stores both
+    directions; ``self_pairs`` adds (i, i) pairs with a zero vector. Samples are
+    (system, first_atom, second_atom, cell_shift_a/b/c), one component ``pair_xyz``
+    and one property ``distance`` = 1. The pairs come from the GPU cell list."""
+
+    def __init__(self, *, cutoff, full_neighbor_list, self_pairs=False):
+        parameters = {"cutoff": cutoff, "full_neighbor_list": full_neighbor_list, "self_pairs": self_pairs}
+        super().__init__("neighbor_list", json.dumps(_to_json(parameters)))

@@ -57,6 +57,84 @@ int device_lds_limit() {
 
 }  // namespace
 
+// GPU neighbour lists of the systems of a host batch as half lists of featomic_pair_t:
+// j > i, or j == i with the shift in the positive half-space
+// (featomic/src/systems/neighbors.rs:253-286). Pairs come sorted by (first, second,
+// shift a, b, c); the reference sorts by (first, second) only
+// (neighbors.rs:391-396), leaving pairs of one atom pair with several shifts in an
+// unspecified order.
+std::vector<std::vector<featomic_pair_t>> gpu_half_pairs(const HostBatch& batch, double cutoff, Workspace& ws) {
+    const int64_t n_atoms = batch.n_atoms();
+    const size_t n_systems = batch.n_systems();
+    std::vector<std::vector<featomic_pair_t>> result(n_systems);
+    if (n_atoms == 0) return result;
+    hipStream_t stream = nullptr;
+    double* d_pos = ws.get<double>("positions", 3 * n_atoms);
+    int* d_sys = ws.get<int>("atom_system", n_atoms);
+    std::vector<int> atom_system(n_atoms);
+    for (size_t s = 0; s < n_systems; s++) {
+        for (int64_t a = batch.offsets[s]; a < batch.offsets[s + 1]; a++) atom_system[a] = static_cast<int>(s);
+    }
+    RA_HIP_CHECK(hipMemcpy(d_pos, batch.positions.data(), sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice));
+    RA_HIP_CHECK(hipMemcpy(d_sys, atom_system.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
+    auto nl = build_neighbors(ws, batch, cutoff, d_pos, d_sys, stream);
+    std::vector<long long> offsets(n_atoms + 1);
+    std::vector<int> j(nl.n_entries), sh(3 * nl.n_entries);
+    std::vector<double> v(3 * nl.n_entries), d(nl.n_entries);
+    RA_HIP_CHECK(hipMemcpy(offsets.data(), nl.offsets, sizeof(long long) * (n_atoms + 1), hipMemcpyDeviceToHost));
+    if (nl.n_entries) {
+        RA_HIP_CHECK(hipMemcpy(j.data(), nl.entry_j, sizeof(int) * nl.n_entries, hipMemcpyDeviceToHost));
+        RA_HIP_CHECK(hipMemcpy(sh.data(), nl.entry_shift, sizeof(int) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
+        RA_HIP_CHECK(hipMemcpy(v.data(), nl.entry_vec, sizeof(double) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
+        RA_HIP_CHECK(hipMemcpy(d.data(), nl.entry_dist, sizeof(double) * nl.n_entries, hipMemcpyDeviceToHost));
+    }
+    for (size_t s = 0; s < n_systems; s++) {
+        std::vector<featomic_pair_t>& out = result[s];
+        const int64_t first_atom = batch.offsets[s];
+        for (int64_t gi = first_atom; gi < batch.offsets[s + 1]; gi++) {
+            // entry_j holds batch-global atom indices
+            const int i = static_cast<int>(gi - first_atom);
+            for (long long e = offsets[gi]; e < offsets[gi + 1]; e++) {
+                const int jj = static_cast<int>(j[e] - first_atom);
+                const int* t = &sh[3 * e];
+                bool keep = jj > i;
+                if (jj == i) {
+                    const int sum = t[0] + t[1] + t[2];
+                    keep = !(sum < 0 || (sum == 0 && (t[2] < 0 || (t[2] == 0 && t[1] < 0))));
+                }
+                if (!keep) continue;
+                featomic_pair_t p;
+                p.first = static_cast<uintptr_t>(i);
+                p.second = static_cast<uintptr_t>(jj);
+                p.distance = d[e];
+                p.vector[0] = v[3 * e];
+                p.vector[1] = v[3 * e + 1];
+                p.vector[2] = v[3 * e + 2];
+                p.cell_shift_indices[0] = t[0];
+                p.cell_shift_indices[1] = t[1];
+                p.cell_shift_indices[2] = t[2];
+                out.push_back(p);
+                if (p.distance * p.distance < 1e-3) {
+                    log_message(FEATOMIC_LOG_LEVEL_WARN, "atoms " + std::to_string(p.first) + " and " +
+                                                             std::to_string(p.second) + " are very close to one another (" +
+                                                             std::to_string(p.distance) + " A)");
+                }
+            }
+        }
+    }
+    return result;
+}
+
+std::vector<featomic_pair_t> gpu_half_pairs(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff) {
+    HostBatch batch;
+    batch.offsets = {0, static_cast<int64_t>(n_atoms)};
+    if (n_atoms) batch.positions.assign(positions, positions + 3 * n_atoms);
+    batch.types.assign(n_atoms, 0);
+    batch.cells.assign(cell, cell + 9);
+    Workspace ws;
+    return std::move(gpu_half_pairs(batch, cutoff, ws)[0]);
+}
+
 // ---------------------------------------------------------------------------- construction
 
 Calculator::Calculator(const std::string& registry_name, const std::string& parameters)
@@ -74,9 +152,14 @@ Calculator::Calculator(const std::string& registry_name, const std::string& para
         // the l = 0 spherical expansion, relabelled (soap/radial_spectrum.rs:75-135)
         kind_ = Kind::RadialSpectrum;
         soap_ = parse_soap_params(parameters, true, true);
+    } else if (registry_name == "neighbor_list") {
+        // pair vectors of the neighbour list (neighbor_list.rs:31-44)
+        kind_ = Kind::NeighborList;
+        nl_ = parse_neighbor_list_params(parameters);
+        cutoffs_.push_back(nl_.cutoff);
+        return;
     } else if (registry_name == "atomic_composition" || registry_name == "dummy_calculator" ||
-               registry_name == "neighbor_list" || registry_name == "sorted_distances" ||
-               registry_name == "spherical_expansion_by_pair") {
+               registry_name == "sorted_distances" || registry_name == "spherical_expansion_by_pair") {
         throw Error::invalid_parameter("the '" + registry_name + "' calculator is not provided by this GPU implementation");
     } else {
         throw Error::invalid_parameter("unknown calculator with name '" + registry_name + "'");
@@ -130,6 +213,7 @@ const char* Calculator::name() const {
     case Kind::PowerSpectrum: return "SOAP power spectrum";
     case Kind::LodeSphericalExpansion: return "lode spherical expansion";
     case Kind::RadialSpectrum: return "radial spectrum";
+    case Kind::NeighborList: return "neighbors list";
     }
     return "";
 }
@@ -220,7 +304,7 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
 // ---------------------------------------------------------------------------- radial surrogate
 
 SurrogateReport Calculator::train_radial_surrogate(const SurrogateOptions& options) {
-    if (kind_ == Kind::LodeSphericalExpansion) {
+    if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList) {
         throw Error::invalid_parameter("the radial surrogate is only available for the SOAP calculators");
     }
     check_surrogate_options(options);
@@ -826,10 +910,11 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
 mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& grads) {
     hipStream_t stream = nullptr;
+    if (kind_ == Kind::NeighborList) return run_neighbor_list(batch, grads);
     if (kind_ == Kind::LodeSphericalExpansion) {
         if (grads.strain || grads.cell) {
             throw Error::invalid_parameter("the lode spherical expansion calculator does not support gradients with respect to " +
-                                           std::string(grads.strain ? "strain" : "cell"));
+                                           std::string(grads.cell ? "cell" : "strain"));
         }
         return precision == 32 ? run_lode<float>(batch, grads, stream, nullptr)
                                : run_lode<double>(batch, grads, stream, nullptr);
@@ -840,6 +925,9 @@ mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& gr
 
 void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
                                 featomic_amd_device_output_t* output) {
+    if (kind_ == Kind::NeighborList) {
+        throw Error::invalid_parameter("the neighbors list calculator returns host TensorMaps only (use featomic_calculator_compute)");
+    }
     if (kind_ == Kind::LodeSphericalExpansion) {
         if (precision == 32) {
             run_lode<float>(batch, grads, stream, output);
@@ -865,7 +953,9 @@ const std::vector<std::array<int, 3>>& soap_window_shapes() {
 }
 
 std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads, hipStream_t stream, int repeats) {
-    if (kind_ == Kind::LodeSphericalExpansion) throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
+    if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList) {
+        throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
+    }
     if (repeats < 1 || repeats > 100) throw Error::invalid_parameter("autotune repeats must be in [1, 100]");
     featomic_amd_device_output_t out{};
     // best time of one phase over `repeats` runs after a warm-up run (which also sizes
@@ -968,6 +1058,7 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
 
 void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
     if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
+    if (kind_ == Kind::NeighborList) throw Error::invalid_parameter("the positions VJP covers the SOAP and LODE calculators");
     vjp_grad_ = grad_values;
     vjp_out_ = out;
     main_ms_sum_ = -1.0;

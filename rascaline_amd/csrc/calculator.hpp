@@ -55,7 +55,7 @@ struct HostBatch {
     int64_t n_atoms() const { return offsets.empty() ? 0 : offsets.back(); }
 };
 
-enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion, RadialSpectrum };
+enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion, RadialSpectrum, NeighborList };
 
 // Launch shapes of the SOAP kernels that the autotuner chooses (0 / -1: the
 // built-in choice).
@@ -87,7 +87,7 @@ public:
     const std::string& parameters() const { return parameters_; }
     const std::vector<double>& cutoffs() const { return cutoffs_; }
     Kind kind() const { return kind_; }
-    double nl_cutoff() const { return soap_.cutoff; }
+    double nl_cutoff() const { return kind_ == Kind::NeighborList ? nl_.cutoff : soap_.cutoff; }
 
     int precision = 64;
     // fast form: mixed-precision (bfloat16 hi + lo on MFMA, FP32 accumulation) contractions
@@ -130,6 +130,7 @@ private:
     Kind kind_;
     SoapParams soap_;
     LodeParams lode_;
+    NeighborListParams nl_;
     RadialTables tables_;
     std::vector<double> self_contribution_;  // SOAP l=0 self term / LODE centre contribution
     Workspace ws_;
@@ -160,6 +161,8 @@ private:
     template <typename T>
     mts_tensormap_t* run_soap(const HostBatch& batch, const Gradients& gradients, hipStream_t stream, bool to_host,
                               featomic_amd_device_output_t* output);
+    // neighbor_list calculator: pair vectors from the GPU half list, host TensorMap
+    mts_tensormap_t* run_neighbor_list(const HostBatch& batch, const Gradients& gradients);
     template <typename T>
     mts_tensormap_t* run_lode(const HostBatch& batch, const Gradients& gradients, hipStream_t stream,
                               featomic_amd_device_output_t* output);
@@ -185,6 +188,14 @@ struct DeviceNeighbors {
     double* entry_vec = nullptr;    // 3 per entry
     double* entry_dist = nullptr;
 };
+
+// Sends a message to the logging callback set with featomic_set_logging_callback.
+void log_message(int level, const std::string& message);
+
+// The GPU neighbour list of one system (host positions and cell) as a half list.
+std::vector<featomic_pair_t> gpu_half_pairs(const double* positions, uintptr_t n_atoms, const double* cell, double cutoff);
+// The same for every system of a host batch (one GPU build for the batch).
+std::vector<std::vector<featomic_pair_t>> gpu_half_pairs(const HostBatch& batch, double cutoff, Workspace& ws);
 
 // Uploads the batch geometry and builds the full neighbour list (GPU cell list,
 // or the caller's half list when provided).

@@ -2,6 +2,7 @@
 // featomic/src/c_api/*.rs) over the GPU calculators, plus the extension entry
 // points declared in include/featomic_amd.h.
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -20,11 +21,21 @@ struct featomic_calculator_t {
 };
 
 namespace {
+featomic_logging_callback_t LOGGING_CALLBACK = nullptr;
+}  // namespace
+
+namespace rascaline_amd {
+void log_message(int level, const std::string& message) {
+    if (LOGGING_CALLBACK) LOGGING_CALLBACK(level, message.c_str());
+}
+}  // namespace rascaline_amd
+
+namespace {
 
 using rascaline_amd::Error;
+using rascaline_amd::log_message;
 
 thread_local std::string LAST_ERROR;
-featomic_logging_callback_t LOGGING_CALLBACK = nullptr;
 
 // time spans for featomic_profiling_*, c_api/profiling.rs:17-91
 struct Profiler {
@@ -50,10 +61,6 @@ struct ScopedSpan {
         s.total_ms += ms;
     }
 };
-
-void log_message(int level, const std::string& message) {
-    if (LOGGING_CALLBACK) LOGGING_CALLBACK(level, message.c_str());
-}
 
 // catch every exception at the ABI (c_api/status.rs:82-88)
 template <typename F>
@@ -147,9 +154,12 @@ rascaline_amd::Gradients parse_gradients(const char* const* gradients, uintptr_t
                                            "\", should be one of \"positions\", \"cell\", or \"strain\"");
         }
     }
-    if (kind == rascaline_amd::Kind::LodeSphericalExpansion && (g.strain || g.cell)) {
-        throw Error::invalid_parameter(std::string("the lode spherical expansion calculator does not support gradients with respect to ") +
-                                       (g.strain ? "strain" : "cell"));
+    // calculator.rs:342-380: only the SOAP calculators have cell and strain gradients
+    // (cell is checked first)
+    if ((kind == rascaline_amd::Kind::LodeSphericalExpansion || kind == rascaline_amd::Kind::NeighborList) && (g.strain || g.cell)) {
+        const std::string name = kind == rascaline_amd::Kind::NeighborList ? "neighbors list" : "lode spherical expansion";
+        throw Error::invalid_parameter("the " + name + " calculator does not support gradients with respect to " +
+                                       (g.cell ? "cell" : "strain"));
     }
     return g;
 }
@@ -234,6 +244,10 @@ featomic_status_t featomic_calculator_compute(featomic_calculator_t* calculator,
                         "instead. Please review the documentation carefully.");
         }
         const bool soap = impl.kind() != rascaline_amd::Kind::LodeSphericalExpansion;
+        if (impl.kind() == rascaline_amd::Kind::NeighborList && !(impl.nl_cutoff() > 0.0 && std::isfinite(impl.nl_cutoff()))) {
+            // neighbor_list.rs:67-68 asserts this before anything else
+            throw Error::internal("assertion failed: self.cutoff > 0.0 && self.cutoff.is_finite()");
+        }
         auto batch = gather_systems(systems, systems_count, options.use_native_system, soap ? impl.nl_cutoff() : 0.0, soap);
         *descriptor = impl.compute(batch, grads);
     });
@@ -411,62 +425,7 @@ featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_at
         check_pointer(n_pairs, "n_pairs");
         if (n_atoms > 0) check_pointer(positions, "positions");
         if (!(cutoff > 0.0)) throw Error::invalid_parameter("cutoff must be positive");
-        rascaline_amd::HostBatch batch;
-        batch.offsets = {0, static_cast<int64_t>(n_atoms)};
-        batch.positions.assign(positions, positions + 3 * n_atoms);
-        batch.types.assign(n_atoms, 0);
-        batch.cells.assign(cell, cell + 9);
-        rascaline_amd::Workspace ws;
-        hipStream_t stream = nullptr;
-        double* d_pos = ws.get<double>("positions", 3 * std::max<uintptr_t>(n_atoms, 1));
-        int* d_sys = ws.get<int>("atom_system", std::max<uintptr_t>(n_atoms, 1));
-        std::vector<int> zeros(n_atoms, 0);
-        if (n_atoms) {
-            RA_HIP_CHECK(hipMemcpy(d_pos, positions, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice));
-            RA_HIP_CHECK(hipMemcpy(d_sys, zeros.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
-        }
-        auto nl = rascaline_amd::build_neighbors(ws, batch, cutoff, d_pos, d_sys, stream);
-        std::vector<long long> offsets(n_atoms + 1);
-        std::vector<int> j(nl.n_entries), s(3 * nl.n_entries);
-        std::vector<double> v(3 * nl.n_entries), d(nl.n_entries);
-        RA_HIP_CHECK(hipMemcpy(offsets.data(), nl.offsets, sizeof(long long) * (n_atoms + 1), hipMemcpyDeviceToHost));
-        if (nl.n_entries) {
-            RA_HIP_CHECK(hipMemcpy(j.data(), nl.entry_j, sizeof(int) * nl.n_entries, hipMemcpyDeviceToHost));
-            RA_HIP_CHECK(hipMemcpy(s.data(), nl.entry_shift, sizeof(int) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
-            RA_HIP_CHECK(hipMemcpy(v.data(), nl.entry_vec, sizeof(double) * 3 * nl.n_entries, hipMemcpyDeviceToHost));
-            RA_HIP_CHECK(hipMemcpy(d.data(), nl.entry_dist, sizeof(double) * nl.n_entries, hipMemcpyDeviceToHost));
-        }
-        // half list: j > i, or j == i with the shift in the positive half-space
-        // (featomic/src/systems/neighbors.rs:253-286)
-        std::vector<featomic_pair_t> out;
-        for (uintptr_t i = 0; i < n_atoms; i++) {
-            for (long long e = offsets[i]; e < offsets[i + 1]; e++) {
-                const int jj = j[e];
-                const int* sh = &s[3 * e];
-                bool keep = jj > static_cast<int>(i);
-                if (jj == static_cast<int>(i)) {
-                    const int sum = sh[0] + sh[1] + sh[2];
-                    keep = !(sum < 0 || (sum == 0 && (sh[2] < 0 || (sh[2] == 0 && sh[1] < 0))));
-                }
-                if (!keep) continue;
-                featomic_pair_t p;
-                p.first = i;
-                p.second = static_cast<uintptr_t>(jj);
-                p.distance = d[e];
-                p.vector[0] = v[3 * e];
-                p.vector[1] = v[3 * e + 1];
-                p.vector[2] = v[3 * e + 2];
-                p.cell_shift_indices[0] = sh[0];
-                p.cell_shift_indices[1] = sh[1];
-                p.cell_shift_indices[2] = sh[2];
-                out.push_back(p);
-                if (p.distance * p.distance < 1e-3) {
-                    log_message(FEATOMIC_LOG_LEVEL_WARN, "atoms " + std::to_string(p.first) + " and " +
-                                                             std::to_string(p.second) + " are very close to one another (" +
-                                                             std::to_string(p.distance) + " A)");
-                }
-            }
-        }
+        std::vector<featomic_pair_t> out = rascaline_amd::gpu_half_pairs(positions, n_atoms, cell, cutoff);
         *n_pairs = out.size();
         *pairs = static_cast<featomic_pair_t*>(std::malloc(sizeof(featomic_pair_t) * std::max<size_t>(out.size(), 1)));
         if (!out.empty()) std::memcpy(*pairs, out.data(), sizeof(featomic_pair_t) * out.size());

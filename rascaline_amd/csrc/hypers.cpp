@@ -218,4 +218,22 @@ LodeParams parse_lode_params(const std::string& text) {
     return p;
 }
 
+NeighborListParams parse_neighbor_list_params(const std::string& text) {
+    Json root = parse_json(text);
+    if (!root.is_object()) throw Error::json("invalid type, expected a map");
+    NeighborListParams p;
+    p.cutoff = number(root, "cutoff", "parameters");
+    for (const char* key : {"full_neighbor_list", "self_pairs"}) {
+        const Json& v = require(root, key, "parameters");
+        bool b = false;
+        try {
+            b = v.as_bool();
+        } catch (const JsonError&) {
+            throw Error::json(std::string("invalid type for `") + key + "`, expected a boolean");
+        }
+        (std::string(key) == "self_pairs" ? p.self_pairs : p.full_neighbor_list) = b;
+    }
+    return p;
+}
+
 }  // namespace rascaline_amd

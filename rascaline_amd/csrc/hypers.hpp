@@ -71,6 +71,14 @@ struct LodeParams {
     int L2() const { return (max_angular + 1) * (max_angular + 1); }
 };
 
+// NeighborList parameters (featomic/src/calculators/neighbor_list.rs:31-44): three
+// required fields, unknown fields ignored
+struct NeighborListParams {
+    double cutoff = 0.0;
+    bool full_neighbor_list = false;
+    bool self_pairs = false;
+};
+
 // `deny_unknown_fields = true` mirrors serde's attribute on the SE parameters;
 // the power spectrum parameters do not deny unknown top-level fields
 // (featomic/src/calculators/soap/power_spectrum.rs:33-35).
@@ -78,5 +86,6 @@ struct LodeParams {
 // parsed as a spherical expansion with max_angular = 0
 SoapParams parse_soap_params(const std::string& json, bool deny_unknown_fields, bool radial_spectrum = false);
 LodeParams parse_lode_params(const std::string& json);
+NeighborListParams parse_neighbor_list_params(const std::string& json);
 
 }  // namespace rascaline_amd

@@ -255,3 +255,41 @@ def test_radial_spectrum_parameters():
     bad = dict(radial, basis={"type": "TensorProduct", "max_angular": 2, "radial": {"type": "Gto", "max_radial": 3}})
     assert not C.lib().featomic_calculator(b"soap_radial_spectrum", json.dumps(bad).encode())
     assert "unknown field" in last_error()
+
+
+def test_neighbor_list_parameters():
+    """NeighborList parameters (featomic/src/calculators/neighbor_list.rs:31-44): three
+    required fields, unknown fields ignored (no deny_unknown_fields); name "neighbors
+    list" (neighbor_list.rs:56-58); one cutoff."""
+    params = {"cutoff": 3.5, "full_neighbor_list": True, "self_pairs": False}
+    calc = make("neighbor_list", dict(params, extra="ignored"))
+    assert calc, last_error()
+    try:
+        buf = ctypes.create_string_buffer(64)
+        assert C.lib().featomic_calculator_name(calc, buf, 64) == FEATOMIC_SUCCESS
+        assert buf.value == b"neighbors list"
+        cutoffs = ctypes.POINTER(ctypes.c_double)()
+        count = ctypes.c_size_t()
+        assert C.lib().featomic_calculator_cutoffs(calc, ctypes.byref(cutoffs), ctypes.byref(count)) == FEATOMIC_SUCCESS
+        assert [cutoffs[k] for k in range(count.value)] == [3.5]
+    finally:
+        C.lib().featomic_calculator_free(calc)
+    for missing in params:
+        assert not make("neighbor_list", {k: v for k, v in params.items() if k != missing})
+        assert last_error() == f"json error: missing field `{missing}`"
+    assert not make("neighbor_list", dict(params, self_pairs=1))
+    assert last_error().startswith("json error: invalid type for `self_pairs`")
+    nl = calculators.NeighborList(cutoff=2.0, full_neighbor_list=False)
+    assert json.loads(nl.parameters) == {"cutoff": 2.0, "full_neighbor_list": False, "self_pairs": False}
+
+
+def test_neighbor_list_rejects_cell_gradients():
+    """Only positions gradients (neighbor_list.rs:100-106); the check runs before
+    any system is read (calculator.rs:342-380)."""
+    import rascaline_amd as ra
+
+    nl = ra.NeighborList(cutoff=2.0, full_neighbor_list=False)
+    system = ra.System(np.array([1, 1]), np.zeros((2, 3)), np.eye(3) * 5.0)
+    for grad in ("cell", "strain"):
+        with pytest.raises(C.FeatomicError, match=f"the neighbors list calculator does not support gradients with respect to {grad}"):
+            nl.compute(system, gradients=[grad])

@@ -1,0 +1,147 @@
+"""GPU tests of the NeighborList calculator (featomic/src/calculators/neighbor_list.rs):
+pair vectors from the GPU cell list, relabelled per atom-type pair.
+
+* the reference's known answers (neighbor_list.rs:640-958: half and full lists of
+  the water and CH test systems, self pairs, an empty result) on labels, and the
+  oracle bit for bit on values;
+* random batches (periodic, triclinic, cluster systems; half / full, with and
+  without self pairs, position gradients): keys, samples, gradient samples and every
+  value bit-identical to the FP64 oracle (the pair vectors come from the same FP64
+  arithmetic, the gradients are 0 / +-1);
+* use_native_system = False (the caller's pairs) gives the same TensorMap.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import featomic_oracle as orc
+import rascaline_amd as ra
+from helpers import oracle_to_tensormap, random_system
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def water():
+    return (np.array([[0.0, 0.0, 0.0], [0.0, 0.75545, -0.58895], [0.0, -0.75545, -0.58895]]),
+            np.array([-42, 1, 1], dtype=np.int32), np.eye(3) * 3.0)
+
+
+def ch():
+    return np.array([[0.0, 0.0, 0.0], [0.0, 1.2, 0.0]]), np.array([6, 1], dtype=np.int32), np.eye(3) * 10.0
+
+
+def compute(systems, params, gradients=(), native=True):
+    calc = ra.NeighborList(**params)
+    if native:
+        ra_sys = [ra.System(t, p, c) for p, t, c in systems]
+    else:
+        ra_sys = [ra.System(t, p, c, pairs=ra.neighbors(p, c, params["cutoff"])) for p, t, c in systems]
+    return calc.compute(ra_sys, gradients=list(gradients), use_native_system=native)
+
+
+def assert_bit_identical(got, systems, params, gradients=()):
+    ref = oracle_to_tensormap(orc.neighbor_list([orc.System(p, t, c) for p, t, c in systems], params, gradients))
+    assert got.keys.names == ref.keys.names
+    assert np.array_equal(got.keys.values, ref.keys.values), (got.keys.values, ref.keys.values)
+    for ba, be in zip(got.blocks(), ref.blocks()):
+        assert ba.samples == be.samples
+        assert ba.properties == be.properties
+        assert len(ba.components) == 1 and ba.components[0] == be.components[0]
+        va, ve = np.asarray(ba.values), np.asarray(be.values)
+        assert va.shape == ve.shape
+        assert np.array_equal(va.view(np.uint64), ve.view(np.uint64))
+        for param in gradients:
+            ga, ge = ba.gradient(param), be.gradient(param)
+            assert ga.samples == ge.samples
+            assert [c.names for c in ga.components] == [["xyz"], ["pair_xyz"]]
+            assert np.array_equal(np.asarray(ga.values), np.asarray(ge.values))
+    return ref
+
+
+def samples_of(tm, key):
+    keys = [tuple(k) for k in np.asarray(tm.keys.values).tolist()]
+    return np.asarray(tm.block(keys.index(tuple(key))).samples.values).tolist()
+
+
+def test_half_and_full_water():
+    # neighbor_list.rs:640-770
+    say("NeighborList water, half and full")
+    half = {"cutoff": 2.0, "full_neighbor_list": False, "self_pairs": False}
+    tm = compute([water()], half, ["positions"])
+    assert np.asarray(tm.keys.values).tolist() == [[-42, 1], [1, 1]]
+    assert samples_of(tm, (-42, 1)) == [[0, 0, 1, 0, 0, 0], [0, 0, 2, 0, 0, 0]]
+    assert samples_of(tm, (1, 1)) == [[0, 1, 2, 0, 0, 0], [0, 1, 2, 0, 1, 0]]
+    hh = np.asarray(tm.block(1).values)[..., 0]
+    np.testing.assert_allclose(hh, [[0.0, -1.5109, 0.0], [0.0, 1.4891, 0.0]], rtol=1e-6)
+    assert_bit_identical(tm, [water()], half, ["positions"])
+
+    full = dict(half, full_neighbor_list=True)
+    tm = compute([water()], full, ["positions"])
+    assert np.asarray(tm.keys.values).tolist() == [[-42, 1], [1, -42], [1, 1]]
+    assert samples_of(tm, (1, -42)) == [[0, 1, 0, 0, 0, 0], [0, 2, 0, 0, 0, 0]]
+    assert samples_of(tm, (1, 1)) == [[0, 1, 2, 0, 0, 0], [0, 2, 1, 0, 0, 0], [0, 1, 2, 0, 1, 0], [0, 2, 1, 0, -1, 0]]
+    np.testing.assert_allclose(np.asarray(tm.block(1).values)[..., 0],
+                               [[0.0, -0.75545, 0.58895], [0.0, 0.75545, 0.58895]], rtol=1e-6)
+    assert_bit_identical(tm, [water()], full, ["positions"])
+
+
+def test_periodic_self_images_ch():
+    # neighbor_list.rs:772-838
+    say("NeighborList CH, cutoff larger than the cell")
+    half = {"cutoff": 12.0, "full_neighbor_list": False, "self_pairs": False}
+    tm = compute([ch()], half, ["positions"])
+    assert np.asarray(tm.keys.values).tolist() == [[1, 1], [1, 6], [6, 6]]
+    assert samples_of(tm, (1, 1)) == [[0, 1, 1, 0, 0, 1], [0, 1, 1, 0, 1, 0], [0, 1, 1, 1, 0, 0]]
+    np.testing.assert_allclose(np.asarray(tm.block(0).values)[..., 0], [[0, 0, 10], [0, 10, 0], [10, 0, 0]], rtol=1e-6)
+    assert_bit_identical(tm, [ch()], half, ["positions"])
+    full = dict(half, full_neighbor_list=True)
+    tm = compute([ch()], full, ["positions"])
+    assert np.asarray(tm.keys.values).tolist() == [[1, 1], [1, 6], [6, 1], [6, 6]]
+    assert samples_of(tm, (1, 1)) == [[0, 1, 1, 0, 0, 1], [0, 1, 1, 0, 0, -1], [0, 1, 1, 0, 1, 0],
+                                      [0, 1, 1, 0, -1, 0], [0, 1, 1, 1, 0, 0], [0, 1, 1, -1, 0, 0]]
+    assert_bit_identical(tm, [ch()], full, ["positions"])
+
+
+def test_self_pairs_and_empty():
+    # neighbor_list.rs:906-958
+    say("NeighborList self pairs, empty result")
+    params = {"cutoff": 2.0, "full_neighbor_list": True, "self_pairs": True}
+    tm = compute([water()], params, ["positions"])
+    assert np.asarray(tm.keys.values).tolist() == [[-42, -42], [-42, 1], [1, -42], [1, 1]]
+    assert np.asarray(tm.block(3).samples.values).tolist() == [
+        [0, 1, 2, 0, 0, 0], [0, 2, 1, 0, 0, 0], [0, 1, 2, 0, 1, 0], [0, 2, 1, 0, -1, 0], [0, 1, 1, 0, 0, 0], [0, 2, 2, 0, 0, 0]]
+    # true self pairs have no gradient rows
+    assert len(tm.block(0).gradient("positions").samples) == 0
+    assert_bit_identical(tm, [water()], params, ["positions"])
+    tm = compute([water()], {"cutoff": 0.1, "full_neighbor_list": True, "self_pairs": False})
+    assert np.asarray(tm.keys.values).reshape(-1, 2).shape == (0, 2)
+    assert len(tm.blocks()) == 0
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("self_pairs", [False, True])
+def test_random_batch_vs_oracle(full, self_pairs):
+    say("NeighborList random batch, full", full, "self pairs", self_pairs)
+    rng = np.random.default_rng(11 + 2 * full + self_pairs)
+    systems = [random_system(rng, 24, "periodic", 7.0), random_system(rng, 17, "triclinic", 6.0, types=(1, 8)),
+               random_system(rng, 20, "cluster", 9.0), random_system(rng, 3, "periodic", 2.5, types=(6,))]
+    params = {"cutoff": 3.7, "full_neighbor_list": full, "self_pairs": self_pairs}
+    tm = compute(systems, params, ["positions"])
+    ref = assert_bit_identical(tm, systems, params, ["positions"])
+    assert sum(len(b.samples) for b in ref.blocks()) > 100
+
+
+def test_user_pairs_match_native():
+    say("NeighborList use_native_system = False")
+    rng = np.random.default_rng(5)
+    systems = [random_system(rng, 20, "triclinic", 6.5), random_system(rng, 12, "cluster", 7.0)]
+    params = {"cutoff": 3.2, "full_neighbor_list": True, "self_pairs": True}
+    a = compute(systems, params, ["positions"], native=False)
+    assert_bit_identical(a, systems, params, ["positions"])

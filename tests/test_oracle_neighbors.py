@@ -123,3 +123,120 @@ def test_brute_force_bit_exact(seed):
         assert (int(p["first"]), int(p["second"]), tuple(int(x) for x in p["shift"])) == (i, j, s)
         assert tuple(p["vector"]) == tuple(v)
         assert p["distance"] == d
+
+
+# ---------------------------------------------------------------- NeighborList calculator
+# Known answers of featomic/src/calculators/neighbor_list.rs:640-958, on the test
+# systems of featomic/src/systems/test_utils.rs:29-43.
+
+def water():
+    return orc.System([[0.0, 0.0, 0.0], [0.0, 0.75545, -0.58895], [0.0, -0.75545, -0.58895]], [-42, 1, 1],
+                      np.eye(3) * 3.0)
+
+
+def ch():
+    return orc.System([[0.0, 0.0, 0.0], [0.0, 1.2, 0.0]], [6, 1], np.eye(3) * 10.0)
+
+
+def nl_block(tm, key):
+    keys = [tuple(k) for k in np.asarray(tm["keys"]).tolist()]
+    return tm["blocks"][keys.index(tuple(key))]
+
+
+def test_neighbor_list_half_water():
+    # neighbor_list.rs:640-692
+    tm = orc.neighbor_list([water()], {"cutoff": 2.0, "full_neighbor_list": False, "self_pairs": False})
+    assert np.asarray(tm["keys"]).tolist() == [[-42, 1], [1, 1]]
+    oh = nl_block(tm, (-42, 1))
+    assert oh["properties"].tolist() == [[1]]
+    assert [c[0] for c in oh["components"]] == ["pair_xyz"]
+    assert oh["samples"].tolist() == [[0, 0, 1, 0, 0, 0], [0, 0, 2, 0, 0, 0]]
+    np.testing.assert_allclose(oh["values"][..., 0], [[0.0, 0.75545, -0.58895], [0.0, -0.75545, -0.58895]], rtol=1e-6)
+    hh = nl_block(tm, (1, 1))
+    assert hh["samples"].tolist() == [[0, 1, 2, 0, 0, 0], [0, 1, 2, 0, 1, 0]]
+    np.testing.assert_allclose(hh["values"][..., 0], [[0.0, -1.5109, 0.0], [0.0, 1.4891, 0.0]], rtol=1e-6)
+
+
+def test_neighbor_list_full_water():
+    # neighbor_list.rs:694-770
+    tm = orc.neighbor_list([water()], {"cutoff": 2.0, "full_neighbor_list": True, "self_pairs": False})
+    assert np.asarray(tm["keys"]).tolist() == [[-42, 1], [1, -42], [1, 1]]
+    oh = nl_block(tm, (-42, 1))
+    assert oh["samples"].tolist() == [[0, 0, 1, 0, 0, 0], [0, 0, 2, 0, 0, 0]]
+    np.testing.assert_allclose(oh["values"][..., 0], [[0.0, 0.75545, -0.58895], [0.0, -0.75545, -0.58895]], rtol=1e-6)
+    ho = nl_block(tm, (1, -42))
+    assert ho["samples"].tolist() == [[0, 1, 0, 0, 0, 0], [0, 2, 0, 0, 0, 0]]
+    np.testing.assert_allclose(ho["values"][..., 0], [[0.0, -0.75545, 0.58895], [0.0, 0.75545, 0.58895]], rtol=1e-6)
+    hh = nl_block(tm, (1, 1))
+    assert hh["samples"].tolist() == [[0, 1, 2, 0, 0, 0], [0, 2, 1, 0, 0, 0], [0, 1, 2, 0, 1, 0], [0, 2, 1, 0, -1, 0]]
+    np.testing.assert_allclose(hh["values"][..., 0],
+                               [[0.0, -1.5109, 0.0], [0.0, 1.5109, 0.0], [0.0, 1.4891, 0.0], [0.0, -1.4891, 0.0]],
+                               rtol=1e-6)
+
+
+def test_neighbor_list_periodic_ch():
+    # neighbor_list.rs:772-838: pairs of an atom with its own images
+    half = orc.neighbor_list([ch()], {"cutoff": 12.0, "full_neighbor_list": False, "self_pairs": False})
+    assert np.asarray(half["keys"]).tolist() == [[1, 1], [1, 6], [6, 6]]
+    hh = nl_block(half, (1, 1))
+    assert hh["samples"].tolist() == [[0, 1, 1, 0, 0, 1], [0, 1, 1, 0, 1, 0], [0, 1, 1, 1, 0, 0]]
+    np.testing.assert_allclose(hh["values"][..., 0], [[0, 0, 10], [0, 10, 0], [10, 0, 0]], rtol=1e-6)
+    full = orc.neighbor_list([ch()], {"cutoff": 12.0, "full_neighbor_list": True, "self_pairs": False})
+    assert np.asarray(full["keys"]).tolist() == [[1, 1], [1, 6], [6, 1], [6, 6]]
+    hh = nl_block(full, (1, 1))
+    assert hh["samples"].tolist() == [[0, 1, 1, 0, 0, 1], [0, 1, 1, 0, 0, -1], [0, 1, 1, 0, 1, 0],
+                                      [0, 1, 1, 0, -1, 0], [0, 1, 1, 1, 0, 0], [0, 1, 1, -1, 0, 0]]
+    np.testing.assert_allclose(hh["values"][..., 0],
+                               [[0, 0, 10], [0, 0, -10], [0, 10, 0], [0, -10, 0], [10, 0, 0], [-10, 0, 0]], rtol=1e-6)
+
+
+def test_neighbor_list_self_pairs():
+    # neighbor_list.rs:906-940
+    tm = orc.neighbor_list([water()], {"cutoff": 2.0, "full_neighbor_list": True, "self_pairs": True})
+    assert np.asarray(tm["keys"]).tolist() == [[-42, -42], [-42, 1], [1, -42], [1, 1]]
+    assert tm["blocks"][3]["samples"].tolist() == [[0, 1, 2, 0, 0, 0], [0, 2, 1, 0, 0, 0], [0, 1, 2, 0, 1, 0],
+                                                   [0, 2, 1, 0, -1, 0], [0, 1, 1, 0, 0, 0], [0, 2, 2, 0, 0, 0]]
+
+
+def test_neighbor_list_empty():
+    # neighbor_list.rs:942-958
+    tm = orc.neighbor_list([water()], {"cutoff": 0.1, "full_neighbor_list": True, "self_pairs": False})
+    assert np.asarray(tm["keys"]).shape == (0, 2)
+    assert tm["blocks"] == []
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("system", ["water", "ch", "random"])
+def test_neighbor_list_gradients_finite_differences(full, system):
+    """Central differences of the pair vectors against the gradient rows (the
+    reference's finite_differences_positions test, neighbor_list.rs:840-864, with a
+    displacement of 1e-6); the CH system at cutoff 12 adds pairs of an atom with its
+    own periodic images, whose derivative is zero."""
+    if system == "water":
+        sys_, cutoff = water(), 1.0
+    elif system == "ch":
+        sys_, cutoff = ch(), 12.0
+    else:
+        rng = np.random.default_rng(3)
+        sys_ = orc.System(rng.uniform(0, 4.0, size=(7, 3)), rng.choice([1, 8], size=7), np.eye(3) * 4.0 + 0.3 * rng.normal(size=(3, 3)))
+        cutoff = 3.1
+    params = {"cutoff": cutoff, "full_neighbor_list": full, "self_pairs": True}
+    ref = orc.neighbor_list([sys_], params, ["positions"])
+    h = 1e-6
+    for atom in range(len(sys_)):
+        for x in range(3):
+            moved = []
+            for sign in (1, -1):
+                pos = sys_.positions.copy()
+                pos[atom, x] += sign * h
+                moved.append(orc.neighbor_list([orc.System(pos, sys_.types, sys_.cell)], params))
+            for b, blk in enumerate(ref["blocks"]):
+                plus, minus = moved[0]["blocks"][b], moved[1]["blocks"][b]
+                assert np.array_equal(plus["samples"], blk["samples"]) and np.array_equal(minus["samples"], blk["samples"])
+                fd = (plus["values"] - minus["values"]) / (2 * h)
+                analytic = np.zeros_like(blk["values"])
+                g = blk["gradients"]["positions"]
+                for r, (sample, _, a) in enumerate(g["samples"]):
+                    if a == atom:
+                        analytic[sample] += g["values"][r, x]
+                np.testing.assert_allclose(analytic, fd, atol=1e-8)
