@@ -16,7 +16,8 @@ the reference's label builders:
 * gradient row values: ``soap/spherical_expansion.rs:426-604``;
 * SOAP power spectrum: ``soap/power_spectrum.rs:82-767`` and ``labels/keys.rs:101-170``;
 * LODE: ``lode/spherical_expansion.rs:463-880``, ``labels/samples/long_range.rs:25-78``;
-* NeighborList: ``calculators/neighbor_list.rs:108-620``.
+* NeighborList: ``calculators/neighbor_list.rs:108-620``;
+* SphericalExpansionByPair: ``soap/spherical_expansion_pair.rs:222-300, 420-840``.
 
 Outputs are plain dictionaries::
 
@@ -123,6 +124,7 @@ def lib():
         L.orc_soap_new.argtypes = [ctypes.POINTER(SoapHypers)]
         L.orc_soap_free.argtypes = [vp]
         L.orc_soap_self_contribution.argtypes = [vp, vp]
+        L.orc_soap_pair_contribution.argtypes = [vp, d, vp, ctypes.c_int, vp, vp]
         L.orc_cutoff_smoothing.restype = d
         L.orc_cutoff_smoothing.argtypes = [vp, d]
         L.orc_cutoff_smoothing_gradient.restype = d
@@ -598,6 +600,114 @@ def soap_spherical_expansion(systems, hypers, gradients=()):
 
 # ----------------------------------------------------------------------------
 # SOAP power spectrum
+
+
+def soap_spherical_expansion_by_pair(systems, hypers, gradients=()):
+    """SphericalExpansionByPair (featomic/src/calculators/soap/spherical_expansion_pair.rs):
+    one sample per directed pair, the contribution of the second atom to the density
+    around the first.
+
+    * keys (o3_lambda, 1, first_atom_type, second_atom_type): the full-neighbour-list
+      type pairs with self pairs (neighbor_list.rs:388-414), each with every l
+      (spherical_expansion_pair.rs:560-583), in that order;
+    * samples: the full neighbour list's samples (neighbor_list.rs:416-489), with the
+      self pairs (i, i, 0, 0, 0) only in the l = 0 blocks (:585-637);
+    * values: f Y_lm I_nl of the pair vector (orc_soap_pair_contribution); the
+      reversed pair gets (-1)^l times the values and -(-1)^l times the vector
+      gradients (PairContribution::inverse_pair, :127-155); self pairs hold the self
+      contribution at l = 0 (:222-300);
+    * position gradients: -g for the first atom, +g for the second, one row (their
+      sum) for a pair of an atom with its own image, none for a self pair (:639-663,
+      :450-493); strain [xyz_2, xyz_1] with data v[a] g[b]; cell [xyz, abc] with data
+      shift[a] g[b] (:495-540).
+    """
+    h = soap_hypers(hypers)
+    lmax, N = h.max_angular, h.n_radial
+    L2 = (lmax + 1) ** 2
+    cutoff = h.cutoff
+    do_grad = bool(gradients)
+    handle = lib().orc_soap_new(ctypes.byref(h))
+    if not handle:
+        raise ValueError("invalid SOAP hyper-parameters")
+    try:
+        self_c = np.zeros(N)
+        lib().orc_soap_self_contribution(handle, _ptr(self_c))
+        per_system = [neighbors(s.positions, s.cell, cutoff) for s in systems]
+        type_pairs = set()
+        for s, pairs in zip(systems, per_system):
+            for p in pairs:
+                a, b = int(s.types[p["first"]]), int(s.types[p["second"]])
+                type_pairs.update({(a, b), (b, a)})
+            type_pairs.update((int(t), int(t)) for t in s.types)
+        type_pairs = sorted(type_pairs)
+        # per type pair: [labels, values [L2][N], vector gradients [3][L2][N] or None, vector]
+        samples = {tp: [] for tp in type_pairs}
+        m1l = np.array([(-1.0) ** l for l in range(lmax + 1) for _ in range(2 * l + 1)])[:, None]
+        for si, (s, pairs) in enumerate(zip(systems, per_system)):
+            for p in pairs:
+                i, j = int(p["first"]), int(p["second"])
+                a, b = int(s.types[i]), int(s.types[j])
+                vec = np.ascontiguousarray(p["vector"], dtype=np.float64)
+                shift = [int(x) for x in p["shift"]]
+                v = np.zeros((L2, N))
+                g = np.zeros((3, L2, N))
+                lib().orc_soap_pair_contribution(handle, float(p["distance"]), _ptr(vec), int(do_grad), _ptr(v),
+                                                 _ptr(g) if do_grad else None)
+                samples[(a, b)].append(([si, i, j] + shift, v, g, vec.copy()))
+                samples[(b, a)].append(([si, j, i] + [-x for x in shift], m1l * v, -m1l[None] * g, -vec))
+            for i, t in enumerate(s.types):
+                v = np.zeros((L2, N))
+                v[0] = self_c
+                samples[(int(t), int(t))].append(([si, i, i, 0, 0, 0], v, None, np.zeros(3)))
+    finally:
+        lib().orc_soap_free(handle)
+
+    names = ["system", "first_atom", "second_atom", "cell_shift_a", "cell_shift_b", "cell_shift_c"]
+    keys, blocks = [], []
+    for tp in type_pairs:
+        for l in range(lmax + 1):
+            sl = slice(l * l, (l + 1) * (l + 1))
+            entries = [e for e in samples[tp] if l == 0 or not _is_self_pair(e[0])]
+            keys.append([l, 1, tp[0], tp[1]])
+            values = np.array([e[1][sl] for e in entries]).reshape(len(entries), 2 * l + 1, N)
+            mu = ("o3_mu", np.arange(-l, l + 1, dtype=np.int32).reshape(-1, 1))
+            blk = _block(values, names, [e[0] for e in entries], [mu], ["n"], np.arange(N).reshape(-1, 1))
+            zero = np.zeros((3, 2 * l + 1, N))
+            grad_of = [zero if e[2] is None else e[2][:, sl] for e in entries]
+            if "positions" in gradients:
+                rows, gv = [], []
+                for k, e in enumerate(entries):
+                    system, first, second, sa, sb, sc = e[0]
+                    if first == second and (sa, sb, sc) == (0, 0, 0):
+                        continue
+                    if first == second:
+                        rows.append([k, system, first])
+                        gv.append(-grad_of[k] + grad_of[k])
+                    else:
+                        rows += [[k, system, first], [k, system, second]]
+                        gv += [-grad_of[k], grad_of[k]]
+                gvals = np.array(gv).reshape(len(rows), 3, 2 * l + 1, N)
+                blk["gradients"]["positions"] = _block(gvals, ["sample", "system", "atom"], rows, [_xyz(), mu],
+                                                       ["n"], np.arange(N).reshape(-1, 1))
+            for param in ("strain", "cell"):
+                if param not in gradients:
+                    continue
+                gvals = np.zeros((len(entries), 3, 3, 2 * l + 1, N))
+                for k, e in enumerate(entries):
+                    w = e[3] if param == "strain" else np.array(e[0][3:6], dtype=np.float64)
+                    gvals[k] = w[:, None, None, None] * grad_of[k][None]
+                comps = ([("xyz_2", np.arange(3, dtype=np.int32).reshape(3, 1)),
+                          ("xyz_1", np.arange(3, dtype=np.int32).reshape(3, 1)), mu] if param == "strain"
+                         else [_xyz(), ("abc", np.arange(3, dtype=np.int32).reshape(3, 1)), mu])
+                blk["gradients"][param] = _block(gvals, ["sample"], np.arange(len(entries)).reshape(-1, 1), comps,
+                                                 ["n"], np.arange(N).reshape(-1, 1))
+            blocks.append(blk)
+    return {"keys_names": ["o3_lambda", "o3_sigma", "first_atom_type", "second_atom_type"],
+            "keys": np.asarray(keys, dtype=np.int32).reshape(len(keys), 4), "blocks": blocks}
+
+
+def _is_self_pair(labels):
+    return labels[1] == labels[2] and labels[3] == 0 and labels[4] == 0 and labels[5] == 0
 
 
 def soap_radial_spectrum(systems, hypers, gradients=()):

@@ -84,6 +84,10 @@ void orc_soap_free(orc_soap* soap);
 /* self contribution (n values, l = 0) */
 void orc_soap_self_contribution(const orc_soap* soap, double* out);
 double orc_cutoff_smoothing(const orc_soap* soap, double r);
+/* one pair's contribution (soap/spherical_expansion_pair.rs:323-422):
+ * values [L2][N] = f * Y_lm * I_nl, grads [3][L2][N] = d values / d pair vector */
+void orc_soap_pair_contribution(const orc_soap* soap, double distance, const double vector[3], int do_grad,
+                                double* values, double* grads);
 double orc_cutoff_smoothing_gradient(const orc_soap* soap, double r);
 
 /*

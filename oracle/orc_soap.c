@@ -141,6 +141,15 @@ static void pair_contribution(const orc_soap* s, double distance, const double v
     }
 }
 
+void orc_soap_pair_contribution(const orc_soap* s, double distance, const double vector[3], int do_grad,
+                                double* values, double* grads) {
+    int lmax = s->h.max_angular, N = s->h.n_radial;
+    int L2 = (lmax + 1) * (lmax + 1);
+    double* work = malloc(sizeof(double) * (4 * L2 + 2 * (lmax + 1) * N));
+    pair_contribution(s, distance, vector, do_grad, values, grads, work);
+    free(work);
+}
+
 static int species_index(int n_species, const int32_t* species, int32_t t) {
     for (int s = 0; s < n_species; s++) {
         if (species[s] == t) return s;

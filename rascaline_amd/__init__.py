@@ -7,6 +7,6 @@ the featomic C API; results are metatensor-style TensorMaps.
 from ._c_lib import FeatomicError  # noqa: F401
 from .calculator_base import CalculatorBase, neighbors  # noqa: F401
 from .calculators import (LodeSphericalExpansion, NeighborList, SoapPowerSpectrum,  # noqa: F401
-                          SoapRadialSpectrum, SphericalExpansion)
+                          SoapRadialSpectrum, SphericalExpansion, SphericalExpansionByPair)
 from .systems import System  # noqa: F401
 from .tensormap import Labels, TensorBlock, TensorMap  # noqa: F401

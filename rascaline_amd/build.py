@@ -26,6 +26,7 @@ SOURCES = [
     ("calculator.cpp", []),
     ("lode_host.cpp", []),
     ("neighbor_list_host.cpp", []),
+    ("by_pair_host.cpp", []),
     ("hypers.cpp", []),
     ("radial.cpp", []),
     ("tensormap.cpp", []),

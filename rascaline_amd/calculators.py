@@ -32,6 +32,18 @@ class SoapPowerSpectrum(CalculatorBase):
         super().__init__("soap_power_spectrum", json.dumps(_to_json(parameters)))
 
 
+class SphericalExpansionByPair(CalculatorBase):
+    """The spherical expansion split per directed pair (i, j, cell shift): the
+    contribution of atom j to the density around atom i, keys (o3_lambda, o3_sigma,
+    first_atom_type, second_atom_type), self pairs in the l = 0 blocks
+    (featomic/src/calculators/soap/spherical_expansion_pair.rs:547-840). Summing the
+    samples over the second atom gives the spherical expansion."""
+
+    def __init__(self, *, cutoff, density, basis):
+        parameters = {"cutoff": cutoff, "density": density, "basis": basis}
+        super().__init__("spherical_expansion_by_pair", json.dumps(_to_json(parameters)))
+
+
 class SoapRadialSpectrum(CalculatorBase):
     """SOAP radial spectrum <n | rho_i^{a}> (the l = 0 spherical expansion), with the
     reference's parameters: cutoff, density and basis = {radial, spline_accuracy}."""

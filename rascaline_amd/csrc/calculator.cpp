@@ -40,8 +40,6 @@ void Workspace::release() {
     slots_.clear();
 }
 
-namespace {
-
 int device_lds_limit() {
     static int limit = -1;
     if (limit < 0) {
@@ -54,8 +52,6 @@ int device_lds_limit() {
     }
     return limit;
 }
-
-}  // namespace
 
 // GPU neighbour lists of the systems of a host batch as half lists of featomic_pair_t:
 // j > i, or j == i with the shift in the positive half-space
@@ -152,6 +148,10 @@ Calculator::Calculator(const std::string& registry_name, const std::string& para
         // the l = 0 spherical expansion, relabelled (soap/radial_spectrum.rs:75-135)
         kind_ = Kind::RadialSpectrum;
         soap_ = parse_soap_params(parameters, true, true);
+    } else if (registry_name == "spherical_expansion_by_pair") {
+        // the expansion's parameters, one sample per pair (soap/spherical_expansion_pair.rs:547-840)
+        kind_ = Kind::ByPair;
+        soap_ = parse_soap_params(parameters, true);
     } else if (registry_name == "neighbor_list") {
         // pair vectors of the neighbour list (neighbor_list.rs:31-44)
         kind_ = Kind::NeighborList;
@@ -159,7 +159,7 @@ Calculator::Calculator(const std::string& registry_name, const std::string& para
         cutoffs_.push_back(nl_.cutoff);
         return;
     } else if (registry_name == "atomic_composition" || registry_name == "dummy_calculator" ||
-               registry_name == "sorted_distances" || registry_name == "spherical_expansion_by_pair") {
+               registry_name == "sorted_distances") {
         throw Error::invalid_parameter("the '" + registry_name + "' calculator is not provided by this GPU implementation");
     } else {
         throw Error::invalid_parameter("unknown calculator with name '" + registry_name + "'");
@@ -214,6 +214,7 @@ const char* Calculator::name() const {
     case Kind::LodeSphericalExpansion: return "lode spherical expansion";
     case Kind::RadialSpectrum: return "radial spectrum";
     case Kind::NeighborList: return "neighbors list";
+    case Kind::ByPair: return "spherical expansion by pair";
     }
     return "";
 }
@@ -343,6 +344,62 @@ static_assert(RA_EXPANSION_ENTRIES >= 1 && RA_EXPANSION_ENTRIES <= 64, "expansio
 #define RA_TILED_WINDOWS {{4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}}
 #endif
 
+// Kernel arguments of the SOAP kernels for a built neighbour list: radial tables
+// (spline, exact per-entry values, or the learned integral), cutoff and scaling
+// parameters, spherical-harmonics coefficients and the cleared error flag.
+template <typename T>
+void Calculator::soap_launch(SoapLaunch<T>& lp, const DeviceNeighbors& nl, const int* d_species, int n_atoms, int S,
+                             hipStream_t stream) {
+    const int L = soap_.L(), n = soap_.n_radial;
+    const DeviceTables<T>& tab = device_tables<T>();
+    lp.stream = stream;
+    lp.n_atoms = n_atoms;
+    lp.n_species = S;
+    lp.L = L;
+    lp.n = n;
+    lp.offsets = nl.offsets;
+    lp.entry_j = nl.entry_j;
+    lp.entry_shift = nl.entry_shift;
+    lp.entry_vec = nl.entry_vec;
+    lp.entry_dist = nl.entry_dist;
+    lp.species = d_species;
+    lp.radial = tab.radial;
+    if (radial_surrogate_active()) {
+        // learned radial integral: every (l, n) value and derivative per neighbour entry
+        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
+        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
+        surrogate_->evaluate<T>(stream, nl.n_entries, nl.entry_dist, ev, ed);
+        lp.radial.use_spline = 0;
+        lp.radial.exact_val = ev;
+        lp.radial.exact_der = ed;
+    } else if (!tab.radial.use_spline) {
+        // exact radial integrals: evaluated once per neighbour entry in FP64
+        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
+        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
+        launch_radial_exact<T>(stream, nl.n_entries, L, n, nl.entry_dist, tables_.sigma2_atomic, tab.radial.gto,
+                               tab.radial.ortho, ev, ed);
+        lp.radial.exact_val = ev;
+        lp.radial.exact_der = ed;
+    }
+    lp.scalars.cutoff = soap_.cutoff;
+    lp.scalars.smoothing = soap_.smoothing == Smoothing::ShiftedCosine ? 1 : 0;
+    lp.scalars.width = soap_.smoothing_width;
+    lp.scalars.has_scaling = soap_.has_scaling ? 1 : 0;
+    lp.scalars.scale = soap_.scaling_scale;
+    lp.scalars.rate = soap_.scaling_rate;
+    lp.scalars.exponent = soap_.scaling_exponent;
+    lp.sh_a = tab.sh_a;
+    lp.sh_b = tab.sh_b;
+    lp.sh_c = tab.sh_c;
+    lp.self_contribution = tab.self_c;
+    int* error_flag = ws_.get<int>("soap_error", 1);
+    RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
+    lp.error_flag = error_flag;
+}
+
+template void Calculator::soap_launch<float>(SoapLaunch<float>&, const DeviceNeighbors&, const int*, int, int, hipStream_t);
+template void Calculator::soap_launch<double>(SoapLaunch<double>&, const DeviceNeighbors&, const int*, int, int, hipStream_t);
+
 template <typename T>
 mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& grads, hipStream_t stream, bool to_host,
                                       featomic_amd_device_output_t* output) {
@@ -456,51 +513,10 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     upload(d_block_sample_base, block_sample_base, stream);
 
     // ---- kernel launch parameters
-    const DeviceTables<T>& tab = device_tables<T>();
     SoapLaunch<T> lp;
-    lp.stream = stream;
-    lp.n_atoms = n_atoms;
-    lp.n_species = S;
-    lp.L = L;
-    lp.n = n;
-    lp.offsets = nl.offsets;
-    lp.entry_j = nl.entry_j;
-    lp.entry_shift = nl.entry_shift;
-    lp.entry_vec = nl.entry_vec;
-    lp.entry_dist = nl.entry_dist;
-    lp.species = d_species;
-    lp.radial = tab.radial;
-    if (radial_surrogate_active()) {
-        // learned radial integral: every (l, n) value and derivative per neighbour entry
-        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
-        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
-        surrogate_->evaluate<T>(stream, nl.n_entries, nl.entry_dist, ev, ed);
-        lp.radial.use_spline = 0;
-        lp.radial.exact_val = ev;
-        lp.radial.exact_der = ed;
-    } else if (!tab.radial.use_spline) {
-        // exact radial integrals: evaluated once per neighbour entry in FP64
-        T* ev = ws_.get<T>("exact_val", std::max<long long>(nl.n_entries, 1) * L * n);
-        T* ed = ws_.get<T>("exact_der", std::max<long long>(nl.n_entries, 1) * L * n);
-        launch_radial_exact<T>(stream, nl.n_entries, L, n, nl.entry_dist, tables_.sigma2_atomic, tab.radial.gto,
-                               tab.radial.ortho, ev, ed);
-        lp.radial.exact_val = ev;
-        lp.radial.exact_der = ed;
-    }
-    lp.scalars.cutoff = soap_.cutoff;
-    lp.scalars.smoothing = soap_.smoothing == Smoothing::ShiftedCosine ? 1 : 0;
-    lp.scalars.width = soap_.smoothing_width;
-    lp.scalars.has_scaling = soap_.has_scaling ? 1 : 0;
-    lp.scalars.scale = soap_.scaling_scale;
-    lp.scalars.rate = soap_.scaling_rate;
-    lp.scalars.exponent = soap_.scaling_exponent;
-    lp.sh_a = tab.sh_a;
-    lp.sh_b = tab.sh_b;
-    lp.sh_c = tab.sh_c;
-    lp.self_contribution = tab.self_c;
-    int* error_flag = ws_.get<int>("soap_error", 1);
-    RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
-    lp.error_flag = error_flag;
+    soap_launch<T>(lp, nl, d_species, n_atoms, S, stream);
+    const DeviceTables<T>& tab = device_tables<T>();
+    int* error_flag = lp.error_flag;
 
     const int lds = device_lds_limit();
     const size_t per_entry = soap_chunk_bytes<T>(L, n, 1);
@@ -911,6 +927,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& grads) {
     hipStream_t stream = nullptr;
     if (kind_ == Kind::NeighborList) return run_neighbor_list(batch, grads);
+    if (kind_ == Kind::ByPair) return precision == 32 ? run_by_pair<float>(batch, grads) : run_by_pair<double>(batch, grads);
     if (kind_ == Kind::LodeSphericalExpansion) {
         if (grads.strain || grads.cell) {
             throw Error::invalid_parameter("the lode spherical expansion calculator does not support gradients with respect to " +
@@ -925,8 +942,9 @@ mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& gr
 
 void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
                                 featomic_amd_device_output_t* output) {
-    if (kind_ == Kind::NeighborList) {
-        throw Error::invalid_parameter("the neighbors list calculator returns host TensorMaps only (use featomic_calculator_compute)");
+    if (kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
+        throw Error::invalid_parameter(std::string("the ") + name() +
+                                       " calculator returns host TensorMaps only (use featomic_calculator_compute)");
     }
     if (kind_ == Kind::LodeSphericalExpansion) {
         if (precision == 32) {
@@ -953,7 +971,7 @@ const std::vector<std::array<int, 3>>& soap_window_shapes() {
 }
 
 std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads, hipStream_t stream, int repeats) {
-    if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList) {
+    if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
         throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
     }
     if (repeats < 1 || repeats > 100) throw Error::invalid_parameter("autotune repeats must be in [1, 100]");
@@ -1058,7 +1076,9 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
 
 void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
     if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
-    if (kind_ == Kind::NeighborList) throw Error::invalid_parameter("the positions VJP covers the SOAP and LODE calculators");
+    if (kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
+        throw Error::invalid_parameter("the positions VJP covers the SOAP and LODE calculators");
+    }
     vjp_grad_ = grad_values;
     vjp_out_ = out;
     main_ms_sum_ = -1.0;

@@ -55,7 +55,11 @@ struct HostBatch {
     int64_t n_atoms() const { return offsets.empty() ? 0 : offsets.back(); }
 };
 
-enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion, RadialSpectrum, NeighborList };
+enum class Kind { SphericalExpansion, PowerSpectrum, LodeSphericalExpansion, RadialSpectrum, NeighborList, ByPair };
+
+struct DeviceNeighbors;
+template <typename T>
+struct SoapLaunch;
 
 // Launch shapes of the SOAP kernels that the autotuner chooses (0 / -1: the
 // built-in choice).
@@ -161,6 +165,12 @@ private:
     template <typename T>
     mts_tensormap_t* run_soap(const HostBatch& batch, const Gradients& gradients, hipStream_t stream, bool to_host,
                               featomic_amd_device_output_t* output);
+    template <typename T>
+    void soap_launch(SoapLaunch<T>& lp, const DeviceNeighbors& nl, const int* d_species, int n_atoms, int S,
+                     hipStream_t stream);
+    // spherical_expansion_by_pair: one sample per directed pair, host TensorMap
+    template <typename T>
+    mts_tensormap_t* run_by_pair(const HostBatch& batch, const Gradients& gradients);
     // neighbor_list calculator: pair vectors from the GPU half list, host TensorMap
     mts_tensormap_t* run_neighbor_list(const HostBatch& batch, const Gradients& gradients);
     template <typename T>
@@ -188,6 +198,9 @@ struct DeviceNeighbors {
     double* entry_vec = nullptr;    // 3 per entry
     double* entry_dist = nullptr;
 };
+
+// LDS bytes per workgroup on the current device.
+int device_lds_limit();
 
 // Sends a message to the logging callback set with featomic_set_logging_callback.
 void log_message(int level, const std::string& message);

@@ -60,6 +60,26 @@ struct SoapLaunch {
 
 template <typename T>
 size_t soap_chunk_bytes(int L, int n, int chunk);
+
+// One sample of spherical_expansion_by_pair: its neighbour entry in the full list
+// (-1: a self pair), its type pair, its row in the l = 0 block and in the l > 0
+// blocks (-1 for a self pair), its first position-gradient row and their count
+// (0: self pair, 1: an atom and its own periodic image, 2).
+struct ByPairSample {
+    long long entry;
+    int type_pair;
+    int row0;
+    int row1;
+    int grad_row;
+    int grad_rows;
+    int pad;
+};
+// values / gradient rows of every sample into the blocks (type pair, l), whose
+// element offsets in each output are *_base[type_pair * L + l]; null outputs are skipped
+template <typename T>
+void launch_by_pair(const SoapLaunch<T>& p, long long n_samples, const ByPairSample* samples, const long long* value_base,
+                    const long long* pos_base, const long long* strain_base, const long long* cell_base, T* values, T* pos,
+                    T* strain, T* cell);
 template <typename T>
 void launch_radial_exact(hipStream_t stream, long long n_entries, int L, int n, const double* entry_dist,
                          double sigma2_atomic, const double* gto, const double* ortho, T* out_val, T* out_der);

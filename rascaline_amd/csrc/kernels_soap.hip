@@ -2940,6 +2940,99 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
                                                               sample_flag, sample_scan, block_sample_base, region_base, out);
 }
 
+
+// spherical_expansion_by_pair: one directed pair per sample. A workgroup takes
+// `a.chunk` consecutive samples; each of the first lanes fills the per-entry data
+// (spherical harmonics, cutoff x radial values) of one sample's neighbour entry into
+// LDS, then the wave writes every sample's rows, lanes over the (l, m, n) slots:
+//   values   f Y_lm I_nl,             block (type pair, l), row row0 / row1
+//   g[x]     d values / d pair vector (soap/spherical_expansion_pair.rs:382-419)
+//   positions -g (first atom), +g (second atom); a single zero row for a pair of an
+//            atom with its own image
+//   strain   v[a] g[x]; cell shift[a] g[x]
+// Self pairs (entry < 0) hold the self contribution at l = 0 and zero strain / cell rows.
+template <typename T>
+__global__ void __launch_bounds__(WAVE) by_pair_kernel(SoapArgs<T> a, long long n_samples,
+                                                       const ByPairSample* __restrict__ samples,
+                                                       const long long* __restrict__ value_base,
+                                                       const long long* __restrict__ pos_base,
+                                                       const long long* __restrict__ strain_base,
+                                                       const long long* __restrict__ cell_base, T* __restrict__ values,
+                                                       T* __restrict__ pos, T* __restrict__ strain, T* __restrict__ cell) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ChunkLayout lay(a.L, a.n, a.chunk);
+    T* chunk = reinterpret_cast<T*>(smem);
+    const long long first = (long long)blockIdx.x * a.chunk;
+    const int cnt = (int)min((long long)a.chunk, n_samples - first);
+    const int lane = threadIdx.x;
+    if (lane < cnt) {
+        const long long e = samples[first + lane].entry;
+        if (e >= 0) entry_data<T>(a, lay, e, -1, chunk + lane * lay.per_entry);
+    }
+    __syncthreads();
+    const int n = a.n;
+    const int slots = lay.L2 * n;
+    for (int c = 0; c < cnt; c++) {
+        const ByPairSample sm = samples[first + c];
+        const T* en = chunk + c * lay.per_entry;
+        for (int s = lane; s < slots; s += WAVE) {
+            const int lm = s / n, nn = s - lm * n;
+            const int l = l_of_lm(lm);
+            const int row = l == 0 ? sm.row0 : sm.row1;
+            if (row < 0) continue;
+            const long long w = (long long)(2 * l + 1) * n;
+            const long long off = (long long)(lm - l * l) * n + nn;
+            const long long blk = (long long)sm.type_pair * a.L + l;
+            if (sm.entry < 0) {
+                values[value_base[blk] + row * w + off] = a.self_contribution[nn];
+                for (int q = 0; q < 9; q++) {
+                    if (strain) strain[strain_base[blk] + (row * 9LL + q) * w + off] = (T)0;
+                    if (cell) cell[cell_base[blk] + (row * 9LL + q) * w + off] = (T)0;
+                }
+                continue;
+            }
+            const T y = en[lay.y + lm];
+            values[value_base[blk] + row * w + off] = en[lay.rv + l * n + nn] * y;
+            if (!pos && !strain && !cell) continue;
+            const T ra = en[lay.ra + l * n + nn] * y, rb = en[lay.rb + l * n + nn];
+            T g[3];
+#pragma unroll
+            for (int x = 0; x < 3; x++) g[x] = ra * en[lay.dir + x] + rb * en[lay.dy + x * lay.L2 + lm];
+            if (pos && sm.grad_rows > 0) {
+                T* out = pos + pos_base[blk] + (long long)sm.grad_row * 3 * w + off;
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    if (sm.grad_rows == 2) {
+                        out[x * w] = -g[x];
+                        out[(3 + x) * w] = g[x];
+                    } else {
+                        out[x * w] = (T)0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    if (strain) strain[strain_base[blk] + (row * 9LL + p * 3 + x) * w + off] = en[lay.vec + p] * g[x];
+                    if (cell) cell[cell_base[blk] + (row * 9LL + p * 3 + x) * w + off] = en[lay.shift + p] * g[x];
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_by_pair(const SoapLaunch<T>& p, long long n_samples, const ByPairSample* samples, const long long* value_base,
+                    const long long* pos_base, const long long* strain_base, const long long* cell_base, T* values, T* pos,
+                    T* strain, T* cell) {
+    if (n_samples == 0) return;
+    SoapArgs<T> a = make_args(p);
+    const unsigned blocks = (unsigned)((n_samples + p.chunk - 1) / p.chunk);
+    by_pair_kernel<T><<<blocks, WAVE, soap_chunk_bytes<T>(p.L, p.n, p.chunk), p.stream>>>(
+        a, n_samples, samples, value_base, pos_base, strain_base, cell_base, values, pos, strain, cell);
+}
+
 #define RA_INSTANTIATE(T)                                                                                              \
     template void launch_soap_center<T>(const SoapLaunch<T>&, int, T*, T*);                                            \
     template void launch_ps_values<T>(const SoapLaunch<T>&, const T*, const unsigned*, const long long*,              \
@@ -2975,7 +3068,9 @@ void launch_se_gather(const SoapLaunch<T>& p, int ncomp, const T* dense, const l
     template void launch_ps_cell_strain<T>(const SoapLaunch<T>&, const T*, const T*, const unsigned*,                 \
                                            const long long*, const long long*, T*);                                    \
     template void launch_se_gather<T>(const SoapLaunch<T>&, int, const T*, const long long*, const long long*,        \
-                                      const long long*, const long long*, T*);
+                                      const long long*, const long long*, T*);                                         \
+    template void launch_by_pair<T>(const SoapLaunch<T>&, long long, const ByPairSample*, const long long*,           \
+                                    const long long*, const long long*, const long long*, T*, T*, T*, T*);
 
 RA_INSTANTIATE(float)
 RA_INSTANTIATE(double)

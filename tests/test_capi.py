@@ -70,6 +70,8 @@ def test_every_declared_function_is_exported():
         ("soap_power_spectrum", SOAP, "SOAP power spectrum"),
         ("spherical_expansion", SOAP, "spherical expansion"),
         ("lode_spherical_expansion", LODE, "lode spherical expansion"),
+        # soap/spherical_expansion_pair.rs:548-550
+        ("spherical_expansion_by_pair", SOAP, "spherical expansion by pair"),
     ],
 )
 def test_name_parameters_cutoffs(name, hypers, expected):
@@ -293,3 +295,10 @@ def test_neighbor_list_rejects_cell_gradients():
     for grad in ("cell", "strain"):
         with pytest.raises(C.FeatomicError, match=f"the neighbors list calculator does not support gradients with respect to {grad}"):
             nl.compute(system, gradients=[grad])
+
+
+def test_by_pair_parameters_deny_unknown_fields():
+    """SphericalExpansionByPair takes the expansion's parameters, which deny unknown
+    fields (soap/spherical_expansion_pair.rs:37-60)."""
+    assert not make("spherical_expansion_by_pair", dict(SOAP, extra=1))
+    assert last_error().startswith("json error: unknown field")
