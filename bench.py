@@ -133,7 +133,8 @@ def main():
     parser.add_argument("--streams", type=int, default=4,
                         help="calculators on their own streams that consecutive chunks alternate between")
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp", "lode-vjp"], default="soap",
+    parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp", "lode-vjp", "by-pair"],
+                        default="soap",
                         help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
                              "expansion + gradients on periodic cells of 500-1500 atoms; bulk: power spectrum + "
                              "gradients of one 20 000-atom cell (12 radial functions, l <= 8, 5 types); "
@@ -141,7 +142,8 @@ def main():
                              "spline and with the learned radial integral; vjp: the 4096 structures' power "
                              "spectrum values followed by the vector-Jacobian product of their position gradients "
                              "; lode-vjp: the LODE cells' values followed by the vector-Jacobian product of "
-                             "their position gradients (each its own JSON line)")
+                             "their position gradients; by-pair: the spherical expansion by pair + position "
+                             "gradients of the first 16 structures (each its own JSON line)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +161,8 @@ def main():
         return vjp_main(args)
     if args.workload == "lode-vjp":
         return lode_vjp_main(args)
+    if args.workload == "by-pair":
+        return by_pair_main(args)
 
     import torch
 
@@ -340,6 +344,84 @@ def lode_cpu_baseline(target_seconds=15.0):
         "sample": f"{done} periodic triclinic cells of 96-128 atoms at 0.1 A^-3 ({atoms} atoms), FP64 oracle, "
                   f"one thread, {t_total:.1f} s (the GPU cells hold 500-1500 atoms; cost grows as N^2 n_k)",
     }
+
+
+def by_pair_cpu_baseline(systems, target_seconds=10.0):
+    """The oracle's spherical expansion by pair + position gradients (one C call per
+    pair, one thread) on the first structures of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import featomic_oracle as orc
+
+    done, samples, t_total = 0, 0, 0.0
+    while t_total < target_seconds and done < len(systems):
+        t0 = time.perf_counter()
+        tm = orc.soap_spherical_expansion_by_pair([orc.System(*systems[done])], HYPERS, ["positions"])
+        t_total += time.perf_counter() - t0
+        samples += sum(len(b["samples"]) for b, k in zip(tm["blocks"], tm["keys"]) if k[0] == 0)
+        done += 1
+    return {"value": samples / t_total, "unit": "pair samples/s", "cores": 1, "kind": "port",
+            "sample": f"the first {done} structures of the workload, FP64, 1 thread, {t_total:.1f} s"}
+
+
+def by_pair_main(args):
+    """SphericalExpansionByPair + position gradients (host TensorMap: the GPU writes
+    every pair's rows, the host lays out the labels and copies the blocks out)."""
+    import torch
+
+    import rascaline_amd as ra
+    from rascaline_amd import _c_lib as C
+    from rascaline_amd.datasets import batched_uneven
+
+    n = args.structures if args.structures != 4096 else 16
+    systems = batched_uneven(seed=0, n_structures=n)
+    n_atoms = sum(len(s[1]) for s in systems)
+    calc = ra.SphericalExpansionByPair(**HYPERS)
+    calc.set_precision(args.precision)
+    ra_systems = [ra.System(t, p, c) for p, t, c in systems]
+    for w in range(args.warmup):
+        tm = calc.compute(ra_systems, gradients=["positions"])
+        log(f"by-pair: warmup step {w} done")
+        del tm
+    samples = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms, device_ms, kernel_bytes = 0.0, 0.0, 0
+    for _ in range(args.steps):
+        tm = calc.compute(ra_systems, gradients=["positions"])
+        t_ms, k_ms, k_bytes = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        C.check(C.lib().featomic_amd_last_timings(calc._as_parameter_, ctypes.byref(t_ms), ctypes.byref(k_ms),
+                                                  ctypes.byref(k_bytes)))
+        kernel_ms += k_ms.value / args.steps
+        device_ms += t_ms.value / args.steps
+        kernel_bytes = k_bytes.value
+        samples = sum(len(b.samples) for b, k in zip(tm.blocks(), tm.keys.values) if k[0] == 0)
+        del tm
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / args.steps
+    rate = samples * args.steps / elapsed
+    log(f"by-pair: {ms:.1f} ms/step, {rate:.3e} pair samples/s, by_pair_kernel {kernel_ms:.2f} ms")
+    result = {
+        "metric": "pair samples/s SOAP spherical expansion by pair + position gradients (host TensorMap)",
+        "value": rate,
+        "unit": "pair samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"the first {n} of the 4096 uneven structures, spherical expansion by pair + "
+                               "position gradients, rc 5.0, n 8, l<=6",
+                   "atoms": n_atoms, "pair_samples": samples},
+        "roofline": hbm_roofline(kernel_ms, kernel_bytes, device_ms,
+                                 "by_pair_kernel (every pair's values and position-gradient rows)"),
+        "cpu_baseline": None if args.no_cpu_baseline else by_pair_cpu_baseline(systems),
+    }
+    print(json.dumps(result), flush=True)
+    return 0
 
 
 def lode_main(args):

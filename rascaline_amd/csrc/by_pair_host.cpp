@@ -31,7 +31,20 @@ mts_tensormap_t* Calculator::run_by_pair(const HostBatch& batch, const Gradients
     const int L = soap_.L(), n = soap_.n_radial;
     auto tm = std::make_unique<mts_tensormap_t>();
     tm->keys = LabelsData({"o3_lambda", "o3_sigma", "first_atom_type", "second_atom_type"}, {});
-    if (n_atoms == 0) return tm.release();
+    // timings read by last_timings: the whole call, and by_pair_kernel
+    for (hipEvent_t* ev : {&ev_start_, &ev_stop_, &ev_main_start_, &ev_main_stop_, &ev_exp_start_, &ev_exp_stop_,
+                           &ev_grad_start_}) {
+        if (!*ev) RA_HIP_CHECK(hipEventCreate(ev));
+    }
+    main_ms_sum_ = -1.0;
+    main_bytes_ = 0;
+    RA_HIP_CHECK(hipEventRecord(ev_start_, stream));
+    if (n_atoms == 0) {
+        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+        RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+        return tm.release();
+    }
     const Species sp = species_of(batch);
     const int S = static_cast<int>(sp.types.size());
 
@@ -188,14 +201,24 @@ mts_tensormap_t* Calculator::run_by_pair(const HostBatch& batch, const Gradients
     T* d_strain = grads.strain ? ws_.get<T>("by_pair_strain", std::max<long long>(cs_total, 1)) : nullptr;
     T* d_cell = grads.cell ? ws_.get<T>("by_pair_cell", std::max<long long>(cs_total, 1)) : nullptr;
     const size_t nb = value_base.size();
+    RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
     launch_by_pair<T>(lp, n_samples, d_samples, d_bases, d_bases + nb, d_bases + 2 * nb, d_bases + 3 * nb, d_values,
                       d_pos, d_strain, d_cell);
     RA_HIP_CHECK(hipGetLastError());
+    RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+    // algorithmic bytes of by_pair_kernel: every output element written once, and per
+    // sample its descriptor (32 B) and neighbour entry (vector, distance, index and
+    // shift: 48 B) read once
+    {
+        const long long written = value_total + (d_pos ? pos_total : 0) + (d_strain ? cs_total : 0) + (d_cell ? cs_total : 0);
+        main_bytes_ = written * static_cast<long long>(sizeof(T)) + n_samples * 80;
+    }
     std::vector<T> h_values, h_pos, h_strain, h_cell;
     download(h_values, d_values, value_total, stream);
     if (d_pos) download(h_pos, d_pos, pos_total, stream);
     if (d_strain) download(h_strain, d_strain, cs_total, stream);
     if (d_cell) download(h_cell, d_cell, cs_total, stream);
+    RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
     RA_HIP_CHECK(hipStreamSynchronize(stream));
 
     // ---- TensorMap
