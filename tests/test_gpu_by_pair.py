@@ -128,3 +128,20 @@ def test_by_pair_user_pairs():
     a = compute(systems, RANDOM, 64, ["positions"], native=False)
     b = compute(systems, RANDOM, 64, ["positions"], native=True)
     compare_tensormaps(a, b, 1e-14, 1e-14, gradients=["positions"])
+
+
+def test_by_pair_isolated_atoms_and_empty_system():
+    """Atoms without neighbours: the l = 0 blocks hold only their self pairs and the
+    l > 0 blocks of those types are empty; a system without atoms adds nothing."""
+    say("by pair: isolated atoms, empty system")
+    isolated = (np.array([[0.0, 0.0, 0.0], [20.0, 0.0, 0.0]]), np.array([1, 8], dtype=np.int32), np.zeros((3, 3)))
+    empty = (np.zeros((0, 3)), np.zeros(0, dtype=np.int32), np.zeros((3, 3)))
+    systems = [isolated, empty] + random_batch(25)[:1]
+    got = compute(systems, RANDOM, 64, GRADS)
+    ref = oracle(systems, RANDOM, GRADS)
+    compare_tensormaps(got, ref, 1e-10, 1e-10, gradients=GRADS)
+    only = compute([isolated], RANDOM, 64, ["positions"])
+    assert [tuple(k) for k in np.asarray(only.keys.values).tolist()] == [(l, 1, t, t) for t in (1, 8) for l in range(5)]
+    for k, blk in zip(np.asarray(only.keys.values), only.blocks()):
+        assert len(blk.samples) == (1 if k[0] == 0 else 0)
+        assert len(blk.gradient("positions").samples) == 0

@@ -145,3 +145,16 @@ def test_user_pairs_match_native():
     params = {"cutoff": 3.2, "full_neighbor_list": True, "self_pairs": True}
     a = compute(systems, params, ["positions"], native=False)
     assert_bit_identical(a, systems, params, ["positions"])
+
+
+def test_isolated_atoms_and_empty_system():
+    say("NeighborList isolated atoms, empty system")
+    isolated = (np.array([[0.0, 0.0, 0.0], [20.0, 0.0, 0.0]]), np.array([1, 8], dtype=np.int32), np.zeros((3, 3)))
+    empty = (np.zeros((0, 3)), np.zeros(0, dtype=np.int32), np.zeros((3, 3)))
+    for self_pairs in (False, True):
+        params = {"cutoff": 3.0, "full_neighbor_list": True, "self_pairs": self_pairs}
+        systems = [isolated, empty, water()]
+        tm = compute(systems, params, ["positions"])
+        assert_bit_identical(tm, systems, params, ["positions"])
+    tm = compute([isolated], {"cutoff": 3.0, "full_neighbor_list": False, "self_pairs": False})
+    assert len(tm.blocks()) == 0
