@@ -1211,14 +1211,41 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
 // Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
 // to NC.
 template <typename T>
-__global__ void entry_prep_kernel(SoapArgs<T> a, long long n_entries, int NC, T* __restrict__ data) {
+__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base);
+
+// A workgroup fills the records of `per_block` consecutive entries in LDS, then
+// writes them out as one contiguous block of 16-byte stores (the slots' own
+// writes are 4- or 8-byte pieces spread over each record).
+template <typename T>
+__global__ void __launch_bounds__(256) entry_prep_kernel(SoapArgs<T> a, long long n_entries, int NC, int per_block,
+                                                         T* __restrict__ data) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* records = reinterpret_cast<T*>(smem);
     const TileEntryLayout lay(a.L, a.n, NC);
     const int L = lay.L;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_entries * 2 * L) return;
-    const long long e = idx / (2 * L);
-    const int slot = (int)(idx - e * 2 * L);
-    T* base = data + e * lay.per_entry;
+    const long long e_first = (long long)blockIdx.x * per_block;
+    const int count = (int)min((long long)per_block, n_entries - e_first);
+    const int tid = threadIdx.x;
+    if (tid < count * 2 * L) {
+        const int local = tid / (2 * L);
+        const int slot = tid - local * 2 * L;
+        const long long e = e_first + local;
+        T* base = records + (size_t)local * lay.per_entry;
+        entry_prep_slot<T>(a, lay, e, slot, NC, base);
+    }
+    __syncthreads();
+    typedef typename Vec4Of<T>::type V;
+    constexpr int VEC = 16 / sizeof(T);
+    const int nvec = count * lay.per_entry / VEC;
+    const V* src = reinterpret_cast<const V*>(records);
+    V* dst = reinterpret_cast<V*>(data + e_first * lay.per_entry);
+    for (int k = tid; k < nvec; k += blockDim.x) dst[k] = src[k];
+}
+
+// One slot of an entry's record (see entry_prep_kernel), written at `base`.
+template <typename T>
+__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base) {
+    const int L = lay.L;
     if (slot < L) {
         const ShTable tb(L);
         const int m = slot;
@@ -2745,8 +2772,10 @@ void launch_entry_prep(const SoapLaunch<T>& p, long long n_entries, T* data) {
     if (n_entries == 0) return;
     SoapArgs<T> a = make_args(p);
     const int NC = (p.n + 3) / 4 * 4;
-    const long long total = n_entries * 2 * p.L;
-    entry_prep_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, p.stream>>>(a, n_entries, NC, data);
+    const int per_block = 256 / (2 * p.L);
+    const size_t shmem = (size_t)per_block * TileEntryLayout(p.L, p.n, NC).per_entry * sizeof(T);
+    const long long blocks = (n_entries + per_block - 1) / per_block;
+    entry_prep_kernel<T><<<(unsigned)blocks, 256, shmem, p.stream>>>(a, n_entries, NC, per_block, data);
 }
 
 template <typename T>
