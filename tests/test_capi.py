@@ -302,3 +302,25 @@ def test_by_pair_parameters_deny_unknown_fields():
     fields (soap/spherical_expansion_pair.rs:37-60)."""
     assert not make("spherical_expansion_by_pair", dict(SOAP, extra=1))
     assert last_error().startswith("json error: unknown field")
+
+
+@pytest.mark.parametrize("name, hypers", [("neighbor_list", {"cutoff": 3.0, "full_neighbor_list": True, "self_pairs": False}),
+                                          ("spherical_expansion_by_pair", SOAP)])
+def test_pair_calculators_have_no_device_path(name, hypers):
+    """The pair calculators return host TensorMaps only: the device-resident entry
+    point rejects them before touching any device memory."""
+    calc = make(name, hypers)
+    assert calc, last_error()
+    try:
+        systems = C.featomic_amd_device_systems_t()
+        offsets = np.zeros(1, dtype=np.int64)
+        cells = np.zeros(9)
+        systems.n_systems = 0
+        systems.atom_offsets = offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        systems.cells = cells.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        output = C.featomic_amd_device_output_t()
+        status = C.lib().featomic_amd_compute_device(calc, ctypes.byref(systems), None, 0, None, ctypes.byref(output))
+        assert status == FEATOMIC_INVALID_PARAMETER_ERROR
+        assert "host TensorMaps only" in last_error()
+    finally:
+        C.lib().featomic_calculator_free(calc)
