@@ -3,7 +3,8 @@ is the vector-Jacobian product of the position gradients.
 
 * torch.autograd.gradcheck (fast mode: a random direction, central finite
   differences of the forward in FP64) on small periodic and cluster systems, for
-  the power spectrum, the spherical expansion and LODE (periodic only);
+  the power spectrum, the spherical expansion, the radial spectrum and LODE
+  (periodic only);
 * loss.backward() on Σ W · values against featomic-torch's contraction of the FP64
   oracle's gradient rows with W (1e-10).
 """
@@ -45,7 +46,7 @@ LODE = {
 }
 
 
-@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion", "lode"])
+@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion", "radial_spectrum", "lode"])
 def test_gradcheck(kind):
     import torch
 
@@ -55,6 +56,9 @@ def test_gradcheck(kind):
         rng = np.random.default_rng(4)
         systems = [random_system(rng, 6, "triclinic", 5.0, types=(1, 8)), random_system(rng, 5, "periodic", 4.5, types=(1, 6))]
         calc = ra.LodeSphericalExpansion(**LODE)
+    elif kind == "radial_spectrum":
+        calc = ra.SoapRadialSpectrum(cutoff=SOAP["cutoff"], density=SOAP["density"],
+                                     basis={"radial": SOAP["basis"]["radial"]})
     else:
         calc = ra.SoapPowerSpectrum(**SOAP) if kind == "power_spectrum" else ra.SphericalExpansion(**SOAP)
     calc.set_precision(64)

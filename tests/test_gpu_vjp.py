@@ -164,3 +164,35 @@ def test_lode_positions_vjp_vs_oracle(precision):
         assert np.all(np.abs(got - expected) <= 1e-10 * (np.abs(expected) + scale))
     else:
         assert err <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_radial_spectrum_positions_vjp_vs_oracle(precision):
+    """The radial spectrum (the l = 0 expansion, relabelled) through the same VJP entry
+    point, against featomic-torch's contraction of the oracle's gradient rows."""
+    import torch
+
+    say("radial spectrum positions VJP, precision", precision)
+    hypers = {"cutoff": SOAP["cutoff"], "density": SOAP["density"],
+              "basis": {"radial": {"type": "Gto", "max_radial": 5}}}
+    systems = batch(45)
+    ref = orc.soap_radial_spectrum([orc.System(p, t, c) for p, t, c in systems], hypers, ["positions"])
+    calc = ra.SoapRadialSpectrum(**hypers)
+    calc.set_precision(precision)
+    rng = np.random.default_rng(6)
+    weights = [rng.normal(size=(len(b["samples"]), int(np.prod(b["values"].shape[1:])))) for b in ref["blocks"]]
+    offsets = np.concatenate([[0], np.cumsum([len(t) for _, t, _ in systems])])
+    expected = reference_vjp(ref, weights, offsets)
+    ds = DeviceSystems(systems)
+    dev = compute_device(calc, ds)
+    assert dev.n_blocks == len(ref["blocks"])
+    assert np.array_equal(dev.keys, np.asarray(ref["keys"]))
+    dtype = torch.float64 if precision == 64 else torch.float32
+    got = positions_vjp(calc, ds, device_weights(dev, weights, torch, dtype)).double().cpu().numpy()
+    scale = np.abs(expected).max()
+    err = np.abs(got - expected).max() / scale
+    say(f"  max |a - b| / max|b| = {err:.2e}")
+    if precision == 64:
+        assert np.all(np.abs(got - expected) <= 1e-10 * (np.abs(expected) + scale))
+    else:
+        assert err <= 1e-4
