@@ -13,7 +13,11 @@ chunks of structures whose outputs fit in HBM (the full position-gradient
 output is several hundred GB); each chunk's output is produced in device memory
 and left there for the consumer.
 
-Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+Prints one JSON line: metric, value (atoms/s), ms_per_step, the roofline of the
+dominant kernel (HIP-event time, algorithmic bytes, measured HBM traffic) and the
+oracle's CPU rate on a bounded sample. --workload selects the other workloads
+(LODE, bulk cell, learned radial integral, backward, spherical expansion by pair),
+each printing its own line.
 """
 
 import argparse
