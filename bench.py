@@ -151,8 +151,8 @@ def main():
     args = parser.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharding over "
+              "several GPUs is a design, not built", file=sys.stderr)
         return 2
 
     if args.workload == "lode":
