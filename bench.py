@@ -132,7 +132,7 @@ def main():
     parser.add_argument("--steps", type=int, default=3)
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--structures", type=int, default=4096)
-    parser.add_argument("--chunk-atoms", type=int, default=48000)
+    parser.add_argument("--chunk-atoms", type=int, default=56000)
     parser.add_argument("--precision", type=int, default=32)
     parser.add_argument("--streams", type=int, default=4,
                         help="calculators on their own streams that consecutive chunks alternate between")
