@@ -138,6 +138,11 @@ def lib():
         L.orc_lode_free.argtypes = [vp]
         L.orc_k_vectors.restype = ctypes.c_int
         L.orc_k_vectors.argtypes = [vp, d, vp]
+        L.orc_lode_density_fourier.restype = d
+        L.orc_lode_density_fourier.argtypes = [vp, d]
+        L.orc_lode_center_contribution.argtypes = [vp, vp]
+        L.orc_lode_k0_contribution.restype = ctypes.c_int
+        L.orc_lode_k0_contribution.argtypes = [vp, vp]
         L.orc_lode_system.restype = ctypes.c_int
         L.orc_lode_system.argtypes = [vp, sz, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
         L.orc_mlp_n_params.restype = ctypes.c_int64
@@ -523,6 +528,28 @@ class SoapComputation:
         return out
 
 
+    def se_position_gradient_all(self, sd, center, atom, nt_index):
+        """se_position_gradient for every l at once, [3, L2, N]; the per-(center, atom)
+        sums over pair images are tabulated once per system."""
+        if center == atom:
+            return sd.self_grads[center, nt_index]
+        table = getattr(sd, "pair_table", None)
+        if table is None:
+            sign = np.concatenate([np.full(2 * l + 1, -1.0 if l % 2 else 1.0) for l in range(self.lmax + 1)])
+            table = {}
+            for p, pair in enumerate(sd.pairs):
+                f, s = int(pair["first"]), int(pair["second"])
+                if f == s:
+                    continue
+                g = sd.pair_grads[p]
+                table[(f, s)] = table[(f, s)] + g if (f, s) in table else g.copy()
+                inv = -sign[None, :, None] * g
+                table[(s, f)] = table[(s, f)] + inv if (s, f) in table else inv
+            sd.pair_table = table
+        out = table.get((center, atom))
+        return out if out is not None else np.zeros((3, self.L2, self.N))
+
+
 def soap_spherical_expansion(systems, hypers, gradients=()):
     comp = SoapComputation(systems, hypers, gradients)
     lmax, N = comp.lmax, comp.N
@@ -776,15 +803,14 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                     samples.append((si, i))
         kappa = math.sqrt(2.0) if t1 != t2 else 1.0
         i1, i2 = comp.tindex(t1), comp.tindex(t2)
+        # power_spectrum.rs:584-618, all samples of the block at once
+        c1s = np.array([comp.data[si].values[i, i1] for (si, i) in samples]).reshape(len(samples), comp.L2, N)
+        c2s = np.array([comp.data[si].values[i, i2] for (si, i) in samples]).reshape(len(samples), comp.L2, N)
         values = np.zeros((len(samples), len(props)))
-        for s_i, (si, i) in enumerate(samples):
-            c1 = comp.data[si].values[i, i1]
-            c2 = comp.data[si].values[i, i2]
-            row = []
-            for l in range(lmax + 1):
-                sl = slice(l * l, (l + 1) * (l + 1))
-                row.append((kappa * np.einsum("ma,mb->ab", c1[sl], c2[sl]) / norms[l]).reshape(-1))
-            values[s_i] = np.concatenate(row)
+        for l in range(lmax + 1):
+            sl = slice(l * l, (l + 1) * (l + 1))
+            values[:, l * N * N:(l + 1) * N * N] = (
+                kappa * np.einsum("sma,smb->sab", c1s[:, sl], c2s[:, sl]) / norms[l]).reshape(len(samples), -1)
         blk = _block(values, ["system", "atom"], samples, [], ["l", "n_1", "n_2"], props)
 
         if "positions" in comp.gradients:
@@ -801,26 +827,29 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                         nbrs.add(i)
                 for j in sorted(nbrs):
                     grad_samples.append((s_i, si, j))
+            # power_spectrum.rs:621-678; rows in chunks, every l at once per chunk
             gv = np.zeros((len(grad_samples), 3, len(props)))
-            for r, (s_i, si, j) in enumerate(grad_samples):
-                sd = comp.data[si]
-                i = samples[s_i][1]
-                t = sd.system.types
-                c1 = sd.values[i, i1]
-                c2 = sd.values[i, i2]
-                row = []
+            zero = np.zeros((3, comp.L2, N))
+            for r0 in range(0, len(grad_samples), 4096):
+                chunk = grad_samples[r0:r0 + 4096]
+                g1s = np.empty((len(chunk), 3, comp.L2, N))
+                g2s = np.empty((len(chunk), 3, comp.L2, N))
+                rows = np.empty(len(chunk), dtype=np.int64)
+                for r, (s_i, si, j) in enumerate(chunk):
+                    sd = comp.data[si]
+                    i = samples[s_i][1]
+                    t = sd.system.types
+                    rows[r] = s_i
+                    # SE gradient sample exists in block t1 iff j == i or type_j == t1
+                    g1s[r] = comp.se_position_gradient_all(sd, i, j, i1) if (j == i or t[j] == t1) else zero
+                    g2s[r] = comp.se_position_gradient_all(sd, i, j, i2) if (j == i or t[j] == t2) else zero
+                c1r, c2r = c1s[rows], c2s[rows]
                 for l in range(lmax + 1):
                     sl = slice(l * l, (l + 1) * (l + 1))
-                    acc = np.zeros((3, N, N))
-                    # SE gradient sample exists in block t1 iff j == i or type_j == t1
-                    if j == i or t[j] == t1:
-                        g1 = comp.se_position_gradient(sd, i, j, i1, l)
-                        acc += np.einsum("mb,xma->xab", c2[sl], g1)
-                    if j == i or t[j] == t2:
-                        g2 = comp.se_position_gradient(sd, i, j, i2, l)
-                        acc += np.einsum("ma,xmb->xab", c1[sl], g2)
-                    row.append((kappa * acc / norms[l]).reshape(3, -1))
-                gv[r] = np.concatenate(row, axis=1)
+                    # Σ_m g1[x, m, a] c2[m, b] + Σ_m c1[m, a] g2[x, m, b]
+                    acc = np.matmul(g1s[:, :, sl].transpose(0, 1, 3, 2), c2r[:, None, sl])
+                    acc += np.matmul(c1r[:, None, sl].transpose(0, 1, 3, 2), g2s[:, :, sl])
+                    gv[r0:r0 + len(chunk), :, l * N * N:(l + 1) * N * N] = (kappa * acc / norms[l]).reshape(len(chunk), 3, -1)
             blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz()], ["l", "n_1", "n_2"], props)
 
         for param, attr in (("cell", "cell_g"), ("strain", "strain")):
@@ -876,6 +905,40 @@ def k_vectors(cell, k_cutoff):
     out = np.zeros((n, 4))
     lib().orc_k_vectors(_ptr(cell), float(k_cutoff), _ptr(out))
     return out
+
+
+def _lode_terms(hypers, fn):
+    h = lode_hypers(hypers)
+    handle = lib().orc_lode_new(ctypes.byref(h))
+    if not handle:
+        raise ValueError("invalid LODE hyper-parameters")
+    try:
+        return fn(handle, h)
+    finally:
+        lib().orc_lode_free(handle)
+
+
+def lode_density_fourier(hypers, k_norms):
+    """rho(k) for each |k| (lode/spherical_expansion.rs:280-342)."""
+    return _lode_terms(hypers, lambda hd, h: np.array([lib().orc_lode_density_fourier(hd, float(k)) for k in k_norms]))
+
+
+def lode_center_contribution(hypers):
+    """Central-atom term per radial function (lode/radial_integral/gto.rs:136-185)."""
+    def fn(hd, h):
+        out = np.zeros(h.n_radial)
+        lib().orc_lode_center_contribution(hd, _ptr(out))
+        return out
+    return _lode_terms(hypers, fn)
+
+
+def lode_k0_contribution(hypers):
+    """k = 0 term per radial function (lode/spherical_expansion.rs:347-390)."""
+    def fn(hd, h):
+        out = np.zeros(h.n_radial)
+        lib().orc_lode_k0_contribution(hd, _ptr(out))
+        return out
+    return _lode_terms(hypers, fn)
 
 
 def lode_spherical_expansion(systems, hypers, gradients=()):

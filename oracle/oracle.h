@@ -135,6 +135,12 @@ orc_lode* orc_lode_new(const orc_lode_hypers* hypers);
 void orc_lode_free(orc_lode* lode);
 /* number of k-vectors for a cell (math/k_vectors.rs:20-83) */
 int orc_k_vectors(const double* cell, double k_cutoff, double* out_directions_norms /* [n][4] or NULL */);
+/* rho(k) of the smeared power-law density (lode/spherical_expansion.rs:280-342) */
+double orc_lode_density_fourier(const orc_lode* lode, double k);
+/* central-atom term of the l = 0 radial basis, [n_radial] (lode/radial_integral/gto.rs:136-185) */
+void orc_lode_center_contribution(const orc_lode* lode, double* out);
+/* k = 0 term, [n_radial]; returns 0 (and zeros) when it vanishes (lode/spherical_expansion.rs:347-390) */
+int orc_lode_k0_contribution(const orc_lode* lode, double* out);
 /*
  * LODE spherical expansion of one (periodic) system, dense layout
  *   values [n_atoms][S][L2][N]

@@ -177,6 +177,35 @@ static double density_fourier(const orc_lode* lode, double k) {
     return factor * v;
 }
 
+double orc_lode_density_fourier(const orc_lode* lode, double k) { return density_fourier(lode, k); }
+
+void orc_lode_center_contribution(const orc_lode* lode, double* out) {
+    for (int n = 0; n < lode->h.n_radial; n++) out[n] = lode->center[n];
+}
+
+/* compute_k0_contributions, lode/spherical_expansion.rs:347-390: factor(p, smearing) times the
+ * l = 0 radial integral at k = 0; zero for 0 < p <= 3. Returns 1 when the term is non-zero. */
+int orc_lode_k0_contribution(const orc_lode* lode, double* out) {
+    int p = lode->h.exponent, N = lode->h.n_radial;
+    double smearing = lode->h.smearing;
+    double factor;
+    if (p == 0) {
+        double s2 = smearing * smearing;
+        factor = pow(2.0 * ORC_PI * s2, 1.5) / pow(ORC_PI * s2, 0.75) / sqrt(4.0 * ORC_PI);
+    } else if (p > 3) {
+        double p_eff = 3.0 - p;
+        factor = 0.5 * ORC_PI * pow(2.0, p_eff) / orc_gamma(0.5 * p) * pow(2.0, (p - 1.0) / 2.0) / -p_eff *
+                 pow(smearing, -p_eff) / pow(smearing, 2.0 * p - 6.0);
+    } else {
+        for (int n = 0; n < N; n++) out[n] = 0.0;
+        return 0;
+    }
+    double ri0[64];
+    orc_radial_compute(lode->radial, 0, 0.0, ri0, NULL);
+    for (int n = 0; n < N; n++) out[n] = factor * ri0[n];
+    return 1;
+}
+
 static int species_index(int n_species, const int32_t* species, int32_t t) {
     for (int s = 0; s < n_species; s++) {
         if (species[s] == t) return s;
@@ -261,26 +290,14 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
     }
 
     /* k = 0 term, lode/spherical_expansion.rs:347-390,658-687 */
-    int p = lode->h.exponent;
-    if (p == 0 || p > 3) {
-        double smearing = lode->h.smearing;
-        double factor;
-        if (p == 0) {
-            double s2 = smearing * smearing;
-            factor = pow(2.0 * ORC_PI * s2, 1.5) / pow(ORC_PI * s2, 0.75) / sqrt(4.0 * ORC_PI);
-        } else {
-            double p_eff = 3.0 - p;
-            factor = 0.5 * ORC_PI * pow(2.0, p_eff) / orc_gamma(0.5 * p) * pow(2.0, (p - 1.0) / 2.0) / -p_eff *
-                     pow(smearing, -p_eff) / pow(smearing, 2.0 * p - 6.0);
-        }
-        double ri0[64];
-        orc_radial_compute(lode->radial, 0, 0.0, ri0, NULL);
+    double k0[64];
+    if (orc_lode_k0_contribution(lode, k0)) {
         /* for each neighbour atom (counted with multiplicity), every center gets the k0 term in
          * block (ct, type_neighbor) */
         for (size_t j = 0; j < n_atoms; j++) {
             int nt = species_index(n_species, species, types[j]);
             for (size_t i = 0; i < n_atoms; i++) {
-                for (int n = 0; n < N; n++) values[((i * S + nt) * L2 + 0) * N + n] += global_factor * (factor * ri0[n]);
+                for (int n = 0; n < N; n++) values[((i * S + nt) * L2 + 0) * N + n] += global_factor * k0[n];
             }
         }
     }
