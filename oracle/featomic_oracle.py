@@ -133,6 +133,8 @@ def lib():
         L.orc_soap_se_system.argtypes = [vp, sz, vp, sz, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
         L.orc_soap_ps_batch.restype = ctypes.c_int64
         L.orc_soap_ps_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(d)]
+        L.orc_ps_gradient_rows.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, d, vp, vp, vp, vp, vp, vp]
         L.orc_lode_new.restype = vp
         L.orc_lode_new.argtypes = [ctypes.POINTER(LodeHypers)]
         L.orc_lode_free.argtypes = [vp]
@@ -528,6 +530,38 @@ class SoapComputation:
         return out
 
 
+    def se_gradient_table(self, sd):
+        """Table of the spherical-expansion position-gradient rows of one system,
+        [entries][3][L2][N]: entry 0 is zero, then the self gradients of every (atom,
+        type) from `self_base`, then one entry per (center, atom != center) with the sum
+        over pair images (spherical_expansion.rs:476-521), indexed by `pair_index`."""
+        cached = getattr(sd, "gradient_table", None)
+        if cached is not None:
+            return cached
+        n = len(sd.system)
+        sign = np.concatenate([np.full(2 * l + 1, -1.0 if l % 2 else 1.0) for l in range(self.lmax + 1)])
+        pair_index = {}
+        order = []
+        for p, pair in enumerate(sd.pairs):
+            f, s = int(pair["first"]), int(pair["second"])
+            if f == s:
+                continue
+            for key in ((f, s), (s, f)):
+                if key not in pair_index:
+                    pair_index[key] = 1 + n * self.S + len(order)
+                    order.append(key)
+        table = np.zeros((1 + n * self.S + len(order), 3, self.L2, self.N))
+        table[1:1 + n * self.S] = sd.self_grads.reshape(n * self.S, 3, self.L2, self.N)
+        for p, pair in enumerate(sd.pairs):
+            f, s = int(pair["first"]), int(pair["second"])
+            if f == s:
+                continue
+            g = sd.pair_grads[p]
+            table[pair_index[(f, s)]] += g
+            table[pair_index[(s, f)]] -= sign[None, :, None] * g
+        sd.gradient_table = (table, 1, pair_index)
+        return sd.gradient_table
+
     def se_position_gradient_all(self, sd, center, atom, nt_index):
         """se_position_gradient for every l at once, [3, L2, N]; the per-(center, atom)
         sums over pair images are tabulated once per system."""
@@ -827,29 +861,39 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                         nbrs.add(i)
                 for j in sorted(nbrs):
                     grad_samples.append((s_i, si, j))
-            # power_spectrum.rs:621-678; rows in chunks, every l at once per chunk
+            # power_spectrum.rs:621-678; the rows of each system are contracted in C from the
+            # table of its spherical-expansion gradient rows (entry 0: no such term)
             gv = np.zeros((len(grad_samples), 3, len(props)))
-            zero = np.zeros((3, comp.L2, N))
-            for r0 in range(0, len(grad_samples), 4096):
-                chunk = grad_samples[r0:r0 + 4096]
-                g1s = np.empty((len(chunk), 3, comp.L2, N))
-                g2s = np.empty((len(chunk), 3, comp.L2, N))
-                rows = np.empty(len(chunk), dtype=np.int64)
-                for r, (s_i, si, j) in enumerate(chunk):
-                    sd = comp.data[si]
+            r0 = 0
+            while r0 < len(grad_samples):
+                si = grad_samples[r0][1]
+                r1 = r0
+                while r1 < len(grad_samples) and grad_samples[r1][1] == si:
+                    r1 += 1
+                sd = comp.data[si]
+                table, self_base, pair_index = comp.se_gradient_table(sd)
+                t = sd.system.types
+                n = r1 - r0
+                center = np.empty(n, dtype=np.int64)
+                idx1 = np.zeros(n, dtype=np.int64)
+                idx2 = np.zeros(n, dtype=np.int64)
+                for r in range(n):
+                    s_i, _, j = grad_samples[r0 + r]
                     i = samples[s_i][1]
-                    t = sd.system.types
-                    rows[r] = s_i
-                    # SE gradient sample exists in block t1 iff j == i or type_j == t1
-                    g1s[r] = comp.se_position_gradient_all(sd, i, j, i1) if (j == i or t[j] == t1) else zero
-                    g2s[r] = comp.se_position_gradient_all(sd, i, j, i2) if (j == i or t[j] == t2) else zero
-                c1r, c2r = c1s[rows], c2s[rows]
-                for l in range(lmax + 1):
-                    sl = slice(l * l, (l + 1) * (l + 1))
-                    # Σ_m g1[x, m, a] c2[m, b] + Σ_m c1[m, a] g2[x, m, b]
-                    acc = np.matmul(g1s[:, :, sl].transpose(0, 1, 3, 2), c2r[:, None, sl])
-                    acc += np.matmul(c1r[:, None, sl].transpose(0, 1, 3, 2), g2s[:, :, sl])
-                    gv[r0:r0 + len(chunk), :, l * N * N:(l + 1) * N * N] = (kappa * acc / norms[l]).reshape(len(chunk), 3, -1)
+                    center[r] = i
+                    if j == i:
+                        idx1[r] = self_base + i * comp.S + i1
+                        idx2[r] = self_base + i * comp.S + i2
+                    else:
+                        # SE gradient sample exists in block t1 iff type_j == t1
+                        k = pair_index.get((i, j), 0)
+                        idx1[r] = k if t[j] == t1 else 0
+                        idx2[r] = k if t[j] == t2 else 0
+                out = np.empty((n, 3, len(props)))
+                lib().orc_ps_gradient_rows(n, lmax + 1, N, comp.S, i1, i2, kappa, _ptr(sd.values), _ptr(center),
+                                           _ptr(idx1), _ptr(idx2), _ptr(table), _ptr(out))
+                gv[r0:r1] = out
+                r0 = r1
             blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz()], ["l", "n_1", "n_2"], props)
 
         for param, attr in (("cell", "cell_g"), ("strain", "strain")):

@@ -118,6 +118,12 @@ int64_t orc_soap_ps_batch(const orc_soap* soap, int n_systems, const int64_t* at
                           int n_species, const int32_t* species, int n_threads,
                           double* checksum);
 
+/* power-spectrum position-gradient rows of one block from the spherical-expansion
+ * gradient rows (table) and coefficients (values), see orc_soap.c */
+void orc_ps_gradient_rows(int64_t n_rows, int L, int N, int S, int i1, int i2, double kappa, const double* values,
+                          const int64_t* center, const int64_t* idx1, const int64_t* idx2, const double* table,
+                          double* out);
+
 /* ---- LODE ---- */
 typedef struct orc_lode_hypers {
     double smearing;

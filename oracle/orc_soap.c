@@ -419,3 +419,40 @@ int64_t orc_soap_ps_batch(const orc_soap* s, int n_systems, const int64_t* atom_
     *checksum = total;
     return total_rows;
 }
+
+/* Power-spectrum position-gradient rows of one block (soap/power_spectrum.rs:621-678):
+ * row r of center c = center[r] holds, per l,
+ *   kappa / ((-1)^l sqrt(2l+1)) * (sum_m g1[x][m][n1] c2[m][n2] + sum_m c1[m][n1] g2[x][m][n2])
+ * with c1, c2 the center's coefficients of the block's two neighbour types and g1, g2 the
+ * spherical-expansion gradient rows table[idx1[r]], table[idx2[r]] (table entry 0 is zero when
+ * the row has no such term). values [n_atoms][S][L2][N], table [entries][3][L2][N],
+ * out [n_rows][3][L N N]. */
+void orc_ps_gradient_rows(int64_t n_rows, int L, int N, int S, int i1, int i2, double kappa, const double* values,
+                          const int64_t* center, const int64_t* idx1, const int64_t* idx2, const double* table,
+                          double* out) {
+    const int L2 = L * L;
+    const int P = L * N * N;
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < n_rows; r++) {
+        const double* c1 = values + ((size_t)center[r] * S + i1) * L2 * N;
+        const double* c2 = values + ((size_t)center[r] * S + i2) * L2 * N;
+        const double* g1 = table + (size_t)idx1[r] * 3 * L2 * N;
+        const double* g2 = table + (size_t)idx2[r] * 3 * L2 * N;
+        double* o = out + (size_t)r * 3 * P;
+        for (int x = 0; x < 3; x++) {
+            for (int l = 0; l < L; l++) {
+                const double norm = (l % 2 == 0 ? 1.0 : -1.0) * sqrt(2.0 * l + 1.0);
+                for (int a = 0; a < N; a++) {
+                    for (int b = 0; b < N; b++) {
+                        double acc = 0.0;
+                        for (int m = l * l; m < (l + 1) * (l + 1); m++) {
+                            acc += g1[((size_t)x * L2 + m) * N + a] * c2[m * N + b];
+                            acc += c1[m * N + a] * g2[((size_t)x * L2 + m) * N + b];
+                        }
+                        o[(size_t)x * P + (l * N + a) * N + b] = kappa * acc / norm;
+                    }
+                }
+            }
+        }
+    }
+}
