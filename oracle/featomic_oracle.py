@@ -146,6 +146,8 @@ def lib():
         L.orc_lode_k0_contribution.restype = ctypes.c_int
         L.orc_lode_k0_contribution.argtypes = [vp, vp]
         L.orc_lode_system.restype = ctypes.c_int
+        L.orc_lode_system_centers.restype = ctypes.c_int
+        L.orc_lode_system_centers.argtypes = [vp, sz, vp, vp, vp, ctypes.c_int, vp, vp, sz, vp, vp, vp]
         L.orc_lode_system.argtypes = [vp, sz, vp, vp, vp, ctypes.c_int, vp, vp, vp, vp]
         L.orc_mlp_n_params.restype = ctypes.c_int64
         L.orc_mlp_n_params.argtypes = [ctypes.c_int]
@@ -983,6 +985,34 @@ def lode_k0_contribution(hypers):
         lib().orc_lode_k0_contribution(hd, _ptr(out))
         return out
     return _lode_terms(hypers, fn)
+
+
+def lode_dense(system, hypers, species, centers=()):
+    """Dense LODE expansion of one system: values [n][S][L2][N] of every atom and the
+    position-gradient rows of the listed centers, grads [c][n][3][L2][N] (row (i, j) =
+    d c_i^{type j} / d r_j) and self_grads [c][S][3][L2][N]."""
+    h = lode_hypers(hypers)
+    handle = lib().orc_lode_new(ctypes.byref(h))
+    if not handle:
+        raise ValueError("invalid LODE hyper-parameters")
+    L2, N = (h.max_angular + 1) ** 2, h.n_radial
+    n, S = len(system), len(species)
+    species = np.ascontiguousarray(species, dtype=np.int32)
+    centers = np.ascontiguousarray(centers, dtype=np.int32)
+    values = np.zeros((n, S, L2, N))
+    grads = np.zeros((len(centers), n, 3, L2, N))
+    self_grads = np.zeros((len(centers), S, 3, L2, N))
+    try:
+        status = lib().orc_lode_system_centers(handle, n, _ptr(system.positions), _ptr(system.types),
+                                               _ptr(system.cell.reshape(9)), S, _ptr(species), _ptr(values),
+                                               len(centers), _ptr(centers) if len(centers) else None,
+                                               _ptr(grads) if len(centers) else None,
+                                               _ptr(self_grads) if len(centers) else None)
+    finally:
+        lib().orc_lode_free(handle)
+    if status != 0:
+        raise ValueError(f"LODE oracle failed with status {status}")
+    return values, grads, self_grads
 
 
 def lode_spherical_expansion(systems, hypers, gradients=()):

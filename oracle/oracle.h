@@ -159,6 +159,13 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
                     const int32_t* types, const double* cell, int n_species,
                     const int32_t* species, double* values, double* grads, double* self_grads);
 
+/* the same with the gradient rows of the listed centers only: grads [n_centers][n_atoms][3][L2][N],
+ * self_grads [n_centers][S][3][L2][N] (values of every atom) */
+int orc_lode_system_centers(const orc_lode* lode, size_t n_atoms, const double* positions,
+                            const int32_t* types, const double* cell, int n_species, const int32_t* species,
+                            double* values, size_t n_centers, const int32_t* centers, double* grads,
+                            double* self_grads);
+
 /* radial-integral surrogate network (orc_mlp.c): 1 -> 64 -> 64 -> n_out, SiLU */
 #define ORC_MLP_HIDDEN 64
 int64_t orc_mlp_n_params(int n_out);

@@ -213,9 +213,11 @@ static int species_index(int n_species, const int32_t* species, int32_t t) {
     return -1;
 }
 
-int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* positions, const int32_t* types,
-                    const double* cell, int n_species, const int32_t* species, double* values, double* grads,
-                    double* self_grads) {
+/* centers == NULL: gradient rows of every center; else of centers[0 .. n_centers), stored at the
+ * center's position in that list */
+static int lode_system(const orc_lode* lode, size_t n_atoms, const double* positions, const int32_t* types,
+                       const double* cell, int n_species, const int32_t* species, double* values, size_t n_centers,
+                       const int32_t* centers, double* grads, double* self_grads) {
     int lmax = lode->h.max_angular, N = lode->h.n_radial;
     int L2 = (lmax + 1) * (lmax + 1);
     size_t blk = (size_t)L2 * N;
@@ -279,8 +281,9 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
     double global_factor = 4.0 * ORC_PI / volume;
 
     memset(values, 0, sizeof(double) * n_atoms * S * blk);
-    if (grads) memset(grads, 0, sizeof(double) * n_atoms * n_atoms * 3 * blk);
-    if (self_grads) memset(self_grads, 0, sizeof(double) * n_atoms * S * 3 * blk);
+    if (!centers) n_centers = n_atoms;
+    if (grads) memset(grads, 0, sizeof(double) * n_centers * n_atoms * 3 * blk);
+    if (self_grads) memset(self_grads, 0, sizeof(double) * n_centers * S * 3 * blk);
 
     /* centre contribution, lode/spherical_expansion.rs:397-449 (l=0, nt == ct) */
     double w = lode->h.center_atom_weight;
@@ -321,7 +324,8 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
         }
         if (!grads && !self_grads) continue;
         /* gradients, lode/spherical_expansion.rs:771-871 */
-        for (size_t i = 0; i < n_atoms; i++) {
+        for (size_t c = 0; c < n_centers; c++) {
+            size_t i = centers ? (size_t)centers[c] : c;
             int ct = species_index(n_species, species, types[i]);
             for (size_t j = 0; j < n_atoms; j++) {
                 int nt_j = species_index(n_species, species, types[j]);
@@ -347,7 +351,8 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
                         }
                     }
                     for (int k = 0; k < nk; k++) kdep[k] *= global_factor * phase * df[k];
-                    double* out = (i == j) ? self_grads + (i * S + t) * 3 * blk : grads + (i * n_atoms + j) * 3 * blk;
+                    double* out = (i == j) ? (self_grads ? self_grads + (c * S + t) * 3 * blk : NULL)
+                                           : (grads ? grads + (c * n_atoms + j) * 3 * blk : NULL);
                     if (!out) continue;
                     for (int lm = l * l; lm < (l + 1) * (l + 1); lm++) {
                         for (int n = 0; n < N; n++) {
@@ -380,4 +385,18 @@ int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* position
     free(ri);
     free(kv);
     return 0;
+}
+
+int orc_lode_system(const orc_lode* lode, size_t n_atoms, const double* positions, const int32_t* types,
+                    const double* cell, int n_species, const int32_t* species, double* values, double* grads,
+                    double* self_grads) {
+    return lode_system(lode, n_atoms, positions, types, cell, n_species, species, values, n_atoms, NULL, grads,
+                       self_grads);
+}
+
+int orc_lode_system_centers(const orc_lode* lode, size_t n_atoms, const double* positions, const int32_t* types,
+                            const double* cell, int n_species, const int32_t* species, double* values,
+                            size_t n_centers, const int32_t* centers, double* grads, double* self_grads) {
+    return lode_system(lode, n_atoms, positions, types, cell, n_species, species, values, n_centers, centers, grads,
+                       self_grads);
 }

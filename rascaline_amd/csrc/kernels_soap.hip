@@ -1319,16 +1319,18 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
 // DC_run[x][l² + l ± m][:] += (df R + f dR) Y d_x + (f / r) R r dY/dx
 // (soap/spherical_expansion_pair.rs:382-419). A run carried from the previous
 // window (slot 0) continues its sums.
+// `shc` is the spherical-harmonics coefficient table (ShTable layout), in global
+// memory or an LDS copy.
 template <typename T, int NC>
-__device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& lay, int nr, const int* run_lo,
+__device__ void plan_entry_stage2(const T* shc, const TileEntryLayout& lay, int nr, const int* run_lo,
                                   const int* run_hi, bool carry_in, const int* leg_l, const int* leg_m, const T* ent,
-                                  T* DC) {
+                                  T* DC, int n_threads = TILE_THREADS) {
     const int leg = lay.leg, L2 = lay.L2;
     const int CS = L2 * NC;
     const ShTable tb(lay.L);
     const T SQRT_2 = (T)1.4142135623730951;
     const int per_x = nr * leg;
-    for (int item = threadIdx.x; item < 3 * per_x; item += TILE_THREADS) {
+    for (int item = threadIdx.x; item < 3 * per_x; item += n_threads) {
         const int x = item / per_x;
         const int rest = item - x * per_x;
         const int r = rest / leg, k = rest - r * leg;
@@ -1350,7 +1352,7 @@ __device__ void plan_entry_stage2(const SoapArgs<T>& a, const TileEntryLayout& l
                 accm[c] = 0;
             }
         }
-        const T c_gl = a.sh_c[tb.gl + l], c_dpa = a.sh_c[tb.dpa + k], c_dpb = a.sh_c[tb.dpb + k];
+        const T c_gl = shc[tb.gl + l], c_dpa = shc[tb.dpa + k], c_dpb = shc[tb.dpb + k];
         for (int e = run_lo[r]; e < run_hi[r]; e++) {
             const T* base = ent + e * lay.per_entry;
             const T* P = base + lay.p;
@@ -1436,6 +1438,287 @@ __device__ __forceinline__ void bf16_split(float x, short& hi, short& lo) {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Gradient rows of the complete runs of one window (the contraction phase of the
+// tiled kernels), run by the workgroup's TILE_THREADS computing threads with no
+// workgroup barrier: per run r (neighbour type run_tj[r], rows run_row[r][k] per
+// present type k) and x, l, the rows s_l (DC_r^T C_t, C_t^T DC_r or both) of
+// soap/power_spectrum.rs:621-678, written through the wave's staging area `stage`.
+template <typename T, int NC, int RP, bool FAST>
+__device__ __forceinline__ void ps_window_rows(int L, int n, int n_present, const int* present, int nc_runs,
+                                               const int* run_tj, const int (*run_row)[TILE_MAX_TYPES],
+                                               const T* C, const T* DC, const T* scale_l, T* stage,
+                                               long long* stage_offs, T* __restrict__ out) {
+    const int tid = threadIdx.x;
+    const int wave_id = tid / WAVE;
+    const int L2 = L * L;
+    const int CS = L2 * NC;
+    const int P = L * n * n;
+    constexpr int nq = NC / 4;
+    const bool staged = n == NC;
+    if constexpr (FAST && sizeof(T) == 4 && NC == 8) {
+        // mixed precision: X[r][g][n1][n2] = Σ_m DC_r[x][m][n1] C_{t_g}[m][n2] for up to 4
+        // runs x 2 types per pair of 16 x 16 tiles, bfloat16 hi + lo products on MFMA with
+        // FP32 accumulation (3 MFMAs per k block of 16); the wave stages X in LDS and
+        // writes each 8 x 8 block in the orientation of its output block
+        const int lane = tid & (WAVE - 1);
+        const int ngroups = (n_present + 1) / 2;
+        const int items = 3 * L * ngroups;
+        T* X = stage;  // [4 runs][2 types][8][8]
+        for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
+            const int g = it % ngroups, lx = it / ngroups;
+            const int l = L - 1 - lx / 3, x = lx % 3;
+            const int g0 = 2 * g, ng = min(2, n_present - g0);
+            const int kcount = 2 * l + 1;
+            const int i16 = lane & 15, kq = 4 * (lane >> 4);
+            for (int tr = 0; tr < (nc_runs + 1) / 2; tr++) {
+                float4_t acc = {0.f, 0.f, 0.f, 0.f};
+                const int r = 2 * tr + (i16 >> 3), n1 = i16 & 7;
+                const int tg = i16 >> 3, n2 = i16 & 7;
+                for (int kb = 0; kb < kcount; kb += 16) {
+                    short4_t ah, al, bh, bl;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int mi = kb + kq + e;
+                        float av = 0.f, bv = 0.f;
+                        if (mi < kcount) {
+                            if (r < nc_runs) av = DC[((size_t)r * 3 + x) * CS + (l * l + mi) * NC + n1];
+                            if (tg < ng) bv = C[(size_t)present[g0 + tg] * CS + (l * l + mi) * NC + n2];
+                        }
+                        short h, lo;
+                        bf16_split(av, h, lo);
+                        ah[e] = h;
+                        al[e] = lo;
+                        bf16_split(bv, h, lo);
+                        bh[e] = h;
+                        bl[e] = lo;
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+                }
+                // D[row][col]: row = 4 (lane / 16) + v within the tile, col = lane % 16
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rho = 16 * tr + kq + v;
+                    X[(((rho >> 3) * 2 + tg) * 8 + (rho & 7)) * 8 + n2] = acc[v];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            const int npieces = nc_runs * ng * 16;
+            for (int p = lane; p < npieces; p += WAVE) {
+                const int r = p / (ng * 16), rem = p - r * ng * 16;
+                const int gg = rem >> 4, n1 = (rem & 15) >> 1, h = rem & 1;
+                const int t = present[g0 + gg], tj = run_tj[r];
+                const T* Xb = X + (r * 2 + gg) * 64;
+                T v[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int n2 = 4 * h + c;
+                    T sum = 0;
+                    if (t >= tj) sum += Xb[n1 * 8 + n2];
+                    if (t <= tj) sum += Xb[n2 * 8 + n1];
+                    v[c] = sum;
+                }
+                const T sc = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+                T* dst = out + (long long)run_row[r][g0 + gg] * 3 * P + (long long)x * P + l * 64 + n1 * 8 + 4 * h;
+                store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        }
+        return;
+    } else if constexpr (FAST && sizeof(T) == 4 && (NC == 12 || NC == 16)) {
+        // mixed precision for n = 12 or 16: one 16 x 16 tile per (run, type),
+        // X[n1][n2] = Σ_m DC_r[x][m][n1] C_t[m][n2] in bfloat16 hi + lo on MFMA. The
+        // type's operand (B) is split once per (l, x, type) item; the wave stages up
+        // to RP * NC / 4 tiles, then writes their n x n blocks as 16-byte pieces
+        constexpr int STAGE_TILES = RP * NC / 4;
+        constexpr int KB_MAX = (2 * TILE_MAX_L - 1 + 15) / 16;
+        constexpr int nq4 = NC / 4;
+        const int lane = tid & (WAVE - 1);
+        const int i16 = lane & 15, kq = 4 * (lane >> 4);
+        const int items = 3 * L * n_present;
+        T* X = stage;  // [STAGE_TILES][16][16]
+        for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
+            const int g = it % n_present, lx = it / n_present;
+            const int l = L - 1 - lx / 3, x = lx % 3;
+            const int kcount = 2 * l + 1;
+            const int t = present[g];
+            const T* Cb = C + (size_t)t * CS + l * l * NC;
+            short4_t bh[KB_MAX], bl[KB_MAX];
+#pragma unroll
+            for (int kb = 0; kb < KB_MAX; kb++) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int mi = 16 * kb + kq + e;
+                    const float bv = (mi < kcount && i16 < NC) ? Cb[mi * NC + i16] : 0.f;
+                    short h, lo;
+                    bf16_split(bv, h, lo);
+                    bh[kb][e] = h;
+                    bl[kb][e] = lo;
+                }
+            }
+            const T sc_same = scale_l[l], sc_diff = scale_l[l] * (T)1.4142135623730951;
+            for (int r0 = 0; r0 < nc_runs; r0 += STAGE_TILES) {
+                const int nr = min(STAGE_TILES, nc_runs - r0);
+                for (int rr = 0; rr < nr; rr++) {
+                    const T* D = DC + ((size_t)(r0 + rr) * 3 + x) * CS + l * l * NC;
+                    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kb = 0; kb < KB_MAX; kb++) {
+                        if (16 * kb >= kcount) break;
+                        short4_t ah, al;
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            const int mi = 16 * kb + kq + e;
+                            const float av = (mi < kcount && i16 < NC) ? D[mi * NC + i16] : 0.f;
+                            short h, lo;
+                            bf16_split(av, h, lo);
+                            ah[e] = h;
+                            al[e] = lo;
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[kb], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[kb], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[kb], acc, 0, 0, 0);
+                    }
+                    // D[row][col]: row = 4 (lane / 16) + v, col = lane % 16
+#pragma unroll
+                    for (int v = 0; v < 4; v++) X[rr * 256 + (kq + v) * 16 + i16] = acc[v];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                const int npieces = nr * NC * nq4;
+                for (int p = lane; p < npieces; p += WAVE) {
+                    const int rr = p / (NC * nq4), rem = p - rr * (NC * nq4);
+                    const int n1 = rem / nq4, h = rem - n1 * nq4;
+                    const int r = r0 + rr, tj = run_tj[r];
+                    const T* Xb = X + rr * 256;
+                    T v[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int n2 = 4 * h + c;
+                        T sum = 0;
+                        if (t >= tj) sum += Xb[n1 * 16 + n2];
+                        if (t <= tj) sum += Xb[n2 * 16 + n1];
+                        v[c] = sum;
+                    }
+                    const T sc = t != tj ? sc_diff : sc_same;
+                    T* dst = out + (long long)run_row[r][g] * 3 * P + (long long)x * P + l * NC * NC + n1 * NC + 4 * h;
+                    store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            }
+        }
+        return;
+    }
+    // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
+    // so that the nq lanes of one (run, type, x, l) tile are adjacent
+    const int NRK = nc_runs * n_present;
+    const int per_lx = NRK * nq;
+    const int total = per_lx * 3 * L;
+    const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
+    const float inv_np = 1.0f / (float)(n_present > 0 ? n_present : 1);
+    for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
+        const int u = ubase + (tid & (WAVE - 1));
+        long long tile = -1;
+        int q = 0;
+        T s = 0;
+        T acc[4][NC];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[rr][c] = 0;
+        if (u < total) {
+            int lx, wq, rk, r, k;
+            small_divmod(u, per_lx, inv_lx, lx, wq);
+            rk = wq / nq;
+            q = wq - rk * nq;
+            small_divmod(rk, n_present, inv_np, r, k);
+            const int lrev = lx / 3;
+            const int x = lx - 3 * lrev;
+            const int l = L - 1 - lrev;
+            const int t = present[k], tj = run_tj[r];
+            const T* D = DC + ((size_t)r * 3 + x) * CS;
+            const T* Ct = C + (size_t)t * CS;
+            if (t >= tj) tile_fma<T, NC, RP>(l, q, D, Ct, acc);
+            if (t <= tj) tile_fma<T, NC, RP>(l, q, Ct, D, acc);
+            s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+            tile = (long long)run_row[r][k] * 3 * P + (long long)x * P + l * n * n;
+        }
+        if (staged) {
+            staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
+        } else if (tile >= 0) {
+            tile_store<T, NC, RP>(out + tile, q, n, s, acc);
+        }
+    }
+}
+
+// The center's own gradient rows (atom == center) of every block (t1 <= t2) present
+// around center i, from its self gradients DS [n_present][3][L2][NC] (LDS) and its
+// coefficients C: s_l (DS_{t1}^T C_{t2} + C_{t1}^T DS_{t2}) (soap/power_spectrum.rs:621-678),
+// run by the TILE_THREADS computing threads with no workgroup barrier.
+template <typename T, int NC, int RP>
+__device__ __forceinline__ void ps_self_rows(int i, int L, int n, int S, int n_present, const int* present,
+                                             const int* __restrict__ self_rows, const T* C, const T* DS,
+                                             const T* scale_l, T* stage, long long* stage_offs, T* __restrict__ out) {
+    const int tid = threadIdx.x;
+    const int wave_id = tid / WAVE;
+    const int CS = L * L * NC;
+    const int P = L * n * n;
+    const int npairs = S * (S + 1) / 2;
+    constexpr int nq = NC / 4;
+    const bool staged = n == NC;
+    const int npp = n_present * (n_present + 1) / 2;
+    const int per_lx = npp * nq;
+    const int total = per_lx * 3 * L;
+    const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
+    for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
+        const int u = ubase + (tid & (WAVE - 1));
+        long long tile = -1;
+        int q = 0;
+        T s = 0;
+        T acc[4][NC];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[rr][c] = 0;
+        if (u < total) {
+            int lx, wq;
+            small_divmod(u, per_lx, inv_lx, lx, wq);
+            const int p = wq / nq;
+            q = wq - p * nq;
+            const int lrev = lx / 3;
+            const int x = lx - 3 * lrev;
+            const int l = L - 1 - lrev;
+            // p -> (k1 <= k2) in row-major order of the upper triangle
+            int k1 = 0, rem = p;
+            while (rem >= n_present - k1) {
+                rem -= n_present - k1;
+                k1++;
+            }
+            const int k2 = k1 + rem;
+            const int t1 = present[k1], t2 = present[k2];
+            const int local = pair_index(t1, t2, S);
+            const long long row = self_rows[(long long)i * npairs + local];
+            tile_fma<T, NC, RP>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
+            tile_fma<T, NC, RP>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
+            s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
+            tile = row * 3 * P + (long long)x * P + l * n * n;
+        }
+        if (staged) {
+            staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
+        } else if (tile >= 0) {
+            tile_store<T, NC, RP>(out + tile, q, n, s, acc);
+        }
+    }
+}
 
 template <typename T, int NC, int RP, bool FAST>
 __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
@@ -1524,214 +1807,12 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
             for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
         }
         lds_barrier();
-        plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
+        plan_entry_stage2<T, NC>(a.sh_c, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
         lds_barrier();
 
         const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
-        if constexpr (FAST && sizeof(T) == 4 && NC == 8) {
-            // mixed precision: X[r][g][n1][n2] = Σ_m DC_r[x][m][n1] C_{t_g}[m][n2] for up to 4
-            // runs x 2 types per pair of 16 x 16 tiles, bfloat16 hi + lo products on MFMA with
-            // FP32 accumulation (3 MFMAs per k block of 16); the wave stages X in LDS and
-            // writes each 8 x 8 block in the orientation of its output block
-            const int lane = tid & (WAVE - 1);
-            const int ngroups = (n_present + 1) / 2;
-            const int items = 3 * L * ngroups;
-            T* X = stage;  // [4 runs][2 types][8][8]
-            for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
-                const int g = it % ngroups, lx = it / ngroups;
-                const int l = L - 1 - lx / 3, x = lx % 3;
-                const int g0 = 2 * g, ng = min(2, n_present - g0);
-                const int kcount = 2 * l + 1;
-                const int i16 = lane & 15, kq = 4 * (lane >> 4);
-                for (int tr = 0; tr < (nc_runs + 1) / 2; tr++) {
-                    float4_t acc = {0.f, 0.f, 0.f, 0.f};
-                    const int r = 2 * tr + (i16 >> 3), n1 = i16 & 7;
-                    const int tg = i16 >> 3, n2 = i16 & 7;
-                    for (int kb = 0; kb < kcount; kb += 16) {
-                        short4_t ah, al, bh, bl;
-#pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            const int mi = kb + kq + e;
-                            float av = 0.f, bv = 0.f;
-                            if (mi < kcount) {
-                                if (r < nc_runs) av = DC[((size_t)r * 3 + x) * CS + (l * l + mi) * NC + n1];
-                                if (tg < ng) bv = C[(size_t)present[g0 + tg] * CS + (l * l + mi) * NC + n2];
-                            }
-                            short h, lo;
-                            bf16_split(av, h, lo);
-                            ah[e] = h;
-                            al[e] = lo;
-                            bf16_split(bv, h, lo);
-                            bh[e] = h;
-                            bl[e] = lo;
-                        }
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
-                    }
-                    // D[row][col]: row = 4 (lane / 16) + v within the tile, col = lane % 16
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int rho = 16 * tr + kq + v;
-                        X[(((rho >> 3) * 2 + tg) * 8 + (rho & 7)) * 8 + n2] = acc[v];
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-                const int npieces = nc_runs * ng * 16;
-                for (int p = lane; p < npieces; p += WAVE) {
-                    const int r = p / (ng * 16), rem = p - r * ng * 16;
-                    const int gg = rem >> 4, n1 = (rem & 15) >> 1, h = rem & 1;
-                    const int t = present[g0 + gg], tj = run_tj[r];
-                    const T* Xb = X + (r * 2 + gg) * 64;
-                    T v[4];
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int n2 = 4 * h + c;
-                        T sum = 0;
-                        if (t >= tj) sum += Xb[n1 * 8 + n2];
-                        if (t <= tj) sum += Xb[n2 * 8 + n1];
-                        v[c] = sum;
-                    }
-                    const T sc = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-                    T* dst = out + (long long)run_row[r][g0 + gg] * 3 * P + (long long)x * P + l * 64 + n1 * 8 + 4 * h;
-                    store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-            }
-            lds_barrier();
-            continue;
-        } else if constexpr (FAST && sizeof(T) == 4 && (NC == 12 || NC == 16)) {
-            // mixed precision for n = 12 or 16: one 16 x 16 tile per (run, type),
-            // X[n1][n2] = Σ_m DC_r[x][m][n1] C_t[m][n2] in bfloat16 hi + lo on MFMA. The
-            // type's operand (B) is split once per (l, x, type) item; the wave stages up
-            // to RP * NC / 4 tiles, then writes their n x n blocks as 16-byte pieces
-            constexpr int STAGE_TILES = RP * NC / 4;
-            constexpr int KB_MAX = (2 * TILE_MAX_L - 1 + 15) / 16;
-            constexpr int nq4 = NC / 4;
-            const int lane = tid & (WAVE - 1);
-            const int i16 = lane & 15, kq = 4 * (lane >> 4);
-            const int items = 3 * L * n_present;
-            T* X = stage;  // [STAGE_TILES][16][16]
-            for (int it = wave_id; it < items; it += TILE_THREADS / WAVE) {
-                const int g = it % n_present, lx = it / n_present;
-                const int l = L - 1 - lx / 3, x = lx % 3;
-                const int kcount = 2 * l + 1;
-                const int t = present[g];
-                const T* Cb = C + (size_t)t * CS + l * l * NC;
-                short4_t bh[KB_MAX], bl[KB_MAX];
-#pragma unroll
-                for (int kb = 0; kb < KB_MAX; kb++) {
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const int mi = 16 * kb + kq + e;
-                        const float bv = (mi < kcount && i16 < NC) ? Cb[mi * NC + i16] : 0.f;
-                        short h, lo;
-                        bf16_split(bv, h, lo);
-                        bh[kb][e] = h;
-                        bl[kb][e] = lo;
-                    }
-                }
-                const T sc_same = scale_l[l], sc_diff = scale_l[l] * (T)1.4142135623730951;
-                for (int r0 = 0; r0 < nc_runs; r0 += STAGE_TILES) {
-                    const int nr = min(STAGE_TILES, nc_runs - r0);
-                    for (int rr = 0; rr < nr; rr++) {
-                        const T* D = DC + ((size_t)(r0 + rr) * 3 + x) * CS + l * l * NC;
-                        float4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                        for (int kb = 0; kb < KB_MAX; kb++) {
-                            if (16 * kb >= kcount) break;
-                            short4_t ah, al;
-#pragma unroll
-                            for (int e = 0; e < 4; e++) {
-                                const int mi = 16 * kb + kq + e;
-                                const float av = (mi < kcount && i16 < NC) ? D[mi * NC + i16] : 0.f;
-                                short h, lo;
-                                bf16_split(av, h, lo);
-                                ah[e] = h;
-                                al[e] = lo;
-                            }
-                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[kb], acc, 0, 0, 0);
-                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[kb], acc, 0, 0, 0);
-                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[kb], acc, 0, 0, 0);
-                        }
-                        // D[row][col]: row = 4 (lane / 16) + v, col = lane % 16
-#pragma unroll
-                        for (int v = 0; v < 4; v++) X[rr * 256 + (kq + v) * 16 + i16] = acc[v];
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-                    const int npieces = nr * NC * nq4;
-                    for (int p = lane; p < npieces; p += WAVE) {
-                        const int rr = p / (NC * nq4), rem = p - rr * (NC * nq4);
-                        const int n1 = rem / nq4, h = rem - n1 * nq4;
-                        const int r = r0 + rr, tj = run_tj[r];
-                        const T* Xb = X + rr * 256;
-                        T v[4];
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            const int n2 = 4 * h + c;
-                            T sum = 0;
-                            if (t >= tj) sum += Xb[n1 * 16 + n2];
-                            if (t <= tj) sum += Xb[n2 * 16 + n1];
-                            v[c] = sum;
-                        }
-                        const T sc = t != tj ? sc_diff : sc_same;
-                        T* dst = out + (long long)run_row[r][g] * 3 * P + (long long)x * P + l * NC * NC + n1 * NC + 4 * h;
-                        store4<T>(dst, v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-                }
-            }
-            lds_barrier();
-            continue;
-        }
-        // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
-        // so that the nq lanes of one (run, type, x, l) tile are adjacent
-        const int NRK = nc_runs * n_present;
-        const int per_lx = NRK * nq;
-        const int total = per_lx * 3 * L;
-        const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
-        const float inv_np = 1.0f / (float)(n_present > 0 ? n_present : 1);
-        for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
-            const int u = ubase + (tid & (WAVE - 1));
-            long long tile = -1;
-            int q = 0;
-            T s = 0;
-            T acc[4][NC];
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-                for (int c = 0; c < NC; c++) acc[rr][c] = 0;
-            if (u < total) {
-                int lx, wq, rk, r, k;
-                small_divmod(u, per_lx, inv_lx, lx, wq);
-                rk = wq / nq;
-                q = wq - rk * nq;
-                small_divmod(rk, n_present, inv_np, r, k);
-                const int lrev = lx / 3;
-                const int x = lx - 3 * lrev;
-                const int l = L - 1 - lrev;
-                const int t = present[k], tj = run_tj[r];
-                const T* D = DC + ((size_t)r * 3 + x) * CS;
-                const T* Ct = C + (size_t)t * CS;
-                if (t >= tj) tile_fma<T, NC, RP>(l, q, D, Ct, acc);
-                if (t <= tj) tile_fma<T, NC, RP>(l, q, Ct, D, acc);
-                s = t != tj ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-                tile = (long long)run_row[r][k] * 3 * P + (long long)x * P + l * n * n;
-            }
-            if (staged) {
-                staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
-            } else if (tile >= 0) {
-                tile_store<T, NC, RP>(out + tile, q, n, s, acc);
-            }
-        }
+        ps_window_rows<T, NC, RP, FAST>(L, n, n_present, present, nc_runs, run_tj, run_row, C, DC, scale_l, stage,
+                                        stage_offs, out);
         lds_barrier();
     }
 
@@ -1746,48 +1827,214 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
         DS[k] = c < n ? dself[(((long long)i * S + present[kt]) * 3 + x) * L2 * n + lm * n + c] : (T)0;
     }
     lds_barrier();
-    const int npp = n_present * (n_present + 1) / 2;
-    const int per_lx = npp * nq;
-    const int total = per_lx * 3 * L;
-    const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
-    for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
-        const int u = ubase + (tid & (WAVE - 1));
-        long long tile = -1;
-        int q = 0;
-        T s = 0;
-        T acc[4][NC];
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) acc[rr][c] = 0;
-        if (u < total) {
-            int lx, wq;
-            small_divmod(u, per_lx, inv_lx, lx, wq);
-            const int p = wq / nq;
-            q = wq - p * nq;
-            const int lrev = lx / 3;
-            const int x = lx - 3 * lrev;
-            const int l = L - 1 - lrev;
-            // p -> (k1 <= k2) in row-major order of the upper triangle
-            int k1 = 0, rem = p;
-            while (rem >= n_present - k1) {
-                rem -= n_present - k1;
-                k1++;
-            }
-            const int k2 = k1 + rem;
-            const int t1 = present[k1], t2 = present[k2];
-            const int local = pair_index(t1, t2, S);
-            const long long row = self_rows[(long long)i * npairs + local];
-            tile_fma<T, NC, RP>(l, q, DS + ((size_t)k1 * 3 + x) * CS, C + (size_t)t2 * CS, acc);
-            tile_fma<T, NC, RP>(l, q, C + (size_t)t1 * CS, DS + ((size_t)k2 * 3 + x) * CS, acc);
-            s = t1 != t2 ? scale_l[l] * (T)1.4142135623730951 : scale_l[l];
-            tile = row * 3 * P + (long long)x * P + l * n * n;
+    ps_self_rows<T, NC, RP>(i, L, n, S, n_present, present, self_rows, C, DS, scale_l, stage, stage_offs, out);
+}
+
+// ---- pipelined schedule of the same gradient rows
+//
+// A fifth wave does every global load of the window loop: while the four
+// computing waves write the rows of window w, it fetches window w + 1 (runs, their
+// rows, the entry records) into the other half of a double buffer. The computing waves thus never load from global memory
+// after their first store, so no load of theirs waits behind the stores still
+// draining (loads and stores retire through one in-order counter), and the
+// fetch latency overlaps the contraction. The center's own rows go first, so that
+// their inputs are read before any row is stored.
+constexpr int PIPE_THREADS = TILE_THREADS + WAVE;
+constexpr int PIPE_MAX_WINDOWS = 64;                              // window descriptors kept in LDS
+constexpr int PIPE_ROWS_PER_LANE = TILE_MAX_RUNS * TILE_MAX_TYPES / WAVE;
+constexpr int SH_TABLE_MAX = TILE_MAX_L * (TILE_MAX_L + 1) + 4 * TILE_MAX_L;  // ShTable(TILE_MAX_L).size
+
+// run table of one window in LDS (ranges relative to the window's first entry)
+struct PipeRuns {
+    int lo[TILE_MAX_RUNS], hi[TILE_MAX_RUNS], tj[TILE_MAX_RUNS], complete[TILE_MAX_RUNS];
+    int row[TILE_MAX_RUNS][TILE_MAX_TYPES];
+    int carry, n_runs;
+};
+
+// elements of one half of the double buffer: the window's entry records, overlaid by
+// the computing waves' store staging while rows are written
+template <typename T>
+__host__ __device__ inline size_t pipe_buffer_elems(int per_entry, int max_entries, int rows_per_pass, int NC) {
+    const size_t entries = (size_t)per_entry * max_entries;
+    const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC + (size_t)TILE_THREADS * sizeof(long long) / sizeof(T);
+    const size_t e = entries > staging ? entries : staging;
+    return (e + 16 / sizeof(T) - 1) / (16 / sizeof(T)) * (16 / sizeof(T));
+}
+
+// index of the k-th set bit of mask
+__device__ __forceinline__ int nth_set_bit(unsigned mask, int k) {
+    for (int t = 0; t < 32; t++) {
+        if ((mask >> t) & 1u) {
+            if (k == 0) return t;
+            k--;
         }
-        if (staged) {
-            staged_tile_store<T, NC, RP>(out, tile, q, s, acc, stage, stage_offs);
-        } else if (tile >= 0) {
-            tile_store<T, NC, RP>(out + tile, q, n, s, acc);
+    }
+    return -1;
+}
+
+// The loading wave's fetch of window w into one half of the double buffer: the entry
+// records go straight from global memory to LDS (global_load_lds, 16 bytes per lane,
+// no registers), the run table through registers. Ends once everything has landed.
+template <typename T>
+__device__ __forceinline__ void pipe_load(int w, long long e0, int S, int n_present, int per_entry, const int4* win_lds,
+                                          const int4* __restrict__ windows, const int4* __restrict__ runs,
+                                          const int* __restrict__ run_rows, const T* __restrict__ entry_data,
+                                          PipeRuns& b, T* ent, int lane) {
+    int4 wd;
+    if (w < PIPE_MAX_WINDOWS) {
+        wd = win_lds[w];
+    } else {
+        wd = windows[e0 + w];
+    }
+    const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
+    const int count = (int)((size_t)ec * per_entry * sizeof(T) / 16);
+    const unsigned char* src = reinterpret_cast<const unsigned char*>(entry_data + (e0 + es) * per_entry);
+    unsigned char* dst = reinterpret_cast<unsigned char*>(ent);
+    for (int k0 = 0; k0 < count; k0 += WAVE) {
+        if (k0 + lane < count) __builtin_amdgcn_global_load_lds(src + 16 * (size_t)(k0 + lane), dst + 16 * k0, 16, 0, 0);
+    }
+    int4 ri = make_int4(0, 0, 0, 0);
+    int rows[PIPE_ROWS_PER_LANE];
+    if (lane < rc) ri = runs[e0 + rs + lane];
+#pragma unroll
+    for (int k = 0; k < PIPE_ROWS_PER_LANE; k++) {
+        const int idx = lane + WAVE * k;
+        if (idx < rc * n_present) {
+            const int r = idx / n_present, kk = idx - r * n_present;
+            rows[k] = run_rows[(e0 + rs + r) * S + kk];
         }
+    }
+    if (lane < rc) {
+        b.lo[lane] = max(ri.x, es) - es;
+        b.hi[lane] = min(ri.x + ri.y, es + ec) - es;
+        b.tj[lane] = ri.z;
+        b.complete[lane] = ri.x + ri.y <= es + ec ? 1 : 0;
+        if (lane == 0) b.carry = ri.x < es ? 1 : 0;
+    }
+    if (lane == 0) b.n_runs = rc;
+#pragma unroll
+    for (int k = 0; k < PIPE_ROWS_PER_LANE; k++) {
+        const int idx = lane + WAVE * k;
+        if (idx < rc * n_present) {
+            const int r = idx / n_present, kk = idx - r * n_present;
+            b.row[r][kk] = rows[k];
+        }
+    }
+    // the records written by the DMA are counted by this wave's vector-memory counter
+    // only: wait for them before the barrier that hands the half over
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
+template <typename T, int NC, int RP, bool FAST>
+__global__ void __launch_bounds__(PIPE_THREADS) ps_gradient_pipe_kernel(
+    SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
+    const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
+    const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
+    const T* __restrict__ entry_data, T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ PipeRuns bufs[2];
+    __shared__ int4 win_lds[PIPE_MAX_WINDOWS];
+    __shared__ int present[TILE_MAX_TYPES];
+    __shared__ T scale_l[TILE_MAX_L];
+    __shared__ T shc[SH_TABLE_MAX];
+    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
+    __shared__ int self_row_lds[TILE_MAX_TYPES * (TILE_MAX_TYPES + 1) / 2];
+
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave_id = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const bool loader = wave_id == TILE_THREADS / WAVE;
+    const int S = a.n_species, L = a.L, n = a.n;
+    const int L2 = L * L;
+    const int CS = L2 * NC;
+    const TileEntryLayout lay(L, n, NC);
+    T* C = reinterpret_cast<T*>(smem);
+    T* region = C + (size_t)S * CS;
+    const size_t dc_size = (size_t)(max_runs > S ? max_runs : S) * 3 * CS;
+    const size_t half = pipe_buffer_elems<T>(lay.per_entry, max_entries, RP, NC);
+    T* const ent0 = region + dc_size;  // halves ent0 + h * half, h = 0, 1
+
+    const long long e0 = a.offsets[i];
+    const unsigned mask = type_mask[i];
+    const int n_present = __popc(mask);
+    const int nw = n_windows[i];
+
+    // tables, coefficients, window descriptors and the self gradients, all loaded
+    // before the first store
+    for (int k = tid; k < S * CS; k += PIPE_THREADS) {
+        const int t = k / CS;
+        const int r = k - t * CS;
+        const int lm = r / NC, c = r - lm * NC;
+        C[k] = c < n ? coef[((long long)i * S + t) * L2 * n + lm * n + c] : (T)0;
+    }
+    if (tid < L) scale_l[tid] = (T)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
+    for (int k = tid; k < lay.leg; k += PIPE_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        leg_l[k] = l;
+        leg_m[k] = k - l * (l + 1) / 2;
+    }
+    {
+        const ShTable tb(L);
+        for (int k = tid; k < tb.size; k += PIPE_THREADS) shc[k] = a.sh_c[k];
+    }
+    if (tid == 0) {
+        int k = 0;
+        for (int t = 0; t < S; t++)
+            if ((mask >> t) & 1u) present[k++] = t;
+    }
+    for (int k = tid; k < min(nw, PIPE_MAX_WINDOWS); k += PIPE_THREADS) win_lds[k] = windows[e0 + k];
+    // the center's own rows per local block (entries of absent types are never read)
+    for (int k = tid; k < S * (S + 1) / 2; k += PIPE_THREADS) self_row_lds[k] = self_rows[(long long)i * (S * (S + 1) / 2) + k];
+    T* DS = region;
+    for (int k = tid; k < n_present * 3 * CS; k += PIPE_THREADS) {
+        const int kt = k / (3 * CS);
+        const int r = k - kt * 3 * CS;
+        const int x = r / CS;
+        const int r2 = r - x * CS;
+        const int lm = r2 / NC, c = r2 - lm * NC;
+        DS[k] = c < n ? dself[(((long long)i * S + nth_set_bit(mask, kt)) * 3 + x) * L2 * n + lm * n + c] : (T)0;
+    }
+    lds_barrier();
+
+    if (loader) {
+        // window 0 into the first half while the center's own rows are written
+        if (nw > 0)
+            pipe_load<T>(0, e0, S, n_present, lay.per_entry, win_lds, windows, runs, run_rows, entry_data, bufs[0], ent0,
+                         lane);
+    } else {
+        T* const e1 = ent0 + half;
+        ps_self_rows<T, NC, RP>(0, L, n, S, n_present, present, self_row_lds, C, DS, scale_l,
+                                e1 + (size_t)wave_id * WAVE * RP * NC,
+                                reinterpret_cast<long long*>(e1 + (size_t)TILE_THREADS * RP * NC) + wave_id * WAVE, out);
+    }
+    lds_barrier();
+
+    T* DC = region;
+    for (int w = 0; w < nw; w++) {
+        const int h = w & 1;
+        T* const cur = ent0 + (size_t)h * half;
+        if (!loader) {
+            plan_entry_stage2<T, NC>(shc, lay, bufs[h].n_runs, bufs[h].lo, bufs[h].hi, bufs[h].carry != 0, leg_l, leg_m,
+                                     cur, DC);
+        }
+        lds_barrier();
+        if (loader) {
+            // window w + 1 into the other half while the rows of window w are written
+            if (w + 1 < nw)
+                pipe_load<T>(w + 1, e0, S, n_present, lay.per_entry, win_lds, windows, runs, run_rows, entry_data,
+                             bufs[h ^ 1], ent0 + (size_t)(h ^ 1) * half, lane);
+        } else {
+            const int rc = bufs[h].n_runs;
+            const int nc_runs = bufs[h].complete[rc - 1] ? rc : rc - 1;
+            ps_window_rows<T, NC, RP, FAST>(L, n, n_present, present, nc_runs, bufs[h].tj, bufs[h].row, C, DC, scale_l,
+                                            cur + (size_t)wave_id * WAVE * RP * NC,
+                                            reinterpret_cast<long long*>(cur + (size_t)TILE_THREADS * RP * NC) + wave_id * WAVE,
+                                            out);
+        }
+        lds_barrier();
     }
 }
 
@@ -1870,7 +2117,7 @@ __global__ void __launch_bounds__(TILE_THREADS) se_gradient_tiled_kernel(
             for (int k = tid; k < count; k += TILE_THREADS) dst[k] = src[k];
         }
         lds_barrier();
-        plan_entry_stage2<T, NC>(a, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
+        plan_entry_stage2<T, NC>(a.sh_c, lay, rc, run_lo, run_hi, carry_flag != 0, leg_l, leg_m, ent, DC);
         lds_barrier();
 
         const int nc_runs = run_complete[rc - 1] ? rc : rc - 1;
@@ -2725,14 +2972,44 @@ bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, c
 }
 
 // `rows_per_pass`: rows of each 4 x NC tile staged at once (1, 2 or 4)
+// FP32 uses the pipelined schedule (ps_gradient_pipe_kernel) when
+// RASCALINE_AMD_PIPELINE=1 (measured slower on the benchmark workload: 161 against
+// 137 ms per step, three workgroups per CU instead of four); FP64, whose double
+// buffer would leave one workgroup per CU, keeps the single-buffered kernel
+bool ps_gradient_pipelined(size_t value_size) {
+    static const bool on = [] {
+        const char* v = getenv("RASCALINE_AMD_PIPELINE");
+        return v && v[0] == '1';
+    }();
+    return on && value_size == 4;
+}
+
 template <typename T>
 size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass) {
     const int NC = (p.n + 3) / 4 * 4;
     const size_t cs = (size_t)p.L * p.L * NC;
     const size_t dc = (size_t)(max_runs > p.n_species ? max_runs : p.n_species) * 3 * cs;
-    const size_t entries = (size_t)TileEntryLayout(p.L, p.n, NC).per_entry * max_entries * sizeof(T);
+    const int per_entry = TileEntryLayout(p.L, p.n, NC).per_entry;
+    if (ps_gradient_pipelined(sizeof(T))) {
+        return ((size_t)p.n_species * cs + dc + 2 * pipe_buffer_elems<T>(per_entry, max_entries, rows_per_pass, NC)) *
+               sizeof(T);
+    }
+    const size_t entries = (size_t)per_entry * max_entries * sizeof(T);
     const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC * sizeof(T) + (size_t)TILE_THREADS * sizeof(void*);
     return ((size_t)p.n_species * cs + dc) * sizeof(T) + (entries > staging ? entries : staging);
+}
+
+// static LDS of the gradient-row kernel in use (bounds the padding the compiler may add)
+template <typename T>
+size_t ps_gradient_tiled_lds(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass) {
+    const size_t fixed = ps_gradient_pipelined(sizeof(T))
+                             ? 2 * sizeof(PipeRuns) + PIPE_MAX_WINDOWS * sizeof(int4) + TILE_MAX_TYPES * sizeof(int) +
+                                   TILE_MAX_L * sizeof(T) + SH_TABLE_MAX * sizeof(T) +
+                                   TILE_MAX_L * (TILE_MAX_L + 1) * sizeof(int) +
+                                   TILE_MAX_TYPES * (TILE_MAX_TYPES + 1) / 2 * sizeof(int)
+                             : TILE_MAX_RUNS * (4 + TILE_MAX_TYPES) * sizeof(int) + (TILE_MAX_TYPES + 1) * sizeof(int) +
+                                   TILE_MAX_L * sizeof(T) + TILE_MAX_L * (TILE_MAX_L + 1) * sizeof(int);
+    return ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass) + fixed + 256;
 }
 
 bool ps_gradient_tiled_supported(int L, int n, int n_species, int max_runs, int max_entries, int rows_per_pass) {
@@ -2792,8 +3069,13 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     // n = 12 and n = 16
     const bool mixed = fast && sizeof(T) == 4 && ((p.n == 8 && max_runs <= 4) || p.n == 12 || p.n == 16);
 #define RA_LAUNCH_TILED(NCV, RPV, FV)                                                                                  \
-    ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                          \
-        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
+    if (ps_gradient_pipelined(sizeof(T))) {                                                                            \
+        ps_gradient_pipe_kernel<T, NCV, RPV, FV><<<p.n_atoms, PIPE_THREADS, shmem, p.stream>>>(                       \
+            a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out); \
+    } else {                                                                                                           \
+        ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                      \
+            a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out); \
+    }
 #define RA_LAUNCH_TILED_RP(NCV)       \
     if (rows_per_pass == 4) {         \
         RA_LAUNCH_TILED(NCV, 4, false);      \
@@ -3072,6 +3354,7 @@ void launch_by_pair(const SoapLaunch<T>& p, long long n_samples, const ByPairSam
                                              const long long*, const long long*, const long long*, const int*,        \
                                              const int*, T*, int*);                                                    \
     template size_t ps_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int, int);                                   \
+    template size_t ps_gradient_tiled_lds<T>(const SoapLaunch<T>&, int, int, int);                                     \
     template void launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, int, const T*, const T*, const unsigned*, \
                                               const int*, const int*, const int*, const int*, const int*, const T*, T*, \
                                               bool);                                                                   \

@@ -192,3 +192,32 @@ def block_relative_error(actual, expected, param=None):
         if scale > 0:
             worst = max(worst, float(np.abs(a - b).max()) / scale)
     return worst
+
+
+def approx_floor_needed(actual, expected, rel=1e-4, scale=None):
+    """Smallest eps_frac for which every element of a block passes the reference tests'
+    approx semantics, |a - b| <= eps or |a - b| <= rel * max(|a|, |b|), with the absolute
+    floor eps = eps_frac * max|expected| of the block (or eps_frac * scale when the arrays
+    are a part of a block whose largest element is `scale`)."""
+    a = np.asarray(actual, dtype=np.float64)
+    b = np.asarray(expected, dtype=np.float64)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    if b.size == 0:
+        return 0.0
+    if scale is None:
+        scale = float(np.abs(b).max())
+    diff = np.abs(a - b)
+    fail = diff > rel * np.maximum(np.abs(a), np.abs(b))
+    if not np.any(fail):
+        return 0.0
+    if scale == 0.0:
+        return float("inf")
+    return float(diff[fail].max() / scale)
+
+
+def assert_block_approx(actual, expected, rel, eps_frac, what="", scale=None):
+    """Element-wise bar: |a - b| <= max(eps_frac * max|expected|, rel * max(|a|, |b|)) over one
+    block's array; returns the floor the block needed (see approx_floor_needed)."""
+    needed = approx_floor_needed(actual, expected, rel, scale)
+    assert needed <= eps_frac, f"{what}: needs eps {needed:.2e} x max|ref| (bar {eps_frac:.0e}) at rel {rel:.0e}"
+    return needed

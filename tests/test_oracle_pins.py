@@ -235,3 +235,29 @@ def test_willatt2018_large_rate_is_unscaled():
     for a, b in zip(plain["blocks"], scaled["blocks"]):
         # s = 1 - O(r^3 / rate): 1e-11 at these distances, below FP64 cancellation in small elements
         assert_relative_eq(b["values"], a["values"], 1e-10, 1e-12 * np.abs(a["values"]).max())
+
+
+def test_lode_center_subset_is_the_full_computation():
+    # orc_lode_system_centers (gradient rows of chosen centers, used for large cells) gives the
+    # same rows as the full computation
+    rng = np.random.default_rng(5)
+    from helpers import random_system
+
+    pos, types, cell = random_system(rng, 9, "triclinic", 5.5)
+    system = orc.System(pos, types, cell)
+    hypers = lode_hypers(1, 1.0, 3.0, 3, 2)
+    species = np.unique(types)
+    values, grads, self_grads = orc.lode_dense(system, hypers, species, centers=range(len(types)))
+    sub_values, sub_grads, sub_self = orc.lode_dense(system, hypers, species, centers=[4, 1])
+    assert np.array_equal(values, sub_values)
+    assert np.array_equal(grads[[4, 1]], sub_grads)
+    assert np.array_equal(self_grads[[4, 1]], sub_self)
+    full = orc.lode_spherical_expansion([system], hypers, ["positions"])
+    # block (center type, neighbour type) l = 0 row of center 4 with neighbour j = 0 (if of that type)
+    t4, t0 = types[4], types[0]
+    b = [tuple(k) for k in full["keys"]].index((0, 1, t4, t0))
+    g = full["blocks"][b]["gradients"]["positions"]
+    samples = [tuple(s) for s in full["blocks"][b]["samples"]]
+    s4 = samples.index((0, 4))
+    row = [tuple(r) for r in g["samples"]].index((s4, 0, 0))
+    assert np.allclose(g["values"][row][:, 0, :], sub_grads[0, 0, :, 0, :], rtol=0, atol=0)
