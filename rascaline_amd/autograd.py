@@ -1,19 +1,30 @@
 """Position gradients through torch's autograd, the role featomic-torch plays for
-the reference (featomic-torch/src/autograd.cpp:135-230 for the backward, 237-300
-for the contraction with dA/dX): the forward returns the calculator's values on the
-GPU; the backward is the vector-Jacobian product of the position gradients
-(`featomic_amd_positions_vjp`), computed without the gradient rows.
+the reference (featomic-torch/src/autograd.cpp:135-230 for the backward, 237-420
+for the contraction with dA/dX and its own backward): the forward returns the
+calculator's values on the GPU; the backward is the vector-Jacobian product of the
+position gradients (`featomic_amd_positions_vjp`), computed without the gradient
+rows; and that backward is itself differentiable with respect to dA/dX, as
+featomic-torch's PositionsGrad::backward is (autograd.cpp:320-420), so that a loss
+on forces (create_graph=True) reaches the weights applied to the descriptor.
 
     positions = torch.tensor(..., dtype=torch.float64, device="cuda", requires_grad=True)
     values, layout = soap_values(calculator, systems, positions)   # flat values, block layout
-    loss = (values * weights).sum()
-    loss.backward()                                                 # positions.grad [atoms, 3]
+    energy = (values * weights).sum()
+    forces = -torch.autograd.grad(energy, positions, create_graph=True)[0]
+    loss = ((forces - reference) ** 2).sum()
+    loss.backward()                                                 # weights.grad
 
 `systems` is a list of (positions, types, cell) tuples giving the types and cells
 (their positions are ignored); `positions` holds every atom's position, systems in
 order. The values keep the calculator's precision (float32 or float64).
+
+As in featomic-torch, second derivatives with respect to the positions are not
+computed: the double backward contributes nothing to positions.grad and warns once.
 """
 
+import warnings
+
+import numpy as np
 import torch
 
 from .device import DeviceSystems, compute_device, positions_vjp
@@ -27,6 +38,7 @@ class BlockLayout:
         self.n_samples = list(result.n_samples)
         self.properties = list(result.properties)
         self.offsets = [result.value_offset(b) for b in range(result.n_blocks)]
+        self.value_size = result.value_size
 
     def block(self, values, b):
         """Block b of the flat values as [samples, properties] (a view)."""
@@ -34,9 +46,94 @@ class BlockLayout:
         return values[off:off + ns * npr].view(ns, npr)
 
 
+class _Holder:
+    layout = None
+
+
+def _lode_row_labels(systems, key):
+    """(sample, system, atom) of the position-gradient rows of one LODE block, in the
+    reference's order (labels/samples/long_range.rs:55-77): for every system holding the
+    neighbour type, every center of the center type, the atoms of the neighbour type and
+    the center itself."""
+    _, _, ct, nt = (int(k) for k in key)
+    rows = []
+    sample = 0
+    for s, (_, types, _) in enumerate(systems):
+        types = np.asarray(types)
+        if not np.any(types == nt):
+            continue
+        for i in np.nonzero(types == ct)[0]:
+            atoms = np.nonzero((types == nt) | (np.arange(len(types)) == i))[0]
+            rows.append(np.stack([np.full(len(atoms), sample), np.full(len(atoms), s), atoms], axis=1))
+            sample += 1
+    return np.concatenate(rows) if rows else np.zeros((0, 3), dtype=np.int64)
+
+
+def positions_jvp(calculator, systems, device_systems, tangent):
+    """dX/dr · v: for every sample, Σ over its position-gradient rows of (dX/dr)[row] ·
+    tangent[atom of row], in the layout of the flat values (the product featomic-torch's
+    PositionsGrad::backward forms, featomic-torch/src/autograd.cpp:380-410, from the
+    calculator's own gradient rows, here on the device)."""
+    result = compute_device(calculator, device_systems, ["positions"])
+    dtype = torch.float32 if result.precision == 32 else torch.float64
+    out = torch.zeros(result.value_size, dtype=dtype, device=tangent.device)
+    starts = torch.from_numpy(device_systems.offsets[:-1].copy()).to(tangent.device)
+    tangent = tangent.to(dtype)
+    for b in range(result.n_blocks):
+        g = result.gradients(b)
+        if g.shape[0] == 0:
+            continue
+        labels = result.gradient_labels(b)
+        if labels is None:
+            labels = torch.from_numpy(_lode_row_labels(systems, result.keys[b])).to(tangent.device)
+        labels = labels.long()
+        atoms = starts[labels[:, 1]] + labels[:, 2]
+        t = tangent[atoms]
+        contribution = g[:, 0] * t[:, 0:1] + g[:, 1] * t[:, 1:2] + g[:, 2] * t[:, 2:3]
+        # rows are sorted by sample: per-sample sums as differences of an FP64 running
+        # sum at the segment ends (a fixed order, so repeated calls agree bit for bit)
+        counts = torch.bincount(labels[:, 0], minlength=result.n_samples[b])
+        ends = counts.cumsum(0)
+        running = torch.cat([torch.zeros((1, contribution.shape[1]), dtype=torch.float64, device=tangent.device),
+                             contribution.double().cumsum(0)])
+        block = (running[ends] - running[ends - counts]).to(dtype)
+        off = result.value_offset(b)
+        out[off:off + block.numel()] = block.reshape(-1)
+    return out
+
+
+_WARNED = [False]
+
+
+class _PositionsGrad(torch.autograd.Function):
+    """dA/dr from dA/dX (featomic-torch's PositionsGrad); its backward gives dB/d(dA/dX)."""
+
+    @staticmethod
+    def forward(ctx, positions, grad_values, calculator, systems, device_systems):
+        ctx.save_for_backward(positions, grad_values)
+        ctx.calculator = calculator
+        ctx.systems = systems
+        ctx.device_systems = device_systems
+        grad = positions_vjp(calculator, device_systems, grad_values.contiguous())
+        return grad.to(torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_grad):
+        positions, grad_values = ctx.saved_tensors
+        if ctx.needs_input_grad[0] and not _WARNED[0]:
+            _WARNED[0] = True
+            warnings.warn("second derivatives with respect to positions are not implemented and are not "
+                          "accumulated during backward() calls", stacklevel=2)
+        d_grad_values = None
+        if ctx.needs_input_grad[1]:
+            d_grad_values = positions_jvp(ctx.calculator, ctx.systems, ctx.device_systems, grad_grad.contiguous())
+            d_grad_values = d_grad_values.to(grad_values.dtype)
+        return None, d_grad_values, None, None, None
+
+
 class _SoapValues(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, positions, calculator, systems):
+    def forward(ctx, positions, calculator, systems, holder):
         ds = DeviceSystems(systems, positions=positions.detach().contiguous())
         result = compute_device(calculator, ds)
         values = torch.empty(result.value_size, dtype=torch.float32 if result.precision == 32 else torch.float64,
@@ -45,28 +142,25 @@ class _SoapValues(torch.autograd.Function):
             off = result.value_offset(b)
             v = result.values(b)
             values[off:off + v.numel()] = v.reshape(-1)
+        holder.layout = BlockLayout(result)
         # saved for torch's in-place modification check; the backward reads the same storage
         ctx.save_for_backward(positions)
         ctx.calculator = calculator
-        ctx.systems = ds
-        ctx.layout = BlockLayout(result)
+        ctx.systems = systems
+        ctx.device_systems = ds
         return values
 
     @staticmethod
     def backward(ctx, grad_values):
-        ctx.saved_tensors  # raises if the positions were modified in place since the forward
-        grad = positions_vjp(ctx.calculator, ctx.systems, grad_values.contiguous())
-        return grad.to(torch.float64), None, None
+        (positions,) = ctx.saved_tensors  # raises if the positions were modified in place since the forward
+        grad = _PositionsGrad.apply(positions, grad_values, ctx.calculator, ctx.systems, ctx.device_systems)
+        return grad, None, None, None
 
 
 def soap_values(calculator, systems, positions):
     """Values of a calculator (SOAP spherical expansion or power spectrum, LODE
     spherical expansion) as one flat tensor plus its BlockLayout; differentiable with
-    respect to `positions`."""
-    values = _SoapValues.apply(positions, calculator, systems)
-    # the layout of this computation (the same batch gives the same layout)
-    layout = values.grad_fn.layout if values.grad_fn is not None else None
-    if layout is None:
-        ds = DeviceSystems(systems, positions=positions.detach().contiguous())
-        layout = BlockLayout(compute_device(calculator, ds))
-    return values, layout
+    respect to `positions`, twice with respect to what multiplies the values."""
+    holder = _Holder()
+    values = _SoapValues.apply(positions, calculator, systems, holder)
+    return values, holder.layout

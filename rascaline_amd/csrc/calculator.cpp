@@ -693,9 +693,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             if (tune.gradient_window >= 0 && tune.gradient_window < static_cast<int>(soap_window_shapes().size())) {
                 windows.push_back({soap_window_shapes()[tune.gradient_window], whole});
             } else {
-                // the pipelined power-spectrum kernel (five waves) fits three workgroups per CU in FP32
-                const size_t fp32_cap = ps && ps_gradient_pipelined(sizeof(T)) ? 54016 : RA_TILED_LDS_BUDGET;
-                const size_t cap = std::min<size_t>(whole, sizeof(T) == 4 ? fp32_cap : 81920);
+                const size_t cap = std::min<size_t>(whole, sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920);
                 for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, cap});
                 for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, whole});
             }

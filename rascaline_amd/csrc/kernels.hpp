@@ -125,9 +125,6 @@ size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_ent
 // dynamic plus static LDS per workgroup of the gradient-row kernel in use
 template <typename T>
 size_t ps_gradient_tiled_lds(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass);
-// whether the power-spectrum gradient rows of values of this size (bytes) use the
-// pipelined schedule (a loading wave)
-bool ps_gradient_pipelined(size_t value_size);
 bool ps_gradient_tiled_supported(int L, int n, int n_species, int max_runs, int max_entries, int rows_per_pass);
 // Planning pass of the tiled gradient kernel; runs / windows are int4 arrays with
 // one slot per neighbour entry, run_rows [entries][n_species], self_rows

@@ -1830,214 +1830,6 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     ps_self_rows<T, NC, RP>(i, L, n, S, n_present, present, self_rows, C, DS, scale_l, stage, stage_offs, out);
 }
 
-// ---- pipelined schedule of the same gradient rows
-//
-// A fifth wave does every global load of the window loop: while the four
-// computing waves write the rows of window w, it fetches window w + 1 (runs, their
-// rows, the entry records) into the other half of a double buffer. The computing waves thus never load from global memory
-// after their first store, so no load of theirs waits behind the stores still
-// draining (loads and stores retire through one in-order counter), and the
-// fetch latency overlaps the contraction. The center's own rows go first, so that
-// their inputs are read before any row is stored.
-constexpr int PIPE_THREADS = TILE_THREADS + WAVE;
-constexpr int PIPE_MAX_WINDOWS = 64;                              // window descriptors kept in LDS
-constexpr int PIPE_ROWS_PER_LANE = TILE_MAX_RUNS * TILE_MAX_TYPES / WAVE;
-constexpr int SH_TABLE_MAX = TILE_MAX_L * (TILE_MAX_L + 1) + 4 * TILE_MAX_L;  // ShTable(TILE_MAX_L).size
-
-// run table of one window in LDS (ranges relative to the window's first entry)
-struct PipeRuns {
-    int lo[TILE_MAX_RUNS], hi[TILE_MAX_RUNS], tj[TILE_MAX_RUNS], complete[TILE_MAX_RUNS];
-    int row[TILE_MAX_RUNS][TILE_MAX_TYPES];
-    int carry, n_runs;
-};
-
-// elements of one half of the double buffer: the window's entry records, overlaid by
-// the computing waves' store staging while rows are written
-template <typename T>
-__host__ __device__ inline size_t pipe_buffer_elems(int per_entry, int max_entries, int rows_per_pass, int NC) {
-    const size_t entries = (size_t)per_entry * max_entries;
-    const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC + (size_t)TILE_THREADS * sizeof(long long) / sizeof(T);
-    const size_t e = entries > staging ? entries : staging;
-    return (e + 16 / sizeof(T) - 1) / (16 / sizeof(T)) * (16 / sizeof(T));
-}
-
-// index of the k-th set bit of mask
-__device__ __forceinline__ int nth_set_bit(unsigned mask, int k) {
-    for (int t = 0; t < 32; t++) {
-        if ((mask >> t) & 1u) {
-            if (k == 0) return t;
-            k--;
-        }
-    }
-    return -1;
-}
-
-// The loading wave's fetch of window w into one half of the double buffer: the entry
-// records go straight from global memory to LDS (global_load_lds, 16 bytes per lane,
-// no registers), the run table through registers. Ends once everything has landed.
-template <typename T>
-__device__ __forceinline__ void pipe_load(int w, long long e0, int S, int n_present, int per_entry, const int4* win_lds,
-                                          const int4* __restrict__ windows, const int4* __restrict__ runs,
-                                          const int* __restrict__ run_rows, const T* __restrict__ entry_data,
-                                          PipeRuns& b, T* ent, int lane) {
-    int4 wd;
-    if (w < PIPE_MAX_WINDOWS) {
-        wd = win_lds[w];
-    } else {
-        wd = windows[e0 + w];
-    }
-    const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
-    const int count = (int)((size_t)ec * per_entry * sizeof(T) / 16);
-    const unsigned char* src = reinterpret_cast<const unsigned char*>(entry_data + (e0 + es) * per_entry);
-    unsigned char* dst = reinterpret_cast<unsigned char*>(ent);
-    for (int k0 = 0; k0 < count; k0 += WAVE) {
-        if (k0 + lane < count) __builtin_amdgcn_global_load_lds(src + 16 * (size_t)(k0 + lane), dst + 16 * k0, 16, 0, 0);
-    }
-    int4 ri = make_int4(0, 0, 0, 0);
-    int rows[PIPE_ROWS_PER_LANE];
-    if (lane < rc) ri = runs[e0 + rs + lane];
-#pragma unroll
-    for (int k = 0; k < PIPE_ROWS_PER_LANE; k++) {
-        const int idx = lane + WAVE * k;
-        if (idx < rc * n_present) {
-            const int r = idx / n_present, kk = idx - r * n_present;
-            rows[k] = run_rows[(e0 + rs + r) * S + kk];
-        }
-    }
-    if (lane < rc) {
-        b.lo[lane] = max(ri.x, es) - es;
-        b.hi[lane] = min(ri.x + ri.y, es + ec) - es;
-        b.tj[lane] = ri.z;
-        b.complete[lane] = ri.x + ri.y <= es + ec ? 1 : 0;
-        if (lane == 0) b.carry = ri.x < es ? 1 : 0;
-    }
-    if (lane == 0) b.n_runs = rc;
-#pragma unroll
-    for (int k = 0; k < PIPE_ROWS_PER_LANE; k++) {
-        const int idx = lane + WAVE * k;
-        if (idx < rc * n_present) {
-            const int r = idx / n_present, kk = idx - r * n_present;
-            b.row[r][kk] = rows[k];
-        }
-    }
-    // the records written by the DMA are counted by this wave's vector-memory counter
-    // only: wait for them before the barrier that hands the half over
-    __builtin_amdgcn_s_waitcnt(0);
-}
-
-template <typename T, int NC, int RP, bool FAST>
-__global__ void __launch_bounds__(PIPE_THREADS) ps_gradient_pipe_kernel(
-    SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
-    const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
-    const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
-    const T* __restrict__ entry_data, T* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ PipeRuns bufs[2];
-    __shared__ int4 win_lds[PIPE_MAX_WINDOWS];
-    __shared__ int present[TILE_MAX_TYPES];
-    __shared__ T scale_l[TILE_MAX_L];
-    __shared__ T shc[SH_TABLE_MAX];
-    __shared__ int leg_l[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
-    __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
-    __shared__ int self_row_lds[TILE_MAX_TYPES * (TILE_MAX_TYPES + 1) / 2];
-
-    const int i = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int lane = tid & (WAVE - 1);
-    const int wave_id = __builtin_amdgcn_readfirstlane(tid / WAVE);
-    const bool loader = wave_id == TILE_THREADS / WAVE;
-    const int S = a.n_species, L = a.L, n = a.n;
-    const int L2 = L * L;
-    const int CS = L2 * NC;
-    const TileEntryLayout lay(L, n, NC);
-    T* C = reinterpret_cast<T*>(smem);
-    T* region = C + (size_t)S * CS;
-    const size_t dc_size = (size_t)(max_runs > S ? max_runs : S) * 3 * CS;
-    const size_t half = pipe_buffer_elems<T>(lay.per_entry, max_entries, RP, NC);
-    T* const ent0 = region + dc_size;  // halves ent0 + h * half, h = 0, 1
-
-    const long long e0 = a.offsets[i];
-    const unsigned mask = type_mask[i];
-    const int n_present = __popc(mask);
-    const int nw = n_windows[i];
-
-    // tables, coefficients, window descriptors and the self gradients, all loaded
-    // before the first store
-    for (int k = tid; k < S * CS; k += PIPE_THREADS) {
-        const int t = k / CS;
-        const int r = k - t * CS;
-        const int lm = r / NC, c = r - lm * NC;
-        C[k] = c < n ? coef[((long long)i * S + t) * L2 * n + lm * n + c] : (T)0;
-    }
-    if (tid < L) scale_l[tid] = (T)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
-    for (int k = tid; k < lay.leg; k += PIPE_THREADS) {
-        int l = 0;
-        while ((l + 1) * (l + 2) / 2 <= k) l++;
-        leg_l[k] = l;
-        leg_m[k] = k - l * (l + 1) / 2;
-    }
-    {
-        const ShTable tb(L);
-        for (int k = tid; k < tb.size; k += PIPE_THREADS) shc[k] = a.sh_c[k];
-    }
-    if (tid == 0) {
-        int k = 0;
-        for (int t = 0; t < S; t++)
-            if ((mask >> t) & 1u) present[k++] = t;
-    }
-    for (int k = tid; k < min(nw, PIPE_MAX_WINDOWS); k += PIPE_THREADS) win_lds[k] = windows[e0 + k];
-    // the center's own rows per local block (entries of absent types are never read)
-    for (int k = tid; k < S * (S + 1) / 2; k += PIPE_THREADS) self_row_lds[k] = self_rows[(long long)i * (S * (S + 1) / 2) + k];
-    T* DS = region;
-    for (int k = tid; k < n_present * 3 * CS; k += PIPE_THREADS) {
-        const int kt = k / (3 * CS);
-        const int r = k - kt * 3 * CS;
-        const int x = r / CS;
-        const int r2 = r - x * CS;
-        const int lm = r2 / NC, c = r2 - lm * NC;
-        DS[k] = c < n ? dself[(((long long)i * S + nth_set_bit(mask, kt)) * 3 + x) * L2 * n + lm * n + c] : (T)0;
-    }
-    lds_barrier();
-
-    if (loader) {
-        // window 0 into the first half while the center's own rows are written
-        if (nw > 0)
-            pipe_load<T>(0, e0, S, n_present, lay.per_entry, win_lds, windows, runs, run_rows, entry_data, bufs[0], ent0,
-                         lane);
-    } else {
-        T* const e1 = ent0 + half;
-        ps_self_rows<T, NC, RP>(0, L, n, S, n_present, present, self_row_lds, C, DS, scale_l,
-                                e1 + (size_t)wave_id * WAVE * RP * NC,
-                                reinterpret_cast<long long*>(e1 + (size_t)TILE_THREADS * RP * NC) + wave_id * WAVE, out);
-    }
-    lds_barrier();
-
-    T* DC = region;
-    for (int w = 0; w < nw; w++) {
-        const int h = w & 1;
-        T* const cur = ent0 + (size_t)h * half;
-        if (!loader) {
-            plan_entry_stage2<T, NC>(shc, lay, bufs[h].n_runs, bufs[h].lo, bufs[h].hi, bufs[h].carry != 0, leg_l, leg_m,
-                                     cur, DC);
-        }
-        lds_barrier();
-        if (loader) {
-            // window w + 1 into the other half while the rows of window w are written
-            if (w + 1 < nw)
-                pipe_load<T>(w + 1, e0, S, n_present, lay.per_entry, win_lds, windows, runs, run_rows, entry_data,
-                             bufs[h ^ 1], ent0 + (size_t)(h ^ 1) * half, lane);
-        } else {
-            const int rc = bufs[h].n_runs;
-            const int nc_runs = bufs[h].complete[rc - 1] ? rc : rc - 1;
-            ps_window_rows<T, NC, RP, FAST>(L, n, n_present, present, nc_runs, bufs[h].tj, bufs[h].row, C, DC, scale_l,
-                                            cur + (size_t)wave_id * WAVE * RP * NC,
-                                            reinterpret_cast<long long*>(cur + (size_t)TILE_THREADS * RP * NC) + wave_id * WAVE,
-                                            out);
-        }
-        lds_barrier();
-    }
-}
-
 // Spherical-expansion position gradients from the planned windows: the same
 // per-center workgroup, window staging and stage-2 sums as the power-spectrum
 // kernel, DC_run = d c_i / d r_j for each neighbour atom j (images summed), whose
@@ -2972,43 +2764,23 @@ bool launch_ps_gradient_fast(const SoapLaunch<T>& p, int chunk, const T* coef, c
 }
 
 // `rows_per_pass`: rows of each 4 x NC tile staged at once (1, 2 or 4)
-// FP32 uses the pipelined schedule (ps_gradient_pipe_kernel) when
-// RASCALINE_AMD_PIPELINE=1 (measured slower on the benchmark workload: 161 against
-// 137 ms per step, three workgroups per CU instead of four); FP64, whose double
-// buffer would leave one workgroup per CU, keeps the single-buffered kernel
-bool ps_gradient_pipelined(size_t value_size) {
-    static const bool on = [] {
-        const char* v = getenv("RASCALINE_AMD_PIPELINE");
-        return v && v[0] == '1';
-    }();
-    return on && value_size == 4;
-}
-
 template <typename T>
 size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass) {
     const int NC = (p.n + 3) / 4 * 4;
     const size_t cs = (size_t)p.L * p.L * NC;
     const size_t dc = (size_t)(max_runs > p.n_species ? max_runs : p.n_species) * 3 * cs;
     const int per_entry = TileEntryLayout(p.L, p.n, NC).per_entry;
-    if (ps_gradient_pipelined(sizeof(T))) {
-        return ((size_t)p.n_species * cs + dc + 2 * pipe_buffer_elems<T>(per_entry, max_entries, rows_per_pass, NC)) *
-               sizeof(T);
-    }
     const size_t entries = (size_t)per_entry * max_entries * sizeof(T);
     const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC * sizeof(T) + (size_t)TILE_THREADS * sizeof(void*);
     return ((size_t)p.n_species * cs + dc) * sizeof(T) + (entries > staging ? entries : staging);
 }
 
-// static LDS of the gradient-row kernel in use (bounds the padding the compiler may add)
+// dynamic plus static LDS of the gradient-row kernel (the static tables, with a margin
+// for the padding the compiler may add)
 template <typename T>
 size_t ps_gradient_tiled_lds(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass) {
-    const size_t fixed = ps_gradient_pipelined(sizeof(T))
-                             ? 2 * sizeof(PipeRuns) + PIPE_MAX_WINDOWS * sizeof(int4) + TILE_MAX_TYPES * sizeof(int) +
-                                   TILE_MAX_L * sizeof(T) + SH_TABLE_MAX * sizeof(T) +
-                                   TILE_MAX_L * (TILE_MAX_L + 1) * sizeof(int) +
-                                   TILE_MAX_TYPES * (TILE_MAX_TYPES + 1) / 2 * sizeof(int)
-                             : TILE_MAX_RUNS * (4 + TILE_MAX_TYPES) * sizeof(int) + (TILE_MAX_TYPES + 1) * sizeof(int) +
-                                   TILE_MAX_L * sizeof(T) + TILE_MAX_L * (TILE_MAX_L + 1) * sizeof(int);
+    const size_t fixed = TILE_MAX_RUNS * (4 + TILE_MAX_TYPES) * sizeof(int) + (TILE_MAX_TYPES + 1) * sizeof(int) +
+                         TILE_MAX_L * sizeof(T) + TILE_MAX_L * (TILE_MAX_L + 1) * sizeof(int);
     return ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass) + fixed + 256;
 }
 
@@ -3069,13 +2841,8 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     // n = 12 and n = 16
     const bool mixed = fast && sizeof(T) == 4 && ((p.n == 8 && max_runs <= 4) || p.n == 12 || p.n == 16);
 #define RA_LAUNCH_TILED(NCV, RPV, FV)                                                                                  \
-    if (ps_gradient_pipelined(sizeof(T))) {                                                                            \
-        ps_gradient_pipe_kernel<T, NCV, RPV, FV><<<p.n_atoms, PIPE_THREADS, shmem, p.stream>>>(                       \
-            a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out); \
-    } else {                                                                                                           \
-        ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                      \
-            a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out); \
-    }
+    ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                          \
+        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
 #define RA_LAUNCH_TILED_RP(NCV)       \
     if (rows_per_pass == 4) {         \
         RA_LAUNCH_TILED(NCV, 4, false);      \

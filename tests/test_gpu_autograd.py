@@ -6,7 +6,11 @@ is the vector-Jacobian product of the position gradients.
   the power spectrum, the spherical expansion, the radial spectrum and LODE
   (periodic only);
 * loss.backward() on Σ W · values against featomic-torch's contraction of the FP64
-  oracle's gradient rows with W (1e-10).
+  oracle's gradient rows with W (1e-10);
+* the double backward (featomic-torch's PositionsGrad::backward): gradcheck of the
+  backward as a function of dA/dX, and a loss on forces whose gradient with respect
+  to the weights is checked against the oracle's gradient rows (FP64 1e-10, FP32
+  2e-5 of the largest element).
 """
 
 import sys
@@ -100,3 +104,78 @@ def test_backward_vs_oracle():
     for b, blk in enumerate(ref["blocks"]):
         v = layout.block(values, b).detach().cpu().numpy()
         assert np.all(np.abs(v - blk["values"]) <= 1e-10 * (np.abs(blk["values"]) + np.abs(blk["values"]).max()))
+
+
+def force_loss_systems():
+    rng = np.random.default_rng(23)
+    return [random_system(rng, 12, "triclinic", 6.0, types=(1, 6, 8)),
+            random_system(rng, 9, "cluster", 6.0, types=(1, 8))]
+
+
+@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion", "lode"])
+def test_double_backward_gradcheck(kind):
+    """The backward as a function of dA/dX, w -> (dX/dr)^T w, is differentiable: its
+    gradient is dX/dr . v (featomic-torch's PositionsGrad::backward); gradcheck with
+    central finite differences of the (linear) backward in FP64."""
+    import torch
+
+    say("double backward gradcheck", kind)
+    systems = small_systems(5)
+    if kind == "lode":
+        rng = np.random.default_rng(6)
+        systems = [random_system(rng, 6, "triclinic", 5.0, types=(1, 8)), random_system(rng, 5, "periodic", 4.5, types=(1, 6))]
+        calc = ra.LodeSphericalExpansion(**LODE)
+    else:
+        calc = ra.SoapPowerSpectrum(**SOAP) if kind == "power_spectrum" else ra.SphericalExpansion(**SOAP)
+    calc.set_precision(64)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda",
+                       requires_grad=True)
+    values, _ = soap_values(calc, systems, pos)
+    w = torch.randn(values.shape, dtype=torch.float64, device="cuda", requires_grad=True)
+
+    def backward_of(w):
+        return torch.autograd.grad(values, pos, w, create_graph=True)[0]
+
+    assert torch.autograd.gradcheck(backward_of, (w,), eps=1e-6, atol=1e-8, rtol=1e-6, fast_mode=True)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_force_loss_vs_oracle(precision):
+    """Training on forces: E = Σ W X, F = -dE/dr (create_graph), loss = Σ u . F; then
+    dloss/dW = -(dX/dr) . u per sample, which the FP64 oracle's gradient rows give."""
+    import torch
+
+    systems = force_loss_systems()
+    calc = ra.SoapPowerSpectrum(**SOAP)
+    calc.set_precision(precision)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda",
+                       requires_grad=True)
+    values, layout = soap_values(calc, systems, pos)
+    rng = np.random.default_rng(3)
+    weights = torch.tensor(rng.normal(size=values.shape), dtype=values.dtype, device="cuda", requires_grad=True)
+    energy = (values * weights).sum()
+    forces = -torch.autograd.grad(energy, pos, create_graph=True)[0]
+    u = torch.tensor(rng.normal(size=forces.shape), dtype=torch.float64, device="cuda")
+    loss = (forces * u).sum()
+    import rascaline_amd.autograd as ag
+
+    ag._WARNED[0] = False  # the warning is given once per process
+    with pytest.warns(UserWarning, match="second derivatives with respect to positions"):
+        loss.backward()
+    ref = orc.soap_power_spectrum([orc.System(p, t, c) for p, t, c in systems], SOAP, ["positions"])
+    assert np.array_equal(layout.keys, np.asarray(ref["keys"]))
+    offsets = np.concatenate([[0], np.cumsum([len(s[1]) for s in systems])])
+    un = u.cpu().numpy()
+    worst = 0.0
+    for b, blk in enumerate(ref["blocks"]):
+        g = blk["gradients"]["positions"]
+        rows = g["values"].reshape(len(g["samples"]), 3, -1)
+        expected = np.zeros((len(blk["samples"]), rows.shape[-1]))
+        for r, (sample, system, atom) in enumerate(g["samples"]):
+            expected[sample] -= un[offsets[system] + atom] @ rows[r]
+        got = layout.block(weights.grad, b).double().cpu().numpy()
+        scale = max(np.abs(expected).max(), 1e-300)
+        bar = 1e-10 if precision == 64 else 2e-5
+        assert np.all(np.abs(got - expected) <= bar * (np.abs(expected) + scale)), b
+        worst = max(worst, float(np.abs(got - expected).max() / scale))
+    say(f"force-loss weight gradient vs oracle rows, FP{precision}: {worst:.1e}")
