@@ -209,6 +209,23 @@ featomic_status_t featomic_amd_mlp_loss_gradient(int32_t n_out, int64_t n_sample
                                                  const float* inputs, const float* targets, double* loss,
                                                  float* gradient);
 
+/* The same pass as featomic_amd_mlp_loss_gradient on device-resident arrays, with
+ * no device allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops that
+ * forbid allocations). `params` [n_params], `inputs` [n_samples] and `targets`
+ * [n_samples][n_out] are device arrays; the mean squared error goes to `loss` (one
+ * double, device) and its gradient to `gradient` ([n_params] floats, device). The
+ * work is queued on `stream` (a hipStream_t, NULL for the default stream) and the
+ * call returns without waiting. The scratch (transposed weights, one partial
+ * gradient per tile of 8 to 32 samples) is a static 32 MiB device array that every
+ * GPU the library runs on holds until the process ends; calls whose scratch would
+ * exceed it (about 14 000 samples at n_out = 216) return
+ * FEATOMIC_INVALID_PARAMETER_ERROR. Every call on a GPU shares that array: calls that
+ * overlap on one GPU (different streams or threads) race. */
+featomic_status_t featomic_amd_mlp_loss_gradient_noalloc(int32_t n_out, int64_t n_samples, const float* params,
+                                                         const float* inputs, const float* targets, double* loss,
+                                                         float* gradient, void* stream);
+
 /* The device inference kernel (FP16 hi + lo on the matrix cores, FP32
  * accumulation) over host inputs: outputs [n_samples][n_out]. */
 featomic_status_t featomic_amd_mlp_forward(int32_t n_out, int64_t n_samples, const float* params, const float* inputs,

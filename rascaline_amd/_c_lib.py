@@ -204,6 +204,9 @@ PROTOTYPES = [
     ("featomic_amd_mlp_loss_gradient", featomic_status_t,
      [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
       ctypes.c_void_p]),
+    ("featomic_amd_mlp_loss_gradient_noalloc", featomic_status_t,
+     [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
     ("featomic_amd_mlp_forward", featomic_status_t,
      [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 ]

@@ -14,6 +14,7 @@
 #include "calculator.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
+#include "kernels_mlp.hpp"
 
 struct featomic_calculator_t {
     std::unique_ptr<rascaline_amd::Calculator> impl;
@@ -404,6 +405,20 @@ featomic_status_t featomic_amd_mlp_loss_gradient(int32_t n_out, int64_t n_sample
         check_pointer(loss, "loss");
         check_pointer(gradient, "gradient");
         *loss = rascaline_amd::mlp_loss_gradient_device(n_out, n_samples, params, inputs, targets, gradient);
+    });
+}
+
+featomic_status_t featomic_amd_mlp_loss_gradient_noalloc(int32_t n_out, int64_t n_samples, const float* params,
+                                                         const float* inputs, const float* targets, double* loss,
+                                                         float* gradient, void* stream) {
+    return guard([&] {
+        check_pointer(params, "params");
+        check_pointer(inputs, "inputs");
+        check_pointer(targets, "targets");
+        check_pointer(loss, "loss");
+        check_pointer(gradient, "gradient");
+        rascaline_amd::mlp_loss_gradient_noalloc(n_out, n_samples, params, inputs, targets, loss, gradient,
+                                                 static_cast<hipStream_t>(stream));
     });
 }
 

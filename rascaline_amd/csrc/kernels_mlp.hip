@@ -12,7 +12,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
+
+#include "device.hpp"
+#include "hypers.hpp"
 
 namespace rascaline_amd {
 
@@ -773,6 +778,98 @@ void launch_uniform_samples(hipStream_t stream, long long n, double r_max, unsig
 void launch_network_inputs(hipStream_t stream, long long n, const double* r, double x_scale, double x_shift, float* x) {
     if (n <= 0) return;
     network_inputs_kernel<<<blocks_for(n), BLOCK, 0, stream>>>(n, r, x_scale, x_shift, x);
+}
+
+// ---- allocation-free training step (featomic_amd_mlp_loss_gradient_noalloc)
+//
+// The scratch of one loss-and-gradient pass (transposed weights, one partial
+// gradient and one partial loss per tile of samples) lives in this file-scope
+// device array instead of device memory allocated per call. Every GPU the library
+// runs on holds its own copy until the process ends; calls that overlap on one GPU
+// share it and race.
+constexpr size_t MLP_SCRATCH_BYTES = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[MLP_SCRATCH_BYTES];
+
+constexpr int MLP_SCRATCH_MAX_DEVICES = 64;
+
+// The device address of `scratch` on the current device, looked up once per device
+// (the array's name in host code is a stand-in, not a device address).
+static unsigned char* scratch_address() {
+    static std::mutex mutex;
+    static void* address[MLP_SCRATCH_MAX_DEVICES] = {};
+    int device = 0;
+    RA_HIP_CHECK(hipGetDevice(&device));
+    if (device < 0 || device >= MLP_SCRATCH_MAX_DEVICES) throw Error::internal("unexpected device index");
+    std::lock_guard<std::mutex> lock(mutex);
+    if (!address[device]) {
+        void* ptr = nullptr;
+        const hipError_t status = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (status != hipSuccess || !ptr) {
+            throw Error::internal(std::string("could not find the scratch array on the device: ") + hipGetErrorString(status));
+        }
+        address[device] = ptr;
+    }
+    return static_cast<unsigned char*>(address[device]);
+}
+
+size_t mlp_noalloc_scratch_bytes() { return MLP_SCRATCH_BYTES; }
+
+// scratch bytes of one pass: w2t [H][H], w3t [H][n_out], partial [tiles][P] floats, then
+// the partial losses [tiles] doubles (8-byte aligned)
+static size_t noalloc_layout(int n_out, long long n_samples, size_t* partial_offset, size_t* loss_offset) {
+    const long long P = mlp_n_params(n_out);
+    const int ts = mlp_train_tile(n_out);
+    const long long tiles = (n_samples + ts - 1) / ts;
+    const size_t floats = (size_t)H * H + (size_t)H * n_out;
+    *partial_offset = floats * sizeof(float);
+    const size_t after = *partial_offset + (size_t)tiles * P * sizeof(float);
+    *loss_offset = (after + 7) / 8 * 8;
+    return *loss_offset + (size_t)tiles * sizeof(double);
+}
+
+long long mlp_noalloc_max_samples(int n_out) {
+    long long lo = 0, hi = 1;
+    size_t a, b;
+    while (noalloc_layout(n_out, hi, &a, &b) <= MLP_SCRATCH_BYTES) hi *= 2;
+    while (hi - lo > 1) {
+        const long long mid = (lo + hi) / 2;
+        if (noalloc_layout(n_out, mid, &a, &b) <= MLP_SCRATCH_BYTES) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+void mlp_loss_gradient_noalloc(int n_out, long long n_samples, const float* params, const float* inputs,
+                               const float* targets, double* loss, float* gradient, hipStream_t stream) {
+    if (n_out < 1 || n_out > MLP_MAX_OUTPUTS) throw Error::invalid_parameter("n_out must be in [1, 512]");
+    if (n_samples < 1) throw Error::invalid_parameter("n_samples must be positive");
+    size_t partial_offset = 0, loss_offset = 0;
+    const size_t bytes = noalloc_layout(n_out, n_samples, &partial_offset, &loss_offset);
+    if (bytes > MLP_SCRATCH_BYTES) {
+        throw Error::invalid_parameter("featomic_amd_mlp_loss_gradient_noalloc takes at most " +
+                                       std::to_string(mlp_noalloc_max_samples(n_out)) + " samples for " +
+                                       std::to_string(n_out) + " outputs (its scratch is " +
+                                       std::to_string(MLP_SCRATCH_BYTES >> 20) + " MiB), got " +
+                                       std::to_string(n_samples));
+    }
+    unsigned char* base = scratch_address();
+    float* w2t = reinterpret_cast<float*>(base);
+    float* w3t = w2t + H * H;
+    float* partial = reinterpret_cast<float*>(base + partial_offset);
+    double* loss_partial = reinterpret_cast<double*>(base + loss_offset);
+    const int ts = mlp_train_tile(n_out);
+    const int tiles = static_cast<int>((n_samples + ts - 1) / ts);
+    // every word the passes read is written first: the transposed weights by the
+    // transpose kernel, each tile's partial gradient and loss by its training workgroup
+    launch_mlp_transpose(stream, n_out, params, w2t, w3t);
+    launch_mlp_train(stream, n_out, n_samples, n_samples, inputs, targets, 0, 0, params, w2t, w3t, partial,
+                     loss_partial);
+    launch_mlp_update(stream, n_out, tiles, partial, loss_partial, n_samples, 0, 0.0f, 1.0f, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, gradient, loss);
+    RA_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace rascaline_amd

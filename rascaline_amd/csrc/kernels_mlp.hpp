@@ -49,6 +49,16 @@ void launch_mlp_update(hipStream_t stream, int n_out, int n_wg, const float* par
 // Transposed copies of W2 and W3 from the flat parameters.
 void launch_mlp_transpose(hipStream_t stream, int n_out, const float* params, float* w2t, float* w3t);
 
+// One loss-and-gradient pass of the training kernels on device arrays with no device
+// allocation: the scratch is a file-scope device array of mlp_noalloc_scratch_bytes()
+// per GPU, shared by every call (overlapping calls on one GPU race). Queued on
+// `stream`; loss (one double) and gradient ([n_params] floats) are device pointers.
+void mlp_loss_gradient_noalloc(int n_out, long long n_samples, const float* params, const float* inputs,
+                               const float* targets, double* loss, float* gradient, hipStream_t stream);
+size_t mlp_noalloc_scratch_bytes();
+// the largest n_samples whose scratch fits
+long long mlp_noalloc_max_samples(int n_out);
+
 // ---- output-layer solve (FP64)
 
 // Hidden features a2 (FP32) of n samples: features[s][64].
