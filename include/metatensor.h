@@ -73,6 +73,14 @@ mts_status_t mts_block_data(mts_block_t* block, mts_array_t* data);
 mts_status_t mts_block_gradient(mts_block_t* block, const char* parameter, mts_block_t** gradient);
 mts_status_t mts_block_gradients_list(const mts_block_t* block, const char* const** parameters, uintptr_t* parameters_count);
 
+/* construction: labels are copied; `data` is taken over (its content copied, then
+ * released with its `destroy` callback; an array without `data` callback gives zeros);
+ * mts_tensormap takes over the blocks and sets the caller's pointers to NULL */
+mts_block_t* mts_block(struct mts_array_t data, mts_labels_t samples, const mts_labels_t* components,
+                       uintptr_t components_count, mts_labels_t properties);
+mts_status_t mts_block_free(mts_block_t* block);
+mts_tensormap_t* mts_tensormap(mts_labels_t keys, mts_block_t** blocks, uintptr_t blocks_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -174,6 +174,10 @@ PROTOTYPES = [
     ("mts_block_labels", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(mts_labels_t)]),
     ("mts_block_data", ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(mts_array_t)]),
     ("mts_block_gradient", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("mts_block", ctypes.c_void_p,
+     [mts_array_t, mts_labels_t, ctypes.POINTER(mts_labels_t), ctypes.c_size_t, mts_labels_t]),
+    ("mts_block_free", ctypes.c_int32, [ctypes.c_void_p]),
+    ("mts_tensormap", ctypes.c_void_p, [mts_labels_t, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
     ("mts_block_gradients_list", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p)), ctypes.POINTER(ctypes.c_size_t)]),
     ("featomic_amd_calculator_set_precision", featomic_status_t, [ctypes.c_void_p, ctypes.c_int32]),

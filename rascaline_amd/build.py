@@ -30,6 +30,7 @@ SOURCES = [
     ("hypers.cpp", []),
     ("radial.cpp", []),
     ("tensormap.cpp", []),
+    ("selection.cpp", []),
     ("capi.cpp", []),
 ]
 

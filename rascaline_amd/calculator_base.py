@@ -73,6 +73,68 @@ def tensormap_from_ptr(ptr):
         L.mts_tensormap_free(ptr)
 
 
+class _SelectionArgs:
+    """ctypes views of the selections of one compute call, alive as long as this object."""
+
+    def __init__(self):
+        self._keep = []
+        self._tensors = []
+
+    def labels(self, labels):
+        names = [n.encode("utf8") for n in labels.names]
+        name_array = (ctypes.c_char_p * max(len(names), 1))(*names)
+        values = np.ascontiguousarray(labels.values, dtype=np.int32)
+        self._keep += [name_array, values]
+        raw = C.mts_labels_t()
+        raw.internal_ptr_ = None
+        raw.names = ctypes.cast(name_array, ctypes.POINTER(ctypes.c_char_p))
+        raw.values = values.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        raw.size = len(names)
+        raw.count = values.shape[0]
+        self._keep.append(raw)
+        return raw
+
+    def tensor(self, tensor):
+        """A library-side TensorMap with the labels of `tensor` (zero data): the
+        predefined selection (featomic's LabelsSelection::Predefined)."""
+        L = C.lib()
+        blocks = []
+        for block in tensor.blocks():
+            components = [self.labels(c) for c in block.components]
+            comp_array = (C.mts_labels_t * max(len(components), 1))(*components)
+            self._keep.append(comp_array)
+            ptr = L.mts_block(C.mts_array_t(), self.labels(block.samples), comp_array, len(components),
+                              self.labels(block.properties))
+            if not ptr:
+                for b in blocks:
+                    L.mts_block_free(b)
+                raise C.FeatomicError(L.mts_last_error().decode("utf8"), 1)
+            blocks.append(ptr)
+        array = (ctypes.c_void_p * max(len(blocks), 1))(*blocks)
+        ptr = L.mts_tensormap(self.labels(tensor.keys), array, len(blocks))
+        if not ptr:
+            for b in blocks:
+                L.mts_block_free(b)
+            raise C.FeatomicError(L.mts_last_error().decode("utf8"), 1)
+        self._tensors.append(ptr)
+        return ptr
+
+    def selection(self, value, out):
+        if value is None:
+            return
+        if isinstance(value, Labels):
+            out.subset = ctypes.pointer(self.labels(value))
+        elif isinstance(value, TensorMap):
+            out.predefined = self.tensor(value)
+        else:
+            raise TypeError(f"expected Labels or TensorMap as selection, got {type(value)}")
+
+    def release(self):
+        for ptr in self._tensors:
+            C.lib().mts_tensormap_free(ptr)
+        self._tensors = []
+
+
 class CalculatorBase:
     """Base class of all calculators (featomic ``CalculatorBase``)."""
 
@@ -207,14 +269,25 @@ class CalculatorBase:
         array = (C.featomic_system_t * len(adapters))(*[a.struct for a in adapters])
         gradients = list(gradients or [])
         grads = (ctypes.c_char_p * max(len(gradients), 1))(*[g.encode("utf8") for g in gradients])
-        if selected_samples is not None or selected_properties is not None or selected_keys is not None:
-            raise C.FeatomicError("samples, properties and keys selection are not supported by this GPU implementation", 1)
         options = C.featomic_calculation_options_t()
         options.gradients = ctypes.cast(grads, ctypes.POINTER(ctypes.c_char_p))
         options.gradients_count = len(gradients)
         options.use_native_system = bool(use_native_system)
-        out = ctypes.c_void_p()
-        C.check(self._lib.featomic_calculator_compute(self._as_parameter_, ctypes.byref(out), array, len(adapters), options))
+        # selections (calculator_base.py:230-285): Labels for a subset, TensorMap for a
+        # predefined selection, Labels for the keys
+        selections = _SelectionArgs()
+        try:
+            selections.selection(selected_samples, options.selected_samples)
+            selections.selection(selected_properties, options.selected_properties)
+            if selected_keys is not None:
+                if not isinstance(selected_keys, Labels):
+                    raise TypeError(f"expected Labels for selected_keys, got {type(selected_keys)}")
+                options.selected_keys = ctypes.pointer(selections.labels(selected_keys))
+            out = ctypes.c_void_p()
+            C.check(self._lib.featomic_calculator_compute(self._as_parameter_, ctypes.byref(out), array, len(adapters),
+                                                          options))
+        finally:
+            selections.release()
         if not out.value:
             return None
         return tensormap_from_ptr(out)

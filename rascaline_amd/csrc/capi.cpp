@@ -15,6 +15,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "kernels_mlp.hpp"
+#include "selection.hpp"
 
 struct featomic_calculator_t {
     std::unique_ptr<rascaline_amd::Calculator> impl;
@@ -232,10 +233,6 @@ featomic_status_t featomic_calculator_compute(featomic_calculator_t* calculator,
         }
         check_pointer(systems, "systems");
         if (options.gradients_count > 0) check_pointer(options.gradients, "options.gradients");
-        if (options.selected_samples.subset || options.selected_samples.predefined ||
-            options.selected_properties.subset || options.selected_properties.predefined || options.selected_keys) {
-            throw Error::invalid_parameter("samples, properties and keys selection are not supported by this GPU implementation");
-        }
         auto& impl = *calculator->impl;
         const auto grads = parse_gradients(options.gradients, options.gradients_count, impl.kind());
         if (grads.cell && !std::getenv("FEATOMIC_NO_WARN_CELL_GRADIENTS")) {
@@ -250,7 +247,14 @@ featomic_status_t featomic_calculator_compute(featomic_calculator_t* calculator,
             throw Error::internal("assertion failed: self.cutoff > 0.0 && self.cutoff.is_finite()");
         }
         auto batch = gather_systems(systems, systems_count, options.use_native_system, soap ? impl.nl_cutoff() : 0.0, soap);
-        *descriptor = impl.compute(batch, grads);
+        mts_tensormap_t* full = impl.compute(batch, grads);
+        if (rascaline_amd::has_selections(options.selected_keys, options.selected_samples, options.selected_properties)) {
+            // the complete result, then the selected keys, samples and properties of it
+            *descriptor = rascaline_amd::apply_selections(full, options.selected_keys, options.selected_samples,
+                                                          options.selected_properties);
+        } else {
+            *descriptor = full;
+        }
     });
 }
 

@@ -1,0 +1,310 @@
+// Key, sample and property selections of featomic_calculator_compute
+// (featomic/src/calculator.rs:23-158 for LabelsSelection, :287-496 for their use in
+// prepare), applied to the complete result of a calculation: the selected keys in
+// the order given (or the default keys matching a partial selection), then per
+// block the default samples / properties matching a subset selection, or the labels
+// of a predefined selection's block, in its order. Gradient
+// rows follow their samples; a sample a predefined selection adds to the default
+// ones has zero values and no position-gradient rows, a key the selection adds has
+// the samples of no atom (the calculators' sample rules give none for a key with no
+// default entry).
+#include "selection.hpp"
+
+#include <map>
+#include <unordered_map>
+
+#include "hypers.hpp"
+
+namespace rascaline_amd {
+namespace {
+
+struct VecHash {
+    size_t operator()(const std::vector<int32_t>& v) const {
+        size_t h = 1469598103934665603ull;
+        for (int32_t x : v) {
+            h ^= static_cast<uint32_t>(x);
+            h *= 1099511628211ull;
+        }
+        return h;
+    }
+};
+
+LabelsData from_mts(const mts_labels_t& labels, const char* what) {
+    if (labels.size > 0 && !labels.names) throw Error::invalid_parameter(std::string(what) + " has no names");
+    if (labels.size > 0 && labels.count > 0 && !labels.values) throw Error::invalid_parameter(std::string(what) + " has no values");
+    std::vector<std::string> names;
+    for (uintptr_t k = 0; k < labels.size; k++) {
+        if (!labels.names[k]) throw Error::invalid_parameter(std::string(what) + " has a NULL name");
+        names.emplace_back(labels.names[k]);
+    }
+    std::vector<int32_t> values;
+    if (labels.size > 0 && labels.count > 0) values.assign(labels.values, labels.values + labels.size * labels.count);
+    return LabelsData(std::move(names), std::move(values));
+}
+
+std::string join(const std::vector<std::string>& names, const char* sep) {
+    std::string out;
+    for (size_t k = 0; k < names.size(); k++) {
+        if (k) out += sep;
+        out += names[k];
+    }
+    return out;
+}
+
+std::vector<int32_t> entry(const LabelsData& l, size_t i) {
+    return std::vector<int32_t>(l.values.begin() + i * l.size(), l.values.begin() + (i + 1) * l.size());
+}
+
+// the entries of `labels` matching one entry of `selection` on the selection's names
+// (calculator.rs:94-110: Labels::select, with its error for unknown names): with all
+// the names, the selection's entries that `labels` holds, in the selection's order;
+// with some of them, the matching entries of `labels`, in its order
+LabelsData select_subset(const LabelsData& labels, const LabelsData& selection, const std::string& kind) {
+    if (selection.names == labels.names) {
+        std::unordered_map<std::vector<int32_t>, int, VecHash> present;
+        for (size_t i = 0; i < labels.count(); i++) present.emplace(entry(labels, i), 1);
+        std::vector<int32_t> values;
+        for (size_t i = 0; i < selection.count(); i++) {
+            std::vector<int32_t> e = entry(selection, i);
+            if (present.count(e)) values.insert(values.end(), e.begin(), e.end());
+        }
+        return LabelsData(labels.names, std::move(values));
+    }
+    std::vector<size_t> columns;
+    for (const auto& name : selection.names) {
+        size_t c = 0;
+        while (c < labels.names.size() && labels.names[c] != name) c++;
+        if (c == labels.names.size()) {
+            throw Error::invalid_parameter("'" + name + "' in " + kind + " selection is not part of the " + kind +
+                                           " of this calculator");
+        }
+        columns.push_back(c);
+    }
+    std::unordered_map<std::vector<int32_t>, int, VecHash> wanted;
+    for (size_t i = 0; i < selection.count(); i++) wanted.emplace(entry(selection, i), 1);
+    std::vector<int32_t> values;
+    std::vector<int32_t> key(columns.size());
+    for (size_t i = 0; i < labels.count(); i++) {
+        for (size_t c = 0; c < columns.size(); c++) key[c] = labels.values[i * labels.size() + columns[c]];
+        if (wanted.count(key)) {
+            values.insert(values.end(), labels.values.begin() + i * labels.size(),
+                          labels.values.begin() + (i + 1) * labels.size());
+        }
+    }
+    return LabelsData(labels.names, std::move(values));
+}
+
+// one block's labels from a predefined selection (calculator.rs:113-156)
+LabelsData select_predefined(const mts_tensormap_t& predefined, const LabelsData& keys, size_t key_index,
+                             const std::vector<std::string>& default_names, const std::string& kind, bool samples) {
+    const std::vector<int32_t> key = entry(keys, key_index);
+    for (size_t b = 0; b < predefined.blocks.size(); b++) {
+        if (entry(predefined.keys, b) != key) continue;
+        const LabelsData& labels = samples ? predefined.blocks[b]->samples : predefined.blocks[b]->properties;
+        if (labels.names != default_names) {
+            throw Error::invalid_parameter("invalid predefined " + kind + " names: expected [" + join(default_names, ", ") +
+                                           "], but got [" + join(labels.names, ", ") + "]");
+        }
+        return labels;
+    }
+    std::string print;
+    for (size_t k = 0; k < keys.names.size(); k++) {
+        if (k) print += ", ";
+        print += keys.names[k] + "=" + std::to_string(key[k]);
+    }
+    throw Error::invalid_parameter("expected a block for (" + print + ") in predefined " + kind + " selection");
+}
+
+void check_predefined_keys(const mts_tensormap_t& predefined, const LabelsData& keys, const std::string& kind) {
+    if (predefined.keys.names != keys.names) {
+        throw Error::invalid_parameter("invalid key names in predefined " + kind + ": expected [" + join(keys.names, ",") +
+                                       "], but got [" + join(predefined.keys.names, ", ") + "]");
+    }
+}
+
+size_t product(const std::vector<LabelsData>& components) {
+    size_t p = 1;
+    for (const auto& c : components) p *= c.count();
+    return p;
+}
+
+// rows x (per-row inner values) x properties, copied through row and property maps
+// (-1: zero)
+ArrayData remap(const ArrayData& old, const std::vector<long long>& rows, size_t inner, size_t old_props,
+                const std::vector<long long>& props, std::vector<uintptr_t> shape) {
+    ArrayData out;
+    out.shape = std::move(shape);
+    out.data.assign(rows.size() * inner * props.size(), 0.0);
+    for (size_t r = 0; r < rows.size(); r++) {
+        if (rows[r] < 0) continue;
+        for (size_t c = 0; c < inner; c++) {
+            const double* src = old.data.data() + ((size_t)rows[r] * inner + c) * old_props;
+            double* dst = out.data.data() + (r * inner + c) * props.size();
+            for (size_t p = 0; p < props.size(); p++) dst[p] = props[p] >= 0 ? src[props[p]] : 0.0;
+        }
+    }
+    return out;
+}
+
+std::vector<uintptr_t> shape_of(size_t rows, const std::vector<LabelsData>& components, size_t props) {
+    std::vector<uintptr_t> shape{rows};
+    for (const auto& c : components) shape.push_back(c.count());
+    shape.push_back(props);
+    return shape;
+}
+
+std::unique_ptr<mts_block_t> empty_like(const mts_block_t& tmpl, const std::vector<int32_t>& key,
+                                        const std::vector<std::string>& key_names) {
+    auto block = std::make_unique<mts_block_t>();
+    block->samples = LabelsData(tmpl.samples.names, {});
+    block->components = tmpl.components;
+    // the spherical expansions' o3_mu component follows the key's o3_lambda
+    for (size_t k = 0; k < key_names.size(); k++) {
+        if (key_names[k] != "o3_lambda") continue;
+        for (auto& c : block->components) {
+            if (c.names.size() == 1 && c.names[0] == "o3_mu") {
+                std::vector<int32_t> mu;
+                for (int m = -key[k]; m <= key[k]; m++) mu.push_back(m);
+                c = LabelsData({"o3_mu"}, mu);
+            }
+        }
+    }
+    block->properties = tmpl.properties;
+    block->array.shape = shape_of(0, block->components, block->properties.count());
+    for (size_t g = 0; g < tmpl.gradients.size(); g++) {
+        const mts_block_t& tg = *tmpl.gradients[g];
+        auto grad = std::make_unique<mts_block_t>();
+        grad->samples = LabelsData(tg.samples.names, {});
+        grad->components.clear();
+        const size_t extra = tg.components.size() - tmpl.components.size();
+        for (size_t c = 0; c < extra; c++) grad->components.push_back(tg.components[c]);
+        for (const auto& c : block->components) grad->components.push_back(c);
+        grad->properties = block->properties;
+        grad->array.shape = shape_of(0, grad->components, grad->properties.count());
+        block->add_gradient(tmpl.gradient_names[g], std::move(grad));
+    }
+    return block;
+}
+
+std::unique_ptr<mts_block_t> select_block(const mts_block_t& old, const LabelsData& samples, const LabelsData& properties) {
+    std::unordered_map<std::vector<int32_t>, long long, VecHash> sample_index, property_index;
+    for (size_t i = 0; i < old.samples.count(); i++) sample_index.emplace(entry(old.samples, i), (long long)i);
+    for (size_t i = 0; i < old.properties.count(); i++) property_index.emplace(entry(old.properties, i), (long long)i);
+    std::vector<long long> rows(samples.count()), props(properties.count());
+    for (size_t i = 0; i < samples.count(); i++) {
+        auto it = sample_index.find(entry(samples, i));
+        rows[i] = it == sample_index.end() ? -1 : it->second;
+    }
+    for (size_t i = 0; i < properties.count(); i++) {
+        auto it = property_index.find(entry(properties, i));
+        if (it == property_index.end()) {
+            throw Error::invalid_parameter("a selected property is not one this calculator computes");
+        }
+        props[i] = it->second;
+    }
+    auto block = std::make_unique<mts_block_t>();
+    block->samples = samples;
+    block->components = old.components;
+    block->properties = properties;
+    const size_t inner = product(old.components);
+    block->array = remap(old.array, rows, inner, old.properties.count(), props,
+                         shape_of(rows.size(), old.components, props.size()));
+    for (size_t g = 0; g < old.gradients.size(); g++) {
+        const mts_block_t& og = *old.gradients[g];
+        const size_t ginner = product(og.components);
+        auto grad = std::make_unique<mts_block_t>();
+        grad->components = og.components;
+        grad->properties = properties;
+        std::vector<long long> grows;
+        std::vector<int32_t> gvalues;
+        const size_t width = og.samples.size();
+        if (old.gradient_names[g] == "positions") {
+            // rows (sample, system, atom), grouped by sample: the rows of every selected
+            // sample, relabelled with its new position
+            std::vector<long long> first(old.samples.count(), -1), count(old.samples.count(), 0);
+            for (size_t r = 0; r < og.samples.count(); r++) {
+                const int32_t s = og.samples.values[r * width];
+                if (first[s] < 0) first[s] = (long long)r;
+                count[s]++;
+            }
+            for (size_t i = 0; i < rows.size(); i++) {
+                if (rows[i] < 0) continue;
+                for (long long r = first[rows[i]]; r >= 0 && r < first[rows[i]] + count[rows[i]]; r++) {
+                    grows.push_back(r);
+                    gvalues.push_back((int32_t)i);
+                    for (size_t c = 1; c < width; c++) gvalues.push_back(og.samples.values[r * width + c]);
+                }
+            }
+        } else {
+            // one row per sample (strain, cell)
+            for (size_t i = 0; i < rows.size(); i++) {
+                grows.push_back(rows[i]);
+                gvalues.push_back((int32_t)i);
+            }
+        }
+        grad->samples = LabelsData(og.samples.names, std::move(gvalues));
+        grad->array = remap(og.array, grows, ginner, og.properties.count(), props,
+                            shape_of(grows.size(), og.components, props.size()));
+        block->add_gradient(old.gradient_names[g], std::move(grad));
+    }
+    return block;
+}
+
+}  // namespace
+
+bool has_selections(const mts_labels_t* selected_keys, const featomic_labels_selection_t& samples,
+                    const featomic_labels_selection_t& properties) {
+    return selected_keys || samples.subset || samples.predefined || properties.subset || properties.predefined;
+}
+
+mts_tensormap_t* apply_selections(mts_tensormap_t* full_ptr, const mts_labels_t* selected_keys,
+                                  const featomic_labels_selection_t& samples,
+                                  const featomic_labels_selection_t& properties) {
+    std::unique_ptr<mts_tensormap_t> full(full_ptr);
+    if ((samples.subset && samples.predefined) || (properties.subset && properties.predefined)) {
+        throw Error::invalid_parameter("can not have both subset and predefined selection at the same time");
+    }
+    // keys (calculator.rs:288-308)
+    LabelsData keys = full->keys;
+    if (selected_keys) {
+        const LabelsData selection = from_mts(*selected_keys, "selected keys");
+        if (selection.count() == 0) throw Error::invalid_parameter("selected keys can not be empty");
+        keys = selection.names == full->keys.names ? selection : select_subset(full->keys, selection, "keys");
+    }
+    std::unordered_map<std::vector<int32_t>, size_t, VecHash> old_block;
+    for (size_t b = 0; b < full->blocks.size(); b++) old_block.emplace(entry(full->keys, b), b);
+    if (samples.predefined) check_predefined_keys(*samples.predefined, keys, "samples");
+    if (properties.predefined) check_predefined_keys(*properties.predefined, keys, "properties");
+    const LabelsData sample_selection = samples.subset ? from_mts(*samples.subset, "selected samples") : LabelsData();
+    const LabelsData property_selection =
+        properties.subset ? from_mts(*properties.subset, "selected properties") : LabelsData();
+
+    auto out = std::make_unique<mts_tensormap_t>();
+    out->keys = keys;
+    for (size_t k = 0; k < keys.count(); k++) {
+        const std::vector<int32_t> key = entry(keys, k);
+        auto it = old_block.find(key);
+        std::unique_ptr<mts_block_t> empty;
+        const mts_block_t* old = nullptr;
+        if (it != old_block.end()) {
+            old = full->blocks[it->second].get();
+        } else {
+            if (full->blocks.empty()) {
+                throw Error::invalid_parameter("selected keys hold no key this calculation produces");
+            }
+            empty = empty_like(*full->blocks[0], key, keys.names);
+            old = empty.get();
+        }
+        LabelsData s = old->samples, p = old->properties;
+        if (samples.subset) s = select_subset(old->samples, sample_selection, "samples");
+        if (samples.predefined) s = select_predefined(*samples.predefined, keys, k, old->samples.names, "samples", true);
+        if (properties.subset) p = select_subset(old->properties, property_selection, "properties");
+        if (properties.predefined) {
+            p = select_predefined(*properties.predefined, keys, k, old->properties.names, "properties", false);
+        }
+        out->blocks.push_back(select_block(*old, s, p));
+    }
+    return out.release();
+}
+
+}  // namespace rascaline_amd

@@ -1,7 +1,9 @@
 // metatensor-core compatible accessors over the host TensorMap.
 #include "tensormap.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <string>
 
 namespace {
@@ -132,6 +134,86 @@ mts_status_t mts_block_gradients_list(const mts_block_t* block, const char* cons
     *parameters = block->gradient_name_ptrs.data();
     *count = block->gradient_name_ptrs.size();
     return MTS_SUCCESS;
+}
+
+// ---- construction (metatensor-core's mts_block / mts_tensormap), used to build the
+// predefined selections of featomic_calculator_compute. Labels are copied; the data
+// array's content is copied (zeros when it has no `data` callback) and the array
+// released, as metatensor-core takes it over.
+
+static bool copy_labels(const mts_labels_t& in, rascaline_amd::LabelsData& out) {
+    if (in.size > 0 && !in.names) return false;
+    if (in.size > 0 && in.count > 0 && !in.values) return false;
+    std::vector<std::string> names;
+    for (uintptr_t k = 0; k < in.size; k++) {
+        if (!in.names[k]) return false;
+        names.emplace_back(in.names[k]);
+    }
+    std::vector<int32_t> values;
+    if (in.size > 0 && in.count > 0) values.assign(in.values, in.values + in.size * in.count);
+    out = rascaline_amd::LabelsData(std::move(names), std::move(values));
+    return true;
+}
+
+mts_block_t* mts_block(mts_array_t data, mts_labels_t samples, const mts_labels_t* components,
+                       uintptr_t components_count, mts_labels_t properties) {
+    auto block = std::make_unique<mts_block_t>();
+    bool ok = copy_labels(samples, block->samples) && copy_labels(properties, block->properties);
+    if (components_count > 0 && !components) ok = false;
+    for (uintptr_t c = 0; ok && c < components_count; c++) {
+        rascaline_amd::LabelsData l;
+        ok = copy_labels(components[c], l);
+        block->components.push_back(std::move(l));
+    }
+    if (ok) {
+        block->array.shape.push_back(block->samples.count());
+        for (const auto& c : block->components) block->array.shape.push_back(c.count());
+        block->array.shape.push_back(block->properties.count());
+        size_t size = 1;
+        for (auto n : block->array.shape) size *= n;
+        block->array.data.assign(size, 0.0);
+        if (data.data && data.shape) {
+            const uintptr_t* shape = nullptr;
+            uintptr_t ndim = 0;
+            double* values = nullptr;
+            ok = data.shape(data.ptr, &shape, &ndim) == MTS_SUCCESS && ndim == block->array.shape.size() &&
+                 std::equal(shape, shape + ndim, block->array.shape.begin());
+            if (ok && size > 0) {
+                ok = data.data(data.ptr, &values) == MTS_SUCCESS && values;
+                if (ok) std::copy(values, values + size, block->array.data.begin());
+            }
+        }
+    }
+    if (data.destroy) data.destroy(data.ptr);
+    if (!ok) {
+        mts_error("invalid labels or data for a new block");
+        return nullptr;
+    }
+    return block.release();
+}
+
+mts_status_t mts_block_free(mts_block_t* block) {
+    delete block;
+    return MTS_SUCCESS;
+}
+
+mts_tensormap_t* mts_tensormap(mts_labels_t keys, mts_block_t** blocks, uintptr_t blocks_count) {
+    auto tensor = std::make_unique<mts_tensormap_t>();
+    if (!copy_labels(keys, tensor->keys) || (blocks_count > 0 && !blocks) || tensor->keys.count() != blocks_count) {
+        mts_error("invalid keys or blocks for a new tensor map");
+        return nullptr;
+    }
+    for (uintptr_t b = 0; b < blocks_count; b++) {
+        if (!blocks[b]) {
+            mts_error("got a NULL block");
+            return nullptr;
+        }
+    }
+    for (uintptr_t b = 0; b < blocks_count; b++) {
+        tensor->blocks.emplace_back(blocks[b]);
+        blocks[b] = nullptr;
+    }
+    return tensor.release();
 }
 
 }  // extern "C"
