@@ -121,9 +121,87 @@ def cpu_baseline(systems, target_seconds=25.0):
         "unit": "atoms/s",
         "cores": cores,
         "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": f"first {n} of the benchmark's structures ({atoms} atoms, {rows} gradient rows), FP64, "
                   f"{cores} threads, {dt:.1f} s",
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fd:
+            for line in fd:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def accuracy_sample(systems, n_structures=24):
+    """The FP64 oracle's power spectrum + position gradients of the first structures of the
+    workload, the reference for the accuracy of each benchmarked form."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import featomic_oracle as orc
+
+    return orc.soap_power_spectrum([orc.System(p, t, c) for (p, t, c) in systems[:n_structures]], HYPERS,
+                                   ["positions"])
+
+
+def form_accuracy(out, reference, rel=1e-4):
+    """Error of a device result (the first chunk, whose first systems are the reference's)
+    against the FP64 oracle: per block, the smallest absolute floor (as a fraction of the
+    block's largest element) with which every element passes |a - b| <= max(floor, rel
+    max(|a|, |b|)), and the largest block-relative error max|a - b| / max|b|."""
+    import numpy as np
+
+    from rascaline_amd.device import DeviceResult
+
+    dev = DeviceResult(out)
+    keys = [tuple(k) for k in dev.keys]
+    worst = {"values_floor": 0.0, "gradients_floor": 0.0, "values_block_relative": 0.0,
+             "gradients_block_relative": 0.0}
+    for key, block in zip(reference["keys"], reference["blocks"]):
+        b = keys.index(tuple(key))
+        ns = len(block["samples"])
+        g = block["gradients"]["positions"]
+        nr = len(g["samples"])
+        assert np.array_equal(dev.sample_labels(b)[:ns].cpu().numpy(), block["samples"])
+        assert np.array_equal(dev.gradient_labels(b)[:nr].cpu().numpy(), g["samples"])
+        pairs = (("values", dev.values(b)[:ns], block["values"]), ("gradients", dev.gradients(b)[:nr], g["values"]))
+        for name, got, ref in pairs:
+            got = got.double().cpu().numpy().reshape(ref.shape)
+            if not ref.size:
+                continue
+            scale = np.abs(ref).max()
+            diff = np.abs(got - ref)
+            fail = diff > rel * np.maximum(np.abs(got), np.abs(ref))
+            floor = float(diff[fail].max() / scale) if np.any(fail) else 0.0
+            worst[name + "_floor"] = max(worst[name + "_floor"], floor)
+            worst[name + "_block_relative"] = max(worst[name + "_block_relative"], float(diff.max() / scale))
+    return worst
+
+
+def end_to_end(systems, precision, n_structures=64, repeats=2):
+    """atoms/s through featomic_calculator_compute: host systems in, host TensorMap (float64
+    arrays) out, on the first structures of the workload (the whole workload's float64
+    gradients would not fit in host memory)."""
+    import rascaline_amd as ra
+
+    sample = systems[:n_structures]
+    calc = ra.SoapPowerSpectrum(**HYPERS)
+    calc.set_precision(precision)
+    ra_systems = [ra.System(t, p, c) for (p, t, c) in sample]
+    calc.compute(ra_systems, gradients=["positions"])
+    best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        calc.compute(ra_systems, gradients=["positions"])
+        best = min(best, time.perf_counter() - t0)
+    atoms = sum(len(t) for (_, t, _) in sample)
+    return {"value": atoms / best, "unit": "atoms/s",
+            "sample": f"first {len(sample)} structures ({atoms} atoms), featomic_calculator_compute with host "
+                      f"systems and a host TensorMap (float64 arrays), FP{precision} kernels, best of {repeats}"}
 
 
 def main():
@@ -185,13 +263,26 @@ def main():
     log(f"{len(batches)} chunks of up to {args.chunk_atoms} atoms")
     from rascaline_amd import _c_lib as C
 
+    def make_calculator(fast):
+        calc = ra.SoapPowerSpectrum(**HYPERS)
+        calc.set_precision(args.precision)
+        calc.set_fast_form(fast)
+        if fast:
+            # the fast form's learned radial integral, trained on the GPU (outside the timed
+            # steps); the spline stays in use when the network misses the acceptance tolerance
+            report = calc.train_radial_surrogate()
+            surrogate.update({"accepted": bool(report["accepted"]), "max_value_error": report["max_value_error"],
+                              "max_derivative_error": report["max_derivative_error"],
+                              "train_ms": report["train_ms"]})
+        return calc
+
+    surrogate = {}
+
     def measure(fast):
         """Warmup, then exactly `steps` timed steps over all chunks (bracketed by device
         synchronisation), then one pass with the library's hipEvent timings for the
         roofline of the gradient kernel."""
-        calc = ra.SoapPowerSpectrum(**HYPERS)
-        calc.set_precision(args.precision)
-        calc.set_fast_form(fast)
+        calc = make_calculator(fast)
         name = "fast" if fast else "plain"
         # launch shapes chosen by the autotuner on the first chunk (outside the timed steps)
         tuning_report = calc.autotune(batches[0], repeats=2)
@@ -202,9 +293,7 @@ def main():
         # `streams` chunks, in its stream's order)
         lanes = [(calc, stream)]
         for _ in range(args.streams - 1):
-            extra = ra.SoapPowerSpectrum(**HYPERS)
-            extra.set_precision(args.precision)
-            extra.set_fast_form(fast)
+            extra = make_calculator(fast)
             extra.tuning = calc.tuning
             lanes.append((extra, torch.cuda.Stream().cuda_stream))
 
@@ -226,6 +315,12 @@ def main():
         ms = 1e3 * elapsed / args.steps
         rate = n_atoms * args.steps / elapsed
         log(f"{name}: timed {args.steps} steps: {ms:.1f} ms/step, {rate:.3e} atoms/s")
+
+        # accuracy of this form on the first structures of the first chunk
+        out = run_chunk(calc, batches[0], stream, grads)
+        torch.cuda.synchronize()
+        accuracy = form_accuracy(out, reference)
+        log(f"{name}: accuracy against the FP64 oracle {accuracy}")
 
         total_bytes, kernel_ms, device_ms = 0, 0.0, 0.0
         rows_total, samples_total = 0, 0
@@ -257,18 +352,30 @@ def main():
             "algorithmic_bytes_per_step": total_bytes,
             "tuning": tuning_report["chosen"],
         }
+        roofline["accuracy"] = accuracy
         return ms, rate, roofline, rows_total, samples_total
 
-    # both forms meet the metric's 1e-4 bar (tests/test_gpu_soap.py::test_ps_fast_form_vs_oracle);
-    # the line reports the faster one and lists both
+    t0 = time.perf_counter()
+    reference = accuracy_sample(systems)
+    log(f"FP64 oracle of the first 24 structures for the accuracy checks: {time.perf_counter() - t0:.1f} s")
+    # the line reports the faster form among those within the metric's bar (1e-4 relative with
+    # the floor the FP32 arithmetic itself needs, 3e-5 of the block's scale) and lists both
     fast_available = args.precision == 32
     plain_ms, plain_rate, plain_roofline, rows_total, samples_total = measure(False)
     forms = {"plain": (plain_ms, plain_rate, plain_roofline)}
     if fast_available:
         fast_ms, fast_rate, fast_roofline, _, _ = measure(True)
         forms["fast"] = (fast_ms, fast_rate, fast_roofline)
-    best = max(forms, key=lambda f: forms[f][1])
+
+    def within_bar(rl):
+        acc = rl["accuracy"]
+        return acc["values_floor"] <= 3e-5 and acc["gradients_floor"] <= 3e-5
+
+    eligible = [f for f in forms if within_bar(forms[f][2])] or ["plain"]
+    best = max(eligible, key=lambda f: forms[f][1])
     ms_per_step, value, roofline = forms[best]
+    e2e = end_to_end(systems, args.precision)
+    log(f"end to end: {e2e}")
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -293,7 +400,7 @@ def main():
                         f"0.08-0.12 A^-3, half clusters in 15 A boxes; 4 types), SOAP power spectrum + position "
                         f"gradients, rc 5.0, n 8, l<=6",
             "form": (("plain FP32" if best == "plain" else
-                      "fast: FP32 with the power-spectrum gradient contraction on MFMA in bfloat16 hi+lo")
+                      "fast: learned radial integral + power-spectrum contractions on MFMA in bfloat16 hi+lo")
                      if fast_available else "FP64"),
             "atoms": n_atoms,
             "chunks": len(batches),
@@ -303,10 +410,15 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "end_to_end": e2e,
         "forms": {name: {"value": rate, "unit": "atoms/s", "ms_per_step": ms, "roofline_frac": rl["frac"],
-                         "kernel_ms_per_step": rl["kernel_ms_per_step"], "tuning": rl["tuning"]}
+                         "kernel_ms_per_step": rl["kernel_ms_per_step"], "tuning": rl["tuning"],
+                         "accuracy": rl["accuracy"]}
                   for name, (ms, rate, rl) in forms.items()},
     }
+    if "fast" in forms:
+        result["forms"]["fast"]["speedup"] = forms["fast"][1] / forms["plain"][1]
+        result["forms"]["fast"]["radial_surrogate"] = surrogate
     print(json.dumps(result), flush=True)
     return 0
 

@@ -929,13 +929,10 @@ __device__ __forceinline__ void store4(T* p, T a0, T a1, T a2, T a3) {
 __device__ __forceinline__ void small_divmod(int u, int d, float inv, int& q, int& r) {
     q = (int)((float)u * inv);
     r = u - q * d;
-    if (r < 0) {
-        q--;
-        r += d;
-    } else if (r >= d) {
-        q++;
-        r -= d;
-    }
+    // the float quotient is off by at most one: correct with selects, not branches
+    const int up = r >= d ? 1 : 0, down = r < 0 ? 1 : 0;
+    q += up - down;
+    r += (down - up) * d;
 }
 
 // Rows of the 4 x NC register tile of lane-quad q, for RP rows staged per pass:
@@ -968,6 +965,35 @@ __device__ __forceinline__ void tile_fma(int l, int q, const T* __restrict__ R, 
         for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int c = 0; c < NC; c++) acc[r][c] += rv[r] * cv[c];
+    }
+}
+
+// acc[a][c] = Σ_m R[m][q + 2a] * Cm[m][c] over m in [l², (l+1)²) for NC = 8, RP = 1
+// (tile_fma with the first term assigned rather than added to zeros).
+__device__ __forceinline__ void tile_product_f8(int l, int q, const float* __restrict__ R,
+                                                const float* __restrict__ Cm, float (&acc)[4][8]) {
+    const int m0 = l * l;
+    {
+        float rv[4], cv[8];
+#pragma unroll
+        for (int a = 0; a < 4; a++) rv[a] = R[m0 * 8 + q + 2 * a];
+        load4<float>(Cm + m0 * 8, cv);
+        load4<float>(Cm + m0 * 8 + 4, cv + 4);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[a][c] = rv[a] * cv[c];
+    }
+    for (int m = m0 + 1; m < (l + 1) * (l + 1); m++) {
+        float rv[4], cv[8];
+#pragma unroll
+        for (int a = 0; a < 4; a++) rv[a] = R[m * 8 + q + 2 * a];
+        load4<float>(Cm + m * 8, cv);
+        load4<float>(Cm + m * 8 + 4, cv + 4);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[a][c] += rv[a] * cv[c];
     }
 }
 
@@ -1029,6 +1055,66 @@ __device__ __forceinline__ void staged_tile_store(T* __restrict__ out, long long
 #pragma unroll
         for (int s2 = 0; s2 < PER_OWNER; s2++) {
             if (off[s2] >= 0) *reinterpret_cast<V*>(out + off[s2]) = val[s2];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+}
+
+// Whole-tile staged store for 8 x 8 FP32 tiles (n = 8, one staged row per pass): every
+// Floats per staged 8 x 8 tile in the oriented store: 64 plus a pad of 8, so that the
+// four tiles one read instruction covers start 8 banks apart (the transposed reads
+// and the row writes then meet at most two-way bank conflicts).
+constexpr int ORIENTED_SLOT = 72;
+// LDS staging floats per wave of the gradient-row kernel: the oriented store's 8
+// padded slots, or one RP x NC row block per lane.
+template <typename T, int NC, int RP>
+__host__ __device__ constexpr int stage_floats_per_wave() {
+    return (sizeof(T) == 4 && NC == 8 && RP == 1) ? 8 * ORIENTED_SLOT : WAVE * RP * NC;
+}
+
+// lane holds the rows tile_row(q, 0..3) of X (already scaled) for the output block at
+// element offset `tile` (-1: none), and `flags` says which orientations the block
+// sums: bit 0 X, bit 1 X^T. In each of four passes 16 lanes (8 tiles) stage their
+// whole tiles, then the wave writes those 8 blocks as 16-byte pieces from consecutive
+// lanes, each piece read from X, from X^T or from both, so that a product is formed
+// once whatever the block's orientation.
+__device__ __forceinline__ void staged_tile_store_oriented(float* __restrict__ out, long long tile, int flags, int q,
+                                                           const float (&acc)[4][8], float* stage, long long* stage_off) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int slot = (lane >> 1) & 7;
+#pragma unroll
+    for (int pass = 0; pass < 4; pass++) {
+        if ((lane >> 4) == pass && tile >= 0) {
+            float* X = stage + slot * ORIENTED_SLOT;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int row = q + 2 * a;  // tile_row<8, 1>(q, a)
+                store4<float>(X + row * 8, acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+                store4<float>(X + row * 8 + 4, acc[a][4], acc[a][5], acc[a][6], acc[a][7]);
+            }
+        }
+        if ((lane >> 4) == pass && q == 0) stage_off[slot] = tile >= 0 ? (tile | ((long long)flags << 60)) : -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int p = s2 * WAVE + lane;
+            const int u = p >> 4, n1 = (p & 15) >> 1, h = p & 1;
+            const long long off = stage_off[u];
+            if (off >= 0) {
+                const int fl = (int)(off >> 60);
+                const float* X = stage + u * ORIENTED_SLOT;
+                float v[4] = {0.f, 0.f, 0.f, 0.f};
+                if (fl & 1) load4<float>(X + n1 * 8 + 4 * h, v);
+                if (fl & 2) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) v[c] += X[(4 * h + c) * 8 + n1];
+                }
+                store4<float>(out + (off & ((1ll << 60) - 1)) + n1 * 8 + 4 * h, v[0], v[1], v[2], v[3]);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
@@ -1618,6 +1704,46 @@ __device__ __forceinline__ void ps_window_rows(int L, int n, int n_present, cons
         }
         return;
     }
+    if constexpr (!FAST && sizeof(T) == 4 && NC == 8 && RP == 1) {
+        if (n == NC) {
+            // FP32, n = 8: one product X = DC_r^T C_t per (run, type, x, l) tile, for every
+            // lane of the wave alike; the block's orientation is chosen when it is written
+            const int NRK = nc_runs * n_present;
+            const int per_lx = NRK * nq;
+            const int total = per_lx * 3 * L;
+            const float inv_lx = 1.0f / (float)(per_lx > 0 ? per_lx : 1);
+            const float inv_np = 1.0f / (float)(n_present > 0 ? n_present : 1);
+            for (int ubase = wave_id * WAVE; ubase < total; ubase += TILE_THREADS) {
+                const int u = ubase + (tid & (WAVE - 1));
+                long long tile = -1;
+                int q = 0, flags = 0;
+                float acc[4][8];  // written only for u < total; staged only then
+                if (u < total) {
+                    int lx, wq, r, k;
+                    small_divmod(u, per_lx, inv_lx, lx, wq);
+                    const int rk = wq / nq;
+                    q = wq - rk * nq;
+                    small_divmod(rk, n_present, inv_np, r, k);
+                    const int lrev = lx / 3;
+                    const int x = lx - 3 * lrev;
+                    const int l = L - 1 - lrev;
+                    const int t = present[k], tj = run_tj[r];
+                    tile_product_f8(l, q, reinterpret_cast<const float*>(DC) + ((size_t)r * 3 + x) * CS,
+                                    reinterpret_cast<const float*>(C) + (size_t)t * CS, acc);
+                    const float sc = t != tj ? (float)scale_l[l] * 1.4142135623730951f : (float)scale_l[l];
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+                        for (int c = 0; c < 8; c++) acc[rr][c] *= sc;
+                    flags = (t >= tj ? 1 : 0) | (t <= tj ? 2 : 0);
+                    tile = (long long)run_row[r][k] * 3 * P + (long long)x * P + l * 64;
+                }
+                staged_tile_store_oriented(reinterpret_cast<float*>(out), tile, flags, q, acc,
+                                           reinterpret_cast<float*>(stage), stage_offs);
+            }
+            return;
+        }
+    }
     // gradient rows of the complete runs: units (l descending, x) x (run, type) x quad,
     // so that the nq lanes of one (run, type, x, l) tile are adjacent
     const int NRK = nc_runs * n_present;
@@ -1749,8 +1875,9 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     T* ent = region + dc_size;
     // staging of the stores overlays the entry data (free while rows are emitted)
     const int wave_id = tid / WAVE;
-    T* stage = ent + (size_t)wave_id * WAVE * RP * NC;
-    long long* stage_offs = reinterpret_cast<long long*>(ent + (size_t)(TILE_THREADS / WAVE) * WAVE * RP * NC) + wave_id * WAVE;
+    constexpr int SPW = stage_floats_per_wave<T, NC, RP>();
+    T* stage = ent + (size_t)wave_id * SPW;
+    long long* stage_offs = reinterpret_cast<long long*>(ent + (size_t)(TILE_THREADS / WAVE) * SPW) + wave_id * WAVE;
     const bool staged = n == NC;
 
     const long long e0 = a.offsets[i];
@@ -2771,7 +2898,8 @@ size_t ps_gradient_tiled_shmem(const SoapLaunch<T>& p, int max_runs, int max_ent
     const size_t dc = (size_t)(max_runs > p.n_species ? max_runs : p.n_species) * 3 * cs;
     const int per_entry = TileEntryLayout(p.L, p.n, NC).per_entry;
     const size_t entries = (size_t)per_entry * max_entries * sizeof(T);
-    const size_t staging = (size_t)TILE_THREADS * rows_per_pass * NC * sizeof(T) + (size_t)TILE_THREADS * sizeof(void*);
+    const int spw = (sizeof(T) == 4 && NC == 8 && rows_per_pass == 1) ? 8 * ORIENTED_SLOT : WAVE * rows_per_pass * NC;
+    const size_t staging = (size_t)(TILE_THREADS / WAVE) * spw * sizeof(T) + (size_t)TILE_THREADS * sizeof(void*);
     return ((size_t)p.n_species * cs + dc) * sizeof(T) + (entries > staging ? entries : staging);
 }
 
