@@ -31,6 +31,7 @@ SOURCES = [
     ("radial.cpp", []),
     ("tensormap.cpp", []),
     ("selection.cpp", []),
+    ("host_transfer.cpp", []),
     ("capi.cpp", []),
 ]
 

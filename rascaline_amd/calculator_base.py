@@ -23,7 +23,19 @@ def _labels(raw):
     return Labels(names, values)
 
 
-def _read_block(ptr):
+class _TensorMapOwner:
+    """Keeps a library-owned mts_tensormap_t alive while numpy views of its arrays exist."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.ptr:
+            C.lib().mts_tensormap_free(self.ptr)
+            self.ptr = None
+
+
+def _read_block(ptr, owner):
     L = C.lib()
     arr = C.mts_array_t()
     C.check_mts(L.mts_block_data(ptr, ctypes.byref(arr)))
@@ -35,7 +47,10 @@ def _read_block(ptr):
     C.check_mts(arr.data(arr.ptr, ctypes.byref(data_ptr)))
     size = int(np.prod(shape)) if shape else 0
     if size:
-        values = np.ctypeslib.as_array(data_ptr, shape=(size,)).reshape(shape).copy()
+        # a view of the library's array (no copy); the buffer keeps the TensorMap alive
+        buf = (ctypes.c_double * size).from_address(ctypes.addressof(data_ptr.contents))
+        buf._owner = owner
+        values = np.frombuffer(buf, dtype=np.float64).reshape(shape)
     else:
         values = np.zeros(shape)
     labels = []
@@ -52,25 +67,25 @@ def _read_block(ptr):
         param = names[k]
         g = ctypes.c_void_p()
         C.check_mts(L.mts_block_gradient(ptr, param, ctypes.byref(g)))
-        block.add_gradient(param.decode("utf8"), _read_block(g))
+        block.add_gradient(param.decode("utf8"), _read_block(g, owner))
     return block
 
 
 def tensormap_from_ptr(ptr):
-    """Copy a library-owned mts_tensormap_t into Python, then release it."""
+    """A Python TensorMap over a library-owned mts_tensormap_t: labels are copied, the
+    block arrays are numpy views of the library's float64 arrays, which are released
+    with the last of them."""
     L = C.lib()
-    try:
-        raw = C.mts_labels_t()
-        C.check_mts(L.mts_tensormap_keys(ptr, ctypes.byref(raw)))
-        keys = _labels(raw)
-        blocks = []
-        for b in range(len(keys)):
-            bp = ctypes.c_void_p()
-            C.check_mts(L.mts_tensormap_block_by_id(ptr, ctypes.byref(bp), b))
-            blocks.append(_read_block(bp))
-        return TensorMap(keys, blocks)
-    finally:
-        L.mts_tensormap_free(ptr)
+    owner = _TensorMapOwner(ptr)
+    raw = C.mts_labels_t()
+    C.check_mts(L.mts_tensormap_keys(ptr, ctypes.byref(raw)))
+    keys = _labels(raw)
+    blocks = []
+    for b in range(len(keys)):
+        bp = ctypes.c_void_p()
+        C.check_mts(L.mts_tensormap_block_by_id(ptr, ctypes.byref(bp), b))
+        blocks.append(_read_block(bp, owner))
+    return TensorMap(keys, blocks)
 
 
 class _SelectionArgs:

@@ -842,13 +842,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     if (!to_host) return nullptr;
 
-    // ---- download and assemble the TensorMap
-    std::vector<T> h_values, h_grads, h_strain, h_cell;
+    // ---- download the labels, then assemble the TensorMap: the blocks' float64 arrays
+    // are written by the staged transfers below
     std::vector<int32_t> h_slabels, h_glabels;
-    download(h_values, d_values, values_size, stream);
-    if (d_grads) download(h_grads, d_grads, grads_size, stream);
-    if (d_strain) download(h_strain, d_strain, cs_size, stream);
-    if (d_cell) download(h_cell, d_cell, cs_size, stream);
     download(h_slabels, sample_labels, 2 * total_samples, stream);
     if (grad_labels) download(h_glabels, grad_labels, 3 * total_rows, stream);
     int flag = 0;
@@ -868,6 +864,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     const LabelsData properties = ps ? LabelsData({"l", "n_1", "n_2"}, prop_values) : LabelsData({"n"}, prop_values);
     const LabelsData xyz = LabelsData::range("xyz", 3);
+    std::vector<HostSegment> seg_values, seg_grads, seg_strain, seg_cell;
 
     for (size_t k = 0; k < refs.size(); k++) {
         const int b = refs[k].b, l = refs[k].l;
@@ -886,7 +883,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             shape = {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(2 * l + 1), static_cast<uintptr_t>(n)};
         }
         auto block = make_block(LabelsData({"system", "atom"}, sl), comps, properties, shape);
-        for (long long x = 0; x < ns * w; x++) block->array.data[x] = static_cast<double>(h_values[out_voff[k] + x]);
+        seg_values.push_back({out_voff[k], ns * w, block->array.data.data()});
 
         if (grads.positions) {
             std::vector<int32_t> gl(h_glabels.begin() + 3 * r0, h_glabels.begin() + 3 * (r0 + nr));
@@ -895,13 +892,12 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             std::vector<uintptr_t> gshape = {static_cast<uintptr_t>(nr), 3};
             for (size_t d = 1; d < shape.size(); d++) gshape.push_back(shape[d]);
             auto g = make_block(LabelsData({"sample", "system", "atom"}, gl), gcomps, properties, gshape);
-            for (long long x = 0; x < nr * 3 * w; x++) g->array.data[x] = static_cast<double>(h_grads[out_goff[k] + x]);
+            seg_grads.push_back({out_goff[k], nr * 3 * w, g->array.data.data()});
             block->add_gradient("positions", std::move(g));
         }
         for (int which = 0; which < 2; which++) {
             const bool want = which == 0 ? grads.cell : grads.strain;
             if (!want) continue;
-            const std::vector<T>& src = which == 0 ? h_cell : h_strain;
             std::vector<LabelsData> gcomps;
             if (which == 0) {
                 gcomps = {xyz, LabelsData::range("abc", 3)};
@@ -913,11 +909,15 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             for (size_t d = 1; d < shape.size(); d++) gshape.push_back(shape[d]);
             auto g = make_block(LabelsData::range("sample", static_cast<int>(ns)), gcomps, properties, gshape);
             const long long base = ps ? s0 * 9 * P : cs_base[b * L + l];
-            for (long long x = 0; x < ns * 9 * w; x++) g->array.data[x] = static_cast<double>(src[base + x]);
+            (which == 0 ? seg_cell : seg_strain).push_back({base, ns * 9 * w, g->array.data.data()});
             block->add_gradient(which == 0 ? "cell" : "strain", std::move(g));
         }
         tm->blocks.push_back(std::move(block));
     }
+    download_to_blocks<T>(d_values, std::move(seg_values), stream, staging_);
+    if (d_grads) download_to_blocks<T>(d_grads, std::move(seg_grads), stream, staging_);
+    if (d_strain) download_to_blocks<T>(d_strain, std::move(seg_strain), stream, staging_);
+    if (d_cell) download_to_blocks<T>(d_cell, std::move(seg_cell), stream, staging_);
     (void)key_size;
     return tm.release();
 }

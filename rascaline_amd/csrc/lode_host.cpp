@@ -445,11 +445,9 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         return nullptr;
     }
 
-    // ---- download and assemble the TensorMap
-    std::vector<T> h_values, h_grads;
-    download(h_values, d_values, static_cast<size_t>(values_size), stream);
-    if (d_grads) download(h_grads, d_grads, static_cast<size_t>(grads_size), stream);
-    RA_HIP_CHECK(hipStreamSynchronize(stream));
+    // ---- assemble the TensorMap; the blocks' float64 arrays are written by the staged
+    // transfers at the end
+    std::vector<HostSegment> seg_values, seg_grads;
 
     auto tm = std::make_unique<mts_tensormap_t>();
     std::vector<int32_t> keys;
@@ -483,17 +481,20 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
             auto block = make_block(samples, {mu_labels(l)}, properties,
                                     {static_cast<uintptr_t>(ns), static_cast<uintptr_t>(2 * l + 1), static_cast<uintptr_t>(n)});
             const long long vb = value_base[static_cast<size_t>(l) * S * S + pr];
-            for (long long x = 0; x < ns * w; x++) block->array.data[x] = static_cast<double>(h_values[vb + x]);
+            seg_values.push_back({vb, ns * w, block->array.data.data()});
             if (grads.positions) {
                 auto g = make_block(gsamples, {xyz, mu_labels(l)}, properties,
                                     {static_cast<uintptr_t>(nr), 3, static_cast<uintptr_t>(2 * l + 1), static_cast<uintptr_t>(n)});
                 const long long gb = grad_base[static_cast<size_t>(l) * S * S + pr];
-                for (long long x = 0; x < nr * 3 * w; x++) g->array.data[x] = static_cast<double>(h_grads[gb + x]);
+                seg_grads.push_back({gb, nr * 3 * w, g->array.data.data()});
                 block->add_gradient("positions", std::move(g));
             }
             tm->blocks.push_back(std::move(block));
         }
     }
+    download_to_blocks<T>(d_values, std::move(seg_values), stream, staging_);
+    if (d_grads) download_to_blocks<T>(d_grads, std::move(seg_grads), stream, staging_);
+    RA_HIP_CHECK(hipStreamSynchronize(stream));
     tm->keys = LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, keys);
     return tm.release();
 }

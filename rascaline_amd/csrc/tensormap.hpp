@@ -1,6 +1,7 @@
 // Host TensorMap returned through the C API (metatensor-core compatible subset).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -47,10 +48,44 @@ struct LabelsData {
     }
 };
 
+// float64 storage of a block's array. allocate() leaves the elements
+// uninitialised: the producers of a block's data write every element, and value-
+// initialising arrays of tens of GB would cost as much as filling them.
+class DoubleBuffer {
+public:
+    DoubleBuffer() = default;
+    DoubleBuffer(DoubleBuffer&&) noexcept = default;
+    DoubleBuffer& operator=(DoubleBuffer&&) noexcept = default;
+    DoubleBuffer& operator=(const std::vector<double>& v) {
+        allocate(v.size());
+        std::copy(v.begin(), v.end(), ptr_.get());
+        return *this;
+    }
+    void allocate(size_t n) {
+        ptr_.reset(n ? new double[n] : nullptr);
+        size_ = n;
+    }
+    void assign(size_t n, double value) {
+        allocate(n);
+        std::fill_n(ptr_.get(), n, value);
+    }
+    double* data() { return ptr_.get(); }
+    const double* data() const { return ptr_.get(); }
+    size_t size() const { return size_; }
+    double& operator[](size_t k) { return ptr_[k]; }
+    const double& operator[](size_t k) const { return ptr_[k]; }
+    double* begin() { return ptr_.get(); }
+    double* end() { return ptr_.get() + size_; }
+
+private:
+    std::unique_ptr<double[]> ptr_;
+    size_t size_ = 0;
+};
+
 // Contiguous float64 array owned by a block.
 struct ArrayData {
     std::vector<uintptr_t> shape;
-    std::vector<double> data;
+    DoubleBuffer data;
 };
 
 }  // namespace rascaline_amd
