@@ -14,6 +14,7 @@
 #include "../../include/featomic.h"
 #include "../../include/featomic_amd.h"
 #include "device.hpp"
+#include "host_transfer.hpp"
 #include "hypers.hpp"
 #include "radial.hpp"
 #include "surrogate.hpp"
@@ -31,6 +32,9 @@ public:
         return static_cast<T*>(get(name, count * sizeof(T) + 16));
     }
     void release();
+    // a frozen workspace hands out its existing buffers only: a request that would
+    // allocate or grow one throws (buffers captured in a graph must not move)
+    bool frozen = false;
 
 private:
     struct Slot {
@@ -38,6 +42,41 @@ private:
         size_t bytes = 0;
     };
     std::map<std::string, Slot> slots_;
+};
+
+// How a device pass runs. Eager: sizes read back from the device as the pass needs
+// them (host synchronisations). Record: the same, keeping the sizes and the host-
+// derived inputs of the batch in a BatchRecord. Replay: no synchronisation, upload or
+// allocation: sizes and inputs come from the record (the form captured in a graph).
+enum class PassMode { Eager, Record, Replay };
+
+// Sizes and host-derived inputs of one device batch, recorded by a synchronising
+// pass, and the description of its device output.
+struct BatchRecord {
+    long long n_entries = -1;                 // neighbour-list entries
+    std::vector<long long> block_totals;      // (samples, rows) offsets per block
+    Workspace inputs;                         // device copies of the batch's uploads
+    std::map<std::string, size_t> input_counts;
+    // host description of the output (featomic_amd_device_output_t arrays)
+    std::vector<int32_t> keys;
+    size_t key_size = 0;
+    std::vector<int64_t> n_samples, n_rows, value_offsets, gradient_offsets, block_properties;
+    std::vector<int64_t> sample_label_offsets, gradient_label_offsets;
+    featomic_amd_device_output_t output{};
+    void describe(featomic_amd_device_output_t* out) const;
+};
+
+// Replay flag bits (featomic_amd_batch_graph_check)
+constexpr int REPLAY_NEIGHBORS_CHANGED = 1;
+constexpr int REPLAY_BLOCKS_CHANGED = 2;
+constexpr int REPLAY_NEIGHBOR_ERROR = 4;
+constexpr int REPLAY_KERNEL_ERROR = 8;
+
+// The pass mode of a neighbour-list build (Eager when record is null).
+struct PassContext {
+    PassMode mode = PassMode::Eager;
+    BatchRecord* record = nullptr;
+    int* replay_flags = nullptr;  // device word the Replay checks OR into
 };
 
 // Systems gathered from the caller, concatenated.
@@ -113,6 +152,15 @@ public:
     void positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out);
 
     SoapTuning tuning;
+
+    // record / replay of device batches (batch graphs): set the mode of the following
+    // compute_device calls; replays use the record's sizes and inputs and OR
+    // inconsistencies into replay_flags. A calculator bound to a graph refuses eager
+    // device calls, which could move the buffers the graph uses.
+    void set_pass(PassMode mode, BatchRecord* record, int* replay_flags);
+    bool bound_to_graph = false;
+    Workspace& workspace() { return ws_; }
+
     // times the candidate launch shapes on a device-resident batch (the best of
     // `repeats` runs each, device time), keeps the fastest in `tuning` and
     // returns a JSON report of every candidate
@@ -138,6 +186,7 @@ private:
     RadialTables tables_;
     std::vector<double> self_contribution_;  // SOAP l=0 self term / LODE centre contribution
     Workspace ws_;
+    PinnedStaging staging_;  // pinned buffers of the host TensorMap transfers
 
     // host mirror of the last device-resident output
     struct DeviceResult {
@@ -150,6 +199,16 @@ private:
         const int32_t* sample_labels = nullptr;
         const int32_t* gradient_labels = nullptr;
     } device_result_;
+
+    PassMode mode_ = PassMode::Eager;
+    BatchRecord* record_ = nullptr;
+    int* replay_flags_ = nullptr;
+    // device buffer `name` holding `host`: uploaded into the workspace (Eager), into the
+    // batch record (Record), or the record's copy (Replay, no upload)
+    template <typename U>
+    U* batch_input(const std::string& name, const std::vector<U>& host, hipStream_t stream);
+    // timing events are left out of replays (a captured graph holds no event records)
+    void mark(hipEvent_t event, hipStream_t stream);
 
     std::unique_ptr<RadialSurrogate> surrogate_;
     bool use_surrogate_ = false;
@@ -213,7 +272,7 @@ std::vector<std::vector<featomic_pair_t>> gpu_half_pairs(const HostBatch& batch,
 // Uploads the batch geometry and builds the full neighbour list (GPU cell list,
 // or the caller's half list when provided).
 DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cutoff, const double* d_positions,
-                                const int* d_atom_system, hipStream_t stream);
+                                const int* d_atom_system, hipStream_t stream, const PassContext& pass = PassContext());
 
 
 // Device copies of the radial and spherical-harmonics tables (per precision).
@@ -280,7 +339,7 @@ inline std::unique_ptr<mts_block_t> make_block(LabelsData samples, std::vector<L
     b->array.shape = std::move(shape);
     size_t total = 1;
     for (auto s : b->array.shape) total *= s;
-    b->array.data.assign(total, 0.0);
+    b->array.data.allocate(total);
     return b;
 }
 

@@ -36,6 +36,18 @@ void launch_sample_labels(hipStream_t stream, int n_blocks, int n_atoms, const l
 
 // exclusive prefix sum of `n` values into out[0..n] (out[n] = total)
 void exclusive_scan(hipStream_t stream, const long long* in, long long* out, long long n);
+// the same, with its temporary storage from `ws` (no stream-ordered allocation: the
+// form a captured graph can hold)
+class Workspace;
+void exclusive_scan(hipStream_t stream, const long long* in, long long* out, long long n, Workspace& ws);
+// zero `bytes` (a multiple of 4) with a kernel rather than hipMemsetAsync (graph-safe)
+void fill_zero(void* p, size_t bytes, hipStream_t stream);
+// replay consistency checks: atomically OR `bit` into *flag when value[0] != expected,
+// when value[k] != expected[k] for some k < n, or when *src != 0
+void check_equal(hipStream_t stream, const long long* value, long long expected, int bit, int* flag);
+void check_equal_array(hipStream_t stream, const long long* value, const long long* expected, long long n, int bit,
+                       int* flag);
+void or_flag(hipStream_t stream, const int* src, int bit, int* flag);
 
 // ---- SOAP (kernels_soap.hip)
 template <typename T>

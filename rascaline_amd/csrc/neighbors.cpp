@@ -62,13 +62,14 @@ __global__ void int_to_i64(const int* in, long long* out, long long n) {
 }  // namespace
 
 DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cutoff, const double* d_positions,
-                                const int* d_atom_system, hipStream_t stream) {
+                                const int* d_atom_system, hipStream_t stream, const PassContext& pass) {
     DeviceNeighbors nl;
     const int n_atoms = static_cast<int>(batch.n_atoms());
     const size_t n_systems = batch.n_systems();
     nl.offsets = ws.get<long long>("nl_offsets", n_atoms + 1);
 
     if (batch.has_pairs) {
+        if (pass.mode != PassMode::Eager) throw Error::invalid_parameter("batch graphs need the GPU neighbour list");
         // caller-provided half lists: expand to the sorted full list on the host
         struct Entry {
             int j;
@@ -167,8 +168,21 @@ DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cu
         n_bins += static_cast<long long>(p.n_cells[0]) * p.n_cells[1] * p.n_cells[2];
     }
 
-    NLSystem* d_systems = ws.get<NLSystem>("nl_systems", n_systems);
-    RA_HIP_CHECK(hipMemcpyAsync(d_systems, params.data(), sizeof(NLSystem) * n_systems, hipMemcpyHostToDevice, stream));
+    // cell-list parameters: uploaded (Eager), kept in the batch record (Record), or the
+    // record's copy (Replay)
+    NLSystem* d_systems = nullptr;
+    if (pass.mode == PassMode::Eager) {
+        d_systems = ws.get<NLSystem>("nl_systems", n_systems);
+    } else {
+        d_systems = pass.record->inputs.get<NLSystem>("nl_systems", n_systems);
+        if (pass.mode == PassMode::Record) pass.record->input_counts["nl_systems"] = n_systems;
+        if (pass.mode == PassMode::Replay && pass.record->input_counts["nl_systems"] != n_systems) {
+            throw Error::internal("replayed batch differs from its record (systems)");
+        }
+    }
+    if (pass.mode != PassMode::Replay) {
+        RA_HIP_CHECK(hipMemcpyAsync(d_systems, params.data(), sizeof(NLSystem) * n_systems, hipMemcpyHostToDevice, stream));
+    }
     long long* atom_bin = ws.get<long long>("nl_atom_bin", n_atoms);
     int* atom_wrap = ws.get<int>("nl_atom_wrap", 3 * static_cast<size_t>(n_atoms));
     int* bin_count = ws.get<int>("nl_bin_count", n_bins);
@@ -178,21 +192,28 @@ DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cu
     int* bin_atoms = ws.get<int>("nl_bin_atoms", n_atoms);
     long long* counts = ws.get<long long>("nl_counts", n_atoms);
     int* error_flag = ws.get<int>("nl_error", 1);
-    RA_HIP_CHECK(hipMemsetAsync(bin_count, 0, sizeof(int) * n_bins, stream));
-    RA_HIP_CHECK(hipMemsetAsync(bin_fill, 0, sizeof(int) * n_bins, stream));
-    RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
+    fill_zero(bin_count, sizeof(int) * n_bins, stream);
+    fill_zero(bin_fill, sizeof(int) * n_bins, stream);
+    fill_zero(error_flag, sizeof(int), stream);
 
     launch_nl_assign_bins(stream, n_atoms, d_positions, d_atom_system, d_systems, atom_bin, atom_wrap, bin_count);
     if (n_bins) int_to_i64<<<(unsigned)((n_bins + 255) / 256), 256, 0, stream>>>(bin_count, bin_count64, n_bins);
-    exclusive_scan(stream, bin_count64, bin_start, n_bins);
+    exclusive_scan(stream, bin_count64, bin_start, n_bins, ws);
     launch_nl_fill_bins(stream, n_atoms, atom_bin, bin_start, bin_fill, bin_atoms);
     const double cutoff2 = cutoff * cutoff;
     launch_nl_candidates(stream, false, n_atoms, d_positions, d_atom_system, d_systems, atom_bin, atom_wrap, bin_start,
                          bin_count, bin_atoms, cutoff2, counts, nullptr, nullptr, error_flag);
-    exclusive_scan(stream, counts, nl.offsets, n_atoms);
+    exclusive_scan(stream, counts, nl.offsets, n_atoms, ws);
     long long n_entries = 0;
-    RA_HIP_CHECK(hipMemcpyAsync(&n_entries, nl.offsets + n_atoms, sizeof(long long), hipMemcpyDeviceToHost, stream));
-    RA_HIP_CHECK(hipStreamSynchronize(stream));
+    if (pass.mode == PassMode::Replay) {
+        // the recorded count; a different one is reported by the graph's check
+        n_entries = pass.record->n_entries;
+        check_equal(stream, nl.offsets + n_atoms, n_entries, REPLAY_NEIGHBORS_CHANGED, pass.replay_flags);
+    } else {
+        RA_HIP_CHECK(hipMemcpyAsync(&n_entries, nl.offsets + n_atoms, sizeof(long long), hipMemcpyDeviceToHost, stream));
+        RA_HIP_CHECK(hipStreamSynchronize(stream));
+        if (pass.mode == PassMode::Record) pass.record->n_entries = n_entries;
+    }
     nl.n_entries = n_entries;
     const size_t ne = static_cast<size_t>(n_entries);
     unsigned long long* keys = ws.get<unsigned long long>("nl_keys", ne);
@@ -207,6 +228,10 @@ DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cu
     launch_nl_finalize(stream, n_entries, n_atoms, nl.offsets, keys, d_positions, d_atom_system, d_systems, entry_center,
                        nl.entry_j, nl.entry_shift, nl.entry_vec, nl.entry_dist);
     RA_HIP_CHECK(hipGetLastError());
+    if (pass.mode == PassMode::Replay) {
+        or_flag(stream, error_flag, REPLAY_NEIGHBOR_ERROR, pass.replay_flags);
+        return nl;
+    }
     int flag = 0;
     RA_HIP_CHECK(hipMemcpyAsync(&flag, error_flag, sizeof(int), hipMemcpyDeviceToHost, stream));
     RA_HIP_CHECK(hipStreamSynchronize(stream));
