@@ -367,13 +367,53 @@ def main():
         fast_ms, fast_rate, fast_roofline, _, _ = measure(True)
         forms["fast"] = (fast_ms, fast_rate, fast_roofline)
 
+    def measure_graph(fast, tuning):
+        """The same pass captured into one hipGraph (featomic_amd_batch_graph_*): the
+        record pass and the capture happen outside the timed steps; each timed step is
+        one graph launch that recomputes every chunk (neighbour lists to gradient rows)
+        on `streams` calculators."""
+        from rascaline_amd.device import BatchGraph
+
+        gc.collect()
+        torch.cuda.synchronize()
+        lanes = []
+        for _ in range(args.streams):
+            c = make_calculator(fast)
+            c.tuning = tuning
+            lanes.append(c)
+        t0 = time.perf_counter()
+        graph = BatchGraph(lanes, batches, ["positions"])
+        log(f"{'fast' if fast else 'plain'} graph: recorded and captured {len(batches)} chunks on {len(lanes)} "
+            f"calculators in {time.perf_counter() - t0:.1f} s")
+        try:
+            for _ in range(args.warmup):
+                graph.launch()
+            graph.check()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                graph.launch()
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
+            graph.check()
+        finally:
+            graph.close()
+        ms = 1e3 * elapsed / args.steps
+        log(f"{'fast' if fast else 'plain'} graph: timed {args.steps} launches: {ms:.1f} ms/step, "
+            f"{n_atoms * args.steps / elapsed:.3e} atoms/s")
+        return ms, n_atoms * args.steps / elapsed
+
     def within_bar(rl):
         acc = rl["accuracy"]
         return acc["values_floor"] <= 3e-5 and acc["gradients_floor"] <= 3e-5
 
     eligible = [f for f in forms if within_bar(forms[f][2])] or ["plain"]
     best = max(eligible, key=lambda f: forms[f][1])
+    graph_ms, graph_rate = measure_graph(best == "fast", forms[best][2]["tuning"])
     ms_per_step, value, roofline = forms[best]
+    launch = "eager"
+    if graph_rate > value:
+        ms_per_step, value, launch = graph_ms, graph_rate, "graph"
     e2e = end_to_end(systems, args.precision)
     log(f"end to end: {e2e}")
 
@@ -403,6 +443,8 @@ def main():
                       "fast: learned radial integral + power-spectrum contractions on MFMA in bfloat16 hi+lo")
                      if fast_available else "FP64"),
             "atoms": n_atoms,
+            "launch": ("one hipGraph launch per step (featomic_amd_batch_graph)" if launch == "graph" else
+                       "chunk by chunk (featomic_amd_compute_device)"),
             "chunks": len(batches),
             "streams": args.streams,
             "power_spectrum_samples": samples_total,
@@ -416,6 +458,7 @@ def main():
                          "accuracy": rl["accuracy"]}
                   for name, (ms, rate, rl) in forms.items()},
     }
+    result["forms"][best]["graph"] = {"ms_per_step": graph_ms, "value": graph_rate, "unit": "atoms/s"}
     if "fast" in forms:
         result["forms"]["fast"]["speedup"] = forms["fast"][1] / forms["plain"][1]
         result["forms"]["fast"]["radial_surrogate"] = surrogate

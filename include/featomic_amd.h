@@ -97,6 +97,46 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
                                               const char* const* gradients, uintptr_t gradients_count, void* stream,
                                               featomic_amd_device_output_t* output);
 
+/* A pass of SOAP calculators (spherical expansion or power spectrum, positions
+ * gradients or none) over many device-resident batches, captured into one
+ * hipGraph: thousands of structures of uneven sizes go through one graph launch.
+ * No counterpart in the reference, whose calculators run on the host; it serves
+ * callers that recompute the same batches many times (benchmarks, repeated
+ * inference over a fixed set of structures).
+ *
+ * Creation runs every batch once with host synchronisations (the record pass: the
+ * neighbour count, block sizes and host-derived inputs of each batch), then
+ * captures a replay of all of them: batch k runs on calculators[k % n_calculators]
+ * on a stream of its own, consecutive batches of one calculator in order, the
+ * calculators concurrently. The graph keeps its own copy of the batches' positions
+ * (taken at creation), so every replay recomputes the same batches, from the
+ * neighbour list to the gradient rows, with the recorded sizes. The calculators
+ * belong to the graph until it is freed: using them directly fails meanwhile.
+ * `stream` orders the copies of the positions. */
+typedef struct featomic_amd_batch_graph_t featomic_amd_batch_graph_t;
+
+featomic_status_t featomic_amd_batch_graph_create(featomic_calculator_t* const* calculators, uintptr_t n_calculators,
+                                                  const featomic_amd_device_systems_t* batches, uintptr_t n_batches,
+                                                  const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                                  featomic_amd_batch_graph_t** graph);
+
+/* Launch one replay of the graph on `stream` (asynchronous). */
+featomic_status_t featomic_amd_batch_graph_launch(featomic_amd_batch_graph_t* graph, void* stream);
+
+/* Wait for the device, then fail (FEATOMIC_INTERNAL_ERROR) if a replay found
+ * sizes other than the recorded ones or a kernel reported an error. */
+featomic_status_t featomic_amd_batch_graph_check(featomic_amd_batch_graph_t* graph);
+
+/* Layout and device pointers of batch `batch`'s output, as
+ * featomic_amd_compute_device describes it. Batch k shares its calculator's
+ * buffers with batches k + n_calculators, k + 2 n_calculators, ...: after a
+ * replay, the outputs of the last batch of each calculator are the ones held. */
+featomic_status_t featomic_amd_batch_graph_output(const featomic_amd_batch_graph_t* graph, uintptr_t batch,
+                                                  featomic_amd_device_output_t* output);
+
+/* Release the graph and its calculators' binding to it. */
+featomic_status_t featomic_amd_batch_graph_free(featomic_amd_batch_graph_t* graph);
+
 /* Kernel-level timing of the last featomic_amd_compute_device call, measured
  * with hipEvents on the compute stream: total device time and the time of the
  * dominant output kernel (power spectrum gradient rows / values), in ms.

@@ -197,6 +197,14 @@ PROTOTYPES = [
     ("featomic_amd_compute_device", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.POINTER(ctypes.c_char_p), ctypes.c_size_t,
       ctypes.c_void_p, ctypes.POINTER(featomic_amd_device_output_t)]),
+    ("featomic_amd_batch_graph_create", featomic_status_t,
+     [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.POINTER(featomic_amd_device_systems_t), ctypes.c_size_t,
+      ctypes.POINTER(ctypes.c_char_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("featomic_amd_batch_graph_launch", featomic_status_t, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("featomic_amd_batch_graph_check", featomic_status_t, [ctypes.c_void_p]),
+    ("featomic_amd_batch_graph_output", featomic_status_t,
+     [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(featomic_amd_device_output_t)]),
+    ("featomic_amd_batch_graph_free", featomic_status_t, [ctypes.c_void_p]),
     ("featomic_amd_last_timings", featomic_status_t,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     ("featomic_amd_surrogate_options_default", featomic_status_t, [ctypes.POINTER(featomic_amd_surrogate_options_t)]),

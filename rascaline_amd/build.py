@@ -32,6 +32,7 @@ SOURCES = [
     ("tensormap.cpp", []),
     ("selection.cpp", []),
     ("host_transfer.cpp", []),
+    ("batch_graph.cpp", []),
     ("capi.cpp", []),
 ]
 

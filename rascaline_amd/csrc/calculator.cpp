@@ -11,6 +11,8 @@
 
 namespace rascaline_amd {
 
+static void check_unbound(const Calculator& c);
+
 // ---------------------------------------------------------------------------- workspace
 
 Workspace::~Workspace() { release(); }
@@ -19,6 +21,7 @@ void* Workspace::get(const std::string& name, size_t bytes) {
     Slot& slot = slots_[name];
     if (bytes == 0) bytes = 16;
     if (slot.bytes < bytes) {
+        if (frozen) throw Error::internal("the buffer '" + name + "' of a frozen workspace would have to grow");
         // kernels queued by an earlier call may still use the old buffer
         if (slot.ptr) {
             RA_HIP_CHECK(hipDeviceSynchronize());
@@ -305,6 +308,7 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
 // ---------------------------------------------------------------------------- radial surrogate
 
 SurrogateReport Calculator::train_radial_surrogate(const SurrogateOptions& options) {
+    check_unbound(*this);
     if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList) {
         throw Error::invalid_parameter("the radial surrogate is only available for the SOAP calculators");
     }
@@ -393,7 +397,7 @@ void Calculator::soap_launch(SoapLaunch<T>& lp, const DeviceNeighbors& nl, const
     lp.sh_c = tab.sh_c;
     lp.self_contribution = tab.self_c;
     int* error_flag = ws_.get<int>("soap_error", 1);
-    RA_HIP_CHECK(hipMemsetAsync(error_flag, 0, sizeof(int), stream));
+    fill_zero(error_flag, sizeof(int), stream);
     lp.error_flag = error_flag;
 }
 
@@ -422,7 +426,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         RA_HIP_CHECK(hipEventCreate(&ev_exp_stop_));
         RA_HIP_CHECK(hipEventCreate(&ev_grad_start_));
     }
-    RA_HIP_CHECK(hipEventRecord(ev_start_, stream));
+    mark(ev_start_, stream);
 
     auto tm = to_host ? std::make_unique<mts_tensormap_t>() : nullptr;
     if (tm) {
@@ -431,12 +435,12 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                         : LabelsData({"o3_lambda", "o3_sigma", "center_type", "neighbor_type"}, {});
     }
     if (n_atoms == 0) {
-        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
-        RA_HIP_CHECK(hipEventRecord(ev_exp_stop_, stream));
-        RA_HIP_CHECK(hipEventRecord(ev_grad_start_, stream));
-        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
-        RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
-        RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+        mark(ev_exp_start_, stream);
+        mark(ev_exp_stop_, stream);
+        mark(ev_grad_start_, stream);
+        mark(ev_main_start_, stream);
+        mark(ev_main_stop_, stream);
+        mark(ev_stop_, stream);
         main_bytes_ = 0;
         if (output) {
             device_result_ = DeviceResult();
@@ -457,19 +461,21 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     const double* d_positions = batch.device_positions;
     if (!d_positions) {
+        if (mode_ != PassMode::Eager) throw Error::invalid_parameter("batch graphs need device-resident positions");
         double* p = ws_.get<double>("positions", 3 * static_cast<size_t>(n_atoms));
         upload(p, batch.positions, stream);
         d_positions = p;
     }
-    int* d_atom_system = ws_.get<int>("atom_system", n_atoms);
-    int* d_atom_local = ws_.get<int>("atom_local", n_atoms);
-    int* d_species = ws_.get<int>("species", n_atoms);
-    upload(d_atom_system, atom_system, stream);
-    upload(d_atom_local, atom_local, stream);
-    upload(d_species, sp.index, stream);
+    const int* d_atom_system = batch_input("atom_system", atom_system, stream);
+    const int* d_atom_local = batch_input("atom_local", atom_local, stream);
+    const int* d_species = batch_input("species", sp.index, stream);
 
     // ---- neighbour list
-    const DeviceNeighbors nl = build_neighbors(ws_, batch, soap_.cutoff, d_positions, d_atom_system, stream);
+    PassContext pass;
+    pass.mode = mode_;
+    pass.record = record_;
+    pass.replay_flags = replay_flags_;
+    const DeviceNeighbors nl = build_neighbors(ws_, batch, soap_.cutoff, d_positions, d_atom_system, stream, pass);
 
     // ---- metadata: block membership, sample and gradient-row offsets
     unsigned* type_mask = ws_.get<unsigned>("type_mask", n_atoms);
@@ -486,8 +492,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             for (int nt = 0; nt < S; nt++) block_types.insert(block_types.end(), {ct, nt, nt});
         }
     }
-    int* d_block_types = ws_.get<int>("block_types", block_types.size());
-    upload(d_block_types, block_types, stream);
+    const int* d_block_types = batch_input("block_types", block_types, stream);
     const size_t NBN = static_cast<size_t>(NB) * n_atoms;
     long long* sample_flag = ws_.get<long long>("sample_flag", NBN);
     long long* row_count = ws_.get<long long>("row_count", NBN);
@@ -495,13 +500,25 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     long long* row_scan = ws_.get<long long>("row_scan", NBN + 1);
     launch_block_membership(stream, n_atoms, S, ps ? 1 : 0, NB, d_species, type_mask, unique_count, d_block_types,
                             sample_flag, row_count);
-    exclusive_scan(stream, sample_flag, sample_scan, static_cast<long long>(NBN));
-    exclusive_scan(stream, row_count, row_scan, static_cast<long long>(NBN));
+    exclusive_scan(stream, sample_flag, sample_scan, static_cast<long long>(NBN), ws_);
+    exclusive_scan(stream, row_count, row_scan, static_cast<long long>(NBN), ws_);
     long long* d_totals = ws_.get<long long>("block_totals", 2 * (NB + 1));
     launch_gather_block_totals(stream, NB, n_atoms, sample_scan, row_scan, d_totals);
     std::vector<long long> totals;
-    download(totals, d_totals, 2 * (NB + 1), stream);
-    RA_HIP_CHECK(hipStreamSynchronize(stream));
+    if (mode_ == PassMode::Replay) {
+        // the recorded block sizes; different ones are reported by the graph's check
+        totals = record_->block_totals;
+        if (totals.size() != static_cast<size_t>(2 * (NB + 1))) throw Error::internal("replayed batch differs from its record (blocks)");
+        check_equal_array(stream, d_totals, batch_input("block_totals", totals, stream), 2 * (NB + 1), REPLAY_BLOCKS_CHANGED,
+                          replay_flags_);
+    } else {
+        download(totals, d_totals, 2 * (NB + 1), stream);
+        RA_HIP_CHECK(hipStreamSynchronize(stream));
+        if (mode_ == PassMode::Record) {
+            record_->block_totals = totals;
+            batch_input("block_totals", totals, stream);
+        }
+    }
     std::vector<long long> block_sample_base(NB + 1), block_row_base(NB + 1);
     for (int b = 0; b <= NB; b++) {
         block_sample_base[b] = totals[2 * b];
@@ -509,8 +526,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     }
     const long long total_samples = block_sample_base[NB];
     const long long total_rows = block_row_base[NB];
-    long long* d_block_sample_base = ws_.get<long long>("block_sample_base", NB + 1);
-    upload(d_block_sample_base, block_sample_base, stream);
+    const long long* d_block_sample_base = batch_input("block_sample_base", block_sample_base, stream);
 
     // ---- kernel launch parameters
     SoapLaunch<T> lp;
@@ -554,17 +570,17 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(lds) / 2) {
             max_entries /= 2;
         }
-        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
+        mark(ev_exp_start_, stream);
         if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
             launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);
         } else {
             launch_soap_center<T>(lp, 0, coef, dself);
         }
     } else {
-        RA_HIP_CHECK(hipEventRecord(ev_exp_start_, stream));
+        mark(ev_exp_start_, stream);
         launch_soap_center<T>(lp, 0, coef, dself);
     }
-    RA_HIP_CHECK(hipEventRecord(ev_exp_stop_, stream));
+    mark(ev_exp_stop_, stream);
     T* dstrain = nullptr;
     T* dcell = nullptr;
     if (grads.strain) {
@@ -578,6 +594,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     RA_HIP_CHECK(hipGetLastError());
 
     if (vjp_grad_) {
+        if (mode_ != PassMode::Eager) throw Error::invalid_parameter("batch graphs compute values and gradient rows only");
         // vector-Jacobian product of the position gradients: dA/dc, then per-entry
         // contributions and their per-atom sums; no gradient rows
         if (!entry_data) throw Error::invalid_parameter("the positions vector-Jacobian product needs max_radial < 16, max_angular < 16 and at most 32 atomic types");
@@ -608,12 +625,12 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             throw Error::invalid_parameter("too many atomic types or radial / angular functions for the GPU kernels' local memory");
         }
         T* g = ws_.get<T>("vjp_g", 3 * static_cast<size_t>(std::max<long long>(nl.n_entries, 1)));
-        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        mark(ev_main_start_, stream);
         launch_vjp_pair<T>(lp, vjp_entries, entry_data, H, g);
         launch_vjp_gather<T>(stream, n_atoms, nl.offsets, nl.entry_j, nl.entry_shift, g, static_cast<T*>(vjp_out_));
-        RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+        mark(ev_main_stop_, stream);
         RA_HIP_CHECK(hipGetLastError());
-        RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+        mark(ev_stop_, stream);
         // algorithmic bytes: dA/dX read once, dA/dr written once, neighbour entries read once
         main_bytes_ = (ps ? total_samples * P : total_samples * static_cast<long long>(L2) * n) * static_cast<long long>(sizeof(T)) +
                       3LL * n_atoms * static_cast<long long>(sizeof(T)) + nl.n_entries * 48;
@@ -660,10 +677,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     long long* d_region = nullptr;
     long long* d_se_grad = nullptr;
     if (!ps) {
-        d_region = ws_.get<long long>("se_value_base", value_base.size());
-        d_se_grad = ws_.get<long long>("se_grad_base", se_grad_adj.size());
-        upload(d_region, value_base, stream);
-        upload(d_se_grad, se_grad_adj, stream);
+        d_region = batch_input("se_value_base", value_base, stream);
+        d_se_grad = batch_input("se_grad_base", se_grad_adj, stream);
     }
 
     const size_t slots_bytes = slots * sizeof(T);
@@ -674,9 +689,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
     }
     bool timed_start = false;
-    RA_HIP_CHECK(hipEventRecord(ev_grad_start_, stream));
+    mark(ev_grad_start_, stream);
     if (!grads.positions) {
-        RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        mark(ev_main_start_, stream);
         timed_start = true;
     }
     if (grads.positions) {
@@ -715,7 +730,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 launch_ps_grad_plan(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, sample_scan,
                                     row_scan, d_block_sample_base, d_atom_system, d_atom_local, w[0], w[1], d_runs,
                                     d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels, ps);
-                RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+                mark(ev_main_start_, stream);
                 timed_start = true;
                 if (ps) {
                     launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows,
@@ -728,7 +743,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 break;
             }
         }
-        if (!timed_start) RA_HIP_CHECK(hipEventRecord(ev_main_start_, stream));
+        if (!timed_start) mark(ev_main_start_, stream);
         if (ps && !done) {
             // register-blocked kernel; chunk sized for ~48 KB of LDS per workgroup
             const size_t fixed = ps_gradient_shmem<T>(lp, 0);
@@ -749,7 +764,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                                     d_se_grad, d_atom_system, d_atom_local, d_grads, grad_labels);
         }
     }
-    RA_HIP_CHECK(hipEventRecord(ev_main_stop_, stream));
+    mark(ev_main_stop_, stream);
     // algorithmic bytes of the gradient-row kernel: gradient values and labels written once,
     // per-center coefficients and self gradients read once, neighbour entries read once
     main_bytes_ = grads.positions ? grads_size * static_cast<long long>(sizeof(T)) + total_rows * 12 +
@@ -770,7 +785,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         }
     }
     RA_HIP_CHECK(hipGetLastError());
-    RA_HIP_CHECK(hipEventRecord(ev_stop_, stream));
+    if (mode_ == PassMode::Replay) or_flag(stream, error_flag, REPLAY_KERNEL_ERROR, replay_flags_);
+    mark(ev_stop_, stream);
 
     // ---- describe the blocks
     std::vector<int32_t> keys;
@@ -839,6 +855,20 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         output->sample_label_offsets = device_result_.sample_label_offsets.data();
         output->gradient_label_offsets = grad_labels ? device_result_.gradient_label_offsets.data() : nullptr;
         output->precision = sizeof(T) == 4 ? 32 : 64;
+        if (record_) {
+            // the batch's output description, kept for the graph's replays (the
+            // pointers of the capture pass are the ones replays write through)
+            record_->keys = device_result_.keys;
+            record_->key_size = device_result_.key_size;
+            record_->n_samples = device_result_.n_samples;
+            record_->n_rows = device_result_.n_rows;
+            record_->value_offsets = device_result_.value_offsets;
+            record_->gradient_offsets = device_result_.gradient_offsets;
+            record_->block_properties = device_result_.block_properties;
+            record_->sample_label_offsets = device_result_.sample_label_offsets;
+            record_->gradient_label_offsets = device_result_.gradient_label_offsets;
+            record_->output = *output;
+        }
     }
     if (!to_host) return nullptr;
 
@@ -922,9 +952,64 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     return tm.release();
 }
 
+// ---------------------------------------------------------------------------- record / replay
+
+void Calculator::set_pass(PassMode mode, BatchRecord* record, int* replay_flags) {
+    if (mode != PassMode::Eager && !record) throw Error::internal("record and replay passes need a batch record");
+    mode_ = mode;
+    record_ = mode == PassMode::Eager ? nullptr : record;
+    replay_flags_ = mode == PassMode::Replay ? replay_flags : nullptr;
+    ws_.frozen = mode == PassMode::Replay;
+}
+
+template <typename U>
+U* Calculator::batch_input(const std::string& name, const std::vector<U>& host, hipStream_t stream) {
+    if (mode_ == PassMode::Eager) {
+        U* p = ws_.get<U>(name, host.size());
+        upload(p, host, stream);
+        return p;
+    }
+    U* p = record_->inputs.get<U>(name, host.size());
+    if (mode_ == PassMode::Record) {
+        record_->input_counts[name] = host.size();
+        upload(p, host, stream);
+    } else {
+        const auto it = record_->input_counts.find(name);
+        if (it == record_->input_counts.end() || it->second != host.size()) {
+            throw Error::internal("replayed batch differs from its record (" + name + ")");
+        }
+    }
+    return p;
+}
+
+void Calculator::mark(hipEvent_t event, hipStream_t stream) {
+    if (mode_ != PassMode::Replay) RA_HIP_CHECK(hipEventRecord(event, stream));
+}
+
+void BatchRecord::describe(featomic_amd_device_output_t* out) const {
+    *out = output;
+    out->n_blocks = n_samples.size();
+    out->keys = keys.data();
+    out->key_size = key_size;
+    out->n_samples = n_samples.data();
+    out->n_gradient_rows = n_rows.data();
+    out->value_offsets = value_offsets.data();
+    out->gradient_offsets = gradient_offsets.data();
+    out->block_properties = block_properties.data();
+    out->sample_label_offsets = sample_label_offsets.data();
+    out->gradient_label_offsets = output.gradient_label_offsets ? gradient_label_offsets.data() : nullptr;
+}
+
+static void check_unbound(const Calculator& c) {
+    if (c.bound_to_graph) {
+        throw Error::invalid_parameter("this calculator runs inside a batch graph: free the graph before using it directly");
+    }
+}
+
 // ---------------------------------------------------------------------------- entry points
 
 mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& grads) {
+    check_unbound(*this);
     hipStream_t stream = nullptr;
     if (kind_ == Kind::NeighborList) return run_neighbor_list(batch, grads);
     if (kind_ == Kind::ByPair) return precision == 32 ? run_by_pair<float>(batch, grads) : run_by_pair<double>(batch, grads);
@@ -942,6 +1027,10 @@ mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& gr
 
 void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
                                 featomic_amd_device_output_t* output) {
+    if (mode_ == PassMode::Eager) check_unbound(*this);
+    if (mode_ != PassMode::Eager && kind_ != Kind::SphericalExpansion && kind_ != Kind::PowerSpectrum) {
+        throw Error::invalid_parameter("batch graphs support the SOAP spherical expansion and power spectrum");
+    }
     if (kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
         throw Error::invalid_parameter(std::string("the ") + name() +
                                        " calculator returns host TensorMaps only (use featomic_calculator_compute)");
@@ -971,6 +1060,7 @@ const std::vector<std::array<int, 3>>& soap_window_shapes() {
 }
 
 std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads, hipStream_t stream, int repeats) {
+    check_unbound(*this);
     if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
         throw Error::invalid_parameter("the autotuner covers the SOAP calculators");
     }
@@ -1075,6 +1165,7 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
 }
 
 void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
+    check_unbound(*this);
     if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
     if (kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
         throw Error::invalid_parameter("the positions VJP covers the SOAP and LODE calculators");

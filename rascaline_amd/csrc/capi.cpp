@@ -11,6 +11,7 @@
 
 #include "../../include/featomic.h"
 #include "../../include/featomic_amd.h"
+#include "batch_graph.hpp"
 #include "calculator.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
@@ -20,6 +21,10 @@
 struct featomic_calculator_t {
     std::unique_ptr<rascaline_amd::Calculator> impl;
     std::string name_storage;
+};
+
+struct featomic_amd_batch_graph_t {
+    std::unique_ptr<rascaline_amd::BatchGraph> impl;
 };
 
 namespace {
@@ -453,6 +458,24 @@ featomic_status_t featomic_amd_neighbors(const double* positions, uintptr_t n_at
 
 void featomic_amd_free(void* pointer) { std::free(pointer); }
 
+// host description of a device-resident batch
+static rascaline_amd::HostBatch device_batch(const featomic_amd_device_systems_t* systems, const char* what) {
+    check_pointer(systems, what);
+    check_pointer(systems->atom_offsets, "systems.atom_offsets");
+    check_pointer(systems->cells, "systems.cells");
+    rascaline_amd::HostBatch batch;
+    batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
+    const int64_t n_atoms = batch.n_atoms();
+    if (n_atoms > 0) {
+        check_pointer(systems->positions, "systems.positions");
+        check_pointer(systems->types, "systems.types");
+    }
+    batch.device_positions = systems->positions;
+    batch.types.assign(systems->types, systems->types + n_atoms);
+    batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
+    return batch;
+}
+
 featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
                                               const char* const* gradients, uintptr_t gradients_count, void* stream,
                                               featomic_amd_device_output_t* output) {
@@ -460,44 +483,72 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
         check_pointer(calculator, "calculator");
         check_pointer(systems, "systems");
         check_pointer(output, "output");
-        check_pointer(systems->atom_offsets, "systems.atom_offsets");
-        check_pointer(systems->cells, "systems.cells");
+        const rascaline_amd::HostBatch batch = device_batch(systems, "systems");
         auto& impl = *calculator->impl;
         const auto grads = parse_gradients(gradients, gradients_count, impl.kind());
         if (grads.strain || grads.cell) throw Error::invalid_parameter("featomic_amd_compute_device supports positions gradients only");
-        rascaline_amd::HostBatch batch;
-        batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
-        const int64_t n_atoms = batch.n_atoms();
-        if (n_atoms > 0) {
-            check_pointer(systems->positions, "systems.positions");
-            check_pointer(systems->types, "systems.types");
-        }
-        batch.device_positions = systems->positions;
-        batch.types.assign(systems->types, systems->types + n_atoms);
-        batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
         impl.compute_device(batch, grads, static_cast<hipStream_t>(stream), output);
     });
+}
+
+featomic_status_t featomic_amd_batch_graph_create(featomic_calculator_t* const* calculators, uintptr_t n_calculators,
+                                                  const featomic_amd_device_systems_t* batches, uintptr_t n_batches,
+                                                  const char* const* gradients, uintptr_t gradients_count, void* stream,
+                                                  featomic_amd_batch_graph_t** graph) {
+    return guard([&] {
+        check_pointer(calculators, "calculators");
+        check_pointer(graph, "graph");
+        if (n_calculators == 0) throw Error::invalid_parameter("a batch graph needs at least one calculator");
+        if (n_batches > 0) check_pointer(batches, "batches");
+        *graph = nullptr;
+        std::vector<rascaline_amd::Calculator*> lanes;
+        for (uintptr_t l = 0; l < n_calculators; l++) {
+            check_pointer(calculators[l], "calculators[i]");
+            lanes.push_back(calculators[l]->impl.get());
+        }
+        const auto grads = parse_gradients(gradients, gradients_count, lanes[0]->kind());
+        std::vector<rascaline_amd::HostBatch> host;
+        for (uintptr_t k = 0; k < n_batches; k++) host.push_back(device_batch(&batches[k], "batches[i]"));
+        auto g = std::make_unique<featomic_amd_batch_graph_t>();
+        g->impl = std::make_unique<rascaline_amd::BatchGraph>(lanes, host, grads, static_cast<hipStream_t>(stream));
+        *graph = g.release();
+    });
+}
+
+featomic_status_t featomic_amd_batch_graph_launch(featomic_amd_batch_graph_t* graph, void* stream) {
+    return guard([&] {
+        check_pointer(graph, "graph");
+        graph->impl->launch(static_cast<hipStream_t>(stream));
+    });
+}
+
+featomic_status_t featomic_amd_batch_graph_check(featomic_amd_batch_graph_t* graph) {
+    return guard([&] {
+        check_pointer(graph, "graph");
+        graph->impl->check();
+    });
+}
+
+featomic_status_t featomic_amd_batch_graph_output(const featomic_amd_batch_graph_t* graph, uintptr_t batch,
+                                                  featomic_amd_device_output_t* output) {
+    return guard([&] {
+        check_pointer(graph, "graph");
+        check_pointer(output, "output");
+        graph->impl->output(batch, output);
+    });
+}
+
+featomic_status_t featomic_amd_batch_graph_free(featomic_amd_batch_graph_t* graph) {
+    return guard([&] { delete graph; });
 }
 
 featomic_status_t featomic_amd_positions_vjp(featomic_calculator_t* calculator, const featomic_amd_device_systems_t* systems,
                                              const void* grad_values, void* stream, void* positions_grad) {
     return guard([&] {
         check_pointer(calculator, "calculator");
-        check_pointer(systems, "systems");
-        check_pointer(systems->atom_offsets, "systems.atom_offsets");
-        check_pointer(systems->cells, "systems.cells");
+        const rascaline_amd::HostBatch batch = device_batch(systems, "systems");
         check_pointer(grad_values, "grad_values");
         check_pointer(positions_grad, "positions_grad");
-        rascaline_amd::HostBatch batch;
-        batch.offsets.assign(systems->atom_offsets, systems->atom_offsets + systems->n_systems + 1);
-        const int64_t n_atoms = batch.n_atoms();
-        if (n_atoms > 0) {
-            check_pointer(systems->positions, "systems.positions");
-            check_pointer(systems->types, "systems.types");
-        }
-        batch.device_positions = systems->positions;
-        batch.types.assign(systems->types, systems->types + n_atoms);
-        batch.cells.assign(systems->cells, systems->cells + 9 * systems->n_systems);
         calculator->impl->positions_vjp(batch, static_cast<hipStream_t>(stream), grad_values, positions_grad);
     });
 }

@@ -163,3 +163,62 @@ def positions_vjp(calculator, systems, grad_values, stream=None):
                                                ctypes.c_void_p(grad_values.data_ptr()), ctypes.c_void_p(s.cuda_stream),
                                                ctypes.c_void_p(out.data_ptr())))
     return out
+
+
+class BatchGraph:
+    """One pass of SOAP calculators over many device batches captured into a single
+    hipGraph (``featomic_amd_batch_graph_create``). Batch k runs on
+    ``calculators[k % len(calculators)]``; the graph copies the batches' positions
+    when it is created and every ``launch`` recomputes all batches from them. The
+    calculators belong to the graph until ``close``.
+
+        graph = BatchGraph([calc_a, calc_b], batches, ["positions"])
+        graph.launch()                 # asynchronous, on torch's current stream
+        graph.check()                  # waits; raises if a replay was inconsistent
+        result = graph.output(k)       # DeviceResult of batch k (last batches of each lane)
+        graph.close()
+    """
+
+    def __init__(self, calculators, batches, gradients=(), stream=None):
+        import torch
+
+        self._batches = list(batches)
+        self._calculators = list(calculators)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        calcs = (ctypes.c_void_p * len(self._calculators))(*[c._as_parameter_.value for c in self._calculators])
+        structs = (C.featomic_amd_device_systems_t * max(len(self._batches), 1))()
+        for k, b in enumerate(self._batches):
+            structs[k] = b.struct
+        names = [g.encode("utf8") for g in gradients]
+        arr = (ctypes.c_char_p * max(len(names), 1))(*names)
+        handle = ctypes.c_void_p()
+        C.check(C.lib().featomic_amd_batch_graph_create(calcs, len(self._calculators), structs, len(self._batches), arr,
+                                                        len(names), ctypes.c_void_p(s.cuda_stream),
+                                                        ctypes.byref(handle)))
+        self._handle = handle
+        self._keep = (calcs, structs, arr)
+
+    def launch(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        C.check(C.lib().featomic_amd_batch_graph_launch(self._handle, ctypes.c_void_p(s.cuda_stream)))
+
+    def check(self):
+        C.check(C.lib().featomic_amd_batch_graph_check(self._handle))
+
+    def output(self, batch):
+        out = C.featomic_amd_device_output_t()
+        C.check(C.lib().featomic_amd_batch_graph_output(self._handle, int(batch), ctypes.byref(out)))
+        return DeviceResult(out)
+
+    def close(self):
+        if self._handle:
+            C.check(C.lib().featomic_amd_batch_graph_free(self._handle))
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
