@@ -62,6 +62,11 @@ class SoapHypers(ctypes.Structure):
         ("max_angular", ctypes.c_int),
         ("n_radial", ctypes.c_int),
         ("spline_accuracy", ctypes.c_double),
+        ("channel_n", ctypes.POINTER(ctypes.c_int)),
+        ("channel_points", ctypes.POINTER(ctypes.c_int)),
+        ("channel_positions", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+        ("channel_values", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+        ("channel_derivatives", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
     ]
 
 
@@ -243,8 +248,8 @@ def soap_hypers(h):
     else:
         raise ValueError(sm)
     dens = h["density"]
-    assert dens["type"] == "Gaussian"
-    out.gaussian_width = float(dens["width"])
+    assert dens["type"] in ("Gaussian", "DiracDelta")
+    out.gaussian_width = float(dens["width"]) if dens["type"] == "Gaussian" else 0.0
     sc = dens.get("scaling")
     if sc is not None:
         out.has_scaling = 1
@@ -253,12 +258,63 @@ def soap_hypers(h):
         out.scaling_exponent = float(sc["exponent"])
     out.center_atom_weight = float(dens.get("center_atom_weight", 1.0))
     basis = h["basis"]
-    assert basis["type"] == "TensorProduct"
-    out.max_angular = int(basis["max_angular"])
-    out.n_radial = int(basis["radial"]["max_radial"]) + 1
     acc = basis.get("spline_accuracy", 1e-8)
     out.spline_accuracy = -1.0 if acc is None else float(acc)
+    channels = basis_channels(basis)
+    out.max_angular = max(l for l, _ in channels)
+    out.n_radial = max(_radial_size(r) for _, r in channels)
+    one_gto = basis["type"] == "TensorProduct" and basis["radial"]["type"] == "Gto"
+    if not one_gto:
+        # per-channel radial integrals (radial_integral/mod.rs:138-183)
+        L = out.max_angular + 1
+        by_l = dict(channels)
+        n_arr = (ctypes.c_int * L)()
+        p_arr = (ctypes.c_int * L)()
+        pos_arr = (ctypes.POINTER(ctypes.c_double) * L)()
+        val_arr = (ctypes.POINTER(ctypes.c_double) * L)()
+        der_arr = (ctypes.POINTER(ctypes.c_double) * L)()
+        keep = [n_arr, p_arr, pos_arr, val_arr, der_arr]
+        for l in range(L):
+            radial = by_l.get(l)
+            if radial is None:
+                continue
+            n_arr[l] = _radial_size(radial)
+            if radial["type"] == "Tabulated":
+                # points sorted by position (tabulated.rs:206-208)
+                pts = sorted(radial["points"], key=lambda p: p["position"])
+                pos = np.ascontiguousarray([p["position"] for p in pts], dtype=np.float64)
+                val = np.ascontiguousarray([p["values"] for p in pts], dtype=np.float64)
+                der = np.ascontiguousarray([p["derivatives"] for p in pts], dtype=np.float64)
+                keep += [pos, val, der]
+                p_arr[l] = len(pts)
+                pos_arr[l] = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                val_arr[l] = val.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                der_arr[l] = der.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        out.channel_n = ctypes.cast(n_arr, ctypes.POINTER(ctypes.c_int))
+        out.channel_points = ctypes.cast(p_arr, ctypes.POINTER(ctypes.c_int))
+        out.channel_positions = ctypes.cast(pos_arr, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))
+        out.channel_values = ctypes.cast(val_arr, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))
+        out.channel_derivatives = ctypes.cast(der_arr, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))
+        out._keep = keep
     return out
+
+
+def _radial_size(radial):
+    """SoapRadialBasis::size (basis/radial/mod.rs:40-48)."""
+    if radial["type"] == "Gto":
+        return int(radial["max_radial"]) + 1
+    return len(radial["points"][0]["values"])
+
+
+def basis_channels(basis):
+    """[(o3_lambda, radial basis)] of a SphericalExpansionBasis, ascending
+    (basis/mod.rs:26-40): 0..max_angular sharing one radial basis, or the Explicit
+    basis' by_angular map."""
+    if basis.get("type", "TensorProduct") == "TensorProduct" and "by_angular" not in basis:
+        if "max_angular" not in basis:
+            return [(0, basis["radial"])]  # the radial spectrum's basis
+        return [(l, basis["radial"]) for l in range(int(basis["max_angular"]) + 1)]
+    return sorted((int(k), v) for k, v in basis["by_angular"].items())
 
 
 def lode_hypers(h):
@@ -486,6 +542,8 @@ class SoapComputation:
         self.h = soap_hypers(hypers)
         self.lmax = self.h.max_angular
         self.N = self.h.n_radial
+        # (l, n_l) of the basis' angular channels
+        self.channels = [(l, _radial_size(r)) for l, r in basis_channels(hypers["basis"])]
         self.L2 = (self.lmax + 1) ** 2
         self.species = _species(systems)
         self.S = len(self.species)
@@ -631,31 +689,32 @@ def soap_spherical_expansion(systems, hypers, gradients=()):
             for j in sorted(nbrs):
                 grad_samples.append((s_i, si, j))
         nti = comp.tindex(nt)
-        for l in range(lmax + 1):
+        for l, nl in comp.channels:
             keys.append((l, 1, ct, nt))
             sl = slice(l * l, (l + 1) * (l + 1))
-            values = np.zeros((len(samples), 2 * l + 1, N))
+            values = np.zeros((len(samples), 2 * l + 1, nl))
             for s_i, (si, i) in enumerate(samples):
-                values[s_i] = comp.data[si].values[i, nti, sl, :]
+                values[s_i] = comp.data[si].values[i, nti, sl, :nl]
             mu = ("o3_mu", np.arange(-l, l + 1, dtype=np.int32).reshape(-1, 1))
-            blk = _block(values, ["system", "atom"], samples, [mu], ["n"], np.arange(N).reshape(-1, 1))
+            ns = np.arange(nl).reshape(-1, 1)
+            blk = _block(values, ["system", "atom"], samples, [mu], ["n"], ns)
             if "positions" in comp.gradients:
-                gv = np.zeros((len(grad_samples), 3, 2 * l + 1, N))
+                gv = np.zeros((len(grad_samples), 3, 2 * l + 1, nl))
                 for r, (s_i, si, j) in enumerate(grad_samples):
                     i = samples[s_i][1]
-                    gv[r] = comp.se_position_gradient(comp.data[si], i, j, nti, l)
-                blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz(), mu], ["n"], np.arange(N).reshape(-1, 1))
+                    gv[r] = comp.se_position_gradient(comp.data[si], i, j, nti, l)[..., :nl]
+                blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz(), mu], ["n"], ns)
             for param, attr, c0 in (("cell", "cell_g", "abc"), ("strain", "strain", "xyz_1")):
                 if param not in comp.gradients:
                     continue
-                gv = np.zeros((len(samples), 3, 3, 2 * l + 1, N))
+                gv = np.zeros((len(samples), 3, 3, 2 * l + 1, nl))
                 for s_i, (si, i) in enumerate(samples):
-                    gv[s_i] = getattr(comp.data[si], attr)[i, nti, :, :, sl, :]
+                    gv[s_i] = getattr(comp.data[si], attr)[i, nti, :, :, sl, :nl]
                 if param == "cell":
                     comps = [_xyz(), ("abc", np.arange(3).reshape(3, 1)), mu]
                 else:
                     comps = [("xyz_2", np.arange(3).reshape(3, 1)), ("xyz_1", np.arange(3).reshape(3, 1)), mu]
-                blk["gradients"][param] = _block(gv, ["sample"], np.arange(len(samples)).reshape(-1, 1), comps, ["n"], np.arange(N).reshape(-1, 1))
+                blk["gradients"][param] = _block(gv, ["sample"], np.arange(len(samples)).reshape(-1, 1), comps, ["n"], ns)
             blocks.append(blk)
     return {"keys_names": ["o3_lambda", "o3_sigma", "center_type", "neighbor_type"],
             "keys": np.array(keys, dtype=np.int32).reshape(len(keys), 4), "blocks": blocks}
@@ -727,16 +786,17 @@ def soap_spherical_expansion_by_pair(systems, hypers, gradients=()):
 
     names = ["system", "first_atom", "second_atom", "cell_shift_a", "cell_shift_b", "cell_shift_c"]
     keys, blocks = [], []
+    channels = [(l, _radial_size(r)) for l, r in basis_channels(hypers["basis"])]
     for tp in type_pairs:
-        for l in range(lmax + 1):
+        for l, nl in channels:
             sl = slice(l * l, (l + 1) * (l + 1))
             entries = [e for e in samples[tp] if l == 0 or not _is_self_pair(e[0])]
             keys.append([l, 1, tp[0], tp[1]])
-            values = np.array([e[1][sl] for e in entries]).reshape(len(entries), 2 * l + 1, N)
+            values = np.array([e[1][sl, :nl] for e in entries]).reshape(len(entries), 2 * l + 1, nl)
             mu = ("o3_mu", np.arange(-l, l + 1, dtype=np.int32).reshape(-1, 1))
-            blk = _block(values, names, [e[0] for e in entries], [mu], ["n"], np.arange(N).reshape(-1, 1))
-            zero = np.zeros((3, 2 * l + 1, N))
-            grad_of = [zero if e[2] is None else e[2][:, sl] for e in entries]
+            blk = _block(values, names, [e[0] for e in entries], [mu], ["n"], np.arange(nl).reshape(-1, 1))
+            zero = np.zeros((3, 2 * l + 1, nl))
+            grad_of = [zero if e[2] is None else e[2][:, sl, :nl] for e in entries]
             if "positions" in gradients:
                 rows, gv = [], []
                 for k, e in enumerate(entries):
@@ -749,13 +809,13 @@ def soap_spherical_expansion_by_pair(systems, hypers, gradients=()):
                     else:
                         rows += [[k, system, first], [k, system, second]]
                         gv += [-grad_of[k], grad_of[k]]
-                gvals = np.array(gv).reshape(len(rows), 3, 2 * l + 1, N)
+                gvals = np.array(gv).reshape(len(rows), 3, 2 * l + 1, nl)
                 blk["gradients"]["positions"] = _block(gvals, ["sample", "system", "atom"], rows, [_xyz(), mu],
-                                                       ["n"], np.arange(N).reshape(-1, 1))
+                                                       ["n"], np.arange(nl).reshape(-1, 1))
             for param in ("strain", "cell"):
                 if param not in gradients:
                     continue
-                gvals = np.zeros((len(entries), 3, 3, 2 * l + 1, N))
+                gvals = np.zeros((len(entries), 3, 3, 2 * l + 1, nl))
                 for k, e in enumerate(entries):
                     w = e[3] if param == "strain" else np.array(e[0][3:6], dtype=np.float64)
                     gvals[k] = w[:, None, None, None] * grad_of[k][None]
@@ -763,7 +823,7 @@ def soap_spherical_expansion_by_pair(systems, hypers, gradients=()):
                           ("xyz_1", np.arange(3, dtype=np.int32).reshape(3, 1)), mu] if param == "strain"
                          else [_xyz(), ("abc", np.arange(3, dtype=np.int32).reshape(3, 1)), mu])
                 blk["gradients"][param] = _block(gvals, ["sample"], np.arange(len(entries)).reshape(-1, 1), comps,
-                                                 ["n"], np.arange(N).reshape(-1, 1))
+                                                 ["n"], np.arange(nl).reshape(-1, 1))
             blocks.append(blk)
     return {"keys_names": ["o3_lambda", "o3_sigma", "first_atom_type", "second_atom_type"],
             "keys": np.asarray(keys, dtype=np.int32).reshape(len(keys), 4), "blocks": blocks}
@@ -826,7 +886,11 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                     if b >= a:
                         keyset.add((int(t[i]), a, b))
 
-    props = [(l, n1, n2) for l in range(lmax + 1) for n1 in range(N) for n2 in range(N)]
+    # properties (power_spectrum.rs:498-523): (l, n_1, n_2) over the basis' channels;
+    # `cols` picks them from the dense [lmax + 1][N][N] layout computed below
+    props = [(l, n1, n2) for l, nl in comp.channels for n1 in range(nl) for n2 in range(nl)]
+    cols = np.array([l * N * N + n1 * N + n2 for (l, n1, n2) in props], dtype=np.int64)
+    dense = (lmax + 1) * N * N
     norms = np.array([(1.0 if l % 2 == 0 else -1.0) * math.sqrt(2 * l + 1) for l in range(lmax + 1)])
 
     keys, blocks = [], []
@@ -842,12 +906,12 @@ def soap_power_spectrum(systems, hypers, gradients=()):
         # power_spectrum.rs:584-618, all samples of the block at once
         c1s = np.array([comp.data[si].values[i, i1] for (si, i) in samples]).reshape(len(samples), comp.L2, N)
         c2s = np.array([comp.data[si].values[i, i2] for (si, i) in samples]).reshape(len(samples), comp.L2, N)
-        values = np.zeros((len(samples), len(props)))
+        values = np.zeros((len(samples), dense))
         for l in range(lmax + 1):
             sl = slice(l * l, (l + 1) * (l + 1))
             values[:, l * N * N:(l + 1) * N * N] = (
                 kappa * np.einsum("sma,smb->sab", c1s[:, sl], c2s[:, sl]) / norms[l]).reshape(len(samples), -1)
-        blk = _block(values, ["system", "atom"], samples, [], ["l", "n_1", "n_2"], props)
+        blk = _block(values[:, cols], ["system", "atom"], samples, [], ["l", "n_1", "n_2"], props)
 
         if "positions" in comp.gradients:
             grad_samples = []
@@ -891,10 +955,10 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                         k = pair_index.get((i, j), 0)
                         idx1[r] = k if t[j] == t1 else 0
                         idx2[r] = k if t[j] == t2 else 0
-                out = np.empty((n, 3, len(props)))
+                out = np.empty((n, 3, dense))
                 lib().orc_ps_gradient_rows(n, lmax + 1, N, comp.S, i1, i2, kappa, _ptr(sd.values), _ptr(center),
                                            _ptr(idx1), _ptr(idx2), _ptr(table), _ptr(out))
-                gv[r0:r1] = out
+                gv[r0:r1] = out[:, :, cols]
                 r0 = r1
             blk["gradients"]["positions"] = _block(gv, ["sample", "system", "atom"], grad_samples, [_xyz()], ["l", "n_1", "n_2"], props)
 
@@ -911,7 +975,7 @@ def soap_power_spectrum(systems, hypers, gradients=()):
                     sl = slice(l * l, (l + 1) * (l + 1))
                     acc = np.einsum("mb,xyma->xyab", c2[sl], g1[:, :, sl]) + np.einsum("ma,xymb->xyab", c1[sl], g2[:, :, sl])
                     row.append((kappa * acc / norms[l]).reshape(3, 3, -1))
-                gv[s_i] = np.concatenate(row, axis=2)
+                gv[s_i] = np.concatenate(row, axis=2)[:, :, cols]
             if param == "cell":
                 comps = [_xyz(), ("abc", np.arange(3).reshape(3, 1))]
             else:

@@ -56,6 +56,10 @@ orc_radial* orc_soap_radial_new(double cutoff, double gaussian_width, int max_an
 /* LODE GTO, k-space; spline_accuracy <= 0 means exact */
 orc_radial* orc_lode_radial_new(double radius, int max_angular, int n_radial,
                                 double k_cutoff, double spline_accuracy);
+/* a tabulated radial integral: the Hermite spline through the given points, the same
+ * for every l (shared/basis/radial/tabulated.rs:151-166); points sorted by position */
+orc_radial* orc_radial_tabulated(int n, int n_points, const double* positions, const double* values,
+                                 const double* derivatives);
 void orc_radial_compute(const orc_radial* radial, int l, double r, double* values, double* grads);
 int orc_radial_n_knots(const orc_radial* radial, int l);
 /* knot positions, values, derivatives ([n_knots], [n_knots][n], [n_knots][n]) */
@@ -76,6 +80,17 @@ typedef struct orc_soap_hypers {
     int max_angular;
     int n_radial;
     double spline_accuracy; /* <= 0: exact */
+    /* Per angular channel l = 0..max_angular, its own radial integral
+     * (SoapRadialIntegralCacheByAngular, soap/radial_integral/mod.rs:138-183):
+     * channel_n[l] radial functions (0: l is not a channel of the Explicit basis),
+     * GTO x Gaussian when channel_points[l] == 0, else the tabulated spline points of
+     * a DiracDelta density (spline.rs:46-58). channel_n == NULL: TensorProduct with one
+     * GTO basis of n_radial functions. n_radial is the largest channel_n. */
+    const int* channel_n;
+    const int* channel_points;
+    const double* const* channel_positions;   /* [points] */
+    const double* const* channel_values;      /* [points][channel_n[l]] */
+    const double* const* channel_derivatives; /* [points][channel_n[l]] */
 } orc_soap_hypers;
 
 typedef struct orc_soap orc_soap;

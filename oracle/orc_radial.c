@@ -23,7 +23,8 @@ struct orc_radial {
     double* gto;
     /* orthonormalisation O (row-major): values <- values . O^T */
     double* ortho;
-    /* spline tables per l */
+    /* spline tables per l (a tabulated integral: one table, used for every l) */
+    int tabulated;
     int spline;
     int* n_knots;
     double** pos;
@@ -404,7 +405,30 @@ orc_radial* orc_lode_radial_new(double radius, int max_angular, int n_radial, do
     return r;
 }
 
+orc_radial* orc_radial_tabulated(int n, int n_points, const double* positions, const double* values,
+                                 const double* derivatives) {
+    if (n < 1 || n_points < 2) return NULL;
+    orc_radial* r = calloc(1, sizeof(orc_radial));
+    r->n = n;
+    r->stop = positions[n_points - 1];
+    r->tabulated = 1;
+    r->spline = 1;
+    r->n_knots = calloc(1, sizeof(int));
+    r->pos = calloc(1, sizeof(double*));
+    r->val = calloc(1, sizeof(double*));
+    r->der = calloc(1, sizeof(double*));
+    r->n_knots[0] = n_points;
+    r->pos[0] = malloc(sizeof(double) * n_points);
+    r->val[0] = malloc(sizeof(double) * n_points * n);
+    r->der[0] = malloc(sizeof(double) * n_points * n);
+    memcpy(r->pos[0], positions, sizeof(double) * n_points);
+    memcpy(r->val[0], values, sizeof(double) * n_points * n);
+    memcpy(r->der[0], derivatives, sizeof(double) * n_points * n);
+    return r;
+}
+
 void orc_radial_compute(const orc_radial* r, int l, double x, double* values, double* grads) {
+    if (r->tabulated) l = 0;
     if (r->spline) {
         spline_eval(r->n_knots[l], r->pos[l], r->val[l], r->der[l], r->n, x, values, grads);
     } else {

@@ -12,25 +12,75 @@
 
 struct orc_soap {
     orc_soap_hypers h;
-    orc_radial* radial;
+    orc_radial* radial;   /* TensorProduct: one GTO basis for every channel */
+    orc_radial** by_l;    /* per channel l: its radial integral, or NULL (not a channel) */
+    int* n_l;             /* per channel l: its radial size */
 };
-
-orc_soap* orc_soap_new(const orc_soap_hypers* hypers) {
-    orc_soap* s = calloc(1, sizeof(orc_soap));
-    s->h = *hypers;
-    s->radial = orc_soap_radial_new(hypers->cutoff, hypers->gaussian_width, hypers->max_angular,
-                                    hypers->n_radial, hypers->spline_accuracy);
-    if (!s->radial) {
-        free(s);
-        return NULL;
-    }
-    return s;
-}
 
 void orc_soap_free(orc_soap* s) {
     if (!s) return;
     orc_radial_free(s->radial);
+    if (s->by_l) {
+        for (int l = 0; l <= s->h.max_angular; l++) orc_radial_free(s->by_l[l]);
+        free(s->by_l);
+    }
+    free(s->n_l);
     free(s);
+}
+
+orc_soap* orc_soap_new(const orc_soap_hypers* hypers) {
+    orc_soap* s = calloc(1, sizeof(orc_soap));
+    s->h = *hypers;
+    if (!hypers->channel_n) {
+        s->radial = orc_soap_radial_new(hypers->cutoff, hypers->gaussian_width, hypers->max_angular,
+                                        hypers->n_radial, hypers->spline_accuracy);
+        if (!s->radial) {
+            free(s);
+            return NULL;
+        }
+        return s;
+    }
+    /* SoapRadialIntegralCacheByAngular::new, soap/radial_integral/mod.rs:138-183 */
+    int L = hypers->max_angular + 1;
+    s->by_l = calloc(L, sizeof(orc_radial*));
+    s->n_l = calloc(L, sizeof(int));
+    for (int l = 0; l < L; l++) {
+        int n = hypers->channel_n[l];
+        s->n_l[l] = n;
+        if (n == 0) continue;
+        if (hypers->channel_points[l] > 0) {
+            s->by_l[l] = orc_radial_tabulated(n, hypers->channel_points[l], hypers->channel_positions[l],
+                                              hypers->channel_values[l], hypers->channel_derivatives[l]);
+        } else {
+            s->by_l[l] = orc_soap_radial_new(hypers->cutoff, hypers->gaussian_width, l, n, hypers->spline_accuracy);
+        }
+        if (!s->by_l[l]) {
+            orc_soap_free(s);
+            return NULL;
+        }
+    }
+    return s;
+}
+
+/* radial integral of channel l at r, in the [n_radial] layout (zero beyond the
+ * channel's own size, and for a channel outside the basis) */
+static void soap_radial(const orc_soap* s, int l, double r, double* values, double* grads) {
+    if (!s->by_l) {
+        orc_radial_compute(s->radial, l, r, values, grads);
+        return;
+    }
+    int N = s->h.n_radial, n = s->n_l[l];
+    double tv[64], tg[64];
+    for (int i = 0; i < N; i++) {
+        values[i] = 0.0;
+        if (grads) grads[i] = 0.0;
+    }
+    if (n == 0) return;
+    orc_radial_compute(s->by_l[l], l, r, tv, grads ? tg : NULL);
+    for (int i = 0; i < n; i++) {
+        values[i] = tv[i];
+        if (grads) grads[i] = tg[i];
+    }
 }
 
 /* featomic/src/calculators/soap/cutoff.rs:48-64 */
@@ -82,7 +132,7 @@ static double scaling_functions_gradient(const orc_soap* s, double r) {
 void orc_soap_self_contribution(const orc_soap* s, double* out) {
     int n = s->h.n_radial;
     double* ri = malloc(sizeof(double) * n);
-    orc_radial_compute(s->radial, 0, 0.0, ri, NULL);
+    soap_radial(s, 0, 0.0, ri, NULL);
     double z[3] = {0.0, 0.0, 1.0};
     double y[1024];
     orc_spherical_harmonics(s->h.max_angular, z, y, NULL);
@@ -115,7 +165,7 @@ static void pair_contribution(const orc_soap* s, double distance, const double v
     double* ri = dy + 3 * L2;         /* (lmax+1) N */
     double* dri = ri + (lmax + 1) * N;
     for (int l = 0; l <= lmax; l++) {
-        orc_radial_compute(s->radial, l, distance, ri + l * N, do_grad ? dri + l * N : NULL);
+        soap_radial(s, l, distance, ri + l * N, do_grad ? dri + l * N : NULL);
     }
     orc_spherical_harmonics(lmax, dir, y, do_grad ? dy : NULL);
     double f = scaling_functions(s, distance);

@@ -23,6 +23,21 @@ def _labels(raw):
     return Labels(names, values)
 
 
+def _radial_size(radial):
+    if radial.get("type") == "Tabulated":
+        return len(radial["points"][0]["values"]) if radial.get("points") else 0
+    return int(radial.get("max_radial", 0)) + 1
+
+
+def _basis_shape(basis):
+    """(L, n) of the tensor-product shape a basis is computed in: the largest angular
+    channel + 1 and the largest radial size."""
+    if "by_angular" in basis:
+        channels = basis["by_angular"]
+        return max(int(k) for k in channels) + 1, max(_radial_size(r) for r in channels.values())
+    return int(basis.get("max_angular", 0)) + 1, _radial_size(basis["radial"])
+
+
 class _TensorMapOwner:
     """Keeps a library-owned mts_tensormap_t alive while numpy views of its arrays exist."""
 
@@ -213,9 +228,7 @@ class CalculatorBase:
             if not hasattr(opts, key):
                 raise ValueError(f"unknown radial surrogate option '{key}'")
             setattr(opts, key, value)
-        params = json.loads(self.parameters)
-        L = params["basis"]["max_angular"] + 1
-        n = params["basis"]["radial"]["max_radial"] + 1
+        L, n = _basis_shape(json.loads(self.parameters)["basis"])
         value_errors = np.zeros((L, n))
         derivative_errors = np.zeros((L, n))
         report = C.featomic_amd_surrogate_report_t()

@@ -7,6 +7,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "selection.hpp"
 #include "special.hpp"
 
 namespace rascaline_amd {
@@ -178,8 +179,7 @@ Calculator::Calculator(const std::string& registry_name, const std::string& para
         // w * f(0) * Y_00 * I_n0(0)
         std::vector<double> ri(soap_.n_radial), rg(soap_.n_radial);
         if (tables_.use_spline) {
-            const SplineChannel& ch = tables_.channels[0];
-            for (int k = 0; k < soap_.n_radial; k++) ri[k] = ch.values[k];
+            evaluate_channel(tables_.channels[0], soap_.n_radial, 0.0, ri.data(), nullptr);
         } else {
             tables_.exact(0, 0.0, ri.data(), rg.data());
         }
@@ -311,6 +311,9 @@ SurrogateReport Calculator::train_radial_surrogate(const SurrogateOptions& optio
     check_unbound(*this);
     if (kind_ == Kind::LodeSphericalExpansion || kind_ == Kind::NeighborList) {
         throw Error::invalid_parameter("the radial surrogate is only available for the SOAP calculators");
+    }
+    if (soap_.padded() || soap_.radial[0].tabulated) {
+        throw Error::invalid_parameter("the radial surrogate learns one GTO basis shared by every angular channel");
     }
     check_surrogate_options(options);
     auto surrogate = std::make_unique<RadialSurrogate>(soap_.max_angular, soap_.n_radial, soap_.cutoff);
@@ -1012,7 +1015,11 @@ mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& gr
     check_unbound(*this);
     hipStream_t stream = nullptr;
     if (kind_ == Kind::NeighborList) return run_neighbor_list(batch, grads);
-    if (kind_ == Kind::ByPair) return precision == 32 ? run_by_pair<float>(batch, grads) : run_by_pair<double>(batch, grads);
+    if (kind_ == Kind::ByPair) {
+        mts_tensormap_t* tm = precision == 32 ? run_by_pair<float>(batch, grads) : run_by_pair<double>(batch, grads);
+        if (soap_.padded()) tm = restrict_to_basis(tm, soap_.lambdas, [this](int l) { return soap_.n_of(l); });
+        return tm;
+    }
     if (kind_ == Kind::LodeSphericalExpansion) {
         if (grads.strain || grads.cell) {
             throw Error::invalid_parameter("the lode spherical expansion calculator does not support gradients with respect to " +
@@ -1021,8 +1028,10 @@ mts_tensormap_t* Calculator::compute(const HostBatch& batch, const Gradients& gr
         return precision == 32 ? run_lode<float>(batch, grads, stream, nullptr)
                                : run_lode<double>(batch, grads, stream, nullptr);
     }
-    return precision == 32 ? run_soap<float>(batch, grads, stream, true, nullptr)
-                           : run_soap<double>(batch, grads, stream, true, nullptr);
+    mts_tensormap_t* tm = precision == 32 ? run_soap<float>(batch, grads, stream, true, nullptr)
+                                          : run_soap<double>(batch, grads, stream, true, nullptr);
+    if (soap_.padded()) tm = restrict_to_basis(tm, soap_.lambdas, [this](int l) { return soap_.n_of(l); });
+    return tm;
 }
 
 void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, hipStream_t stream,
@@ -1042,6 +1051,11 @@ void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, 
             run_lode<double>(batch, grads, stream, output);
         }
         return;
+    }
+    if (soap_.padded()) {
+        throw Error::invalid_parameter(
+            "an Explicit basis whose channels differ from a tensor product is computed in the padded tensor-product "
+            "layout: use featomic_calculator_compute for its TensorMap");
     }
     main_ms_sum_ = -1.0;
     if (precision == 32) {
@@ -1166,6 +1180,9 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
 
 void Calculator::positions_vjp(const HostBatch& batch, hipStream_t stream, const void* grad_values, void* out) {
     check_unbound(*this);
+    if (kind_ != Kind::LodeSphericalExpansion && soap_.padded()) {
+        throw Error::invalid_parameter("the positions vector-Jacobian product needs a tensor-product basis");
+    }
     if (!grad_values || !out) throw Error::invalid_parameter("grad_values and out must not be NULL");
     if (kind_ == Kind::NeighborList || kind_ == Kind::ByPair) {
         throw Error::invalid_parameter("the positions VJP covers the SOAP and LODE calculators");

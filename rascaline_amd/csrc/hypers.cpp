@@ -1,5 +1,6 @@
 #include "hypers.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <initializer_list>
 
@@ -101,6 +102,158 @@ void parse_basis(const Json& basis, int* max_angular, int* n_radial, double* gto
     if (*max_angular > 30) throw Error::invalid_parameter("max_angular is limited to 30 in this implementation");
 }
 
+// One SOAP radial basis (shared/basis/radial/mod.rs:10-48): Gto {max_radial, radius}
+// or Tabulated {points} (radial/tabulated.rs:82-210, the same checks and messages).
+RadialSpec parse_radial_spec(const Json& radial, double cutoff) {
+    RadialSpec spec;
+    const std::string rtype = tag(radial, "radial basis");
+    if (rtype == "Gto") {
+        deny_unknown(radial, {"type", "max_radial", "radius"});
+        spec.n = unsigned_int(radial, "max_radial", "radial basis") + 1;
+        const Json* radius = radial.find("radius");
+        if (radius && !radius->is_null()) {
+            if (!radius->is_number()) throw Error::json("invalid type for `radius`, expected f64");
+            if (radius->as_number() != cutoff) {
+                // featomic/src/calculators/soap/radial_integral/gto.rs:39-47
+                throw Error::internal("GTO radius must be the same as the cutoff radius in SOAP, or should not provided");
+            }
+        }
+    } else if (rtype == "Tabulated") {
+        deny_unknown(radial, {"type", "points"});
+        const Json& points = require(radial, "points", "radial basis");
+        if (!points.is_array()) throw Error::json("invalid type for `points`, expected a sequence");
+        struct Point {
+            double position;
+            std::vector<double> values, derivatives;
+        };
+        std::vector<Point> list;
+        auto numbers = [](const Json& p, const char* key) {
+            const Json& a = require(p, key, "spline point");
+            if (!a.is_array()) throw Error::json(std::string("invalid type for `") + key + "`, expected a sequence");
+            std::vector<double> out;
+            for (const Json& x : a.as_array()) {
+                if (!x.is_number()) throw Error::json(std::string("invalid type in `") + key + "`, expected f64");
+                out.push_back(x.as_number());
+            }
+            return out;
+        };
+        for (const Json& p : points.as_array()) {
+            if (!p.is_object()) throw Error::json("invalid type for a spline point, expected a map");
+            deny_unknown(p, {"position", "values", "derivatives"});
+            list.push_back({number(p, "position", "spline point"), numbers(p, "values"), numbers(p, "derivatives")});
+        }
+        if (list.size() < 2) throw Error::invalid_parameter("we need at least two points to define a 'Tabulated' radial basis");
+        const size_t size = list[0].values.size();
+        for (const Point& p : list) {
+            if (!std::isfinite(p.position)) {
+                throw Error::invalid_parameter("expected all points 'position' in 'Tabulated' radial basis to be finite numbers, got " +
+                                               std::to_string(p.position));
+            }
+            if (p.values.size() != size) {
+                throw Error::invalid_parameter("expected all points 'values' in 'Tabulated' radial basis to have the same size, got " +
+                                               std::to_string(p.values.size()) + " and " + std::to_string(size));
+            }
+            if (p.derivatives.size() != size) {
+                throw Error::invalid_parameter("expected all points 'derivatives' in 'Tabulated' radial basis to have the same size, got " +
+                                               std::to_string(p.derivatives.size()) + " and " + std::to_string(size));
+            }
+        }
+        std::stable_sort(list.begin(), list.end(), [](const Point& a, const Point& b) { return a.position < b.position; });
+        spec.tabulated = true;
+        spec.n = static_cast<int>(size);
+        for (const Point& p : list) {
+            spec.positions.push_back(p.position);
+            spec.values.insert(spec.values.end(), p.values.begin(), p.values.end());
+            spec.derivatives.insert(spec.derivatives.end(), p.derivatives.begin(), p.derivatives.end());
+        }
+    } else {
+        throw Error::json("unknown variant `" + rtype + "` for radial basis");
+    }
+    if (spec.n > 64) throw Error::invalid_parameter("at most 64 radial basis functions are supported in this implementation");
+    return spec;
+}
+
+// SOAP SphericalExpansionBasis (shared/basis/mod.rs:9-116): TensorProduct or Explicit
+// (with `radial_only`, the radial spectrum's {radial, spline_accuracy})
+void parse_soap_basis(const Json& basis, SoapParams& p, bool radial_only) {
+    if (radial_only) {
+        deny_unknown(basis, {"radial", "spline_accuracy"});
+        p.lambdas = {0};
+        p.radial = {parse_radial_spec(require(basis, "radial", "basis"), p.cutoff)};
+    } else {
+        const std::string type = tag(basis, "basis");
+        if (type == "TensorProduct") {
+            deny_unknown(basis, {"type", "max_angular", "radial", "spline_accuracy"});
+            const int max_angular = unsigned_int(basis, "max_angular", "basis");
+            if (max_angular > 30) throw Error::invalid_parameter("max_angular is limited to 30 in this implementation");
+            const RadialSpec spec = parse_radial_spec(require(basis, "radial", "basis"), p.cutoff);
+            for (int l = 0; l <= max_angular; l++) {
+                p.lambdas.push_back(l);
+                p.radial.push_back(spec);
+            }
+        } else if (type == "Explicit") {
+            deny_unknown(basis, {"type", "by_angular", "spline_accuracy"});
+            const Json& by = require(basis, "by_angular", "basis");
+            if (!by.is_object()) throw Error::json("invalid type for `by_angular`, expected a map");
+            std::vector<std::pair<int, RadialSpec>> channels;
+            for (const auto& kv : by.as_object()) {
+                const std::string& key = kv.first;
+                if (key.empty() || key.find_first_not_of("0123456789") != std::string::npos || key.size() > 6) {
+                    throw Error::json("invalid digit found in string `" + key + "` for an angular channel");
+                }
+                const int l = std::stoi(key);
+                if (l > 30) throw Error::invalid_parameter("max_angular is limited to 30 in this implementation");
+                for (const auto& c : channels) {
+                    if (c.first == l) throw Error::json("duplicate angular channel " + key + " in `by_angular`");
+                }
+                channels.push_back({l, parse_radial_spec(kv.second, p.cutoff)});
+            }
+            if (channels.empty()) throw Error::invalid_parameter("the Explicit basis needs at least one angular channel");
+            std::sort(channels.begin(), channels.end(),
+                      [](const std::pair<int, RadialSpec>& a, const std::pair<int, RadialSpec>& b) { return a.first < b.first; });
+            for (auto& c : channels) {
+                p.lambdas.push_back(c.first);
+                p.radial.push_back(std::move(c.second));
+            }
+            p.explicit_basis = true;
+        } else {
+            throw Error::json("unknown variant `" + type + "` for basis");
+        }
+    }
+    p.max_angular = p.lambdas.back();
+    p.n_radial = 0;
+    for (const auto& r : p.radial) p.n_radial = std::max(p.n_radial, r.n);
+    const Json* acc = basis.find("spline_accuracy");
+    p.use_spline = true;
+    p.spline_accuracy = 1e-8;  // featomic/src/calculators/shared/basis/mod.rs:39
+    if (acc) {
+        if (acc->is_null()) {
+            p.use_spline = false;
+        } else if (acc->is_number()) {
+            p.spline_accuracy = acc->as_number();
+            if (p.spline_accuracy < 0.0) {
+                throw Error::invalid_parameter("got invalid accuracy in spline (" + std::to_string(p.spline_accuracy) +
+                                               "), it must be positive");
+            }
+        } else {
+            throw Error::json("invalid type for `spline_accuracy`");
+        }
+    }
+    // (density, radial basis) pairs the SOAP radial integral supports
+    // (soap/radial_integral/mod.rs:85-113)
+    for (const auto& r : p.radial) {
+        const bool ok = (p.density == DensityKind::Gaussian && !r.tabulated) ||
+                        (p.density == DensityKind::DiracDelta && r.tabulated);
+        if (!ok) throw Error::invalid_parameter("this combination of basis and density is not supported in SOAP");
+    }
+    // a tabulated radial integral is always the spline itself; the GTO integral of
+    // an Explicit basis with several radial sizes is splined here
+    if (!p.use_spline && p.padded()) {
+        throw Error::invalid_parameter(
+            "an Explicit basis whose channels differ in size needs spline_accuracy in this implementation");
+    }
+}
+
 }  // namespace
 
 SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields, bool radial_spectrum) {
@@ -137,8 +290,15 @@ SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields, 
     const Json& density = require(root, "density", "parameters");
     std::string dt = tag(density, "density");
     if (dt == "Gaussian") {
+        deny_unknown(density, {"type", "width", "scaling", "center_atom_weight"});
         p.gaussian_width = number(density, "width", "density");
-    } else if (dt == "DiracDelta" || dt == "SmearedPowerLaw") {
+        if (!(p.gaussian_width > 1e-16) || !std::isfinite(p.gaussian_width)) {
+            throw Error::invalid_parameter("Gaussian density width must be positive and finite");
+        }
+    } else if (dt == "DiracDelta") {
+        deny_unknown(density, {"type", "scaling", "center_atom_weight"});
+        p.density = DensityKind::DiracDelta;
+    } else if (dt == "SmearedPowerLaw") {
         throw Error::invalid_parameter("this combination of basis and density is not supported in SOAP");
     } else {
         throw Error::json("unknown variant `" + dt + "` for density");
@@ -161,20 +321,9 @@ SoapParams parse_soap_params(const std::string& text, bool deny_unknown_fields, 
         if (!caw->is_number()) throw Error::json("invalid type for `center_atom_weight`, expected f64");
         p.center_atom_weight = caw->as_number();
     }
-    if (!(p.gaussian_width > 1e-16) || !std::isfinite(p.gaussian_width)) {
-        throw Error::invalid_parameter("Gaussian density width must be positive and finite");
-    }
 
     // Basis
-    const Json& basis = require(root, "basis", "parameters");
-    double gto_radius = 0.0;
-    bool has_radius = false;
-    parse_basis(basis, &p.max_angular, &p.n_radial, &gto_radius, &has_radius, &p.use_spline, &p.spline_accuracy,
-                radial_spectrum);
-    if (has_radius && gto_radius != p.cutoff) {
-        // featomic/src/calculators/soap/radial_integral/gto.rs:39-47
-        throw Error::internal("GTO radius must be the same as the cutoff radius in SOAP, or should not provided");
-    }
+    parse_soap_basis(require(root, "basis", "parameters"), p, radial_spectrum);
     return p;
 }
 

@@ -35,6 +35,20 @@ struct Error : std::runtime_error {
 
 enum class Smoothing { Step, ShiftedCosine };
 
+// featomic/src/calculators/shared/density.rs:27-60
+enum class DensityKind { Gaussian, DiracDelta };
+
+// One radial basis of the SOAP expansion (featomic's SoapRadialBasis,
+// shared/basis/radial/mod.rs:10-48): GTO with n functions, or a tabulated radial
+// integral given as Hermite spline points (shared/basis/radial/tabulated.rs:43-81),
+// sorted by position.
+struct RadialSpec {
+    bool tabulated = false;
+    int n = 0;
+    std::vector<double> positions;                // tabulated: [points]
+    std::vector<double> values, derivatives;      // tabulated: [points][n]
+};
+
 // SphericalExpansionParameters / PowerSpectrumParameters
 // (featomic/src/calculators/soap/spherical_expansion_pair.rs:37-60,
 //  featomic/src/calculators/soap/power_spectrum.rs:35-43)
@@ -50,6 +64,39 @@ struct SoapParams {
     int n_radial = 0;             // max_radial + 1
     bool use_spline = true;
     double spline_accuracy = 1e-8;
+
+    DensityKind density = DensityKind::Gaussian;
+    // the basis: its angular channels (ascending) and the radial basis of each. A
+    // TensorProduct basis has channels 0..max_angular sharing one radial basis; an
+    // Explicit basis (shared/basis/mod.rs:95-116) lists its channels, each with its
+    // own. max_angular and n_radial are the largest channel and radial size: the
+    // tensor-product shape the kernels compute.
+    bool explicit_basis = false;
+    std::vector<int> lambdas;
+    std::vector<RadialSpec> radial;
+    // the computed shape holds channels or radial functions the basis does not have
+    // (an Explicit basis with missing channels or smaller radial bases); the output
+    // keeps only the basis' own
+    bool padded() const {
+        if (static_cast<int>(lambdas.size()) != max_angular + 1) return true;
+        for (const auto& r : radial) {
+            if (r.n != n_radial) return true;
+        }
+        return false;
+    }
+    // radial size of channel l (0 when l is not a channel of the basis)
+    int n_of(int l) const {
+        for (size_t k = 0; k < lambdas.size(); k++) {
+            if (lambdas[k] == l) return radial[k].n;
+        }
+        return 0;
+    }
+    const RadialSpec* radial_of(int l) const {
+        for (size_t k = 0; k < lambdas.size(); k++) {
+            if (lambdas[k] == l) return &radial[k];
+        }
+        return nullptr;
+    }
 
     int L() const { return max_angular + 1; }
     int L2() const { return (max_angular + 1) * (max_angular + 1); }

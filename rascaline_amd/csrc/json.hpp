@@ -37,6 +37,7 @@ public:
     bool is_object() const { return kind_ == Kind::Object; }
     bool is_number() const { return kind_ == Kind::Number; }
     bool is_string() const { return kind_ == Kind::String; }
+    bool is_array() const { return kind_ == Kind::Array; }
 
     bool as_bool() const {
         if (kind_ != Kind::Bool) throw JsonError("expected a boolean");

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <string>
 
 #include "special.hpp"
@@ -191,22 +192,90 @@ void RadialTables::exact(int l, double x, double* values, double* grads) const {
     }
 }
 
+namespace {
+
+// GTO x Gaussian tables with n radial functions (every channel 0..max_angular)
+RadialTables soap_gto_tables(const SoapParams& p, int n, bool splined) {
+    RadialTables t;
+    t.lode = false;
+    t.max_angular = p.max_angular;
+    t.n = n;
+    t.stop = p.cutoff;
+    t.sigma2_atomic = p.gaussian_width * p.gaussian_width;
+    t.gto.resize(n);
+    for (int i = 0; i < n; i++) {
+        const double s = gto_width(p.cutoff, i, n);
+        t.gto[i] = 1.0 / (2.0 * s * s);
+    }
+    t.ortho = gto_orthonormalization(p.cutoff, n);
+    t.use_spline = splined;
+    return t;
+}
+
+// a channel of m radial functions widened to n (zero functions after the first m)
+SplineChannel widen(const SplineChannel& ch, int m, int n) {
+    if (m == n) return ch;
+    SplineChannel out;
+    out.positions = ch.positions;
+    const size_t K = ch.positions.size();
+    out.values.assign(K * n, 0.0);
+    out.derivatives.assign(K * n, 0.0);
+    for (size_t k = 0; k < K; k++) {
+        for (int i = 0; i < m; i++) {
+            out.values[k * n + i] = ch.values[k * m + i];
+            out.derivatives[k * n + i] = ch.derivatives[k * m + i];
+        }
+    }
+    return out;
+}
+
+}  // namespace
+
+void evaluate_channel(const SplineChannel& ch, int n, double x, double* values, double* grads) {
+    hermite(ch, n, x, values, grads);
+}
+
 RadialTables build_soap_radial(const SoapParams& p) {
+    const bool one_gto = !p.padded() && !p.radial[0].tabulated;
+    if (one_gto) {
+        // one GTO basis for channels 0..max_angular: splined or exact
+        RadialTables t = soap_gto_tables(p, p.n_radial, p.use_spline);
+        if (t.use_spline) {
+            for (int l = 0; l <= p.max_angular; l++) t.channels.push_back(build_channel(t, l, p.spline_accuracy));
+        }
+        return t;
+    }
+    // an Explicit or tabulated basis: one spline per channel, from that channel's own
+    // radial basis (a tabulated one is used as given, SoapRadialIntegralSpline::
+    // from_tabulated, soap/radial_integral/spline.rs:46-58), widened to the largest
+    // radial size; a channel missing from the basis is zero
     RadialTables t;
     t.lode = false;
     t.max_angular = p.max_angular;
     t.n = p.n_radial;
     t.stop = p.cutoff;
     t.sigma2_atomic = p.gaussian_width * p.gaussian_width;
-    t.gto.resize(t.n);
-    for (int i = 0; i < t.n; i++) {
-        const double s = gto_width(p.cutoff, i, t.n);
-        t.gto[i] = 1.0 / (2.0 * s * s);
-    }
-    t.ortho = gto_orthonormalization(p.cutoff, t.n);
-    t.use_spline = p.use_spline;
-    if (t.use_spline) {
-        for (int l = 0; l <= p.max_angular; l++) t.channels.push_back(build_channel(t, l, p.spline_accuracy));
+    t.use_spline = true;
+    std::map<int, RadialTables> gto_by_size;
+    for (int l = 0; l <= p.max_angular; l++) {
+        const RadialSpec* spec = p.radial_of(l);
+        SplineChannel ch;
+        if (!spec) {
+            ch.positions = {0.0, p.cutoff};
+            ch.values.assign(2 * static_cast<size_t>(t.n), 0.0);
+            ch.derivatives.assign(2 * static_cast<size_t>(t.n), 0.0);
+        } else if (spec->tabulated) {
+            SplineChannel given;
+            given.positions = spec->positions;
+            given.values = spec->values;
+            given.derivatives = spec->derivatives;
+            ch = widen(given, spec->n, t.n);
+        } else {
+            auto it = gto_by_size.find(spec->n);
+            if (it == gto_by_size.end()) it = gto_by_size.emplace(spec->n, soap_gto_tables(p, spec->n, true)).first;
+            ch = widen(build_channel(it->second, l, p.spline_accuracy), spec->n, t.n);
+        }
+        t.channels.push_back(std::move(ch));
     }
     return t;
 }

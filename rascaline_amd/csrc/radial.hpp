@@ -42,6 +42,10 @@ struct RadialTables {
 // added until the mean absolute or mean relative error at the midpoints falls
 // below the requested accuracy (at most 10 000 knots).
 RadialTables build_soap_radial(const SoapParams& params);
+
+// Hermite spline of one channel at x (values / grads of size n; grads may be null),
+// with the reference's interval choice (the last knot <= x)
+void evaluate_channel(const SplineChannel& channel, int n, double x, double* values, double* grads);
 RadialTables build_lode_radial(const LodeParams& params);
 
 // LODE centre contribution (featomic/src/calculators/lode/radial_integral/gto.rs:136-185)

@@ -10,6 +10,7 @@
 // default entry).
 #include "selection.hpp"
 
+#include <algorithm>
 #include <map>
 #include <unordered_map>
 
@@ -251,6 +252,45 @@ std::unique_ptr<mts_block_t> select_block(const mts_block_t& old, const LabelsDa
 }
 
 }  // namespace
+
+mts_tensormap_t* restrict_to_basis(mts_tensormap_t* full_ptr, const std::vector<int>& lambdas,
+                                   const std::function<int(int)>& n_of) {
+    std::unique_ptr<mts_tensormap_t> full(full_ptr);
+    auto out = std::make_unique<mts_tensormap_t>();
+    out->keys = LabelsData(full->keys.names, {});
+    const size_t kw = full->keys.size();
+    size_t lambda_col = kw;
+    for (size_t c = 0; c < kw; c++) {
+        if (full->keys.names[c] == "o3_lambda") lambda_col = c;
+    }
+    auto is_channel = [&](int l) { return std::find(lambdas.begin(), lambdas.end(), l) != lambdas.end(); };
+    for (size_t b = 0; b < full->blocks.size(); b++) {
+        const std::vector<int32_t> key = entry(full->keys, b);
+        const int l = lambda_col < kw ? key[lambda_col] : -1;
+        if (lambda_col < kw && !is_channel(l)) continue;
+        const mts_block_t& old = *full->blocks[b];
+        const LabelsData& props = old.properties;
+        std::vector<int32_t> kept;
+        if (props.names.size() == 1 && props.names[0] == "n") {
+            const int n_l = n_of(l);
+            for (size_t p = 0; p < props.count(); p++) {
+                if (props.values[p] < n_l) kept.push_back(props.values[p]);
+            }
+        } else if (props.names.size() == 3 && props.names[0] == "l") {
+            for (size_t p = 0; p < props.count(); p++) {
+                const int32_t* v = &props.values[3 * p];
+                const int n_l = is_channel(v[0]) ? n_of(v[0]) : 0;
+                if (v[1] < n_l && v[2] < n_l) kept.insert(kept.end(), v, v + 3);
+            }
+        } else {
+            throw Error::internal("unexpected properties for an Explicit basis");
+        }
+        out->blocks.push_back(select_block(old, old.samples, LabelsData(props.names, std::move(kept))));
+        out->keys.values.insert(out->keys.values.end(), key.begin(), key.end());
+    }
+    out->keys.finish();
+    return out.release();
+}
 
 bool has_selections(const mts_labels_t* selected_keys, const featomic_labels_selection_t& samples,
                     const featomic_labels_selection_t& properties) {

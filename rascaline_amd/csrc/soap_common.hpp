@@ -119,8 +119,9 @@ __device__ void radial_integral(const DeviceRadial<T>& rad, int l, long long e, 
         const int K = rad.n_knots[l] - 1;
         const T* pos = rad.knot_pos + rad.knot_offset[l];
         // uniform grid: direct index, then adjust against the stored knot positions
-        // so the interval is the one the reference's binary search picks
-        int k = (int)(r * K / rad.stop);
+        // so the interval is the one the reference's binary search picks (a
+        // tabulated channel may have its own, non-uniform knots)
+        int k = (int)((r - pos[0]) * K / ((double)pos[K] - pos[0]));
         if (k < 0) k = 0;
         if (k > K - 1) k = K - 1;
         while (k > 0 && (T)r < pos[k]) k--;

@@ -324,3 +324,57 @@ def test_pair_calculators_have_no_device_path(name, hypers):
         assert "host TensorMaps only" in last_error()
     finally:
         C.lib().featomic_calculator_free(calc)
+
+
+def _points(n_points, size, start=0.0, stop=4.0):
+    return [{"position": start + (stop - start) * k / max(n_points - 1, 1), "values": [0.1 * k] * size,
+             "derivatives": [0.0] * size} for k in range(n_points)]
+
+
+TABULATED = {"type": "Tabulated", "points": _points(5, 3)}
+
+
+@pytest.mark.parametrize(
+    "density, basis, message",
+    [
+        # (density, radial basis) pairs of soap/radial_integral/mod.rs:85-113
+        ({"type": "DiracDelta"}, {"type": "TensorProduct", "max_angular": 2, "radial": {"type": "Gto", "max_radial": 3}},
+         "this combination of basis and density is not supported in SOAP"),
+        ({"type": "Gaussian", "width": 0.3}, {"type": "TensorProduct", "max_angular": 2, "radial": TABULATED},
+         "this combination of basis and density is not supported in SOAP"),
+        # tabulated.rs:171-208
+        ({"type": "DiracDelta"}, {"type": "TensorProduct", "max_angular": 2,
+                                  "radial": {"type": "Tabulated", "points": _points(1, 3)}},
+         "we need at least two points to define a 'Tabulated' radial basis"),
+        ({"type": "DiracDelta"}, {"type": "TensorProduct", "max_angular": 2,
+                                  "radial": {"type": "Tabulated", "points": _points(3, 3) + _points(2, 2)}},
+         "expected all points 'values' in 'Tabulated' radial basis to have the same size"),
+        ({"type": "DiracDelta"}, {"type": "Explicit", "by_angular": {"x": TABULATED}}, "json error"),
+        ({"type": "DiracDelta"}, {"type": "Explicit", "by_angular": {}}, "at least one angular channel"),
+        ({"type": "Gaussian", "width": 0.3},
+         {"type": "Explicit", "by_angular": {"0": {"type": "Gto", "max_radial": 3}, "2": {"type": "Gto", "max_radial": 5}},
+          "spline_accuracy": None},
+         "needs spline_accuracy"),
+    ],
+)
+def test_invalid_basis_and_density(density, basis, message):
+    hypers = dict(SOAP, density=density, basis=basis)
+    calc = make("spherical_expansion", hypers)
+    assert not calc
+    assert message in last_error(), last_error()
+
+
+@pytest.mark.parametrize("name", ["spherical_expansion", "soap_power_spectrum", "spherical_expansion_by_pair"])
+def test_explicit_and_tabulated_bases_are_accepted(name):
+    """DiracDelta + Tabulated (TensorProduct or Explicit) and Explicit GTO bases build
+    calculators; their parameters round-trip through featomic_calculator_parameters."""
+    for density, basis in (
+        ({"type": "DiracDelta"}, {"type": "TensorProduct", "max_angular": 3, "radial": TABULATED}),
+        ({"type": "DiracDelta", "center_atom_weight": 0.5}, {"type": "Explicit", "by_angular": {"0": TABULATED, "3": TABULATED}}),
+        ({"type": "Gaussian", "width": 0.3},
+         {"type": "Explicit", "by_angular": {"1": {"type": "Gto", "max_radial": 3}, "0": {"type": "Gto", "max_radial": 5}}}),
+    ):
+        hypers = dict(SOAP, density=density, basis=basis)
+        calc = make(name, hypers)
+        assert calc, last_error()
+        C.lib().featomic_calculator_free(calc)
