@@ -202,7 +202,7 @@ featomic_status_t featomic_amd_calculator_get_tuning(featomic_calculator_t* calc
  * spline stays in use otherwise. */
 typedef struct featomic_amd_surrogate_options_t {
     int64_t n_samples;       /* training distances, uniform on [0, cutoff) (default 1000000) */
-    int64_t steps;           /* Adam steps (default 6000) */
+    int64_t steps;           /* Adam steps (default 20000) */
     int32_t batch;           /* distances per Adam step (default 2048) */
     double learning_rate;    /* peak Adam rate, cosine decay to 1e-3 of it (default 3e-3) */
     double ridge;            /* output-layer ridge, relative to trace(G) / 65 (default 1e-12) */

@@ -45,9 +45,10 @@ class _TensorMapOwner:
         self.ptr = ptr
 
     def __del__(self):
-        if self.ptr:
-            C.lib().mts_tensormap_free(self.ptr)
-            self.ptr = None
+        # at interpreter exit the library may already be unloaded
+        if self.ptr and C._LIB is not None:
+            C._LIB.mts_tensormap_free(self.ptr)
+        self.ptr = None
 
 
 def _read_block(ptr, owner):

@@ -30,7 +30,7 @@ namespace rascaline_amd {
 
 struct SurrogateOptions {
     long long n_samples = 1000000;
-    long long steps = 6000;
+    long long steps = 20000;
     int batch = 2048;
     double learning_rate = 3e-3;
     double ridge = 1e-12;

@@ -166,7 +166,7 @@ def test_profiling_and_logging_entry_points():
 def test_radial_surrogate_host_checks():
     opts = C.featomic_amd_surrogate_options_t()
     assert C.lib().featomic_amd_surrogate_options_default(ctypes.byref(opts)) == FEATOMIC_SUCCESS
-    assert (opts.n_samples, opts.steps, opts.batch, opts.n_validation) == (1000000, 6000, 2048, 20001)
+    assert (opts.n_samples, opts.steps, opts.batch, opts.n_validation) == (1000000, 20000, 2048, 20001)
     assert (opts.learning_rate, opts.ridge, opts.tolerance, opts.seed) == (3e-3, 1e-12, 1e-3, 1)
     calc = make("soap_power_spectrum", SOAP)
     try:

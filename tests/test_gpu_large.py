@@ -14,10 +14,9 @@
   l <= 8, FP32, network trained on 1e6 distances and accepted by the spline judge.
   - position gradients (about 9.4 M rows, 110 GB) translation invariant per
     (l, block) within 1e-4; values and the first rows of every gradient block
-    within 1e-3 of the spline path per (l, block) (the judge's tolerance on the
-    radial integral);
+    within 1e-4 of the spline path per (l, block);
   - values on a 10 000-atom cell of the same density against the FP64 oracle,
-    within 1e-3 per (l, block).
+    within 1e-4 per (l, block).
 """
 
 import gc
@@ -188,7 +187,7 @@ def test_surrogate_large_cell_full_size():
         worst_v = max([worst_v] + list(ev.values()))
         worst_g = max([worst_g] + list(eg.values()))
     say(f"    surrogate vs spline (worst block): values {worst_v:.2e}, first gradient rows {worst_g:.2e}")
-    assert worst_v <= 1e-3 and worst_g <= 1e-3
+    assert worst_v <= 1e-4 and worst_g <= 1e-4
     calc.use_radial_surrogate(True)
     del dev
     gc.collect()
@@ -203,4 +202,4 @@ def test_surrogate_large_cell_full_size():
         e = rb["values"].reshape(a.shape)
         worst = max(worst, float(np.abs(a - e).max()) / float(np.abs(e).max()))
     say(f"    10 000-atom cell, network vs FP64 oracle (worst block): values {worst:.2e}")
-    assert worst <= 1e-3
+    assert worst <= 1e-4

@@ -6,13 +6,18 @@
   against the oracle's FP64 forward: |y - y_ref| <= 1e-4 max|y_ref|.
 * Training on the GTO radial integral of the large-cell workload (cutoff 6 A,
   Gaussian width 0.3 A, 12 radial functions, l <= 8): accepted by the spline judge
-  with every per-(l, n) error <= 1e-3 of that channel's largest magnitude, and at
-  least 10x better than the same network without the Adam phase.
+  with every per-(l, n) error <= 2e-4 (values) and 5e-4 (derivatives) of that
+  channel's largest magnitude, and at least 10x better than the same network
+  without the Adam phase.
 * Spherical expansion with position gradients through the network against the FP64
-  oracle (block-relative error <= 1e-3); switching back to the spline restores the
-  plain results (FP32 within 1e-4 block-relative, FP64 within 1e-10 (|b| + max|b|));
-  the spline fallback when the judge rejects the network (bit-identical to a
-  calculator without one).
+  oracle (block-relative error <= 1e-4, values and gradients); switching back to
+  the spline restores the plain results (FP32 within 1e-4 block-relative, FP64
+  within 1e-10 (|b| + max|b|)); the spline fallback when the judge rejects the
+  network (bit-identical to a calculator without one).
+* What the network gains over the method without it: with spline_accuracy null the
+  calculator evaluates the exact GTO integral (FP64 1F1 per entry, l and n); the
+  network replaces that evaluation at least 3x faster on a 3000-atom cell, within
+  1e-4 of the FP64 oracle, and in the time of the spline.
 """
 
 import ctypes
@@ -123,7 +128,7 @@ def test_surrogate_training_accepted_and_gain():
     calc, report = trained_calculator()
     assert report["accepted"]
     assert report["value_errors"].shape == (9, 12)
-    assert report["max_value_error"] <= 1e-3 and report["max_derivative_error"] <= 1e-3
+    assert report["max_value_error"] <= 2e-4 and report["max_derivative_error"] <= 5e-4
     # the same network and output-layer solve without the Adam phase
     untrained = ra.SphericalExpansion(**LARGE_CELL)
     base = untrained.train_radial_surrogate(steps=0, tolerance=1.0)
@@ -154,7 +159,7 @@ def test_spherical_expansion_through_surrogate(precision):
     ev = block_relative_error(tm, expected)
     eg = block_relative_error(tm, expected, "positions")
     say(f"  block-relative error: values {ev:.2e}, gradients {eg:.2e}")
-    assert ev <= 1e-3 and eg <= 1e-3
+    assert ev <= 1e-4 and eg <= 1e-4
     # switching back to the spline gives the plain computation
     calc.use_radial_surrogate(False)
     try:
@@ -191,3 +196,53 @@ def test_judge_rejects_and_spline_stays():
     for ba, bb in zip(out.blocks(), ref.blocks()):
         assert np.array_equal(ba.values, bb.values)
         assert np.array_equal(ba.gradient("positions").values, bb.gradient("positions").values)
+
+
+def test_gain_over_the_exact_radial_integral():
+    """The network in place of the exact radial integral (spline_accuracy null): faster
+    by at least 3x on the spherical expansion + gradients of a 3000-atom cell, within
+    1e-4 of the FP64 oracle; and in the time of the spline."""
+    import copy
+    import time
+
+    import torch
+    from rascaline_amd.device import DeviceSystems, compute_device
+
+    exact_hypers = copy.deepcopy(LARGE_CELL)
+    exact_hypers["basis"]["spline_accuracy"] = None
+    rng = np.random.default_rng(1)
+    pos, types, cell = random_system(rng, 3000, "periodic", (3000 / 0.1) ** (1 / 3), types=(1, 6, 7, 8), min_dist=0.7)
+    batch = DeviceSystems([(pos, types, cell)])
+
+    def best_ms(calc, reps=5):
+        compute_device(calc, batch, ["positions"])
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            compute_device(calc, batch, ["positions"])
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return 1e3 * best
+
+    exact = ra.SphericalExpansion(**exact_hypers)
+    exact.set_precision(32)
+    network = ra.SphericalExpansion(**exact_hypers)
+    network.set_precision(32)
+    report = network.train_radial_surrogate()
+    assert report["accepted"]
+    spline, _ = trained_calculator()
+    spline.use_radial_surrogate(False)
+    try:
+        t_exact, t_network, t_spline = best_ms(exact), best_ms(network), best_ms(spline)
+    finally:
+        spline.use_radial_surrogate(True)
+    systems = large_cell_systems(5)
+    expected = oracle_to_tensormap(orc.soap_spherical_expansion(systems, LARGE_CELL, ["positions"]))
+    tm = network.compute([ra.System(s.types, s.positions, s.cell) for s in systems], gradients=["positions"])
+    ev, eg = block_relative_error(tm, expected), block_relative_error(tm, expected, "positions")
+    say(f"  3000-atom cell: exact {t_exact:.2f} ms, network {t_network:.2f} ms, spline {t_spline:.2f} ms; "
+        f"network error values {ev:.2e}, gradients {eg:.2e}")
+    assert t_network * 3.0 <= t_exact
+    assert t_network <= 1.25 * t_spline
+    assert ev <= 1e-4 and eg <= 1e-4
