@@ -215,6 +215,12 @@ def main():
     parser.add_argument("--streams", type=int, default=4,
                         help="calculators on their own streams that consecutive chunks alternate between")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--forms", choices=["both", "plain"], default="both",
+                        help="plain: leave out the fast form (profiling runs)")
+    parser.add_argument("--no-graph", action="store_true", help="leave out the batch-graph timing")
+    parser.add_argument("--no-end-to-end", action="store_true", help="leave out the featomic_calculator_compute line")
+    parser.add_argument("--gradient-window", type=int, default=None,
+                        help="use this gradient window shape instead of the autotuner's choice")
     parser.add_argument("--workload", choices=["soap", "lode", "bulk", "surrogate", "vjp", "lode-vjp", "by-pair"],
                         default="soap",
                         help="soap: the headline (SOAP power spectrum + gradients); lode: LODE spherical "
@@ -285,8 +291,12 @@ def main():
         calc = make_calculator(fast)
         name = "fast" if fast else "plain"
         # launch shapes chosen by the autotuner on the first chunk (outside the timed steps)
-        tuning_report = calc.autotune(batches[0], repeats=2)
-        log(f"{name}: autotuned {tuning_report['chosen']}")
+        if args.gradient_window is None:
+            tuning_report = calc.autotune(batches[0], repeats=2)
+            log(f"{name}: autotuned {tuning_report['chosen']}")
+        else:
+            calc.tuning = {"expansion_entries": 8, "gradient_window": args.gradient_window}
+            tuning_report = {"chosen": calc.tuning}
         # consecutive chunks alternate between calculators on their own streams, so that
         # one chunk's neighbour list, expansion and values overlap the previous chunk's
         # write-bound gradient rows (each calculator's buffers are reused every
@@ -360,7 +370,7 @@ def main():
     log(f"FP64 oracle of the first 24 structures for the accuracy checks: {time.perf_counter() - t0:.1f} s")
     # the line reports the faster form among those within the metric's bar (1e-4 relative with
     # the floor the FP32 arithmetic itself needs, 3e-5 of the block's scale) and lists both
-    fast_available = args.precision == 32
+    fast_available = args.precision == 32 and args.forms == "both"
     plain_ms, plain_rate, plain_roofline, rows_total, samples_total = measure(False)
     forms = {"plain": (plain_ms, plain_rate, plain_roofline)}
     if fast_available:
@@ -409,12 +419,12 @@ def main():
 
     eligible = [f for f in forms if within_bar(forms[f][2])] or ["plain"]
     best = max(eligible, key=lambda f: forms[f][1])
-    graph_ms, graph_rate = measure_graph(best == "fast", forms[best][2]["tuning"])
+    graph_ms, graph_rate = (None, 0.0) if args.no_graph else measure_graph(best == "fast", forms[best][2]["tuning"])
     ms_per_step, value, roofline = forms[best]
     launch = "eager"
     if graph_rate > value:
         ms_per_step, value, launch = graph_ms, graph_rate, "graph"
-    e2e = end_to_end(systems, args.precision)
+    e2e = None if args.no_end_to_end else end_to_end(systems, args.precision)
     log(f"end to end: {e2e}")
 
     baseline = None
@@ -439,9 +449,8 @@ def main():
             "workload": f"{len(systems)} uneven structures (8-256 atoms; half periodic cubic/triclinic at "
                         f"0.08-0.12 A^-3, half clusters in 15 A boxes; 4 types), SOAP power spectrum + position "
                         f"gradients, rc 5.0, n 8, l<=6",
-            "form": (("plain FP32" if best == "plain" else
-                      "fast: learned radial integral + power-spectrum contractions on MFMA in bfloat16 hi+lo")
-                     if fast_available else "FP64"),
+            "form": ("FP64" if args.precision == 64 else "plain FP32" if best == "plain" else
+                     "fast: learned radial integral + power-spectrum contractions on MFMA in bfloat16 hi+lo"),
             "atoms": n_atoms,
             "launch": ("one hipGraph launch per step (featomic_amd_batch_graph)" if launch == "graph" else
                        "chunk by chunk (featomic_amd_compute_device)"),
@@ -458,7 +467,8 @@ def main():
                          "accuracy": rl["accuracy"]}
                   for name, (ms, rate, rl) in forms.items()},
     }
-    result["forms"][best]["graph"] = {"ms_per_step": graph_ms, "value": graph_rate, "unit": "atoms/s"}
+    if graph_ms is not None:
+        result["forms"][best]["graph"] = {"ms_per_step": graph_ms, "value": graph_rate, "unit": "atoms/s"}
     if "fast" in forms:
         result["forms"]["fast"]["speedup"] = forms["fast"][1] / forms["plain"][1]
         result["forms"]["fast"]["radial_surrogate"] = surrogate
