@@ -84,10 +84,15 @@ typedef struct featomic_amd_device_output_t {
     int32_t precision;                 /* FEATOMIC_AMD_FLOAT32 or FEATOMIC_AMD_FLOAT64 */
     const int64_t* sample_label_offsets;    /* host, first sample label of each block, or NULL */
     const int64_t* gradient_label_offsets;  /* host, first gradient label of each block, or NULL */
+    /* cell gradients (when requested): block b holds n_samples[b] x [3 (xyz)][3 (abc)] rows of
+     * `properties` values from cell_gradients + cell_gradient_offsets[b], one per sample, in
+     * the layout of the host TensorMap's "cell" gradient */
+    const void* cell_gradients;             /* device, or NULL */
+    const int64_t* cell_gradient_offsets;   /* host, in elements, or NULL */
 } featomic_amd_device_output_t;
 
 /* Compute the calculator on a device-resident batch. `gradients` may hold
- * "positions". The result stays valid until the next call on this calculator
+ * "positions" and (SOAP calculators) "cell". The result stays valid until the next call on this calculator
  * or its release. All work is queued on `stream` (a hipStream_t, NULL for the
  * default stream); the call synchronises the stream once after the metadata
  * pass (to size the outputs) and returns without waiting for the kernels that

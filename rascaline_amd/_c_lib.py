@@ -121,6 +121,8 @@ class featomic_amd_device_output_t(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("sample_label_offsets", ctypes.POINTER(ctypes.c_int64)),
         ("gradient_label_offsets", ctypes.POINTER(ctypes.c_int64)),
+        ("cell_gradients", ctypes.c_void_p),
+        ("cell_gradient_offsets", ctypes.POINTER(ctypes.c_int64)),
     ]
 
 

@@ -18,8 +18,14 @@ on forces (create_graph=True) reaches the weights applied to the descriptor.
 (their positions are ignored); `positions` holds every atom's position, systems in
 order. The values keep the calculator's precision (float32 or float64).
 
-As in featomic-torch, second derivatives with respect to the positions are not
-computed: the double backward contributes nothing to positions.grad and warns once.
+With `cell` ([systems][3][3] float64) the values are also differentiable with
+respect to the cells, at fixed Cartesian positions, as featomic-torch's CellGrad
+makes them (featomic-torch/src/autograd.cpp:425-560): the forward keeps the
+calculator's cell gradients dX/dH (SOAP calculators), the backward contracts them
+with dA/dX per system, and that backward is differentiable with respect to dA/dX.
+
+As in featomic-torch, second derivatives with respect to the positions or the cell
+are not computed: the double backward contributes nothing to them and warns once.
 """
 
 import warnings
@@ -103,6 +109,72 @@ def positions_jvp(calculator, systems, device_systems, tangent):
 
 
 _WARNED = [False]
+_WARNED_CELL = [False]
+
+
+def _segment_sums(per_row, counts):
+    """Sums of consecutive runs of rows ([rows, ...] -> [len(counts), ...]) as differences
+    of an FP64 running sum at the run ends: a fixed order, so repeated calls agree bit
+    for bit."""
+    import torch
+
+    ends = counts.cumsum(0)
+    flat = per_row.reshape(per_row.shape[0], -1).double()
+    running = torch.cat([torch.zeros((1, flat.shape[1]), dtype=torch.float64, device=flat.device), flat.cumsum(0)])
+    return (running[ends] - running[ends - counts]).reshape((len(counts),) + tuple(per_row.shape[1:]))
+
+
+class _CellBlocks:
+    """dX/dH of every block ([samples, 3, 3, properties]) with the system of each sample
+    (samples are sorted by system)."""
+
+    def __init__(self, result, n_systems, device):
+        import torch
+
+        self.n_systems = n_systems
+        self.blocks = []
+        for b in range(result.n_blocks):
+            ns, npr = result.n_samples[b], result.properties[b]
+            if ns == 0:
+                continue
+            grad = result.cell_gradients(b).double().clone()
+            systems = result.sample_labels(b)[:, 0].long().clone()
+            counts = torch.bincount(systems, minlength=n_systems)
+            self.blocks.append((result.value_offset(b), ns, npr, grad, counts))
+
+
+class _CellGrad(torch.autograd.Function):
+    """dA/dH from dA/dX (featomic-torch's CellGrad, autograd.cpp:428-489): per system,
+    Σ over its samples and properties of dA/dX[sample] · dX/dH[sample]. Its backward gives
+    dB/d(dA/dX)[sample] = dB/d(dA/dH)[system of sample] : dX/dH[sample] (:493-560)."""
+
+    @staticmethod
+    def forward(ctx, cell, grad_values, cell_blocks):
+        ctx.save_for_backward(cell, grad_values)
+        ctx.cell_blocks = cell_blocks
+        out = torch.zeros((cell_blocks.n_systems, 3, 3), dtype=torch.float64, device=grad_values.device)
+        for off, ns, npr, grad, counts in cell_blocks.blocks:
+            w = grad_values[off:off + ns * npr].reshape(ns, 1, 1, npr).double()
+            per_sample = (grad * w).sum(dim=3)  # [samples, 3, 3]
+            out += _segment_sums(per_sample, counts)
+        return out.to(cell.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_grad):
+        cell, grad_values = ctx.saved_tensors
+        if ctx.needs_input_grad[0] and not _WARNED_CELL[0]:
+            _WARNED_CELL[0] = True
+            warnings.warn("second derivatives with respect to cell matrix are not implemented and are not "
+                          "accumulated during backward() calls", stacklevel=2)
+        d_grad_values = None
+        if ctx.needs_input_grad[1]:
+            d_grad_values = torch.zeros_like(grad_values)
+            gg = grad_grad.double()
+            for off, ns, npr, grad, counts in ctx.cell_blocks.blocks:
+                systems = torch.repeat_interleave(torch.arange(len(counts), device=counts.device), counts)
+                rows = (grad * gg[systems].reshape(ns, 3, 3, 1)).sum(dim=(1, 2))  # [samples, properties]
+                d_grad_values[off:off + ns * npr] = rows.reshape(-1).to(grad_values.dtype)
+        return None, d_grad_values, None
 
 
 class _PositionsGrad(torch.autograd.Function):
@@ -133,9 +205,11 @@ class _PositionsGrad(torch.autograd.Function):
 
 class _SoapValues(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, positions, calculator, systems, holder):
-        ds = DeviceSystems(systems, positions=positions.detach().contiguous())
-        result = compute_device(calculator, ds)
+    def forward(ctx, positions, cell, calculator, systems, holder):
+        cells = None if cell is None else cell.detach().double().cpu().numpy().reshape(len(systems), 3, 3)
+        ds = DeviceSystems(systems, positions=positions.detach().contiguous(), cells=cells)
+        want_cell = cell is not None and cell.requires_grad
+        result = compute_device(calculator, ds, ["cell"] if want_cell else [])
         values = torch.empty(result.value_size, dtype=torch.float32 if result.precision == 32 else torch.float64,
                              device=positions.device)
         for b in range(result.n_blocks):
@@ -143,8 +217,9 @@ class _SoapValues(torch.autograd.Function):
             v = result.values(b)
             values[off:off + v.numel()] = v.reshape(-1)
         holder.layout = BlockLayout(result)
+        ctx.cell_blocks = _CellBlocks(result, len(systems), positions.device) if want_cell else None
         # saved for torch's in-place modification check; the backward reads the same storage
-        ctx.save_for_backward(positions)
+        ctx.save_for_backward(positions, cell)
         ctx.calculator = calculator
         ctx.systems = systems
         ctx.device_systems = ds
@@ -152,15 +227,21 @@ class _SoapValues(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_values):
-        (positions,) = ctx.saved_tensors  # raises if the positions were modified in place since the forward
-        grad = _PositionsGrad.apply(positions, grad_values, ctx.calculator, ctx.systems, ctx.device_systems)
-        return grad, None, None, None
+        positions, cell = ctx.saved_tensors  # raises if they were modified in place since the forward
+        grad_positions = grad_cell = None
+        if ctx.needs_input_grad[0]:
+            grad_positions = _PositionsGrad.apply(positions, grad_values, ctx.calculator, ctx.systems,
+                                                  ctx.device_systems)
+        if ctx.needs_input_grad[1]:
+            grad_cell = _CellGrad.apply(cell, grad_values, ctx.cell_blocks).reshape(cell.shape)
+        return grad_positions, grad_cell, None, None, None
 
 
-def soap_values(calculator, systems, positions):
+def soap_values(calculator, systems, positions, cell=None):
     """Values of a calculator (SOAP spherical expansion or power spectrum, LODE
     spherical expansion) as one flat tensor plus its BlockLayout; differentiable with
-    respect to `positions`, twice with respect to what multiplies the values."""
+    respect to `positions` and (SOAP, when given) `cell` ([systems][3][3] float64),
+    twice with respect to what multiplies the values."""
     holder = _Holder()
-    values = _SoapValues.apply(positions, calculator, systems, holder)
+    values = _SoapValues.apply(positions, cell, calculator, systems, holder)
     return values, holder.layout

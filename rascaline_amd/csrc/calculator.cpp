@@ -793,7 +793,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
 
     // ---- describe the blocks
     std::vector<int32_t> keys;
-    std::vector<int64_t> out_ns, out_nr, out_voff, out_goff, out_props, out_slab, out_glab;
+    std::vector<int64_t> out_ns, out_nr, out_voff, out_goff, out_props, out_slab, out_glab, out_coff;
     struct BlockRef {
         int b, l;
     };
@@ -812,6 +812,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             out_props.push_back(P);
             out_slab.push_back(block_sample_base[b]);
             out_glab.push_back(block_row_base[b]);
+            out_coff.push_back(block_sample_base[b] * 9 * P);
             refs.push_back({b, 0});
         } else {
             for (int l = 0; l < L; l++) {
@@ -828,6 +829,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 // the L blocks of one (center type, neighbour type) pair share its labels
                 out_slab.push_back(block_sample_base[b]);
                 out_glab.push_back(block_row_base[b]);
+                out_coff.push_back(cs_base.empty() ? 0 : cs_base[b * L + l]);
                 refs.push_back({b, l});
             }
         }
@@ -843,6 +845,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         device_result_.block_properties = out_props;
         device_result_.sample_label_offsets = out_slab;
         device_result_.gradient_label_offsets = out_glab;
+        device_result_.cell_gradient_offsets = out_coff;
         output->n_blocks = refs.size();
         output->keys = device_result_.keys.data();
         output->key_size = device_result_.key_size;
@@ -857,6 +860,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         output->gradient_labels = grad_labels;
         output->sample_label_offsets = device_result_.sample_label_offsets.data();
         output->gradient_label_offsets = grad_labels ? device_result_.gradient_label_offsets.data() : nullptr;
+        output->cell_gradients = d_cell;
+        output->cell_gradient_offsets = d_cell ? device_result_.cell_gradient_offsets.data() : nullptr;
         output->precision = sizeof(T) == 4 ? 32 : 64;
         if (record_) {
             // the batch's output description, kept for the graph's replays (the
@@ -870,6 +875,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             record_->block_properties = device_result_.block_properties;
             record_->sample_label_offsets = device_result_.sample_label_offsets;
             record_->gradient_label_offsets = device_result_.gradient_label_offsets;
+            record_->cell_gradient_offsets = device_result_.cell_gradient_offsets;
             record_->output = *output;
         }
     }
@@ -1001,6 +1007,7 @@ void BatchRecord::describe(featomic_amd_device_output_t* out) const {
     out->block_properties = block_properties.data();
     out->sample_label_offsets = sample_label_offsets.data();
     out->gradient_label_offsets = output.gradient_label_offsets ? gradient_label_offsets.data() : nullptr;
+    out->cell_gradient_offsets = output.cell_gradients ? cell_gradient_offsets.data() : nullptr;
 }
 
 static void check_unbound(const Calculator& c) {
@@ -1045,6 +1052,10 @@ void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, 
                                        " calculator returns host TensorMaps only (use featomic_calculator_compute)");
     }
     if (kind_ == Kind::LodeSphericalExpansion) {
+        if (grads.strain || grads.cell) {
+            throw Error::invalid_parameter("the lode spherical expansion calculator does not support gradients with respect to " +
+                                           std::string(grads.cell ? "cell" : "strain"));
+        }
         if (precision == 32) {
             run_lode<float>(batch, grads, stream, output);
         } else {

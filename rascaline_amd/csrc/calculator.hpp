@@ -61,7 +61,7 @@ struct BatchRecord {
     std::vector<int32_t> keys;
     size_t key_size = 0;
     std::vector<int64_t> n_samples, n_rows, value_offsets, gradient_offsets, block_properties;
-    std::vector<int64_t> sample_label_offsets, gradient_label_offsets;
+    std::vector<int64_t> sample_label_offsets, gradient_label_offsets, cell_gradient_offsets;
     featomic_amd_device_output_t output{};
     void describe(featomic_amd_device_output_t* out) const;
 };
@@ -193,7 +193,7 @@ private:
         std::vector<int32_t> keys;
         size_t key_size = 0;
         std::vector<int64_t> n_samples, n_rows, value_offsets, gradient_offsets, block_properties;
-        std::vector<int64_t> sample_label_offsets, gradient_label_offsets;
+        std::vector<int64_t> sample_label_offsets, gradient_label_offsets, cell_gradient_offsets;
         const void* values = nullptr;
         const void* gradients = nullptr;
         const int32_t* sample_labels = nullptr;

@@ -486,7 +486,7 @@ featomic_status_t featomic_amd_compute_device(featomic_calculator_t* calculator,
         const rascaline_amd::HostBatch batch = device_batch(systems, "systems");
         auto& impl = *calculator->impl;
         const auto grads = parse_gradients(gradients, gradients_count, impl.kind());
-        if (grads.strain || grads.cell) throw Error::invalid_parameter("featomic_amd_compute_device supports positions gradients only");
+        if (grads.strain) throw Error::invalid_parameter("featomic_amd_compute_device supports positions and cell gradients");
         impl.compute_device(batch, grads, static_cast<hipStream_t>(stream), output);
     });
 }

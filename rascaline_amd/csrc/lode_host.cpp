@@ -441,6 +441,8 @@ mts_tensormap_t* Calculator::run_lode(const HostBatch& batch, const Gradients& g
         output->gradient_labels = nullptr;
         output->sample_label_offsets = nullptr;
         output->gradient_label_offsets = nullptr;
+        output->cell_gradients = nullptr;
+        output->cell_gradient_offsets = nullptr;
         output->precision = sizeof(T) == 4 ? 32 : 64;
         return nullptr;
     }

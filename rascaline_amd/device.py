@@ -23,9 +23,10 @@ class DeviceSystems:
     """Systems given as (positions [n][3], types [n], cell [3][3]) tuples, with the
     positions copied once to device memory (float64, torch allocation). With
     `positions` (a contiguous float64 GPU tensor [total atoms, 3]), that tensor is
-    used in place of the tuples' positions, without a copy."""
+    used in place of the tuples' positions, without a copy; `cells` ([systems][3][3])
+    replaces the tuples' cells."""
 
-    def __init__(self, systems, device="cuda:0", positions=None):
+    def __init__(self, systems, device="cuda:0", positions=None, cells=None):
         import torch
 
         if positions is None:
@@ -39,7 +40,9 @@ class DeviceSystems:
             self.positions = positions
         self.n_atoms = int(self.positions.shape[0])
         self.types = np.ascontiguousarray(np.concatenate([np.asarray(s[1]) for s in systems]).astype(np.int32))
-        self.cells = np.ascontiguousarray(np.stack([np.asarray(s[2], dtype=np.float64).reshape(9) for s in systems]))
+        if cells is None:
+            cells = [s[2] for s in systems]
+        self.cells = np.ascontiguousarray(np.stack([np.asarray(c, dtype=np.float64).reshape(9) for c in cells]))
         self.offsets = np.zeros(len(systems) + 1, dtype=np.int64)
         self.offsets[1:] = np.cumsum([len(s[1]) for s in systems])
         self.struct = C.featomic_amd_device_systems_t()
@@ -109,6 +112,15 @@ class DeviceResult:
             raise ValueError("no position gradients were computed")
         off = int(self._out.gradient_offsets[b])
         return _view(self._out.gradients + off * self._item, (self.n_gradient_rows[b], 3, self.properties[b]),
+                     self._typestr, self._device)
+
+    def cell_gradients(self, b):
+        """[samples][3 (xyz)][3 (abc)][properties] cell gradients of block b (the host
+        TensorMap's "cell" gradient, one row per sample)."""
+        if not self._out.cell_gradients:
+            raise ValueError("no cell gradients were computed")
+        off = int(self._out.cell_gradient_offsets[b])
+        return _view(self._out.cell_gradients + off * self._item, (self.n_samples[b], 3, 3, self.properties[b]),
                      self._typestr, self._device)
 
     def sample_labels(self, b):

@@ -179,3 +179,104 @@ def test_force_loss_vs_oracle(precision):
         assert np.all(np.abs(got - expected) <= bar * (np.abs(expected) + scale)), b
         worst = max(worst, float(np.abs(got - expected).max() / scale))
     say(f"force-loss weight gradient vs oracle rows, FP{precision}: {worst:.1e}")
+
+
+def periodic_systems(seed):
+    rng = np.random.default_rng(seed)
+    return [random_system(rng, 7, "triclinic", 5.5, types=(1, 6)), random_system(rng, 6, "periodic", 5.0, types=(1, 8))]
+
+
+def cell_tensor(systems, requires_grad=True):
+    import torch
+
+    return torch.tensor(np.stack([np.asarray(s[2], dtype=np.float64) for s in systems]), dtype=torch.float64,
+                        device="cuda", requires_grad=requires_grad)
+
+
+@pytest.mark.parametrize("kind", ["power_spectrum", "spherical_expansion"])
+def test_cell_gradcheck(kind):
+    """featomic-torch's CellGrad: values differentiable with respect to the cells at
+    fixed Cartesian positions; gradcheck against central finite differences of the
+    forward (FP64), with and without the positions."""
+    import torch
+
+    say("cell gradcheck", kind)
+    systems = periodic_systems(21)
+    calc = ra.SoapPowerSpectrum(**SOAP) if kind == "power_spectrum" else ra.SphericalExpansion(**SOAP)
+    calc.set_precision(64)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda")
+    cell = cell_tensor(systems)
+    values, _ = soap_values(calc, systems, pos, cell)
+    w = torch.tensor(np.random.default_rng(4).normal(size=values.shape), dtype=torch.float64, device="cuda")
+
+    def energy_of_cell(c):
+        v, _ = soap_values(calc, systems, pos, c)
+        return (v * w).sum()
+
+    assert torch.autograd.gradcheck(energy_of_cell, (cell,), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+    pos_g = pos.clone().requires_grad_(True)
+
+    def energy_of_both(p, c):
+        v, _ = soap_values(calc, systems, p, c)
+        return (v * w).sum()
+
+    assert torch.autograd.gradcheck(energy_of_both, (pos_g, cell), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_cell_backward_vs_oracle():
+    """dA/dH per system = Σ over its samples of W · dX/dH with the FP64 oracle's cell
+    gradients (CellGrad::forward, autograd.cpp:428-489)."""
+    import torch
+
+    systems = periodic_systems(22)
+    calc = ra.SoapPowerSpectrum(**SOAP)
+    calc.set_precision(64)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda")
+    cell = cell_tensor(systems)
+    values, layout = soap_values(calc, systems, pos, cell)
+    rng = np.random.default_rng(8)
+    w = torch.tensor(rng.normal(size=values.shape), dtype=torch.float64, device="cuda")
+    (values * w).sum().backward()
+    ref = orc.soap_power_spectrum([orc.System(p, t, c) for p, t, c in systems], SOAP, ["cell"])
+    assert np.array_equal(layout.keys, np.asarray(ref["keys"]))
+    expected = np.zeros((len(systems), 3, 3))
+    for b, blk in enumerate(ref["blocks"]):
+        g = blk["gradients"]["cell"]["values"].reshape(len(blk["samples"]), 3, 3, -1)
+        wb = layout.block(w, b).cpu().numpy()
+        for s, (system, _) in enumerate(blk["samples"]):
+            expected[system] += np.einsum("abp,p->ab", g[s], wb[s])
+    got = cell.grad.cpu().numpy()
+    scale = np.abs(expected).max()
+    assert np.all(np.abs(got - expected) <= 1e-10 * (np.abs(expected) + scale)), (got, expected)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_cell_double_backward(precision):
+    """The cell backward as a function of dA/dX is differentiable (CellGrad::backward,
+    autograd.cpp:493-560): a loss on the cell gradient (a stress-like quantity) reaches
+    the weights; gradcheck of w -> dA/dH(w) in FP64, and the second derivative with
+    respect to the cell warns once."""
+    import torch
+
+    import rascaline_amd.autograd as ag
+
+    systems = periodic_systems(23)
+    calc = ra.SoapPowerSpectrum(**SOAP)
+    calc.set_precision(precision)
+    pos = torch.tensor(np.concatenate([s[0] for s in systems]), dtype=torch.float64, device="cuda")
+    cell = cell_tensor(systems)
+    values, _ = soap_values(calc, systems, pos, cell)
+    if precision == 64:
+        w = torch.randn(values.shape, dtype=torch.float64, device="cuda", requires_grad=True)
+
+        def backward_of(w):
+            return torch.autograd.grad(values, cell, w, create_graph=True)[0]
+
+        assert torch.autograd.gradcheck(backward_of, (w,), eps=1e-6, atol=1e-8, rtol=1e-6)
+    weights = torch.randn(values.shape, dtype=values.dtype, device="cuda", requires_grad=True)
+    stress = torch.autograd.grad((values * weights).sum(), cell, create_graph=True)[0]
+    ag._WARNED_CELL[0] = False
+    with pytest.warns(UserWarning, match="second derivatives with respect to cell"):
+        (stress ** 2).sum().backward()
+    assert weights.grad is not None and float(weights.grad.abs().max()) > 0.0
