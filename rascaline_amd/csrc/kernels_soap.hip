@@ -1099,21 +1099,30 @@ __device__ __forceinline__ void staged_tile_store_oriented(float* __restrict__ o
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        // both pieces of the lane at once: the offsets and every LDS read are issued
+        // before the first use (one wait), then the orientations are selected
+        long long off[2];
+        float xd[2][4], xt[2][4];
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             const int p = s2 * WAVE + lane;
             const int u = p >> 4, n1 = (p & 15) >> 1, h = p & 1;
-            const long long off = stage_off[u];
-            if (off >= 0) {
-                const int fl = (int)(off >> 60);
-                const float* X = stage + u * ORIENTED_SLOT;
-                float v[4] = {0.f, 0.f, 0.f, 0.f};
-                if (fl & 1) load4<float>(X + n1 * 8 + 4 * h, v);
-                if (fl & 2) {
+            const float* X = stage + u * ORIENTED_SLOT;
+            off[s2] = stage_off[u];
+            load4<float>(X + n1 * 8 + 4 * h, xd[s2]);
 #pragma unroll
-                    for (int c = 0; c < 4; c++) v[c] += X[(4 * h + c) * 8 + n1];
-                }
-                store4<float>(out + (off & ((1ll << 60) - 1)) + n1 * 8 + 4 * h, v[0], v[1], v[2], v[3]);
+            for (int c = 0; c < 4; c++) xt[s2][c] = X[(4 * h + c) * 8 + n1];
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int p = s2 * WAVE + lane;
+            const int n1 = (p & 15) >> 1, h = p & 1;
+            if (off[s2] >= 0) {
+                const int fl = (int)(off[s2] >> 60);
+                float v[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[c] = ((fl & 1) ? xd[s2][c] : 0.f) + ((fl & 2) ? xt[s2][c] : 0.f);
+                store4<float>(out + (off[s2] & ((1ll << 60) - 1)) + n1 * 8 + 4 * h, v[0], v[1], v[2], v[3]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
