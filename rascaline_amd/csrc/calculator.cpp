@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -345,10 +346,10 @@ void Calculator::use_radial_surrogate(bool enabled) {
 #define RA_EXPANSION_ENTRIES 8
 #endif
 static_assert(RA_EXPANSION_ENTRIES >= 1 && RA_EXPANSION_ENTRIES <= 64, "expansion chunks hold at most one wave of entries");
-// (neighbour runs, entries, staged tile rows) per window of the tiled gradient
-// kernel, tried in order until one fits the LDS budget
+// (neighbour runs, entries, staged tile rows, rank-2) per window of the gradient
+// kernels, tried in order until one applies and fits the LDS budget
 #ifndef RA_TILED_WINDOWS
-#define RA_TILED_WINDOWS {{4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1}}
+#define RA_TILED_WINDOWS {{8, 8, 1, 1}, {4, 16, 1, 0}, {4, 8, 1, 0}, {2, 8, 1, 0}, {1, 4, 1, 0}, {1, 1, 1, 0}}
 #endif
 
 // Kernel arguments of the SOAP kernels for a built neighbour list: radial tables
@@ -705,24 +706,42 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             // workgroups per CU in FP32, 2 in FP64), else the first within the whole
             // LDS (large bases: one or two workgroups per CU, still well ahead of the
             // register-blocked kernel), or the tuned shape within the whole LDS
-            static const int default_windows[][3] = RA_TILED_WINDOWS;
+            // (the rank-2 MFMA kernel where it applies: FP32 power spectrum with n = 8 in
+            // the plain form, the tiled kernel taking the centers with a run longer
+            // than a window)
+            static const int default_windows[][4] = RA_TILED_WINDOWS;
             const size_t whole = lds > 1024 ? static_cast<size_t>(lds) - 1024 : 0;
-            std::vector<std::pair<std::array<int, 3>, size_t>> windows;
-            if (tune.gradient_window >= 0 && tune.gradient_window < static_cast<int>(soap_window_shapes().size())) {
+            std::vector<std::pair<std::array<int, 4>, size_t>> windows;
+            const bool explicit_shape =
+                tune.gradient_window >= 0 && tune.gradient_window < static_cast<int>(soap_window_shapes().size());
+            if (explicit_shape) {
                 windows.push_back({soap_window_shapes()[tune.gradient_window], whole});
             } else {
                 const size_t cap = std::min<size_t>(whole, sizeof(T) == 4 ? RA_TILED_LDS_BUDGET : 81920);
-                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, cap});
-                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2]}, whole});
+                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2], w[3]}, cap});
+                for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2], w[3]}, whole});
             }
             for (const auto& [w, budget] : windows) {
+                const bool rank2 = w[3] != 0 && ps && sizeof(T) == 4 && fast_form == 0 &&
+                                   ps_gradient_rank2_supported(L, n, S, w[1]);
+                // a rank-2 shape of the built-in table is for the rank-2 kernel only; chosen
+                // explicitly, it runs the tiled kernel where the rank-2 one does not apply
+                if (w[3] != 0 && !rank2 && !explicit_shape) continue;
                 if (!ps_gradient_tiled_supported(L, n, S, w[0], w[1], w[2])) continue;
                 // the spherical expansion's rows are the stage-2 sums themselves: no
                 // coefficients or store staging in LDS, one staged row per pass
                 if (!ps && w[2] != 1) continue;
                 const size_t shmem = ps ? ps_gradient_tiled_lds<T>(lp, w[0], w[1], w[2])
                                         : se_gradient_tiled_shmem<T>(lp, w[0], w[1]);
-                if (shmem > budget) continue;
+                if (rank2) {
+                    // the tiled kernel only takes the few centers with long runs
+                    if (shmem > whole) continue;
+                    size_t rank2_lds = 0;
+                    if constexpr (std::is_same<T, float>::value) rank2_lds = ps_gradient_rank2_lds(lp, w[1]);
+                    if (rank2_lds > budget) continue;
+                } else if (shmem > budget) {
+                    continue;
+                }
                 const long long ne = std::max<long long>(nl.n_entries, 1);
                 const int nloc = ps ? S * (S + 1) / 2 : S;
                 int* d_runs = ws_.get<int>("plan_runs", ne * 4);
@@ -730,12 +749,21 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 int* d_windows = ws_.get<int>("plan_windows", ne * 4);
                 int* d_n_windows = ws_.get<int>("plan_n_windows", std::max<long long>(n_atoms, 1));
                 int* d_self_rows = ws_.get<int>("plan_self_rows", std::max<long long>(n_atoms, 1) * nloc);
+                int* d_split = rank2 ? ws_.get<int>("plan_split", std::max<long long>(n_atoms, 1)) : nullptr;
                 launch_ps_grad_plan(stream, n_atoms, S, nl.offsets, nl.entry_j, d_species, type_mask, sample_scan,
                                     row_scan, d_block_sample_base, d_atom_system, d_atom_local, w[0], w[1], d_runs,
-                                    d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels, ps);
+                                    d_run_rows, d_windows, d_n_windows, d_self_rows, grad_labels, ps, d_split);
                 mark(ev_main_start_, stream);
                 timed_start = true;
-                if (ps) {
+                if (rank2) {
+                    if constexpr (std::is_same<T, float>::value) {
+                        launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows,
+                                                    d_windows, d_n_windows, d_self_rows, entry_data, d_grads, false,
+                                                    d_split);
+                        launch_ps_gradient_rank2(lp, w[1], coef, dself, type_mask, d_runs, d_run_rows, d_windows,
+                                                 d_n_windows, d_self_rows, d_split, entry_data, d_grads);
+                    }
+                } else if (ps) {
                     launch_ps_gradient_tiled<T>(lp, w[0], w[1], w[2], coef, dself, type_mask, d_runs, d_run_rows,
                                                 d_windows, d_n_windows, d_self_rows, entry_data, d_grads, fast_form != 0);
                 } else {
@@ -1076,10 +1104,10 @@ void Calculator::compute_device(const HostBatch& batch, const Gradients& grads, 
     }
 }
 
-const std::vector<std::array<int, 3>>& soap_window_shapes() {
-    static const std::vector<std::array<int, 3>> shapes = {
-        {4, 16, 1}, {4, 8, 1}, {2, 8, 1}, {1, 4, 1}, {1, 1, 1},
-        {8, 16, 1}, {4, 32, 1}, {2, 16, 1}, {4, 16, 2}, {4, 4, 1}, {8, 32, 1},
+const std::vector<std::array<int, 4>>& soap_window_shapes() {
+    static const std::vector<std::array<int, 4>> shapes = {
+        {4, 16, 1, 0}, {4, 8, 1, 0}, {2, 8, 1, 0}, {1, 4, 1, 0}, {1, 1, 1, 0},  {8, 16, 1, 0}, {4, 32, 1, 0},
+        {2, 16, 1, 0}, {4, 16, 2, 0}, {4, 4, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 1}, {8, 16, 1, 1},
     };
     return shapes;
 }

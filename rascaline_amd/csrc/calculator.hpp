@@ -113,9 +113,10 @@ struct SoapTuning {
     int gradient_window = -1;
 };
 
-// (runs, entries, rows per pass) window shapes of the tiled power-spectrum
-// gradient kernel that the autotuner tries
-const std::vector<std::array<int, 3>>& soap_window_shapes();
+// (runs, entries, rows per pass, rank-2) window shapes of the gradient kernels that
+// the autotuner tries; rank-2 shapes take the rank-2 MFMA kernel where it applies
+// (FP32 power spectrum, n = 8, plain form) and the tiled kernel elsewhere
+const std::vector<std::array<int, 4>>& soap_window_shapes();
 
 struct Gradients {
     bool positions = false, strain = false, cell = false;

@@ -145,7 +145,7 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
                          const int* species, const unsigned* type_mask, const long long* sample_scan,
                          const long long* row_scan, const long long* block_sample_base, const int* atom_system,
                          const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
-                         int* n_windows, int* self_rows, int* labels, bool power_spectrum);
+                         int* n_windows, int* self_rows, int* labels, bool power_spectrum, int* split = nullptr);
 // spherical-expansion gradient rows from the same plan (power_spectrum = false
 // above: one row per run in run_rows[run * S], self rows at self_rows[i * S + t])
 template <typename T>
@@ -170,7 +170,16 @@ template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
                               const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
-                              T* out, bool fast);
+                              T* out, bool fast, const int* only = nullptr);
+// Rank-2 gradient rows on MFMA (FP32, n = 8) over a plan of whole-run windows of up
+// to `max_entries` (8 or 16) images and 8 runs; the centers the plan flags in `split`
+// (a run longer than a window) are left to launch_ps_gradient_tiled with only = split.
+bool ps_gradient_rank2_supported(int L, int n, int n_species, int max_entries);
+size_t ps_gradient_rank2_lds(const SoapLaunch<float>& p, int max_entries);
+void launch_ps_gradient_rank2(const SoapLaunch<float>& p, int max_entries, const float* coef, const float* dself,
+                              const unsigned* type_mask, const int* runs, const int* run_rows, const int* windows,
+                              const int* n_windows, const int* self_rows, const int* split, const float* entry_data,
+                              float* out);
 template <typename T>
 size_t ps_gradient_shmem(const SoapLaunch<T>& p, int chunk);
 // register-blocked power-spectrum gradient rows; returns false when n > 16

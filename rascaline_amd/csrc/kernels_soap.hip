@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -1144,7 +1145,7 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     const long long* __restrict__ row_scan, const long long* __restrict__ block_sample_base,
     const int* __restrict__ atom_system, const int* __restrict__ atom_local, int max_runs, int max_entries,
     int4* __restrict__ runs, int* __restrict__ run_rows, int4* __restrict__ windows, int* __restrict__ n_windows,
-    int* __restrict__ self_rows, int* __restrict__ labels) {
+    int* __restrict__ self_rows, int* __restrict__ labels, int* __restrict__ split) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int present[TILE_MAX_TYPES];
     // local blocks of a center: (t1 <= t2) type pairs for the power spectrum,
@@ -1178,7 +1179,7 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     __syncthreads();
 
     // uniform planning state
-    bool snapped = false;
+    bool snapped = false, split_run = false;
     int n_runs = 0, n_win = 0;
     int cur_start = 0, cur_entries = 0, cur_run_start = 0, cur_runs = 0;
     auto emit = [&](int start, int entries, int run_start, int count) {
@@ -1235,6 +1236,7 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
                 cur_start = pos;
                 cur_run_start = r;
                 if (rem > max_entries) {
+                    split_run = true;
                     emit(pos, max_entries, r, 1);
                     pos += max_entries;
                     rem -= max_entries;
@@ -1268,7 +1270,10 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     }
     if (open_first >= 0) process_run(open_first, len - open_first, open_j);
     if (cur_runs > 0) emit(cur_start, cur_entries, cur_run_start, cur_runs);
-    if (lane == 0) n_windows[i] = n_win;
+    if (lane == 0) {
+        n_windows[i] = n_win;
+        if (split) split[i] = split_run ? 1 : 0;
+    }
 
     // the center's own rows: before its first neighbour j > i in every block
     if (!snapped) {
@@ -1860,7 +1865,7 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     SoapArgs<T> a, int max_runs, int max_entries, const T* __restrict__ coef, const T* __restrict__ dself,
     const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
     const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
-    const T* __restrict__ entry_data, T* __restrict__ out) {
+    const T* __restrict__ entry_data, T* __restrict__ out, const int* __restrict__ only) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int run_lo[TILE_MAX_RUNS], run_hi[TILE_MAX_RUNS], run_tj[TILE_MAX_RUNS], run_complete[TILE_MAX_RUNS];
     __shared__ int run_row[TILE_MAX_RUNS][TILE_MAX_TYPES];
@@ -1871,6 +1876,8 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     __shared__ int leg_m[TILE_MAX_L * (TILE_MAX_L + 1) / 2];
 
     const int i = blockIdx.x;
+    // with `only`, the centers it flags (the others go to the rank-2 kernel)
+    if (only && only[i] == 0) return;
     const int tid = threadIdx.x;
     const int S = a.n_species, L = a.L, n = a.n;
     const int L2 = L * L;
@@ -1964,6 +1971,595 @@ __global__ void __launch_bounds__(TILE_THREADS) ps_gradient_tiled_kernel(
     }
     lds_barrier();
     ps_self_rows<T, NC, RP>(i, L, n, S, n_present, present, self_rows, C, DS, scale_l, stage, stage_offs, out);
+}
+
+// ---------------------------------------------------------------------------
+// Rank-2 gradient rows on MFMA (FP32, n = 8)
+//
+// One neighbour image e of a run contributes to the expansion gradient
+//     DC_e[x][l^2+l+m][c] = alpha_x[lm] ra_l[c] + beta_x[lm] rb_l[c],
+// alpha_x = Y_lm d_x, beta_x = r dY_lm/dx, ra = df R + f dR, rb = (f / r) R
+// (soap/spherical_expansion_pair.rs:382-419): rank two in (m, c) for each (x, l).
+// A row block of the power-spectrum gradient (soap/power_spectrum.rs:621-678),
+// X = DC^T C_t summed over the run's images, is therefore
+//     X[n1][n2] = sum_e ra_e[n1] u_e[n2] + rb_e[n1] v_e[n2],  u = alpha^T C_t, v = beta^T C_t:
+// the contraction over m shrinks to the 8-vectors u and v, and every 8 x 8 block is
+// a short sum of outer products.
+//
+// One workgroup per center, windows of whole runs (at most E images, 8 runs):
+//  * the images' tables: (Y, T1, T2) per (l, m) with alpha_x = Y d_x and
+//    beta_x = ga_x T1 + gb_x T2 (ga = (sin phi, -cos phi, 0), gb = (-cos theta cos phi,
+//    -cos theta sin phi, sin theta) / 2; spherical_harmonics.rs:256-424), and
+//    (ra, rb) per (l, c);
+//  * per unit (x, l, four present types), one wave forms
+//    W[(t, n2)][(e, a)] = sum_m A_a[e][m] C_t[m][n2] with v_mfma_f32_16x16x4f32
+//    (A = alpha or beta, formed in the lanes from the tables), keeps W in its own
+//    LDS slice, then writes the 8 x 8 blocks with v_mfma_f32_4x4x1f32: 16
+//    independent 4 x 4 products, four blocks per instruction. Block b = (tile, fast
+//    quad fa, slow quad sb): lane 4b + c holds out[4 sb + c][4 fa .. 4 fa + 3], one
+//    16-byte piece of the row, and the 16 lanes of a tile cover its 256 contiguous
+//    bytes, so the accumulators are stored as they are. The orientation of a block
+//    (X for t >= tj, X^T for t <= tj, both when t == tj) is the lanes' choice of
+//    operands: X takes (u, v) on the fast side and (ra, rb) on the slow side, X^T
+//    the reverse.
+// The center's own rows s_l (DS_t1^T C_t2 + C_t1^T DS_t2) come first, with the same
+// 4 x 4 products over m, from the self gradients DS in LDS.
+constexpr int R2_THREADS = 512;
+constexpr int R2_WAVES = R2_THREADS / WAVE;
+constexpr int R2_MAX_L = 16;
+
+// Loader and compute waves of the rank-2 kernel. One loader wave fetches each window's
+// plan and image records (a whole window per burst, one window ahead, into a staging
+// area) and builds its tables into one of two LDS buffers, while seven compute waves
+// write the previous window's rows. Compute waves issue no global load after their
+// first store, so they never wait for their own stores to drain (loads and stores
+// share one in-order counter per wave), and the loader, which never stores, waits
+// only for its own loads.
+constexpr int R2_LOADERS = 1;
+constexpr int R2_COMPUTE = R2_WAVES - R2_LOADERS;
+
+struct Rank2Layout {
+    // float offsets in the dynamic LDS
+    int L, L2, csp, E, wp, S, per_entry;
+    int cp, st, st_runs, st_rows, st_raw, cms, einv, buf0, buf_size, w, total;
+    // offsets within a window buffer
+    int b_hdr, b_lo, b_hi, b_tj, b_row, b_ytab, b_rr, b_geo;
+    __host__ __device__ Rank2Layout(int L_, int S_, int E_) : L(L_), L2(L_ * L_), E(E_), S(S_) {
+        csp = L2 * 8 + 8;  // padded per present type: two types' column reads meet no bank conflict
+        wp = 2 * E + 2;    // (e, a) pairs of one (t, n2) column, padded for conflict-free 8-byte reads
+        per_entry = TileEntryLayout(L, 8, 8).per_entry;
+        b_hdr = 0;         // images, runs
+        b_lo = 4;
+        b_hi = b_lo + TILE_MAX_RUNS;
+        b_tj = b_hi + TILE_MAX_RUNS;
+        b_row = b_tj + TILE_MAX_RUNS;
+        b_ytab = (b_row + TILE_MAX_RUNS * S + 3) / 4 * 4;  // (Y, T1, T2, -) per (image, lm)
+        b_rr = b_ytab + E * L2 * 4;                        // (ra, rb) per (image, l, c)
+        b_geo = b_rr + E * L * 16;                         // d, ga, gb per image
+        buf_size = (b_geo + E * 12 + 3) / 4 * 4;
+        cp = 0;
+        // the loader's staging area: window descriptor and runs (int4 each), run rows,
+        // image records, cos / sin (m phi) and reciprocals
+        st = S * csp;
+        st_runs = st + 4;
+        st_rows = st_runs + 4 * TILE_MAX_RUNS;
+        st_raw = (st_rows + TILE_MAX_RUNS * S + 3) / 4 * 4;
+        cms = st_raw + E * per_entry;
+        einv = cms + E * L * 2;
+        buf0 = einv + E * 4;
+        w = buf0 + 2 * buf_size;  // per compute wave: W [32][wp]
+        total = w + R2_COMPUTE * 32 * wp;
+    }
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__device__ __forceinline__ float4_t mfma4x4(float a, float b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float4_t mfma16x16(float a, float b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// wave-uniform copy of a value every lane holds alike (a scalar register)
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Loader: window w's descriptor, runs, run rows and image records into the staging
+// area, every load issued before the first wait (records of up to E images at L <= 7
+// in registers; larger bases in several passes).
+template <int E>
+__device__ __attribute__((noinline)) void rank2_fetch(const Rank2Layout& ly, int lane, int n_present, long long e0, int w,
+                                            const int4* __restrict__ windows, const int4* __restrict__ runs,
+                                            const int* __restrict__ run_rows, const float* __restrict__ entry_data,
+                                            float* lds) {
+    const int S = ly.S;
+    const int4 wd = windows[e0 + w];
+    const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
+    const float4_t* src = reinterpret_cast<const float4_t*>(entry_data + (e0 + es) * ly.per_entry);
+    const int count = ec * ly.per_entry / 4;
+    constexpr int NV = (E * 152 / 4 + WAVE - 1) / WAVE;  // float4 pieces per lane and pass
+    float4_t* dst = reinterpret_cast<float4_t*>(lds + ly.st_raw);
+    float4_t v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int idx = k * WAVE + lane;
+        if (idx < count) v[k] = src[idx];
+    }
+    int4 run = {0, 0, 0, 0};
+    if (lane < rc) run = runs[e0 + rs + lane];
+    constexpr int NR = TILE_MAX_RUNS * TILE_MAX_TYPES / WAVE;
+    int rowv[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int idx = k * WAVE + lane;
+        if (idx < rc * n_present) {
+            const int r = idx / n_present;
+            rowv[k] = run_rows[(e0 + rs + r) * S + idx - r * n_present];
+        }
+    }
+    int* ist = reinterpret_cast<int*>(lds + ly.st);
+    if (lane == 0) {
+        ist[0] = es;
+        ist[1] = ec;
+        ist[2] = rs;
+        ist[3] = rc;
+    }
+    if (lane < rc) {
+        int* ir = reinterpret_cast<int*>(lds + ly.st_runs) + 4 * lane;
+        ir[0] = run.x;
+        ir[1] = run.y;
+        ir[2] = run.z;
+    }
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int idx = k * WAVE + lane;
+        if (idx < rc * n_present) {
+            const int r = idx / n_present;
+            reinterpret_cast<int*>(lds + ly.st_rows)[r * S + idx - r * n_present] = rowv[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int idx = k * WAVE + lane;
+        if (idx < count) dst[idx] = v[k];
+    }
+#pragma unroll 1
+    for (int idx = NV * WAVE + lane; idx < count; idx += WAVE) dst[idx] = src[idx];
+}
+
+// Loader: the staged window's tables into `buf`: the runs and their rows, and per image
+// (Y, T1, T2) per (l, m) with alpha_x = Y d_x and beta_x = ga_x T1 + gb_x T2
+// (ga = (sin phi, -cos phi, 0), gb = (-cos theta cos phi, -cos theta sin phi, sin theta) / 2;
+// the m = 0 derivative as T2 = -2 c_l P_l1; spherical_harmonics.rs:256-424),
+// (ra, rb) = (df R + f dR, (f / r) R) per (l, c), and (d, ga, gb).
+__device__ __forceinline__ void rank2_tables(const Rank2Layout& ly, const TileEntryLayout& lay, const ShTable& tb,
+                                             const float* shc, const unsigned char* lm_l, int lane, int n_present,
+                                             float* lds, float* buf) {
+    const int L = ly.L, L2 = ly.L2, S = ly.S;
+    const int* ist = reinterpret_cast<const int*>(lds + ly.st);
+    const int es = ist[0], ec = ist[1], rc = ist[3];
+    const float* raw = lds + ly.st_raw;
+    float* cms = lds + ly.cms;
+    float* einv = lds + ly.einv;
+    int* ib = reinterpret_cast<int*>(buf);
+    if (lane == 0) {
+        ib[ly.b_hdr] = ec;
+        ib[ly.b_hdr + 1] = rc;
+    }
+    if (lane < rc) {
+        const int* ir = reinterpret_cast<const int*>(lds + ly.st_runs) + 4 * lane;
+        ib[ly.b_lo + lane] = ir[0] - es;
+        ib[ly.b_hi + lane] = ir[0] + ir[1] - es;
+        ib[ly.b_tj + lane] = ir[2];
+    }
+#pragma unroll 1
+    for (int it = lane; it < rc * n_present; it += WAVE) {
+        const int r = it / n_present, k = it - r * n_present;
+        ib[ly.b_row + r * S + k] = reinterpret_cast<const int*>(lds + ly.st_rows)[r * S + k];
+    }
+    // cos(m phi), sin(m phi) by the recurrence of spherical_harmonics.rs (same signs), and
+    // the reciprocals of the Legendre derivative terms, per image
+#pragma unroll 1
+    for (int it = lane; it < ec * L; it += WAVE) {
+        const int e = it / L, am = it - e * L;
+        const float* g = raw + e * lay.per_entry + lay.trig;
+        const float cos_phi = g[2], sin_phi = g[3];
+        float cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+        const float minus_two_cos = -2 * cos_phi;
+        float sin_m = 0, cos_m = 1;
+        for (int mm = 1; mm <= am; mm++) {
+            sin_m = minus_two_cos * sin_1 - sin_2;
+            cos_m = minus_two_cos * cos_1 - cos_2;
+            sin_2 = sin_1;
+            sin_1 = sin_m;
+            cos_2 = cos_1;
+            cos_1 = cos_m;
+        }
+        cms[it * 2 + 0] = cos_m;
+        cms[it * 2 + 1] = sin_m;
+        if (am == 0) {
+            const float cos_theta = g[0], sin_theta = g[1];
+            einv[e * 4 + 0] = sin_theta > 0.1f ? 1.0f / sin_theta : 0.f;
+            einv[e * 4 + 1] = -0.5f / cos_theta;
+        }
+    }
+    wave_lds_sync();
+    float* ytab = buf + ly.b_ytab;
+#pragma unroll 1
+    for (int it = lane; it < ec * L2; it += WAVE) {
+        const int e = it / L2, lmp = it - e * L2;
+        const int l = lm_l[lmp];
+        const int m = lmp - l * l - l;
+        const float* base = raw + e * lay.per_entry;
+        const float* Pl = base + lay.p;
+        const float sin_theta = base[lay.trig + 1];
+        const int am = m < 0 ? -m : m;
+        const int k = leg_index(l, am);
+        const float p = Pl[k];
+        float Y, T1, T2;
+        if (m == 0) {
+            Y = p / 1.4142135623730951f;
+            T1 = 0.f;
+            T2 = l == 0 ? 0.f : -2.f * (shc[tb.gl + l] * Pl[leg_index(l, 1)]);
+        } else {
+            const float cos_m = cms[(e * L + am) * 2], sin_m = cms[(e * L + am) * 2 + 1];
+            const float pm1 = Pl[leg_index(l, am - 1)];
+            const float pp1 = am < l ? Pl[leg_index(l, am + 1)] : 0.f;
+            const float dp = shc[tb.dpa + k] * pm1 - shc[tb.dpb + k] * pp1;
+            const float pt = (sin_theta > 0.1f) ? ((float)am * einv[e * 4]) * p : einv[e * 4 + 1] * dp;
+            const float c1 = m > 0 ? cos_m : sin_m;
+            const float s1 = m > 0 ? sin_m : -cos_m;
+            Y = p * c1;
+            T1 = pt * s1;
+            T2 = c1 * dp;
+        }
+        store4<float>(ytab + (size_t)(e * L2 + lmp) * 4, Y, T1, T2, 0.f);
+    }
+    float* rr = buf + ly.b_rr;
+#pragma unroll 1
+    for (int it = lane; it < ec * L * 8; it += WAVE) {
+        const int e = it / (L * 8), lc = it - e * L * 8;
+        const float* base = raw + e * lay.per_entry;
+        const float f = base[lay.scal + 0], df = base[lay.scal + 1], f_r = base[lay.scal + 2];
+        const float R = base[lay.rv + lc], dR = base[lay.rd + lc];
+        rr[(e * L * 8 + lc) * 2 + 0] = df * R + f * dR;
+        rr[(e * L * 8 + lc) * 2 + 1] = f_r * R;
+    }
+    float* geo = buf + ly.b_geo;
+#pragma unroll 1
+    for (int e = lane; e < ec; e += WAVE) {
+        const float* base = raw + e * lay.per_entry;
+        const float cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+        const float cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+        float* g = geo + e * 12;
+        g[0] = base[lay.dir + 0];
+        g[1] = base[lay.dir + 1];
+        g[2] = base[lay.dir + 2];
+        g[3] = sin_phi;
+        g[4] = -cos_phi;
+        g[5] = 0.f;
+        g[6] = -0.5f * cos_theta * cos_phi;
+        g[7] = -0.5f * cos_theta * sin_phi;
+        g[8] = 0.5f * sin_theta;
+    }
+    wave_lds_sync();
+}
+
+// (six waves per SIMD: three workgroups per CU; the loader's hoisted addresses must
+// not take the compute waves' registers)
+template <int E>
+__global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) ps_gradient_rank2_kernel(
+    SoapArgs<float> a, const float* __restrict__ coef, const unsigned* __restrict__ type_mask,
+    const int4* __restrict__ runs, const int* __restrict__ run_rows, const int4* __restrict__ windows,
+    const int* __restrict__ n_windows, const int* __restrict__ split, const float* __restrict__ entry_data,
+    float* __restrict__ out) {
+    constexpr int RTM = E / 8;  // row tiles of 16 (image, alpha / beta) rows
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int present[TILE_MAX_TYPES];
+    __shared__ float scale_l[R2_MAX_L];
+    __shared__ float shc[2 * R2_MAX_L * (R2_MAX_L + 1) / 2 + 4 * R2_MAX_L];
+    __shared__ unsigned char lm_l[R2_MAX_L * R2_MAX_L];
+
+    const int i = blockIdx.x;
+    if (split[i] != 0) return;  // a run longer than a window: the tiled kernel's center
+    const int nw = n_windows[i];
+    if (nw == 0) return;        // no neighbour rows (the center's own rows are another kernel's)
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    // wave-uniform (a scalar register), so that every loop over units is uniform
+    const int wave_id = uniform(tid / WAVE);
+    const int S = a.n_species, L = a.L;
+    const int L2 = L * L;
+    const Rank2Layout ly(L, S, E);
+    const TileEntryLayout lay(L, 8, 8);
+    const ShTable tb(L);
+    float* lds = reinterpret_cast<float*>(smem);
+    float* Cp = lds + ly.cp;
+    const int CSP = ly.csp;
+    const int P = L * 64;
+    const long long e0 = a.offsets[i];
+    const unsigned mask = type_mask[i];
+    const int n_present = __popc(mask);
+    const bool loader = wave_id < R2_LOADERS;
+
+    if (tid == 0) {
+        int k = 0;
+        for (int t = 0; t < S; t++)
+            if ((mask >> t) & 1u) present[k++] = t;
+    }
+    if (tid < L) scale_l[tid] = (float)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
+    for (int k = tid; k < tb.size; k += R2_THREADS) shc[k] = a.sh_c[k];
+    for (int k = tid; k < L2; k += R2_THREADS) {
+        int l = 0;
+        while ((l + 1) * (l + 1) <= k) l++;
+        lm_l[k] = (unsigned char)l;
+    }
+    if (loader) rank2_fetch<E>(ly, lane, __popc(mask), e0, 0, windows, runs, run_rows, entry_data, lds);
+    lds_barrier();
+    // coefficients per present type, padded rows of csp floats
+    for (int k = tid; k < n_present * L2 * 8; k += R2_THREADS) {
+        const int kk = k / (L2 * 8), r = k - kk * L2 * 8;
+        Cp[kk * CSP + r] = coef[((long long)i * S + present[kk]) * L2 * 8 + r];
+    }
+    if (loader) {
+        rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0);
+        if (nw > 1) rank2_fetch<E>(ly, lane, n_present, e0, 1, windows, runs, run_rows, entry_data, lds);
+    }
+    lds_barrier();
+
+    // compute-wave lane roles: 4 x 4 blocks (tile lane >> 4, fast quad fa, slow quad sb,
+    // column q); 16 x 16 product operands (row / column j16, k row jk)
+    const int cw = wave_id - R2_LOADERS;
+    const int fa = (lane >> 3) & 1, sb = (lane >> 2) & 1, q = lane & 3;
+    const int tau = lane >> 4;
+    const int j16 = lane & 15, jk = lane >> 4;
+    const int piece = (4 * sb + q) * 8 + 4 * fa;  // the lane's 16-byte piece in an 8 x 8 block
+    const bool alpha_row = (j16 & 1) == 0;
+    const int WP = ly.wp;
+    float* Wv = lds + ly.w + (loader ? 0 : cw) * 32 * WP;
+    const int n_tc = (n_present + 3) / 4;
+
+    if (loader) {
+        // one barrier per window, as the compute waves below
+        for (int w = 0; w < nw; w++) {
+            if (w + 1 < nw) {
+                rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0 + ((w + 1) & 1) * ly.buf_size);
+                if (w + 2 < nw) rank2_fetch<E>(ly, lane, n_present, e0, w + 2, windows, runs, run_rows, entry_data, lds);
+            }
+            lds_barrier();
+        }
+        return;
+    }
+    for (int w = 0; w < nw; w++) {
+        float* buf = lds + ly.buf0 + (w & 1) * ly.buf_size;
+        {
+            const int* ib = reinterpret_cast<const int*>(buf);
+            const int ec = uniform(ib[ly.b_hdr]), rc = uniform(ib[ly.b_hdr + 1]);
+            const float* ytab = buf + ly.b_ytab;
+            const float* rr = buf + ly.b_rr;
+            const float* geo = buf + ly.b_geo;
+            // the lane's rows of the 16 x 16 products: image e = rho / 2, alpha (rho even) or beta
+            const int RT = (2 * ec + 15) / 16;
+            float gd[RTM][3], ga[RTM][3], gb[RTM][3];
+            bool ev[RTM];
+            int ybase[RTM];
+#pragma unroll
+            for (int rt = 0; rt < RTM; rt++) {
+                const int e = (16 * rt + j16) >> 1;
+                ev[rt] = e < ec;
+                const int es_ = ev[rt] ? e : 0;
+                const float* g = geo + es_ * 12;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    gd[rt][c] = g[c];
+                    ga[rt][c] = g[3 + c];
+                    gb[rt][c] = g[6 + c];
+                }
+                ybase[rt] = (es_ * L2 + jk) * 4;
+            }
+            // units: (type chunk, l descending, x), dealt in snake order over the compute waves
+            const int units = 3 * L * n_tc;
+            for (int slot = 0;; slot++) {
+                const int round = slot;
+                const int pos = (round & 1) ? R2_COMPUTE - 1 - cw : cw;
+                const int u = round * R2_COMPUTE + pos;
+                if (u >= units) break;
+                const int tc = u / (3 * L), lx = u - tc * 3 * L;
+                const int l = L - 1 - lx / 3, x = lx % 3;
+                const int lsq = l * l;
+                // (B) W[(t, n2)][(e, a)] = sum_m A_a[e][m] C_t[m][n2] for the chunk's four types
+                float dx[RTM], gax[RTM], gbx[RTM];
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++) {
+                    dx[rt] = x == 0 ? gd[rt][0] : (x == 1 ? gd[rt][1] : gd[rt][2]);
+                    gax[rt] = x == 0 ? ga[rt][0] : (x == 1 ? ga[rt][1] : ga[rt][2]);
+                    gbx[rt] = x == 0 ? gb[rt][0] : (x == 1 ? gb[rt][1] : gb[rt][2]);
+                }
+                const int kk0 = 4 * tc + (j16 >> 3);
+                const bool kv0 = kk0 < n_present, kv1 = kk0 + 2 < n_present;
+                const float* cb0 = Cp + kk0 * CSP + (lane & 7);
+                const float* cb1 = cb0 + 2 * CSP;
+                float4_t accb[RTM][2];
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++) {
+                    accb[rt][0] = float4_t{0.f, 0.f, 0.f, 0.f};
+                    accb[rt][1] = float4_t{0.f, 0.f, 0.f, 0.f};
+                }
+                const int ks_count = (2 * l + 1 + 3) / 4;
+                for (int ks = 0; ks < ks_count; ks++) {
+                    const int mp = 4 * ks + jk;
+                    const bool mv = mp <= 2 * l;
+                    // reads at a valid slot always, zeros selected afterwards
+                    const int lm = lsq + (mv ? mp : 0);
+                    const float c0 = cb0[lm * 8], c1 = cb1[lm * 8];
+                    const float b0 = (mv && kv0) ? c0 : 0.f;
+                    const float b1 = (mv && kv1) ? c1 : 0.f;
+#pragma unroll
+                    for (int rt = 0; rt < RTM; rt++) {
+                        if (rt < RT) {
+                            const float4_t y =
+                                *reinterpret_cast<const float4_t*>(ytab + ybase[rt] + (lsq + 4 * ks) * 4);
+                            float av = alpha_row ? y[0] * dx[rt] : gax[rt] * y[1] + gbx[rt] * y[2];
+                            av = (mv && ev[rt]) ? av : 0.f;
+                            accb[rt][0] = mfma16x16(av, b0, accb[rt][0]);
+                            accb[rt][1] = mfma16x16(av, b1, accb[rt][1]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++) {
+                    if (rt < RT) {
+#pragma unroll
+                        for (int ct = 0; ct < 2; ct++) {
+                            float* wdst = Wv + (16 * ct + j16) * WP + 16 * rt + 4 * jk;
+#pragma unroll
+                            for (int v = 0; v < 4; v++) wdst[v] = accb[rt][ct][v];
+                        }
+                    }
+                }
+                wave_lds_sync();
+                // (C) per run, the 8 x 8 blocks of the chunk's (up to) four types, one per
+                // 16 lanes: X = sum_e ra_e u_e^T + rb_e v_e^T in the block's orientation
+                const int nk = min(4, n_present - 4 * tc);
+                const bool lane_type = tau < nk;
+                const int kslot = 4 * tc + (lane_type ? tau : 0);
+                const int t_lane = present[kslot];
+                const int offWA = (8 * tau + 4 * fa + q) * WP, offWB = (8 * tau + 4 * sb + q) * WP;
+                const int offRA = (l * 8 + 4 * fa + q) * 2, offRB = (l * 8 + 4 * sb + q) * 2;
+                const float sc_same = uniform(scale_l[l]);
+                const float sc_diff = sc_same * 1.4142135623730951f;
+                float* out_unit = out + (long long)x * P + l * 64 + piece;
+                for (int r = 0; r < rc; r++) {
+                    const int tj = uniform(ib[ly.b_tj + r]);
+                    const int lo = uniform(ib[ly.b_lo + r]);
+                    const int cnt = uniform(ib[ly.b_hi + r]) - lo;
+                    const bool f1 = t_lane >= tj, f2 = t_lane <= tj;
+                    const bool both = __ballot(lane_type && f1 && f2) != 0ull;
+                    const int oA = f1 ? offWA : offRA, oB = f1 ? offRB : offWB;
+                    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+                    for (int eo = 0; eo < cnt; eo++) {
+                        const int e = lo + eo;
+                        const float* wb = Wv + 2 * e;
+                        const float* rb = rr + e * (16 * L);
+                        const float2 A = *reinterpret_cast<const float2*>((f1 ? wb : rb) + oA);
+                        const float2 B = *reinterpret_cast<const float2*>((f1 ? rb : wb) + oB);
+                        acc = mfma4x4(A.x, B.x, acc);
+                        acc = mfma4x4(A.y, B.y, acc);
+                        if (both) {
+                            float2 A2 = *reinterpret_cast<const float2*>(rb + offRA);
+                            const float2 B2 = *reinterpret_cast<const float2*>(wb + offWB);
+                            A2.x = f1 && f2 ? A2.x : 0.f;
+                            A2.y = f1 && f2 ? A2.y : 0.f;
+                            acc = mfma4x4(A2.x, B2.x, acc);
+                            acc = mfma4x4(A2.y, B2.y, acc);
+                        }
+                    }
+                    if (lane_type) {
+                        const float sc = t_lane != tj ? sc_diff : sc_same;
+                        float* dst = out_unit + (long long)ib[ly.b_row + r * S + kslot] * 3 * P;
+                        store4<float>(dst, acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
+                    }
+                }
+                wave_lds_sync();
+            }
+        }
+        lds_barrier();
+    }
+}
+
+// The center's own rows s_l (DS_t1^T C_t2 + C_t1^T DS_t2) for every present pair
+// t1 <= t2 (soap/power_spectrum.rs:621-678), from the self gradients DS and the
+// coefficients C in LDS, with 4 x 4 products over m on v_mfma_f32_4x4x1f32: one
+// workgroup per center, four pairs per instruction (block = (pair, fast quad, slow
+// quad), lane 4 b + c holding a 16-byte piece of the row). Centers flagged in
+// `split` are left to the tiled kernel.
+__global__ void __launch_bounds__(R2_THREADS) ps_self_rows_kernel(SoapArgs<float> a, const float* __restrict__ coef,
+                                                                   const float* __restrict__ dself,
+                                                                   const unsigned* __restrict__ type_mask,
+                                                                   const int* __restrict__ self_rows,
+                                                                   const int* __restrict__ split,
+                                                                   float* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int present[TILE_MAX_TYPES];
+    __shared__ float scale_l[R2_MAX_L];
+    const int i = blockIdx.x;
+    if (split[i] != 0) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave_id = uniform(tid / WAVE);
+    const int S = a.n_species, L = a.L;
+    const int L2 = L * L;
+    const int CSP = L2 * 8 + 8;
+    const int P = L * 64;
+    const int npairs = S * (S + 1) / 2;
+    const unsigned mask = type_mask[i];
+    const int n_present = __popc(mask);
+    const int npp = n_present * (n_present + 1) / 2;
+    if (npp == 0) return;
+    float* Cp = reinterpret_cast<float*>(smem);
+    float* DS = Cp + S * CSP;
+    int* pair_row = reinterpret_cast<int*>(DS + S * 3 * CSP);
+    int* pair_k = pair_row + npairs;
+    if (tid == 0) {
+        int k = 0;
+        for (int t = 0; t < S; t++)
+            if ((mask >> t) & 1u) present[k++] = t;
+    }
+    if (tid < L) scale_l[tid] = (float)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
+    lds_barrier();
+    for (int k = tid; k < n_present * L2 * 8; k += R2_THREADS) {
+        const int kk = k / (L2 * 8), r = k - kk * L2 * 8;
+        Cp[kk * CSP + r] = coef[((long long)i * S + present[kk]) * L2 * 8 + r];
+    }
+    for (int k = tid; k < n_present * 3 * L2 * 8; k += R2_THREADS) {
+        const int kx = k / (L2 * 8), r = k - kx * L2 * 8;
+        const int kk = kx / 3, x = kx - 3 * kk;
+        DS[kx * CSP + r] = dself[(((long long)i * S + present[kk]) * 3 + x) * L2 * 8 + r];
+    }
+    if (tid < npp) {
+        int k1 = 0, rem = tid;
+        while (rem >= n_present - k1) {
+            rem -= n_present - k1;
+            k1++;
+        }
+        const int k2 = k1 + rem;
+        pair_k[tid] = k1 | (k2 << 8);
+        pair_row[tid] = self_rows[(long long)i * npairs + pair_index(present[k1], present[k2], S)];
+    }
+    lds_barrier();
+    const int fa = (lane >> 3) & 1, sb = (lane >> 2) & 1, q = lane & 3;
+    const int ng = (npp + 3) / 4;
+    const int units = 3 * L * ng;
+    for (int u = wave_id; u < units; u += R2_WAVES) {
+        const int g = u % ng, lx = u / ng;
+        const int l = L - 1 - lx / 3, x = lx % 3;
+        const int p = 4 * g + (lane >> 4);
+        const bool valid = p < npp;
+        const int pk = pair_k[valid ? p : 0];
+        const int k1 = pk & 255, k2 = pk >> 8;
+        const float* C1 = Cp + k1 * CSP + l * l * 8 + 4 * sb + q;
+        const float* C2 = Cp + k2 * CSP + l * l * 8 + 4 * fa + q;
+        const float* D1 = DS + (k1 * 3 + x) * CSP + l * l * 8 + 4 * sb + q;
+        const float* D2 = DS + (k2 * 3 + x) * CSP + l * l * 8 + 4 * fa + q;
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int mp = 0; mp <= 2 * l; mp++) {
+            acc = mfma4x4(C2[mp * 8], D1[mp * 8], acc);
+            acc = mfma4x4(D2[mp * 8], C1[mp * 8], acc);
+        }
+        if (valid) {
+            const float sc = k1 != k2 ? scale_l[l] * 1.4142135623730951f : scale_l[l];
+            float* dst = out + (long long)pair_row[p] * 3 * P + (long long)x * P + l * 64 + (4 * sb + q) * 8 + 4 * fa;
+            store4<float>(dst, acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
+        }
+    }
 }
 
 // Spherical-expansion position gradients from the planned windows: the same
@@ -2930,7 +3526,7 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
                          const int* species, const unsigned* type_mask, const long long* sample_scan,
                          const long long* row_scan, const long long* block_sample_base, const int* atom_system,
                          const int* atom_local, int max_runs, int max_entries, int* runs, int* run_rows, int* windows,
-                         int* n_windows, int* self_rows, int* labels, bool power_spectrum) {
+                         int* n_windows, int* self_rows, int* labels, bool power_spectrum, int* split) {
     if (n_atoms == 0) return;
     const size_t npairs = (size_t)n_species * (n_species + 1) / 2;
     const size_t shmem = npairs * (8 + 4 * 3);
@@ -2938,12 +3534,12 @@ void launch_ps_grad_plan(hipStream_t stream, int n_atoms, int n_species, const l
         ps_grad_plan_kernel<true><<<n_atoms, WAVE, shmem, stream>>>(
             n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base,
             atom_system, atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows,
-            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels);
+            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels, split);
     } else {
         ps_grad_plan_kernel<false><<<n_atoms, WAVE, shmem, stream>>>(
             n_atoms, n_species, offsets, entry_j, species, type_mask, sample_scan, row_scan, block_sample_base,
             atom_system, atom_local, max_runs, max_entries, reinterpret_cast<int4*>(runs), run_rows,
-            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels);
+            reinterpret_cast<int4*>(windows), n_windows, self_rows, labels, split);
     }
 }
 
@@ -2968,7 +3564,7 @@ template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,
                               const int* windows, const int* n_windows, const int* self_rows, const T* entry_data,
-                              T* out, bool fast) {
+                              T* out, bool fast, const int* only) {
     if (p.n_atoms == 0) return;
     SoapArgs<T> a = make_args(p);
     const size_t shmem = ps_gradient_tiled_shmem<T>(p, max_runs, max_entries, rows_per_pass);
@@ -2979,7 +3575,7 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
     const bool mixed = fast && sizeof(T) == 4 && ((p.n == 8 && max_runs <= 4) || p.n == 12 || p.n == 16);
 #define RA_LAUNCH_TILED(NCV, RPV, FV)                                                                                  \
     ps_gradient_tiled_kernel<T, NCV, RPV, FV><<<p.n_atoms, TILE_THREADS, shmem, p.stream>>>(                          \
-        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out)
+        a, max_runs, max_entries, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, entry_data, out, only)
 #define RA_LAUNCH_TILED_RP(NCV)       \
     if (rows_per_pass == 4) {         \
         RA_LAUNCH_TILED(NCV, 4, false);      \
@@ -3016,6 +3612,46 @@ void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entr
 #undef RA_LAUNCH_MIXED
 #undef RA_LAUNCH_TILED_RP
 #undef RA_LAUNCH_TILED
+}
+
+bool ps_gradient_rank2_supported(int L, int n, int n_species, int max_entries) {
+    return n == 8 && L >= 1 && L <= R2_MAX_L && n_species <= TILE_MAX_TYPES && (max_entries == 8 || max_entries == 16);
+}
+
+// dynamic plus static LDS of the rank-2 kernel
+size_t ps_gradient_rank2_lds(const SoapLaunch<float>& p, int max_entries) {
+    const Rank2Layout ly(p.L, p.n_species, max_entries);
+    const size_t fixed = TILE_MAX_TYPES * sizeof(int) + R2_MAX_L * sizeof(float) +
+                         (2 * R2_MAX_L * (R2_MAX_L + 1) / 2 + 4 * R2_MAX_L) * sizeof(float) + R2_MAX_L * R2_MAX_L;
+    return (size_t)ly.total * sizeof(float) + fixed + 256;
+}
+
+static size_t ps_self_rows_shmem(const SoapLaunch<float>& p) {
+    const size_t csp = (size_t)p.L * p.L * 8 + 8;
+    return ((size_t)p.n_species * 4 * csp + (size_t)p.n_species * (p.n_species + 1)) * sizeof(float);
+}
+
+void launch_ps_gradient_rank2(const SoapLaunch<float>& p, int max_entries, const float* coef, const float* dself,
+                              const unsigned* type_mask, const int* runs, const int* run_rows, const int* windows,
+                              const int* n_windows, const int* self_rows, const int* split, const float* entry_data,
+                              float* out) {
+    if (p.n_atoms == 0) return;
+    if (!ps_gradient_rank2_supported(p.L, p.n, p.n_species, max_entries)) {
+        throw std::invalid_argument("rank-2 gradient kernel: unsupported shape");
+    }
+    SoapArgs<float> a = make_args(p);
+    ps_self_rows_kernel<<<p.n_atoms, R2_THREADS, ps_self_rows_shmem(p), p.stream>>>(a, coef, dself, type_mask,
+                                                                                     self_rows, split, out);
+    const size_t shmem = (size_t)Rank2Layout(p.L, p.n_species, max_entries).total * sizeof(float);
+    const int4* rs = reinterpret_cast<const int4*>(runs);
+    const int4* ws = reinterpret_cast<const int4*>(windows);
+    if (max_entries == 16) {
+        ps_gradient_rank2_kernel<16><<<p.n_atoms, R2_THREADS, shmem, p.stream>>>(
+            a, coef, type_mask, rs, run_rows, ws, n_windows, split, entry_data, out);
+    } else {
+        ps_gradient_rank2_kernel<8><<<p.n_atoms, R2_THREADS, shmem, p.stream>>>(
+            a, coef, type_mask, rs, run_rows, ws, n_windows, split, entry_data, out);
+    }
 }
 
 template <typename T>
@@ -3261,7 +3897,7 @@ void launch_by_pair(const SoapLaunch<T>& p, long long n_samples, const ByPairSam
     template size_t ps_gradient_tiled_lds<T>(const SoapLaunch<T>&, int, int, int);                                     \
     template void launch_ps_gradient_tiled<T>(const SoapLaunch<T>&, int, int, int, const T*, const T*, const unsigned*, \
                                               const int*, const int*, const int*, const int*, const int*, const T*, T*, \
-                                              bool);                                                                   \
+                                              bool, const int*);                                                       \
     template size_t entry_prep_size<T>(const SoapLaunch<T>&, long long);                                               \
     template size_t expansion_tiled_shmem<T>(const SoapLaunch<T>&, int);                                               \
     template size_t se_gradient_tiled_shmem<T>(const SoapLaunch<T>&, int, int);                                        \

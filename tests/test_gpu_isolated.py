@@ -76,7 +76,7 @@ def check(tm, ref, precision, fast, floor):
 def test_isolated_centers_every_launch_shape(references, precision, fast):
     ps_ref, se_ref = references
     syst = [ra.System(s.types, s.positions, s.cell) for s in systems()]
-    windows = list(range(11)) + [-2, -1]
+    windows = list(range(13)) + [-2, -1]
     for kind, ref in (("power spectrum", ps_ref), ("spherical expansion", se_ref)):
         if kind == "spherical expansion" and fast:
             continue
