@@ -2348,21 +2348,14 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
             const float* geo = buf + ly.b_geo;
             // the lane's rows of the 16 x 16 products: image e = rho / 2, alpha (rho even) or beta
             const int RT = (2 * ec + 15) / 16;
-            float gd[RTM][3], ga[RTM][3], gb[RTM][3];
             bool ev[RTM];
-            int ybase[RTM];
+            int ybase[RTM], gbase[RTM];
 #pragma unroll
             for (int rt = 0; rt < RTM; rt++) {
                 const int e = (16 * rt + j16) >> 1;
                 ev[rt] = e < ec;
                 const int es_ = ev[rt] ? e : 0;
-                const float* g = geo + es_ * 12;
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    gd[rt][c] = g[c];
-                    ga[rt][c] = g[3 + c];
-                    gb[rt][c] = g[6 + c];
-                }
+                gbase[rt] = es_ * 12;
                 ybase[rt] = (es_ * L2 + jk) * 4;
             }
             // units: (type chunk, l descending, x), dealt in snake order over the compute waves
@@ -2376,12 +2369,14 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                 const int l = L - 1 - lx / 3, x = lx % 3;
                 const int lsq = l * l;
                 // (B) W[(t, n2)][(e, a)] = sum_m A_a[e][m] C_t[m][n2] for the chunk's four types
+                // the image's d_x, ga_x, gb_x (LDS reads: a select over x would become a
+                // private array in scratch memory, whose loads wait for the wave's stores)
                 float dx[RTM], gax[RTM], gbx[RTM];
 #pragma unroll
                 for (int rt = 0; rt < RTM; rt++) {
-                    dx[rt] = x == 0 ? gd[rt][0] : (x == 1 ? gd[rt][1] : gd[rt][2]);
-                    gax[rt] = x == 0 ? ga[rt][0] : (x == 1 ? ga[rt][1] : ga[rt][2]);
-                    gbx[rt] = x == 0 ? gb[rt][0] : (x == 1 ? gb[rt][1] : gb[rt][2]);
+                    dx[rt] = geo[gbase[rt] + x];
+                    gax[rt] = geo[gbase[rt] + 3 + x];
+                    gbx[rt] = geo[gbase[rt] + 6 + x];
                 }
                 const int kk0 = 4 * tc + (j16 >> 3);
                 const bool kv0 = kk0 < n_present, kv1 = kk0 + 2 < n_present;
