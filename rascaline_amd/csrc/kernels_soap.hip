@@ -2021,7 +2021,7 @@ constexpr int R2_COMPUTE = R2_WAVES - R2_LOADERS;
 struct Rank2Layout {
     // float offsets in the dynamic LDS
     int L, L2, csp, E, wp, S, per_entry;
-    int cp, st, st_runs, st_rows, st_raw, cms, einv, buf0, buf_size, w, total;
+    int cp, st, st_runs, st_rows, st_raw, cms, einv, buf0, buf_size, w, ds, pairs, total;
     // offsets within a window buffer
     int b_hdr, b_lo, b_hi, b_tj, b_row, b_ytab, b_rr, b_geo;
     __host__ __device__ Rank2Layout(int L_, int S_, int E_) : L(L_), L2(L_ * L_), E(E_), S(S_) {
@@ -2048,7 +2048,13 @@ struct Rank2Layout {
         einv = cms + E * L * 2;
         buf0 = einv + E * 4;
         w = buf0 + 2 * buf_size;  // per compute wave: W [32][wp]
-        total = w + R2_COMPUTE * 32 * wp;
+        const int window_end = w + R2_COMPUTE * 32 * wp;
+        // the center's own rows, before the windows: self gradients DS [present][3][csp]
+        // over the second buffer and the W slices, row of each present pair after them
+        ds = buf0 + buf_size;
+        const int ds_end = ds + S * 3 * csp;
+        pairs = window_end > ds_end ? window_end : ds_end;
+        total = pairs + S * S;
     }
 };
 
@@ -2253,14 +2259,68 @@ __device__ __forceinline__ void rank2_tables(const Rank2Layout& ly, const TileEn
     wave_lds_sync();
 }
 
+// The center's own rows s_l (DS_t1^T C_t2 + C_t1^T DS_t2) for every present pair
+// t1 <= t2 (soap/power_spectrum.rs:621-678), from the self gradients DS and the
+// coefficients C in LDS, on v_mfma_f32_16x16x4f32 by the compute waves: one product
+// per (x, l, 2 x 2 block of present-type pairs), rows (type t2, n2) and columns
+// (type t1, n1), so that lane j holds out[n1][4 h .. 4 h + 3] of the pair (t1, t2) — a
+// 16-byte piece of its row — with both terms accumulated in place:
+//   D[(t2, n2)][(t1, n1)] = sum_m C_t2[m][n2] DS_t1[m][n1] + DS_t2[m][n2] C_t1[m][n1].
+__device__ __forceinline__ void rank2_self_rows(int cw, int lane, int L, int n_present, int CSP, int P,
+                                                const float* Cp, const float* DS, const int* pair_row,
+                                                const float* scale_l, float* __restrict__ out) {
+    // blocks (b1 <= b2) of 2 x 2 type pairs: types 2 b1 + (0, 1) on the column side,
+    // 2 b2 + (0, 1) on the row side
+    const int nb = (n_present + 1) / 2;
+    const int nbp = nb * (nb + 1) / 2;
+    const int units = 3 * L * nbp;
+    const int j16 = lane & 15, jk = lane >> 4;
+    for (int u = cw; u < units; u += R2_COMPUTE) {
+        const int bp = u % nbp, lx = u / nbp;
+        const int l = L - 1 - lx / 3, x = lx % 3;
+        int b1 = 0, rem = bp;
+        while (rem >= nb - b1) {
+            rem -= nb - b1;
+            b1++;
+        }
+        const int b2 = b1 + rem;
+        const int k2 = 2 * b2 + (j16 >> 3), k1 = 2 * b1 + (j16 >> 3);
+        const bool kv2 = k2 < n_present, kv1 = k1 < n_present;
+        const int k2s = kv2 ? k2 : 0, k1s = kv1 ? k1 : 0;
+        const float* C2 = Cp + k2s * CSP + l * l * 8 + (lane & 7);
+        const float* D2 = DS + (k2s * 3 + x) * CSP + l * l * 8 + (lane & 7);
+        const float* C1 = Cp + k1s * CSP + l * l * 8 + (lane & 7);
+        const float* D1 = DS + (k1s * 3 + x) * CSP + l * l * 8 + (lane & 7);
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        const int ks_count = (2 * l + 1 + 3) / 4;
+        for (int ks = 0; ks < ks_count; ks++) {
+            const int mp = 4 * ks + jk;
+            const bool mv = mp <= 2 * l;
+            const int off = (mv ? mp : 0) * 8;
+            const float c2 = C2[off], d2 = D2[off], c1 = C1[off], d1 = D1[off];
+            acc = mfma16x16((mv && kv2) ? c2 : 0.f, (mv && kv1) ? d1 : 0.f, acc);
+            acc = mfma16x16((mv && kv2) ? d2 : 0.f, (mv && kv1) ? c1 : 0.f, acc);
+        }
+        // lane j: rows 4 (j / 16) + v = (t2 = 2 b2 + j / 32, n2 = 4 ((j / 16) % 2) + v),
+        // column j % 16 = (t1 = 2 b1 + (j % 16) / 8, n1 = j % 8)
+        const int t2k = 2 * b2 + (lane >> 5), t1k = 2 * b1 + (j16 >> 3);
+        if (t2k < n_present && t1k < n_present && t1k <= t2k) {
+            const float sc = t1k != t2k ? scale_l[l] * 1.4142135623730951f : scale_l[l];
+            float* dst = out + (long long)pair_row[t1k * n_present + t2k] * 3 * P + (long long)x * P + l * 64 +
+                         (lane & 7) * 8 + 4 * ((lane >> 4) & 1);
+            store4<float>(dst, acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
+        }
+    }
+}
+
 // (six waves per SIMD: three workgroups per CU; the loader's hoisted addresses must
 // not take the compute waves' registers)
 template <int E>
 __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) ps_gradient_rank2_kernel(
-    SoapArgs<float> a, const float* __restrict__ coef, const unsigned* __restrict__ type_mask,
-    const int4* __restrict__ runs, const int* __restrict__ run_rows, const int4* __restrict__ windows,
-    const int* __restrict__ n_windows, const int* __restrict__ split, const float* __restrict__ entry_data,
-    float* __restrict__ out) {
+    SoapArgs<float> a, const float* __restrict__ coef, const float* __restrict__ dself,
+    const unsigned* __restrict__ type_mask, const int4* __restrict__ runs, const int* __restrict__ run_rows,
+    const int4* __restrict__ windows, const int* __restrict__ n_windows, const int* __restrict__ self_rows,
+    const int* __restrict__ split, const float* __restrict__ entry_data, float* __restrict__ out) {
     constexpr int RTM = E / 8;  // row tiles of 16 (image, alpha / beta) rows
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int present[TILE_MAX_TYPES];
@@ -2271,7 +2331,8 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
     const int i = blockIdx.x;
     if (split[i] != 0) return;  // a run longer than a window: the tiled kernel's center
     const int nw = n_windows[i];
-    if (nw == 0) return;        // no neighbour rows (the center's own rows are another kernel's)
+    const unsigned mask0 = type_mask[i];
+    if (mask0 == 0u) return;  // no neighbour at all: no rows
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     // wave-uniform (a scalar register), so that every loop over units is uniform
@@ -2302,16 +2363,32 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
         while ((l + 1) * (l + 1) <= k) l++;
         lm_l[k] = (unsigned char)l;
     }
-    if (loader) rank2_fetch<E>(ly, lane, __popc(mask), e0, 0, windows, runs, run_rows, entry_data, lds);
+    if (loader && nw > 0) rank2_fetch<E>(ly, lane, __popc(mask), e0, 0, windows, runs, run_rows, entry_data, lds);
     lds_barrier();
-    // coefficients per present type, padded rows of csp floats
+    // coefficients and self gradients per present type (padded rows of csp floats), and
+    // the row of each present pair, all read before the first store
+    float* DS = lds + ly.ds;
+    int* pair_row = reinterpret_cast<int*>(lds + ly.pairs);
+    const int npairs = S * (S + 1) / 2;
     for (int k = tid; k < n_present * L2 * 8; k += R2_THREADS) {
         const int kk = k / (L2 * 8), r = k - kk * L2 * 8;
         Cp[kk * CSP + r] = coef[((long long)i * S + present[kk]) * L2 * 8 + r];
     }
+    for (int k = tid; k < n_present * 3 * L2 * 8; k += R2_THREADS) {
+        const int kx = k / (L2 * 8), r = k - kx * L2 * 8;
+        const int kk = kx / 3, x = kx - 3 * kk;
+        DS[kx * CSP + r] = dself[(((long long)i * S + present[kk]) * 3 + x) * L2 * 8 + r];
+    }
+    for (int k = tid; k < n_present * n_present; k += R2_THREADS) {
+        const int k1 = k / n_present, k2 = k - k1 * n_present;
+        if (k1 <= k2) pair_row[k] = self_rows[(long long)i * npairs + pair_index(present[k1], present[k2], S)];
+    }
+    if (loader && nw > 0) rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0);
+    lds_barrier();
     if (loader) {
-        rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0);
         if (nw > 1) rank2_fetch<E>(ly, lane, n_present, e0, 1, windows, runs, run_rows, entry_data, lds);
+    } else {
+        rank2_self_rows(wave_id - R2_LOADERS, lane, L, n_present, CSP, P, Cp, DS, pair_row, scale_l, out);
     }
     lds_barrier();
 
@@ -2467,93 +2544,6 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
             }
         }
         lds_barrier();
-    }
-}
-
-// The center's own rows s_l (DS_t1^T C_t2 + C_t1^T DS_t2) for every present pair
-// t1 <= t2 (soap/power_spectrum.rs:621-678), from the self gradients DS and the
-// coefficients C in LDS, with 4 x 4 products over m on v_mfma_f32_4x4x1f32: one
-// workgroup per center, four pairs per instruction (block = (pair, fast quad, slow
-// quad), lane 4 b + c holding a 16-byte piece of the row). Centers flagged in
-// `split` are left to the tiled kernel.
-__global__ void __launch_bounds__(R2_THREADS) ps_self_rows_kernel(SoapArgs<float> a, const float* __restrict__ coef,
-                                                                   const float* __restrict__ dself,
-                                                                   const unsigned* __restrict__ type_mask,
-                                                                   const int* __restrict__ self_rows,
-                                                                   const int* __restrict__ split,
-                                                                   float* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int present[TILE_MAX_TYPES];
-    __shared__ float scale_l[R2_MAX_L];
-    const int i = blockIdx.x;
-    if (split[i] != 0) return;
-    const int tid = threadIdx.x;
-    const int lane = tid & (WAVE - 1);
-    const int wave_id = uniform(tid / WAVE);
-    const int S = a.n_species, L = a.L;
-    const int L2 = L * L;
-    const int CSP = L2 * 8 + 8;
-    const int P = L * 64;
-    const int npairs = S * (S + 1) / 2;
-    const unsigned mask = type_mask[i];
-    const int n_present = __popc(mask);
-    const int npp = n_present * (n_present + 1) / 2;
-    if (npp == 0) return;
-    float* Cp = reinterpret_cast<float*>(smem);
-    float* DS = Cp + S * CSP;
-    int* pair_row = reinterpret_cast<int*>(DS + S * 3 * CSP);
-    int* pair_k = pair_row + npairs;
-    if (tid == 0) {
-        int k = 0;
-        for (int t = 0; t < S; t++)
-            if ((mask >> t) & 1u) present[k++] = t;
-    }
-    if (tid < L) scale_l[tid] = (float)((tid % 2 == 0 ? 1.0 : -1.0) / sqrt(2.0 * tid + 1.0));
-    lds_barrier();
-    for (int k = tid; k < n_present * L2 * 8; k += R2_THREADS) {
-        const int kk = k / (L2 * 8), r = k - kk * L2 * 8;
-        Cp[kk * CSP + r] = coef[((long long)i * S + present[kk]) * L2 * 8 + r];
-    }
-    for (int k = tid; k < n_present * 3 * L2 * 8; k += R2_THREADS) {
-        const int kx = k / (L2 * 8), r = k - kx * L2 * 8;
-        const int kk = kx / 3, x = kx - 3 * kk;
-        DS[kx * CSP + r] = dself[(((long long)i * S + present[kk]) * 3 + x) * L2 * 8 + r];
-    }
-    if (tid < npp) {
-        int k1 = 0, rem = tid;
-        while (rem >= n_present - k1) {
-            rem -= n_present - k1;
-            k1++;
-        }
-        const int k2 = k1 + rem;
-        pair_k[tid] = k1 | (k2 << 8);
-        pair_row[tid] = self_rows[(long long)i * npairs + pair_index(present[k1], present[k2], S)];
-    }
-    lds_barrier();
-    const int fa = (lane >> 3) & 1, sb = (lane >> 2) & 1, q = lane & 3;
-    const int ng = (npp + 3) / 4;
-    const int units = 3 * L * ng;
-    for (int u = wave_id; u < units; u += R2_WAVES) {
-        const int g = u % ng, lx = u / ng;
-        const int l = L - 1 - lx / 3, x = lx % 3;
-        const int p = 4 * g + (lane >> 4);
-        const bool valid = p < npp;
-        const int pk = pair_k[valid ? p : 0];
-        const int k1 = pk & 255, k2 = pk >> 8;
-        const float* C1 = Cp + k1 * CSP + l * l * 8 + 4 * sb + q;
-        const float* C2 = Cp + k2 * CSP + l * l * 8 + 4 * fa + q;
-        const float* D1 = DS + (k1 * 3 + x) * CSP + l * l * 8 + 4 * sb + q;
-        const float* D2 = DS + (k2 * 3 + x) * CSP + l * l * 8 + 4 * fa + q;
-        float4_t acc = {0.f, 0.f, 0.f, 0.f};
-        for (int mp = 0; mp <= 2 * l; mp++) {
-            acc = mfma4x4(C2[mp * 8], D1[mp * 8], acc);
-            acc = mfma4x4(D2[mp * 8], C1[mp * 8], acc);
-        }
-        if (valid) {
-            const float sc = k1 != k2 ? scale_l[l] * 1.4142135623730951f : scale_l[l];
-            float* dst = out + (long long)pair_row[p] * 3 * P + (long long)x * P + l * 64 + (4 * sb + q) * 8 + 4 * fa;
-            store4<float>(dst, acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
-        }
     }
 }
 
@@ -3621,11 +3611,6 @@ size_t ps_gradient_rank2_lds(const SoapLaunch<float>& p, int max_entries) {
     return (size_t)ly.total * sizeof(float) + fixed + 256;
 }
 
-static size_t ps_self_rows_shmem(const SoapLaunch<float>& p) {
-    const size_t csp = (size_t)p.L * p.L * 8 + 8;
-    return ((size_t)p.n_species * 4 * csp + (size_t)p.n_species * (p.n_species + 1)) * sizeof(float);
-}
-
 void launch_ps_gradient_rank2(const SoapLaunch<float>& p, int max_entries, const float* coef, const float* dself,
                               const unsigned* type_mask, const int* runs, const int* run_rows, const int* windows,
                               const int* n_windows, const int* self_rows, const int* split, const float* entry_data,
@@ -3635,17 +3620,15 @@ void launch_ps_gradient_rank2(const SoapLaunch<float>& p, int max_entries, const
         throw std::invalid_argument("rank-2 gradient kernel: unsupported shape");
     }
     SoapArgs<float> a = make_args(p);
-    ps_self_rows_kernel<<<p.n_atoms, R2_THREADS, ps_self_rows_shmem(p), p.stream>>>(a, coef, dself, type_mask,
-                                                                                     self_rows, split, out);
     const size_t shmem = (size_t)Rank2Layout(p.L, p.n_species, max_entries).total * sizeof(float);
     const int4* rs = reinterpret_cast<const int4*>(runs);
     const int4* ws = reinterpret_cast<const int4*>(windows);
     if (max_entries == 16) {
         ps_gradient_rank2_kernel<16><<<p.n_atoms, R2_THREADS, shmem, p.stream>>>(
-            a, coef, type_mask, rs, run_rows, ws, n_windows, split, entry_data, out);
+            a, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, split, entry_data, out);
     } else {
         ps_gradient_rank2_kernel<8><<<p.n_atoms, R2_THREADS, shmem, p.stream>>>(
-            a, coef, type_mask, rs, run_rows, ws, n_windows, split, entry_data, out);
+            a, coef, dself, type_mask, rs, run_rows, ws, n_windows, self_rows, split, entry_data, out);
     }
 }
 
