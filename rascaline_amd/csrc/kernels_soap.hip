@@ -2023,17 +2023,15 @@ struct Rank2Layout {
     int L, L2, csp, E, wp, S, per_entry;
     int cp, st, st_runs, st_rows, st_raw, cms, einv, buf0, buf_size, w, ds, pairs, total;
     // offsets within a window buffer
-    int b_hdr, b_lo, b_hi, b_tj, b_row, b_ytab, b_rr, b_geo;
+    int b_hdr, b_run, b_row, b_ytab, b_rr, b_geo;
     __host__ __device__ Rank2Layout(int L_, int S_, int E_) : L(L_), L2(L_ * L_), E(E_), S(S_) {
         csp = L2 * 8 + 8;  // padded per present type: two types' column reads meet no bank conflict
         wp = 2 * E + 2;    // (e, a) pairs of one (t, n2) column, padded for conflict-free 8-byte reads
         per_entry = TileEntryLayout(L, 8, 8).per_entry;
         b_hdr = 0;         // images, runs
-        b_lo = 4;
-        b_hi = b_lo + TILE_MAX_RUNS;
-        b_tj = b_hi + TILE_MAX_RUNS;
-        b_row = b_tj + TILE_MAX_RUNS;
-        b_ytab = (b_row + TILE_MAX_RUNS * S + 3) / 4 * 4;  // (Y, T1, T2, -) per (image, lm)
+        b_run = 4;                         // per run {neighbour type, first image, images, -}
+        b_row = b_run + 4 * TILE_MAX_RUNS;  // per (run, present type): 64-bit element offset of the row
+        b_ytab = (b_row + 2 * TILE_MAX_RUNS * S + 3) / 4 * 4;  // (Y, T1, T2, -) per (image, lm)
         b_rr = b_ytab + E * L2 * 4;                        // (ra, rb) per (image, l, c)
         b_geo = b_rr + E * L * 16;                         // d, ga, gb per image
         buf_size = (b_geo + E * 12 + 3) / 4 * 4;
@@ -2162,14 +2160,15 @@ __device__ __forceinline__ void rank2_tables(const Rank2Layout& ly, const TileEn
     }
     if (lane < rc) {
         const int* ir = reinterpret_cast<const int*>(lds + ly.st_runs) + 4 * lane;
-        ib[ly.b_lo + lane] = ir[0] - es;
-        ib[ly.b_hi + lane] = ir[0] + ir[1] - es;
-        ib[ly.b_tj + lane] = ir[2];
+        ib[ly.b_run + 4 * lane + 0] = ir[2];
+        ib[ly.b_run + 4 * lane + 1] = ir[0] - es;
+        ib[ly.b_run + 4 * lane + 2] = ir[1];
     }
+    long long* rowoff = reinterpret_cast<long long*>(buf + ly.b_row);
 #pragma unroll 1
     for (int it = lane; it < rc * n_present; it += WAVE) {
         const int r = it / n_present, k = it - r * n_present;
-        ib[ly.b_row + r * S + k] = reinterpret_cast<const int*>(lds + ly.st_rows)[r * S + k];
+        rowoff[r * S + k] = (long long)reinterpret_cast<const int*>(lds + ly.st_rows)[r * S + k] * (3 * 64 * L);
     }
     // cos(m phi), sin(m phi) by the recurrence of spherical_harmonics.rs (same signs), and
     // the reciprocals of the Legendre derivative terms, per image
@@ -2486,6 +2485,8 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                         }
                     }
                 }
+                // W carries the power spectrum's s_l = (-1)^l / sqrt(2 l + 1)
+                const float s_l = uniform(scale_l[l]);
 #pragma unroll
                 for (int rt = 0; rt < RTM; rt++) {
                     if (rt < RT) {
@@ -2493,7 +2494,7 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                         for (int ct = 0; ct < 2; ct++) {
                             float* wdst = Wv + (16 * ct + j16) * WP + 16 * rt + 4 * jk;
 #pragma unroll
-                            for (int v = 0; v < 4; v++) wdst[v] = accb[rt][ct][v];
+                            for (int v = 0; v < 4; v++) wdst[v] = accb[rt][ct][v] * s_l;
                         }
                     }
                 }
@@ -2506,13 +2507,14 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                 const int t_lane = present[kslot];
                 const int offWA = (8 * tau + 4 * fa + q) * WP, offWB = (8 * tau + 4 * sb + q) * WP;
                 const int offRA = (l * 8 + 4 * fa + q) * 2, offRB = (l * 8 + 4 * sb + q) * 2;
-                const float sc_same = uniform(scale_l[l]);
-                const float sc_diff = sc_same * 1.4142135623730951f;
+                const long long* rowoff = reinterpret_cast<const long long*>(buf + ly.b_row) + kslot;
                 float* out_unit = out + (long long)x * P + l * 64 + piece;
-                for (int r = 0; r < rc; r++) {
-                    const int tj = uniform(ib[ly.b_tj + r]);
-                    const int lo = uniform(ib[ly.b_lo + r]);
-                    const int cnt = uniform(ib[ly.b_hi + r]) - lo;
+#pragma unroll
+                for (int r = 0; r < TILE_MAX_RUNS; r++) {
+                    if (r >= rc) break;
+                    // the run (neighbour type, first image, images), wave-uniform
+                    const int4 rd = *reinterpret_cast<const int4*>(ib + ly.b_run + 4 * r);
+                    const int tj = uniform(rd.x), lo = uniform(rd.y), cnt = uniform(rd.z);
                     const bool f1 = t_lane >= tj, f2 = t_lane <= tj;
                     const bool both = __ballot(lane_type && f1 && f2) != 0ull;
                     const int oA = f1 ? offWA : offRA, oB = f1 ? offRB : offWB;
@@ -2535,9 +2537,9 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                         }
                     }
                     if (lane_type) {
-                        const float sc = t_lane != tj ? sc_diff : sc_same;
-                        float* dst = out_unit + (long long)ib[ly.b_row + r * S + kslot] * 3 * P;
-                        store4<float>(dst, acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
+                        // sqrt(2) for the blocks of two different types
+                        const float sc = f1 && f2 ? 1.0f : 1.4142135623730951f;
+                        store4<float>(out_unit + rowoff[r * S], acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
                     }
                 }
                 wave_lds_sync();
