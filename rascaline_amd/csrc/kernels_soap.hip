@@ -2023,14 +2023,14 @@ struct Rank2Layout {
     int L, L2, csp, E, wp, S, per_entry;
     int cp, st, st_runs, st_rows, st_raw, cms, einv, buf0, buf_size, w, ds, pairs, total;
     // offsets within a window buffer
-    int b_hdr, b_run, b_row, b_ytab, b_rr, b_geo;
+    int b_hdr, b_img, b_row, b_ytab, b_rr, b_geo;
     __host__ __device__ Rank2Layout(int L_, int S_, int E_) : L(L_), L2(L_ * L_), E(E_), S(S_) {
         csp = L2 * 8 + 8;  // padded per present type: two types' column reads meet no bank conflict
         wp = 2 * E + 2;    // (e, a) pairs of one (t, n2) column, padded for conflict-free 8-byte reads
         per_entry = TileEntryLayout(L, 8, 8).per_entry;
         b_hdr = 0;         // images, runs
-        b_run = 4;                         // per run {neighbour type, first image, images, -}
-        b_row = b_run + 4 * TILE_MAX_RUNS;  // per (run, present type): 64-bit element offset of the row
+        b_img = 4;              // per image {neighbour type, first of its run, last of its run, run}
+        b_row = b_img + 4 * 16;  // per (run, present type): 64-bit element offset of the row
         b_ytab = (b_row + 2 * TILE_MAX_RUNS * S + 3) / 4 * 4;  // (Y, T1, T2, -) per (image, lm)
         b_rr = b_ytab + E * L2 * 4;                        // (ra, rb) per (image, l, c)
         b_geo = b_rr + E * L * 16;                         // d, ga, gb per image
@@ -2158,11 +2158,16 @@ __device__ __forceinline__ void rank2_tables(const Rank2Layout& ly, const TileEn
         ib[ly.b_hdr] = ec;
         ib[ly.b_hdr + 1] = rc;
     }
-    if (lane < rc) {
-        const int* ir = reinterpret_cast<const int*>(lds + ly.st_runs) + 4 * lane;
-        ib[ly.b_run + 4 * lane + 0] = ir[2];
-        ib[ly.b_run + 4 * lane + 1] = ir[0] - es;
-        ib[ly.b_run + 4 * lane + 2] = ir[1];
+    if (lane < ec) {
+        // the run of image `lane` (runs are consecutive image ranges covering the window)
+        const int* ir = reinterpret_cast<const int*>(lds + ly.st_runs);
+        int r = 0;
+        while (r + 1 < rc && ir[4 * (r + 1)] - es <= lane) r++;
+        const int lo = ir[4 * r] - es, cnt = ir[4 * r + 1];
+        ib[ly.b_img + 4 * lane + 0] = ir[4 * r + 2];
+        ib[ly.b_img + 4 * lane + 1] = lane == lo ? 1 : 0;
+        ib[ly.b_img + 4 * lane + 2] = lane == lo + cnt - 1 ? 1 : 0;
+        ib[ly.b_img + 4 * lane + 3] = r;
     }
     long long* rowoff = reinterpret_cast<long long*>(buf + ly.b_row);
 #pragma unroll 1
@@ -2509,36 +2514,33 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
                 const int offRA = (l * 8 + 4 * fa + q) * 2, offRB = (l * 8 + 4 * sb + q) * 2;
                 const long long* rowoff = reinterpret_cast<const long long*>(buf + ly.b_row) + kslot;
                 float* out_unit = out + (long long)x * P + l * 64 + piece;
+                // image by image (unrolled, no inner loop), the sums of a run's images in the
+                // accumulators, its block stored at its last image
+                const float* wA = Wv + offWA;
+                const float* wB = Wv + offWB;
+                const float* rA = rr + offRA;
+                const float* rB = rr + offRB;
+                float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int r = 0; r < TILE_MAX_RUNS; r++) {
-                    if (r >= rc) break;
-                    // the run (neighbour type, first image, images), wave-uniform
-                    const int4 rd = *reinterpret_cast<const int4*>(ib + ly.b_run + 4 * r);
-                    const int tj = uniform(rd.x), lo = uniform(rd.y), cnt = uniform(rd.z);
-                    const bool f1 = t_lane >= tj, f2 = t_lane <= tj;
-                    const bool both = __ballot(lane_type && f1 && f2) != 0ull;
-                    const int oA = f1 ? offWA : offRA, oB = f1 ? offRB : offWB;
-                    float4_t acc = {0.f, 0.f, 0.f, 0.f};
-                    for (int eo = 0; eo < cnt; eo++) {
-                        const int e = lo + eo;
-                        const float* wb = Wv + 2 * e;
-                        const float* rb = rr + e * (16 * L);
-                        const float2 A = *reinterpret_cast<const float2*>((f1 ? wb : rb) + oA);
-                        const float2 B = *reinterpret_cast<const float2*>((f1 ? rb : wb) + oB);
-                        acc = mfma4x4(A.x, B.x, acc);
-                        acc = mfma4x4(A.y, B.y, acc);
-                        if (both) {
-                            float2 A2 = *reinterpret_cast<const float2*>(rb + offRA);
-                            const float2 B2 = *reinterpret_cast<const float2*>(wb + offWB);
-                            A2.x = f1 && f2 ? A2.x : 0.f;
-                            A2.y = f1 && f2 ? A2.y : 0.f;
-                            acc = mfma4x4(A2.x, B2.x, acc);
-                            acc = mfma4x4(A2.y, B2.y, acc);
-                        }
-                    }
-                    if (lane_type) {
+                for (int e = 0; e < E; e++) {
+                    if (e >= ec) break;
+                    const int4 rd = *reinterpret_cast<const int4*>(ib + ly.b_img + 4 * e);
+                    const int tj = uniform(rd.x), first = uniform(rd.y), last = uniform(rd.z), r = uniform(rd.w);
+                    const bool f1 = t_lane >= tj, f2 = t_lane <= tj, fb = f1 && f2;
+                    const float2 W_a = *reinterpret_cast<const float2*>(wA + 2 * e);
+                    const float2 W_b = *reinterpret_cast<const float2*>(wB + 2 * e);
+                    const float2 R_a = *reinterpret_cast<const float2*>(rA + e * (16 * L));
+                    const float2 R_b = *reinterpret_cast<const float2*>(rB + e * (16 * L));
+                    if (first) acc = float4_t{0.f, 0.f, 0.f, 0.f};
+                    // X (t >= tj): (u, v) fast, (ra, rb) slow; X^T (t < tj) the reverse;
+                    // both when t == tj
+                    acc = mfma4x4(f1 ? W_a.x : R_a.x, f1 ? R_b.x : W_b.x, acc);
+                    acc = mfma4x4(f1 ? W_a.y : R_a.y, f1 ? R_b.y : W_b.y, acc);
+                    acc = mfma4x4(fb ? R_a.x : 0.f, W_b.x, acc);
+                    acc = mfma4x4(fb ? R_a.y : 0.f, W_b.y, acc);
+                    if (last && lane_type) {
                         // sqrt(2) for the blocks of two different types
-                        const float sc = f1 && f2 ? 1.0f : 1.4142135623730951f;
+                        const float sc = fb ? 1.0f : 1.4142135623730951f;
                         store4<float>(out_unit + rowoff[r * S], acc[0] * sc, acc[1] * sc, acc[2] * sc, acc[3] * sc);
                     }
                 }
