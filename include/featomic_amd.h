@@ -31,12 +31,14 @@ extern "C" {
 featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* calculator, int32_t bits);
 
 /* Select the fast form (1) or the plain form (0, default) of the FP32 kernels. The
- * fast form runs the power-spectrum m-contractions on the matrix cores with
+ * fast form runs the power-spectrum values' m-contraction on the matrix cores with
  * bfloat16 hi + lo operand splitting and FP32 accumulation (three products per
  * block, about 16 significant bits per operand; results within 1e-4 of the FP64
- * computation): the values for n a multiple of 4 and at most 16 types, the
- * neighbour gradient rows for n = 8, 12 or 16. Other settings, FP64 and the
- * center's own gradient rows use the plain kernels. */
+ * computation) for n a multiple of 4 and at most 16 types. Its gradient rows take
+ * the fastest contraction the autotuner finds among the matrix-core ones: the
+ * rank-2 kernel (FP32 operands, n = 8, the default of both forms) or the bfloat16
+ * hi + lo tiled kernel (n = 8, 12 or 16). Other settings and FP64 use the plain
+ * kernels. */
 featomic_status_t featomic_amd_calculator_set_fast_form(featomic_calculator_t* calculator, int32_t enabled);
 
 /* Half neighbour list of one system (pairs sorted by first, second, then

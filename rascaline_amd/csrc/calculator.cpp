@@ -722,8 +722,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
                 for (const auto& w : default_windows) windows.push_back({{w[0], w[1], w[2], w[3]}, whole});
             }
             for (const auto& [w, budget] : windows) {
-                const bool rank2 = w[3] != 0 && ps && sizeof(T) == 4 && fast_form == 0 &&
-                                   ps_gradient_rank2_supported(L, n, S, w[1]);
+                // (in both forms: its contraction is on MFMA with FP32 operands)
+                const bool rank2 = w[3] != 0 && ps && sizeof(T) == 4 && ps_gradient_rank2_supported(L, n, S, w[1]);
                 // a rank-2 shape of the built-in table is for the rank-2 kernel only; chosen
                 // explicitly, it runs the tiled kernel where the rank-2 one does not apply
                 if (w[3] != 0 && !rank2 && !explicit_shape) continue;
@@ -1149,14 +1149,15 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
         // 1. gradient rows: the built-in choice, every window shape of the tiled kernels
         //    within the whole LDS, the untiled kernel
         if (grads.positions) {
-            // the fast form keeps its mixed-precision contraction: only the shapes the
-            // mixed kernel covers (n = 8: windows of up to 4 runs; n = 12, 16: all)
+            // the fast form keeps an MFMA contraction: the rank-2 shapes and the shapes the
+            // mixed-precision tiled kernel covers (n = 8: windows of up to 4 runs; n = 12,
+            // 16: all)
             const int n = soap_.n_radial;
             const bool mixed = kind_ == Kind::PowerSpectrum && fast_form != 0 && precision == 32 &&
                                (n == 8 || n == 12 || n == 16);
             std::vector<int> candidates = {-1};
             for (int k = 0; k < static_cast<int>(soap_window_shapes().size()); k++) {
-                if (mixed && n == 8 && soap_window_shapes()[k][0] > 4) continue;
+                if (mixed && n == 8 && soap_window_shapes()[k][0] > 4 && soap_window_shapes()[k][3] == 0) continue;
                 // the spherical-expansion rows leave without store staging
                 if (kind_ != Kind::PowerSpectrum && soap_window_shapes()[k][2] != 1) continue;
                 candidates.push_back(k);
