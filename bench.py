@@ -358,8 +358,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(args.precision, rows_total, name),
-            "kernel": "ps_gradient_tiled_kernel (power-spectrum position-gradient rows)"
-                      + (", bfloat16 hi+lo MFMA contraction" if fast else ""),
+            "kernel": (("ps_gradient_rank2_kernel (power-spectrum position-gradient rows; the tiled kernel "
+                        "takes the centers with a run longer than a window)")
+                       if tuning_report["chosen"]["gradient_window"] in (-1, 11, 12) and args.precision == 32 else
+                       "ps_gradient_tiled_kernel (power-spectrum position-gradient rows"
+                       + (", bfloat16 hi+lo MFMA contraction)" if fast else ")")),
             "kernel_ms_per_step": kernel_ms,
             "device_ms_per_step": device_ms,
             "algorithmic_bytes_per_step": total_bytes,

@@ -2080,11 +2080,13 @@ __device__ __forceinline__ float uniform(float v) {
 // area, every load issued before the first wait (records of up to E images at L <= 7
 // in registers; larger bases in several passes).
 template <int E>
-__device__ __attribute__((noinline)) void rank2_fetch(const Rank2Layout& ly, int lane, int n_present, long long e0, int w,
-                                            const int4* __restrict__ windows, const int4* __restrict__ runs,
-                                            const int* __restrict__ run_rows, const float* __restrict__ entry_data,
-                                            float* lds) {
-    const int S = ly.S;
+__device__ __attribute__((noinline)) void rank2_fetch(int L, int S, int lane, int n_present, long long e0, int w,
+                                                      const int4* __restrict__ windows, const int4* __restrict__ runs,
+                                                      const int* __restrict__ run_rows,
+                                                      const float* __restrict__ entry_data, float* lds) {
+    // arguments by value: a layout passed by reference would live in scratch memory,
+    // written by every wave at the kernel's start
+    const Rank2Layout ly(L, S, E);
     const int4 wd = windows[e0 + w];
     const int es = wd.x, ec = wd.y, rs = wd.z, rc = wd.w;
     const float4_t* src = reinterpret_cast<const float4_t*>(entry_data + (e0 + es) * ly.per_entry);
@@ -2367,7 +2369,7 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
         while ((l + 1) * (l + 1) <= k) l++;
         lm_l[k] = (unsigned char)l;
     }
-    if (loader && nw > 0) rank2_fetch<E>(ly, lane, __popc(mask), e0, 0, windows, runs, run_rows, entry_data, lds);
+    if (loader && nw > 0) rank2_fetch<E>(L, S, lane, __popc(mask), e0, 0, windows, runs, run_rows, entry_data, lds);
     lds_barrier();
     // coefficients and self gradients per present type (padded rows of csp floats), and
     // the row of each present pair, all read before the first store
@@ -2390,7 +2392,7 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
     if (loader && nw > 0) rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0);
     lds_barrier();
     if (loader) {
-        if (nw > 1) rank2_fetch<E>(ly, lane, n_present, e0, 1, windows, runs, run_rows, entry_data, lds);
+        if (nw > 1) rank2_fetch<E>(L, S, lane, n_present, e0, 1, windows, runs, run_rows, entry_data, lds);
     } else {
         rank2_self_rows(wave_id - R2_LOADERS, lane, L, n_present, CSP, P, Cp, DS, pair_row, scale_l, out);
     }
@@ -2413,7 +2415,7 @@ __global__ void __launch_bounds__(R2_THREADS) __attribute__((amdgpu_waves_per_eu
         for (int w = 0; w < nw; w++) {
             if (w + 1 < nw) {
                 rank2_tables(ly, lay, tb, shc, lm_l, lane, n_present, lds, lds + ly.buf0 + ((w + 1) & 1) * ly.buf_size);
-                if (w + 2 < nw) rank2_fetch<E>(ly, lane, n_present, e0, w + 2, windows, runs, run_rows, entry_data, lds);
+                if (w + 2 < nw) rank2_fetch<E>(L, S, lane, n_present, e0, w + 2, windows, runs, run_rows, entry_data, lds);
             }
             lds_barrier();
         }
