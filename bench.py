@@ -357,7 +357,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.precision, rows_total, name),
+            "traffic": measured_traffic(args.precision, rows_total, "vector-alu" if window is not None else
+                                        "fast" if fast else "plain"),
             "kernel": (("ps_gradient_rank2_kernel (power-spectrum position-gradient rows; the tiled kernel "
                         "takes the centers with a run longer than a window)")
                        if tuning_report["chosen"]["gradient_window"] in (-1, 11, 12) and args.precision == 32 else
