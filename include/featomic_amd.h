@@ -185,8 +185,11 @@ featomic_status_t featomic_amd_calculator_autotune(featomic_calculator_t* calcul
                                                   const char* const* gradients, uintptr_t gradients_count, void* stream,
                                                   int32_t repeats, char* report, uintptr_t report_size);
 
-/* Launch shapes in use: `expansion_entries` 0 (built-in) or a power of two up to
- * 64; `gradient_window` -1 (built-in), -2 (untiled gradient kernel) or the index
+/* Launch shapes in use: `expansion_entries` 0 (built-in), a power of two up to
+ * 64 (entries per chunk of the vector-ALU expansion kernel), or -8 / -16 (the
+ * matrix-core expansion kernel with 8 or 16 entries per chunk, FP32 with 8
+ * radial functions and max_angular < 9; elsewhere the vector-ALU kernel);
+ * `gradient_window` -1 (built-in), -2 (untiled gradient kernel) or the index
  * of a window shape of the autotuner's list (as its report numbers them). */
 featomic_status_t featomic_amd_calculator_set_tuning(featomic_calculator_t* calculator, int32_t expansion_entries,
                                                      int32_t gradient_window);

@@ -269,7 +269,7 @@ class CalculatorBase:
 
     @property
     def tuning(self) -> dict:
-        """Launch shapes in use: {"expansion_entries": 0 or 2^k, "gradient_window": -2, -1 or k}."""
+        """Launch shapes in use: {"expansion_entries": 0, 2^k, -8 or -16, "gradient_window": -2, -1 or k}."""
         e, w = ctypes.c_int32(), ctypes.c_int32()
         C.check(self._lib.featomic_amd_calculator_get_tuning(self._as_parameter_, ctypes.byref(e), ctypes.byref(w)))
         return {"expansion_entries": e.value, "gradient_window": w.value}

@@ -346,6 +346,15 @@ void Calculator::use_radial_surrogate(bool enabled) {
 #define RA_EXPANSION_ENTRIES 8
 #endif
 static_assert(RA_EXPANSION_ENTRIES >= 1 && RA_EXPANSION_ENTRIES <= 64, "expansion chunks hold at most one wave of entries");
+// entries per chunk of the matrix-core expansion kernel, the built-in choice where it
+// applies (8 or 16; 0 keeps the vector-ALU kernel). On the 4096-structure workload a
+// 56 000-atom chunk takes 1.59 ms with 8, 1.68 ms with 16, and 2.37 ms on the
+// vector-ALU kernel (8 entries)
+#ifndef RA_EXPANSION_MFMA_ENTRIES
+#define RA_EXPANSION_MFMA_ENTRIES 8
+#endif
+static_assert(RA_EXPANSION_MFMA_ENTRIES == 0 || RA_EXPANSION_MFMA_ENTRIES == 8 || RA_EXPANSION_MFMA_ENTRIES == 16,
+              "the matrix-core expansion kernel takes chunks of 8 or 16 entries");
 // (neighbour runs, entries, staged tile rows, rank-2) per window of the gradient
 // kernels, tried in order until one applies and fits the LDS budget
 #ifndef RA_TILED_WINDOWS
@@ -570,12 +579,21 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         launch_entry_prep<T>(lp, nl.n_entries, entry_data);
         // entries per chunk: the tuned or built-in count, halved until two workgroups fit
         // per CU (large bases: the per-type sums alone take ~62 KB at n = 12, l <= 8)
-        int max_entries = tune.expansion_entries > 0 ? tune.expansion_entries : RA_EXPANSION_ENTRIES;
+        // a negative count asks for the matrix-core kernel with that many entries per
+        // chunk (FP32, n = 8, l_max < 9; elsewhere the tiled kernel with as many)
+        const int asked = tune.expansion_entries < 0 ? -tune.expansion_entries : tune.expansion_entries;
+        const bool mfma = std::is_same<T, float>::value && expansion_mfma_supported(L, n, S) &&
+                          (tune.expansion_entries < 0 || (tune.expansion_entries == 0 && RA_EXPANSION_MFMA_ENTRIES > 0));
+        int max_entries = asked > 0 ? asked : RA_EXPANSION_ENTRIES;
         while (max_entries > 1 && expansion_tiled_shmem<T>(lp, max_entries) + 1024 > static_cast<size_t>(lds) / 2) {
             max_entries /= 2;
         }
         mark(ev_exp_start_, stream);
-        if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
+        if (mfma) {
+            if constexpr (std::is_same<T, float>::value) {
+                launch_expansion_mfma(lp, asked > 0 ? asked : RA_EXPANSION_MFMA_ENTRIES, entry_data, coef, dself);
+            }
+        } else if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
             launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);
         } else {
             launch_soap_center<T>(lp, 0, coef, dself);
@@ -1192,11 +1210,18 @@ std::string Calculator::autotune(const HostBatch& batch, const Gradients& grads,
             }
             report += "]";
         }
-        // 2. neighbour entries per LDS chunk of the expansion kernel
+        // 2. neighbour entries per LDS chunk of the expansion kernel, and the matrix-core
+        //    expansion kernel (negative counts) where it applies
         report += ", \"expansion_entries\": [";
         double best_ms = 1e300;
         bool first = true;
-        for (int e : {4, 8, 16, 32, 64}) {
+        std::vector<int> entry_counts = {4, 8, 16, 32, 64};
+        // (the compute path falls back to the vector-ALU kernel where it does not apply)
+        if (precision == 32 && expansion_mfma_supported(soap_.max_angular + 1, soap_.n_radial, 1)) {
+            entry_counts.push_back(-8);
+            entry_counts.push_back(-16);
+        }
+        for (int e : entry_counts) {
             tuning = best;
             tuning.expansion_entries = e;
             const double ms = measure(0);

@@ -584,8 +584,11 @@ featomic_status_t featomic_amd_calculator_set_tuning(featomic_calculator_t* calc
                                                      int32_t gradient_window) {
     return guard([&] {
         check_pointer(calculator, "calculator");
-        if (expansion_entries < 0 || expansion_entries > 64 || (expansion_entries & (expansion_entries - 1)) != 0) {
-            throw Error::invalid_parameter("expansion_entries must be 0 (built-in) or a power of two up to 64");
+        const bool tiled_count =
+            expansion_entries >= 0 && expansion_entries <= 64 && (expansion_entries & (expansion_entries - 1)) == 0;
+        if (!tiled_count && expansion_entries != -8 && expansion_entries != -16) {
+            throw Error::invalid_parameter(
+                "expansion_entries must be 0 (built-in), a power of two up to 64, or -8 / -16 (matrix-core kernel)");
         }
         if (gradient_window < -2 || gradient_window >= static_cast<int32_t>(rascaline_amd::soap_window_shapes().size())) {
             throw Error::invalid_parameter("gradient_window must be -2, -1 or a window shape index below " +

@@ -842,9 +842,9 @@ struct TileEntryLayout {
         int o = 0;
         rv = o; o += L * NC;
         rd = o; o += L * NC;
-        dir = o; o += 4;
+        dir = o; o += 4;    // direction, neighbour type
         trig = o; o += 4;   // cos theta, sin theta, cos phi, sin phi
-        scal = o; o += 4;   // f, df/dr, f / r
+        scal = o; o += 4;   // f, df/dr, f / r, neighbour atom (record_index)
         p = o; o += leg;
         per_entry = (o + 3) / 4 * 4;
     }
@@ -1310,6 +1310,18 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
 // stores the direction, trigonometric factors and the cutoff / scaling factors.
 // Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
 // to NC.
+// an atom index in a record slot: its bit pattern in FP32, its value in FP64
+template <typename T>
+__device__ __forceinline__ T record_index(int j);
+template <>
+__device__ __forceinline__ float record_index<float>(int j) {
+    return __int_as_float(j);
+}
+template <>
+__device__ __forceinline__ double record_index<double>(int j) {
+    return (double)j;
+}
+
 template <typename T>
 __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base);
 
@@ -1372,7 +1384,10 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
             base[lay.dir + 0] = x;
             base[lay.dir + 1] = y;
             base[lay.dir + 2] = z;
-            base[lay.dir + 3] = 0;
+            // the neighbour's type, and below its atom index (bit pattern in FP32), for
+            // the kernels that group entries by type and skip a center's own images
+            const int j = a.entry_j[e];
+            base[lay.dir + 3] = (T)a.species[j];
             base[lay.trig + 0] = cos_theta;
             base[lay.trig + 1] = sin_theta;
             base[lay.trig + 2] = cos_phi;
@@ -1382,7 +1397,7 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
             base[lay.scal + 0] = (T)f;
             base[lay.scal + 1] = (T)df;
             base[lay.scal + 2] = (T)f / (T)r;
-            base[lay.scal + 3] = 0;
+            base[lay.scal + 3] = record_index<T>(j);
             for (int k = lay.p + lay.leg; k < lay.per_entry; k++) base[k] = 0;
         }
         T value = (T)0.3989422804014327;
@@ -3011,6 +3026,330 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
 }
 
 // ---------------------------------------------------------------------------
+// Spherical expansion c_i and self gradients d c_i / d r_i on the matrix cores (FP32,
+// n = 8, l < 9), from the per-entry data of entry_prep_kernel (soap/
+// spherical_expansion.rs:101-358, spherical_expansion_pair.rs:323-422). One center
+// per 256-thread workgroup, its entries in chunks of E. Per chunk, every thread
+// stages the records (loaded into registers during the previous chunk) and builds
+// the chunk's tables in LDS:
+//   (Y, T1, T2) per (entry, l m), with r dY_lm/dx = ga_x T1 + gb_x T2
+//   (ga = (sin phi, -cos phi, 0), gb = (-cos theta cos phi, -cos theta sin phi,
+//   sin theta) / 2, the m = 0 derivative as T2 = -2 c_l P_l1; spherical_harmonics.rs:
+//   256-424); f R, and (ra, rb) = (df R + f dR, (f / r) R) per (entry, l, c); d, ga, gb
+//   per entry.
+// Wave 0 then sums the values c[t][lm][c] = sum_e f R_l[c] Y_lm and wave 1 + x the
+// self gradients -sum_e ra_l[c] Y_lm d_x + rb_l[c] r dY_lm/dx (not for periodic
+// images of the center itself), as outer products on v_mfma_f32_4x4x1f32: block
+// b = (l, quad mq of m, quad cq of c) takes A = four radial values and B = four
+// angular values, and lane 4 b + j holds out[l^2 + 4 mq + j][4 cq .. 4 cq + 3], one
+// 16-byte piece of the output. The chunk's entries are visited grouped by type, one
+// set of accumulators per type of a group of four; more types take one pass over the
+// entries per group.
+constexpr int EXM_THREADS = 256;
+constexpr int EXM_MAX_L = 9;
+
+struct ExpansionMfmaLayout {
+    // float offsets in the dynamic LDS
+    int L, L2, per_entry, raw, ytab, fr, rab, geo, total;
+    __host__ __device__ ExpansionMfmaLayout(int L_, int E) : L(L_), L2(L_ * L_) {
+        per_entry = TileEntryLayout(L, 8, 8).per_entry;
+        raw = 0;                     // the chunk's records
+        ytab = raw + E * per_entry;  // (Y, T1, T2, -) per (entry, lm)
+        fr = ytab + E * L2 * 4;      // f R per (entry, l, c)
+        rab = fr + E * L * 8;        // (ra, rb) per (entry, l, c)
+        geo = rab + E * L * 16;      // d, ga, gb per entry
+        total = geo + E * 12;
+    }
+};
+
+// 16-block halves of v_mfma_f32_4x4x1f32 needed for the (l, m quad, c quad) blocks
+__host__ __device__ inline int expansion_mfma_halves(int L) {
+    int q = 0;
+    for (int l = 0; l < L; l++) q += (2 * l + 1 + 3) / 4;
+    return (2 * q + 15) / 16;
+}
+
+template <bool B>
+struct BoolTag {
+    static constexpr bool value = B;
+};
+
+template <int E, int NH>
+__global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(SoapArgs<float> a,
+                                                                     const float* __restrict__ entry_data,
+                                                                     float* __restrict__ out_coef,
+                                                                     float* __restrict__ out_grad) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    // the chunk's entries grouped by type within the group of four types (values), and
+    // the same without the center's own periodic images (gradients)
+    __shared__ int vorder[E];
+    __shared__ int gorder[E];
+    __shared__ int vstart[5];
+    __shared__ int gstart[5];
+    float* lds = reinterpret_cast<float*>(smem);
+    // records of up to E entries at l <= 8 (204 floats each), in float4 pieces per thread
+    constexpr int NV = (E * 204 / 4 + EXM_THREADS - 1) / EXM_THREADS;
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int comp = uniform(tid / WAVE);  // 0: values, 1 + x: self gradient along x
+    const int S = a.n_species, L = a.L;
+    const int L2 = L * L;
+    const int slots = L2 * 8;
+    const ExpansionMfmaLayout ly(L, E);
+    const TileEntryLayout lay(L, 8, 8);
+    const ShTable tb(L);
+    const float* shc = a.sh_c;
+    const bool grads = out_grad != nullptr;
+    const long long e0 = a.offsets[i];
+    const int len = (int)(a.offsets[i + 1] - e0);
+    const int ct = a.species[i];
+    float* raw = lds + ly.raw;
+    float* ytab = lds + ly.ytab;
+    float* fr = lds + ly.fr;
+    float* rab = lds + ly.rab;
+    float* geo = lds + ly.geo;
+
+    // this lane's blocks: block b = 16 h + lane / 4 enumerates (l, m quad, c quad) with
+    // the c quad fastest; the A operand is the radial value c = 4 cq + lane % 4, the B
+    // operand the angular value of m index 4 mq + lane % 4, which is also the row of
+    // the lane's accumulators
+    int roff[NH], lmh[NH], cqh[NH];
+    bool valid[NH];
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+        const int b = 16 * h + (lane >> 2);
+        roff[h] = 0;
+        lmh[h] = 0;
+        cqh[h] = 0;
+        valid[h] = false;
+        int first = 0;
+        for (int l = 0; l < L; l++) {
+            const int nb = 2 * ((2 * l + 1 + 3) / 4);
+            if (b >= first && b < first + nb) {
+                const int mq = (b - first) >> 1, cq = (b - first) & 1;
+                const int mi = 4 * mq + (lane & 3);
+                roff[h] = l * 8 + 4 * cq + (lane & 3);
+                valid[h] = mi <= 2 * l;
+                lmh[h] = valid[h] ? l * l + mi : 0;
+                cqh[h] = cq;
+            }
+            first += nb;
+        }
+    }
+
+    float4_t pre[NV];
+    auto fetch = [&](int cb) {
+        const int c = min(E, len - cb);
+        const float4_t* src = reinterpret_cast<const float4_t*>(entry_data + (e0 + cb) * ly.per_entry);
+        const int count = c * ly.per_entry / 4;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int idx = k * EXM_THREADS + tid;
+            if (idx < count) pre[k] = src[idx];
+        }
+    };
+
+    // one instantiation per role, so that the accumulators of each stay in place
+    auto centre = [&](auto role) {
+        constexpr bool VAL = decltype(role)::value;  // values (wave 0) or self gradients
+        const int n_groups = (S + 3) / 4;
+        for (int g = 0; g < n_groups; g++) {
+            float4_t acc[4][NH];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+#pragma unroll
+                for (int h = 0; h < NH; h++) acc[k][h] = float4_t{0.f, 0.f, 0.f, 0.f};
+            }
+            if (len > 0) fetch(0);
+            for (int cb = 0; cb < len; cb += E) {
+                const int cnt = min(E, len - cb);
+                lds_barrier();  // the previous chunk's tables are read no more
+                {
+                    float4_t* dst = reinterpret_cast<float4_t*>(raw);
+                    const int count = cnt * ly.per_entry / 4;
+#pragma unroll
+                    for (int k = 0; k < NV; k++) {
+                        const int idx = k * EXM_THREADS + tid;
+                        if (idx < count) dst[idx] = pre[k];
+                    }
+                }
+                if (cb + E < len) fetch(cb + E);
+                lds_barrier();
+                if (tid < WAVE) {
+                    // wave 0: positions from ballots, entry `tid` of type 4 g + k after the
+                    // chunk's entries of types below it and its own type's earlier entries
+                    const bool have = tid < cnt;
+                    const float* rec = raw + (have ? tid : 0) * ly.per_entry;
+                    const int t = have ? (int)rec[lay.dir + 3] - 4 * g : -1;
+                    const bool own = have && __float_as_int(rec[lay.scal + 3]) == i;
+                    const unsigned long long below = (1ull << tid) - 1ull;
+                    int vbase = 0, gbase = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const unsigned long long vm = __ballot(t == k);
+                        const unsigned long long gm = __ballot(t == k && !own);
+                        if (t == k) {
+                            vorder[vbase + __popcll(vm & below)] = tid;
+                            if (!own) gorder[gbase + __popcll(gm & below)] = tid;
+                        }
+                        if (tid == 0) {
+                            vstart[k] = vbase;
+                            gstart[k] = gbase;
+                        }
+                        vbase += __popcll(vm);
+                        gbase += __popcll(gm);
+                    }
+                    if (tid == 0) {
+                        vstart[4] = vbase;
+                        gstart[4] = gbase;
+                    }
+                }
+                // (Y, T1, T2) per (entry, l m)
+#pragma unroll 1
+                for (int it = tid; it < cnt * L2; it += EXM_THREADS) {
+                    const int e = it / L2, lmp = it - e * L2;
+                    const int l = l_of_lm(lmp);
+                    const int m = lmp - l * l - l;
+                    const int am = m < 0 ? -m : m;
+                    const float* base = raw + e * ly.per_entry;
+                    const float* Pl = base + lay.p;
+                    const float cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+                    const float cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+                    const int k = leg_index(l, am);
+                    const float p = Pl[k];
+                    float Y, T1, T2;
+                    if (m == 0) {
+                        Y = p / 1.4142135623730951f;
+                        T1 = 0.f;
+                        T2 = l == 0 ? 0.f : -2.f * (shc[tb.gl + l] * Pl[leg_index(l, 1)]);
+                    } else {
+                        // cos (m phi), sin (m phi) by the recurrence of spherical_harmonics.rs
+                        float cos_1 = 1, sin_1 = 0, cos_2 = -cos_phi, sin_2 = sin_phi;
+                        const float minus_two_cos = -2 * cos_phi;
+                        float sin_m = 0, cos_m = 1;
+                        for (int mm = 1; mm <= am; mm++) {
+                            sin_m = minus_two_cos * sin_1 - sin_2;
+                            cos_m = minus_two_cos * cos_1 - cos_2;
+                            sin_2 = sin_1;
+                            sin_1 = sin_m;
+                            cos_2 = cos_1;
+                            cos_1 = cos_m;
+                        }
+                        const float pm1 = Pl[leg_index(l, am - 1)];
+                        const float pp1 = am < l ? Pl[leg_index(l, am + 1)] : 0.f;
+                        const float dp = shc[tb.dpa + k] * pm1 - shc[tb.dpb + k] * pp1;
+                        const float pt = (sin_theta > 0.1f) ? (float)am / sin_theta * p : -0.5f / cos_theta * dp;
+                        const float c1 = m > 0 ? cos_m : sin_m;
+                        const float s1 = m > 0 ? sin_m : -cos_m;
+                        Y = p * c1;
+                        T1 = pt * s1;
+                        T2 = c1 * dp;
+                    }
+                    store4<float>(ytab + (size_t)it * 4, Y, T1, T2, 0.f);
+                }
+                // f R and (ra, rb) per (entry, l, c)
+#pragma unroll 1
+                for (int it = tid; it < cnt * L * 8; it += EXM_THREADS) {
+                    const int e = it / (L * 8), lc = it - e * L * 8;
+                    const float* base = raw + e * ly.per_entry;
+                    const float f = base[lay.scal + 0], df = base[lay.scal + 1], f_r = base[lay.scal + 2];
+                    const float R = base[lay.rv + lc], dR = base[lay.rd + lc];
+                    fr[it] = f * R;
+                    rab[2 * it + 0] = df * R + f * dR;
+                    rab[2 * it + 1] = f_r * R;
+                }
+                // d, ga, gb per entry
+                for (int e = tid; e < cnt; e += EXM_THREADS) {
+                    const float* base = raw + e * ly.per_entry;
+                    const float cos_theta = base[lay.trig + 0], sin_theta = base[lay.trig + 1];
+                    const float cos_phi = base[lay.trig + 2], sin_phi = base[lay.trig + 3];
+                    float* gm = geo + e * 12;
+                    gm[0] = base[lay.dir + 0];
+                    gm[1] = base[lay.dir + 1];
+                    gm[2] = base[lay.dir + 2];
+                    gm[3] = sin_phi;
+                    gm[4] = -cos_phi;
+                    gm[5] = 0.f;
+                    gm[6] = -0.5f * cos_theta * cos_phi;
+                    gm[7] = -0.5f * cos_theta * sin_phi;
+                    gm[8] = 0.5f * sin_theta;
+                }
+                lds_barrier();
+                // the type is a constant of each unrolled k, so each set of accumulators
+                // stays in its registers; the chunk's order and type starts are read once
+                // (lane q holds entry q of the order)
+                const int ord = lane < E ? (VAL ? vorder[lane] : gorder[lane]) : 0;
+                const int starts = lane < 5 ? (VAL ? vstart[lane] : gstart[lane]) : 0;
+                if constexpr (VAL) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int lo = __builtin_amdgcn_readlane(starts, k);
+                        const int hi = __builtin_amdgcn_readlane(starts, k + 1);
+                        for (int q = lo; q < hi; q++) {
+                            const int e = __builtin_amdgcn_readlane(ord, q);
+#pragma unroll
+                            for (int h = 0; h < NH; h++) {
+                                const float y = ytab[(size_t)(e * L2 + lmh[h]) * 4];
+                                const float av = fr[e * L * 8 + roff[h]];
+                                acc[k][h] = mfma4x4(av, valid[h] ? y : 0.f, acc[k][h]);
+                            }
+                        }
+                    }
+                } else if (grads) {
+                    const int x = comp - 1;  // waves 1 to 3
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int lo = __builtin_amdgcn_readlane(starts, k);
+                        const int hi = __builtin_amdgcn_readlane(starts, k + 1);
+                        for (int q = lo; q < hi; q++) {
+                            const int e = __builtin_amdgcn_readlane(ord, q);
+                            const float dx = geo[e * 12 + x], gax = geo[e * 12 + 3 + x], gbx = geo[e * 12 + 6 + x];
+#pragma unroll
+                            for (int h = 0; h < NH; h++) {
+                                const float* y4 = ytab + (size_t)(e * L2 + lmh[h]) * 4;
+                                const float Y = y4[0], T1 = y4[1], T2 = y4[2];
+                                const float* r2 = rab + 2 * (e * L * 8 + roff[h]);
+                                acc[k][h] = mfma4x4(r2[0], valid[h] ? Y * dx : 0.f, acc[k][h]);
+                                acc[k][h] = mfma4x4(r2[1], valid[h] ? gax * T1 + gbx * T2 : 0.f, acc[k][h]);
+                            }
+                        }
+                    }
+                }
+            }
+            // the group's sums; the self contribution goes to the l = 0 values of the
+            // center's own type (spherical_expansion_pair.rs:222-252)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int t = 4 * g + k;
+                if (t >= S) break;
+#pragma unroll
+                for (int h = 0; h < NH; h++) {
+                    if (!valid[h]) continue;
+                    float4_t v = acc[k][h];
+                    if constexpr (VAL) {
+                        if (t == ct && lmh[h] == 0) {
+#pragma unroll
+                            for (int q = 0; q < 4; q++) v[q] += a.self_contribution[4 * cqh[h] + q];
+                        }
+                        store4<float>(out_coef + ((long long)i * S + t) * slots + lmh[h] * 8 + 4 * cqh[h], v[0], v[1],
+                                      v[2], v[3]);
+                    } else if (grads) {
+                        store4<float>(out_grad + (((long long)i * S + t) * 3 + comp - 1) * slots + lmh[h] * 8 + 4 * cqh[h],
+                                      -v[0], -v[1], -v[2], -v[3]);
+                    }
+                }
+            }
+        }
+    };
+    if (comp == 0) {
+        centre(BoolTag<true>{});
+    } else {
+        centre(BoolTag<false>{});
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Vector-Jacobian product of the position gradients without the gradient rows:
 // dA/dr_atom = Σ_blocks Σ_rows (dX/dr)[row] · (dA/dX)[sample of row], the
 // contraction of featomic-torch's PositionsGrad (featomic-torch/src/
@@ -3695,6 +4034,40 @@ void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* en
         RA_LAUNCH_EXP(16, 2);
     }
 #undef RA_LAUNCH_EXP
+}
+
+bool expansion_mfma_supported(int L, int n, int S) { return n == 8 && L >= 1 && L <= EXM_MAX_L && S >= 1; }
+
+size_t expansion_mfma_lds(const SoapLaunch<float>& p, int E) {
+    return (size_t)ExpansionMfmaLayout(p.L, E).total * sizeof(float) + 2 * (size_t)E * sizeof(int);
+}
+
+void launch_expansion_mfma(const SoapLaunch<float>& p, int E, const float* entry_data, float* coef, float* dself) {
+    if (p.n_atoms == 0) return;
+    if (!expansion_mfma_supported(p.L, p.n, p.n_species) || (E != 8 && E != 16)) {
+        throw std::invalid_argument("the matrix-core expansion kernel takes n = 8, l < 9 and chunks of 8 or 16 entries");
+    }
+    // the record prefetch holds E records of at most 204 floats per thread group
+    if (ExpansionMfmaLayout(p.L, E).per_entry > 204) throw std::invalid_argument("expansion record larger than staged");
+    SoapArgs<float> a = make_args(p);
+    const size_t shmem = (size_t)ExpansionMfmaLayout(p.L, E).total * sizeof(float);
+    const int nh = expansion_mfma_halves(p.L);
+#define RA_LAUNCH_EXM(EV, NHV) \
+    expansion_mfma_kernel<EV, NHV><<<p.n_atoms, EXM_THREADS, shmem, p.stream>>>(a, entry_data, coef, dself)
+#define RA_LAUNCH_EXM_E(NHV)  \
+    if (E == 8) {             \
+        RA_LAUNCH_EXM(8, NHV); \
+    } else {                  \
+        RA_LAUNCH_EXM(16, NHV); \
+    }
+    switch (nh) {
+        case 1: RA_LAUNCH_EXM_E(1); break;
+        case 2: RA_LAUNCH_EXM_E(2); break;
+        case 3: RA_LAUNCH_EXM_E(3); break;
+        default: RA_LAUNCH_EXM_E(4); break;
+    }
+#undef RA_LAUNCH_EXM_E
+#undef RA_LAUNCH_EXM
 }
 
 template <typename T>

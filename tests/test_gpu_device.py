@@ -164,7 +164,8 @@ def test_every_tuned_launch_shape_and_autotune(precision, fast):
         err = max_block_error(block_arrays(compute_device(calc, ds, ["positions"])), ref)
         say(f"  gradient window {w}: {err:.1e}")
         assert err <= bar, w
-    for e in (4, 16, 32, 64):
+    # negative counts: the matrix-core expansion kernel (FP32; FP64 takes the vector-ALU one)
+    for e in (4, 16, 32, 64, -8, -16):
         calc.tuning = {"expansion_entries": e, "gradient_window": -1}
         err = max_block_error(block_arrays(compute_device(calc, ds, ["positions"])), ref)
         say(f"  expansion entries {e}: {err:.1e}")
