@@ -292,7 +292,8 @@ def main():
         name = "fast" if fast else "plain"
         # launch shapes chosen by the autotuner on the first chunk (outside the timed steps)
         if window is not None or args.gradient_window is not None:
-            calc.tuning = {"expansion_entries": 8,
+            # (the vector-ALU form also takes the vector-ALU expansion kernel, 8 entries per chunk)
+            calc.tuning = {"expansion_entries": 8 if window is not None else 0,
                            "gradient_window": window if window is not None else args.gradient_window}
             tuning_report = {"chosen": calc.tuning}
             if window is not None:

@@ -574,6 +574,8 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     // shared by the expansion and the power-spectrum gradient kernels
     const bool tiled = n <= 16 && L <= 16 && S <= 32;
     T* entry_data = nullptr;
+    // the power spectrum's values come out of the matrix-core expansion kernel itself
+    bool fused_values = false;
     if (tiled) {
         entry_data = ws_.get<T>("entry_data", std::max<size_t>(entry_prep_size<T>(lp, nl.n_entries), 1));
         launch_entry_prep<T>(lp, nl.n_entries, entry_data);
@@ -591,7 +593,10 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
         mark(ev_exp_start_, stream);
         if (mfma) {
             if constexpr (std::is_same<T, float>::value) {
-                launch_expansion_mfma(lp, asked > 0 ? asked : RA_EXPANSION_MFMA_ENTRIES, entry_data, coef, dself);
+                fused_values = ps && !vjp_grad_ && expansion_mfma_values_supported(S);
+                T* values = fused_values ? ws_.get<T>("out_values", std::max<long long>(total_samples * P, 1)) : nullptr;
+                launch_expansion_mfma(lp, asked > 0 ? asked : RA_EXPANSION_MFMA_ENTRIES, entry_data, coef, dself,
+                                      type_mask, sample_flag, sample_scan, values);
             }
         } else if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
             launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);
@@ -706,7 +711,7 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
     const size_t slots_bytes = slots * sizeof(T);
     if (ps) {
         if (static_cast<long long>(S * slots_bytes) > lds) throw Error::invalid_parameter("too many atomic types for the GPU kernels' local memory");
-        launch_ps_values<T>(lp, coef, type_mask, sample_flag, sample_scan, d_values, fast_form != 0, lds);
+        if (!fused_values) launch_ps_values<T>(lp, coef, type_mask, sample_flag, sample_scan, d_values, fast_form != 0, lds);
     } else {
         launch_se_gather<T>(lp, 1, coef, sample_flag, sample_scan, d_block_sample_base, d_region, d_values);
     }

@@ -1338,9 +1338,13 @@ __global__ void __launch_bounds__(256) entry_prep_kernel(SoapArgs<T> a, long lon
     const long long e_first = (long long)blockIdx.x * per_block;
     const int count = (int)min((long long)per_block, n_entries - e_first);
     const int tid = threadIdx.x;
-    if (tid < count * 2 * L) {
-        const int local = tid / (2 * L);
-        const int slot = tid - local * 2 * L;
+    // threads 0-127 take the Legendre slots (entry, m), threads 128-255 the radial slots
+    // (entry, l): each wave runs one of the two paths (per_block * L <= 128)
+    const int part = tid < 128 ? 0 : 1;
+    const int u = tid - 128 * part;
+    if (u < count * L) {
+        const int local = u / L;
+        const int slot = u - local * L + part * L;
         const long long e = e_first + local;
         T* base = records + (size_t)local * lay.per_entry;
         entry_prep_slot<T>(a, lay, e, slot, NC, base);
@@ -1421,7 +1425,11 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
         const int n = lay.n;
         T* va = base + lay.rv + l * NC;
         T* vb = base + lay.rd + l * NC;
-        radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
+        if (n <= 8) {
+            radial_integral_unrolled<T, 8>(a.radial, l, e, a.entry_dist[e], va, vb);
+        } else {
+            radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
+        }
         for (int k = n; k < NC; k++) {
             va[k] = 0;
             vb[k] = 0;
@@ -3047,20 +3055,74 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
 // entries per group.
 constexpr int EXM_THREADS = 256;
 constexpr int EXM_MAX_L = 9;
+constexpr int EXM_MAX_VALUE_TYPES = 16;
 
 struct ExpansionMfmaLayout {
     // float offsets in the dynamic LDS
-    int L, L2, per_entry, raw, ytab, fr, rab, geo, total;
-    __host__ __device__ ExpansionMfmaLayout(int L_, int E) : L(L_), L2(L_ * L_) {
+    int L, L2, per_entry, raw, ytab, fr, rab, geo, csp, cval, total;
+    // SC: types whose coefficients stay in LDS for the power-spectrum values (0: none)
+    __host__ __device__ ExpansionMfmaLayout(int L_, int E, int SC) : L(L_), L2(L_ * L_) {
         per_entry = TileEntryLayout(L, 8, 8).per_entry;
         raw = 0;                     // the chunk's records
         ytab = raw + E * per_entry;  // (Y, T1, T2, -) per (entry, lm)
         fr = ytab + E * L2 * 4;      // f R per (entry, l, c)
         rab = fr + E * L * 8;        // (ra, rb) per (entry, l, c)
         geo = rab + E * L * 16;      // d, ga, gb per entry
-        total = geo + E * 12;
+        csp = L2 * 8 + 8;            // per type, padded: two types' column reads meet no bank conflict
+        cval = (geo + E * 12 + 3) / 4 * 4;  // the center's coefficients [type][lm][c]
+        total = cval + SC * csp;
     }
 };
+
+// Power-spectrum values of one center from its coefficients C in LDS (all types,
+// padded rows of `csp`), written into the samples of its present type pairs
+// (soap/power_spectrum.rs:340-348): one v_mfma_f32_16x16x4f32 product per (l, 2 x 2
+// block of type pairs, 4 m), rows (t2, n2) and columns (t1, n1), so that lane j holds
+// PS[l][n1][4 h .. 4 h + 3] of the pair (t1, t2), a 16-byte piece of its sample.
+__device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int lane, int L, int S, int n_atoms, int i,
+                                                    int ct, unsigned mask, const float* C, int csp,
+                                                    const long long* __restrict__ sample_flag,
+                                                    const long long* __restrict__ sample_scan,
+                                                    float* __restrict__ out) {
+    const int nb = (S + 1) / 2;
+    const int nbp = nb * (nb + 1) / 2;
+    const int npairs = S * (S + 1) / 2;
+    const int P = L * 64;
+    const int j16 = lane & 15, jk = lane >> 4;
+    for (int u = wave; u < L * nbp; u += n_waves) {
+        const int bp = u % nbp, l = u / nbp;
+        int b1 = 0, rem = bp;
+        while (rem >= nb - b1) {
+            rem -= nb - b1;
+            b1++;
+        }
+        const int b2 = b1 + rem;
+        const int k2 = 2 * b2 + (j16 >> 3), k1 = 2 * b1 + (j16 >> 3);
+        const bool kv2 = k2 < S, kv1 = k1 < S;
+        const float* C2 = C + (kv2 ? k2 : 0) * csp + l * l * 8 + (lane & 7);
+        const float* C1 = C + (kv1 ? k1 : 0) * csp + l * l * 8 + (lane & 7);
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < (2 * l + 1 + 3) / 4; ks++) {
+            const int mp = 4 * ks + jk;
+            const bool mv = mp <= 2 * l;
+            const int off = (mv ? mp : 0) * 8;
+            acc = mfma16x16((mv && kv2) ? C2[off] : 0.f, (mv && kv1) ? C1[off] : 0.f, acc);
+        }
+        // lane j: rows 4 (j / 16) + v = (t2 = 2 b2 + j / 32, n2 = 4 ((j / 16) % 2) + v),
+        // column j % 16 = (t1 = 2 b1 + (j % 16) / 8, n1 = j % 8)
+        const int t2 = 2 * b2 + (lane >> 5), t1 = 2 * b1 + (j16 >> 3);
+        if (t2 < S && t1 <= t2 && ((mask >> t1) & 1u) && ((mask >> t2) & 1u)) {
+            const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
+            if (sample_flag[idx]) {
+                const float kappa = t1 != t2 ? 1.4142135623730951f : 1.f;
+                const float norm = (l % 2 == 0 ? 1.f : -1.f) * (float)sqrt((double)(2 * l + 1));
+                float* dst = out + sample_scan[idx] * P + l * 64 + (lane & 7) * 8 + 4 * ((lane >> 4) & 1);
+                store4<float>(dst, (acc[0] * kappa) / norm, (acc[1] * kappa) / norm, (acc[2] * kappa) / norm,
+                              (acc[3] * kappa) / norm);
+            }
+        }
+    }
+}
 
 // 16-block halves of v_mfma_f32_4x4x1f32 needed for the (l, m quad, c quad) blocks
 __host__ __device__ inline int expansion_mfma_halves(int L) {
@@ -3075,10 +3137,10 @@ struct BoolTag {
 };
 
 template <int E, int NH>
-__global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(SoapArgs<float> a,
-                                                                     const float* __restrict__ entry_data,
-                                                                     float* __restrict__ out_coef,
-                                                                     float* __restrict__ out_grad) {
+__global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
+    SoapArgs<float> a, const float* __restrict__ entry_data, float* __restrict__ out_coef, float* __restrict__ out_grad,
+    const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_flag,
+    const long long* __restrict__ sample_scan, float* __restrict__ ps_values) {
     extern __shared__ __align__(16) unsigned char smem[];
     // the chunk's entries grouped by type within the group of four types (values), and
     // the same without the center's own periodic images (gradients)
@@ -3096,7 +3158,9 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(SoapArgs<fl
     const int S = a.n_species, L = a.L;
     const int L2 = L * L;
     const int slots = L2 * 8;
-    const ExpansionMfmaLayout ly(L, E);
+    // with `ps_values`, the power-spectrum values follow from the coefficients in LDS
+    const bool fused = ps_values != nullptr;
+    const ExpansionMfmaLayout ly(L, E, fused ? S : 0);
     const TileEntryLayout lay(L, 8, 8);
     const ShTable tb(L);
     const float* shc = a.sh_c;
@@ -3334,6 +3398,7 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(SoapArgs<fl
                         }
                         store4<float>(out_coef + ((long long)i * S + t) * slots + lmh[h] * 8 + 4 * cqh[h], v[0], v[1],
                                       v[2], v[3]);
+                        if (fused) store4<float>(lds + ly.cval + t * ly.csp + lmh[h] * 8 + 4 * cqh[h], v[0], v[1], v[2], v[3]);
                     } else if (grads) {
                         store4<float>(out_grad + (((long long)i * S + t) * 3 + comp - 1) * slots + lmh[h] * 8 + 4 * cqh[h],
                                       -v[0], -v[1], -v[2], -v[3]);
@@ -3346,6 +3411,11 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(SoapArgs<fl
         centre(BoolTag<true>{});
     } else {
         centre(BoolTag<false>{});
+    }
+    if (fused) {
+        lds_barrier();  // the coefficients of every type are in LDS
+        expansion_ps_values(comp, EXM_THREADS / WAVE, lane, L, S, a.n_atoms, i, ct, type_mask[i], lds + ly.cval, ly.csp,
+                            sample_flag, sample_scan, ps_values);
     }
 }
 
@@ -3884,7 +3954,7 @@ void launch_entry_prep(const SoapLaunch<T>& p, long long n_entries, T* data) {
     if (n_entries == 0) return;
     SoapArgs<T> a = make_args(p);
     const int NC = (p.n + 3) / 4 * 4;
-    const int per_block = 256 / (2 * p.L);
+    const int per_block = 128 / p.L;
     const size_t shmem = (size_t)per_block * TileEntryLayout(p.L, p.n, NC).per_entry * sizeof(T);
     const long long blocks = (n_entries + per_block - 1) / per_block;
     entry_prep_kernel<T><<<(unsigned)blocks, 256, shmem, p.stream>>>(a, n_entries, NC, per_block, data);
@@ -4038,22 +4108,32 @@ void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* en
 
 bool expansion_mfma_supported(int L, int n, int S) { return n == 8 && L >= 1 && L <= EXM_MAX_L && S >= 1; }
 
-size_t expansion_mfma_lds(const SoapLaunch<float>& p, int E) {
-    return (size_t)ExpansionMfmaLayout(p.L, E).total * sizeof(float) + 2 * (size_t)E * sizeof(int);
+bool expansion_mfma_values_supported(int n_species) { return n_species <= EXM_MAX_VALUE_TYPES; }
+
+size_t expansion_mfma_lds(const SoapLaunch<float>& p, int E, bool values) {
+    return (size_t)ExpansionMfmaLayout(p.L, E, values ? p.n_species : 0).total * sizeof(float);
 }
 
-void launch_expansion_mfma(const SoapLaunch<float>& p, int E, const float* entry_data, float* coef, float* dself) {
+void launch_expansion_mfma(const SoapLaunch<float>& p, int E, const float* entry_data, float* coef, float* dself,
+                           const unsigned* type_mask, const long long* sample_flag, const long long* sample_scan,
+                           float* ps_values) {
     if (p.n_atoms == 0) return;
     if (!expansion_mfma_supported(p.L, p.n, p.n_species) || (E != 8 && E != 16)) {
         throw std::invalid_argument("the matrix-core expansion kernel takes n = 8, l < 9 and chunks of 8 or 16 entries");
     }
+    if (ps_values && (!expansion_mfma_values_supported(p.n_species) || !type_mask || !sample_flag || !sample_scan)) {
+        throw std::invalid_argument("the expansion kernel forms power-spectrum values for at most 16 types, with the "
+                                    "type masks and sample offsets");
+    }
     // the record prefetch holds E records of at most 204 floats per thread group
-    if (ExpansionMfmaLayout(p.L, E).per_entry > 204) throw std::invalid_argument("expansion record larger than staged");
+    if (ExpansionMfmaLayout(p.L, E, 0).per_entry > 204) throw std::invalid_argument("expansion record larger than staged");
     SoapArgs<float> a = make_args(p);
-    const size_t shmem = (size_t)ExpansionMfmaLayout(p.L, E).total * sizeof(float);
+    const size_t shmem = expansion_mfma_lds(p, E, ps_values != nullptr);
     const int nh = expansion_mfma_halves(p.L);
-#define RA_LAUNCH_EXM(EV, NHV) \
-    expansion_mfma_kernel<EV, NHV><<<p.n_atoms, EXM_THREADS, shmem, p.stream>>>(a, entry_data, coef, dself)
+#define RA_LAUNCH_EXM(EV, NHV)                                                                        \
+    expansion_mfma_kernel<EV, NHV><<<p.n_atoms, EXM_THREADS, shmem, p.stream>>>(a, entry_data, coef, dself, \
+                                                                                type_mask, sample_flag,   \
+                                                                                sample_scan, ps_values)
 #define RA_LAUNCH_EXM_E(NHV)  \
     if (E == 8) {             \
         RA_LAUNCH_EXM(8, NHV); \
