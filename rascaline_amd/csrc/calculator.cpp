@@ -235,6 +235,7 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
         const int L = rt.max_angular + 1, n = rt.n;
         std::vector<int> koff(L, 0), nk(L, 0);
         std::vector<T> kpos, kval, kder;
+        std::vector<double> kgrid(2 * L, 0.0);
         if (rt.use_spline) {
             int off = 0;
             for (int l = 0; l < L; l++) {
@@ -245,6 +246,11 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
                 for (double x : ch.positions) kpos.push_back(static_cast<T>(x));
                 for (double x : ch.values) kval.push_back(static_cast<T>(x));
                 for (double x : ch.derivatives) kder.push_back(static_cast<T>(x));
+                // the interval estimate of the uniform grid, from the knots as stored
+                const double first = static_cast<double>(kpos[koff[l]]);
+                const double last = static_cast<double>(kpos[koff[l] + nk[l] - 1]);
+                kgrid[2 * l] = first;
+                kgrid[2 * l + 1] = last > first ? (nk[l] - 1) / (last - first) : 0.0;
             }
         }
         t->radial.max_angular = rt.max_angular;
@@ -256,6 +262,7 @@ const Calculator::DeviceTables<T>& Calculator::device_tables() {
         t->radial.ortho = t->put(rt.ortho);
         t->radial.knot_offset = t->put(koff);
         t->radial.n_knots = t->put(nk);
+        t->radial.knot_grid = t->put(kgrid);
         t->radial.knot_pos = t->put(kpos);
         t->radial.knot_val = t->put(kval);
         t->radial.knot_der = t->put(kder);

@@ -44,6 +44,7 @@ struct DeviceRadial {
     const double* ortho;       // exact mode: n x n
     const int* knot_offset;    // per l: first knot
     const int* n_knots;        // per l: number of knots
+    const double* knot_grid;   // per l: first knot, (knots - 1) / (last - first knot), as the kernels round them
     const T* knot_pos;         // [total knots]
     const T* knot_val;         // [total knots][n]
     const T* knot_der;         // [total knots][n]

@@ -1310,6 +1310,70 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
 // stores the direction, trigonometric factors and the cutoff / scaling factors.
 // Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
 // to NC.
+// radial_integral for n = 8, with the interval's knots and every table value loaded
+// together (16-byte pieces) at the interval of the uniform-grid estimate (from the
+// per-channel grid table), and loaded again only when the stored knots move the
+// interval. The interval is that of radial_integral; the derivative's 1 / delta is one
+// reciprocal (within an ulp of radial_integral's divisions).
+template <typename T>
+__device__ void radial_integral_n8(const DeviceRadial<T>& rad, int l, long long e, double r, T* values, T* grads) {
+    T a0[8], b0[8], a1[8], b1[8];
+    if (rad.use_spline) {
+        const int K = rad.n_knots[l] - 1;
+        const int off = rad.knot_offset[l];
+        const T* pos = rad.knot_pos + off;
+        int k = (int)((r - rad.knot_grid[2 * l]) * rad.knot_grid[2 * l + 1]);
+        if (k < 0) k = 0;
+        if (k > K - 1) k = K - 1;
+        T x0, x1;
+        auto load = [&]() {
+            x0 = pos[k];
+            x1 = pos[k + 1];
+            const T* v0 = rad.knot_val + (long long)(off + k) * 8;
+            const T* m0 = rad.knot_der + (long long)(off + k) * 8;
+            load4<T>(v0, a0);
+            load4<T>(v0 + 4, a0 + 4);
+            load4<T>(v0 + 8, a1);
+            load4<T>(v0 + 12, a1 + 4);
+            load4<T>(m0, b0);
+            load4<T>(m0 + 4, b0 + 4);
+            load4<T>(m0 + 8, b1);
+            load4<T>(m0 + 12, b1 + 4);
+        };
+        load();
+        if ((k > 0 && (T)r < x0) || (k < K - 1 && (T)r >= x1)) {
+            while (k > 0 && (T)r < pos[k]) k--;
+            while (k < K - 1 && (T)r >= pos[k + 1]) k++;
+            load();
+        }
+        const T delta = x1 - x0;
+        const T t = ((T)r - x0) / delta, t2 = t * t, t3 = t2 * t;
+        const T h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+        const T d00 = 6 * (t2 - t), d10 = 3 * t2 - 4 * t + 1, d01 = -d00, d11 = 3 * t2 - 2 * t;
+        const T inv_delta = (T)1 / delta;
+        T v[8], g[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            v[i] = h00 * a0[i] + h10 * delta * b0[i] + h01 * a1[i] + h11 * delta * b1[i];
+            g[i] = d00 * a0[i] * inv_delta + d10 * b0[i] + d01 * a1[i] * inv_delta + d11 * b1[i];
+        }
+        store4<T>(values, v[0], v[1], v[2], v[3]);
+        store4<T>(values + 4, v[4], v[5], v[6], v[7]);
+        store4<T>(grads, g[0], g[1], g[2], g[3]);
+        store4<T>(grads + 4, g[4], g[5], g[6], g[7]);
+    } else {
+        const long long base = (e * (rad.max_angular + 1) + l) * 8;
+        load4<T>(rad.exact_val + base, a0);
+        load4<T>(rad.exact_val + base + 4, a0 + 4);
+        load4<T>(rad.exact_der + base, b0);
+        load4<T>(rad.exact_der + base + 4, b0 + 4);
+        store4<T>(values, a0[0], a0[1], a0[2], a0[3]);
+        store4<T>(values + 4, a0[4], a0[5], a0[6], a0[7]);
+        store4<T>(grads, b0[0], b0[1], b0[2], b0[3]);
+        store4<T>(grads + 4, b0[4], b0[5], b0[6], b0[7]);
+    }
+}
+
 // an atom index in a record slot: its bit pattern in FP32, its value in FP64
 template <typename T>
 __device__ __forceinline__ T record_index(int j);
@@ -1425,8 +1489,8 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
         const int n = lay.n;
         T* va = base + lay.rv + l * NC;
         T* vb = base + lay.rd + l * NC;
-        if (n <= 8) {
-            radial_integral_unrolled<T, 8>(a.radial, l, e, a.entry_dist[e], va, vb);
+        if (n == 8) {
+            radial_integral_n8<T>(a.radial, l, e, a.entry_dist[e], va, vb);
         } else {
             radial_integral<T>(a.radial, l, e, a.entry_dist[e], va, vb);
         }

@@ -148,76 +148,6 @@ __device__ void radial_integral(const DeviceRadial<T>& rad, int l, long long e, 
     }
 }
 
-// The same for n <= NMAX, with the interval's knots and every table value loaded
-// together, at the interval of the uniform-grid estimate, and loaded again only when
-// the stored knots move the interval (the general loop above waits for the knots,
-// then for each n's values in turn). The interval and the arithmetic are those of
-// radial_integral, in the same order.
-template <typename T, int NMAX>
-__device__ void radial_integral_unrolled(const DeviceRadial<T>& rad, int l, long long e, double r, T* values,
-                                         T* grads) {
-    const int n = rad.n;
-    if (rad.use_spline) {
-        const int K = rad.n_knots[l] - 1;
-        const int off = rad.knot_offset[l];
-        const T* pos = rad.knot_pos + off;
-        int k = (int)((r - pos[0]) * K / ((double)pos[K] - pos[0]));
-        if (k < 0) k = 0;
-        if (k > K - 1) k = K - 1;
-        T x0, x1;
-        T a0[NMAX], b0[NMAX], a1[NMAX], b1[NMAX];
-        auto load = [&]() {
-            x0 = pos[k];
-            x1 = pos[k + 1];
-            const T* v0 = rad.knot_val + (long long)(off + k) * n;
-            const T* m0 = rad.knot_der + (long long)(off + k) * n;
-#pragma unroll
-            for (int i = 0; i < NMAX; i++) {
-                if (i < n) {
-                    a0[i] = v0[i];
-                    b0[i] = m0[i];
-                    a1[i] = v0[n + i];
-                    b1[i] = m0[n + i];
-                }
-            }
-        };
-        load();
-        if ((k > 0 && (T)r < x0) || (k < K - 1 && (T)r >= x1)) {
-            while (k > 0 && (T)r < pos[k]) k--;
-            while (k < K - 1 && (T)r >= pos[k + 1]) k++;
-            load();
-        }
-        const T delta = x1 - x0;
-        const T t = ((T)r - x0) / delta, t2 = t * t, t3 = t2 * t;
-        const T h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
-        const T d00 = 6 * (t2 - t), d10 = 3 * t2 - 4 * t + 1, d01 = -d00, d11 = 3 * t2 - 2 * t;
-#pragma unroll
-        for (int i = 0; i < NMAX; i++) {
-            if (i < n) {
-                values[i] = h00 * a0[i] + h10 * delta * b0[i] + h01 * a1[i] + h11 * delta * b1[i];
-                if (grads) grads[i] = d00 * a0[i] / delta + d10 * b0[i] + d01 * a1[i] / delta + d11 * b1[i];
-            }
-        }
-    } else {
-        const long long base = (e * (rad.max_angular + 1) + l) * n;
-        T a0[NMAX], b0[NMAX];
-#pragma unroll
-        for (int i = 0; i < NMAX; i++) {
-            if (i < n) {
-                a0[i] = rad.exact_val[base + i];
-                if (grads) b0[i] = rad.exact_der[base + i];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < NMAX; i++) {
-            if (i < n) {
-                values[i] = a0[i];
-                if (grads) grads[i] = b0[i];
-            }
-        }
-    }
-}
-
 // Cutoff smoothing x radial scaling and its derivative
 // (featomic/src/calculators/soap/cutoff.rs:48-79, shared/density.rs:108-140,
 //  soap/spherical_expansion_pair.rs:189-210).
