@@ -1387,7 +1387,8 @@ __device__ __forceinline__ double record_index<double>(int j) {
 }
 
 template <typename T>
-__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base);
+__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base,
+                                const T* sha, const T* shb, const T* shc);
 
 // A workgroup fills the records of `per_block` consecutive entries in LDS, then
 // writes them out as one contiguous block of 16-byte stores (the slots' own
@@ -1402,6 +1403,19 @@ __global__ void __launch_bounds__(256) entry_prep_kernel(SoapArgs<T> a, long lon
     const long long e_first = (long long)blockIdx.x * per_block;
     const int count = (int)min((long long)per_block, n_entries - e_first);
     const int tid = threadIdx.x;
+    // the Legendre recursion's coefficients, in LDS for the recursion's loops
+    __shared__ T sha[TILE_MAX_L * (TILE_MAX_L + 1) / 2 + 1];
+    __shared__ T shb[TILE_MAX_L * (TILE_MAX_L + 1) / 2 + 1];
+    __shared__ T shc[TILE_MAX_L * (TILE_MAX_L + 1) + 4 * TILE_MAX_L];
+    {
+        const ShTable tb(L);
+        for (int k = tid; k <= lay.leg; k += blockDim.x) {
+            sha[k] = a.sh_a[k];
+            shb[k] = a.sh_b[k];
+        }
+        for (int k = tid; k < tb.size; k += blockDim.x) shc[k] = a.sh_c[k];
+    }
+    __syncthreads();
     // threads 0-127 take the Legendre slots (entry, m), threads 128-255 the radial slots
     // (entry, l): each wave runs one of the two paths (per_block * L <= 128)
     const int part = tid < 128 ? 0 : 1;
@@ -1411,7 +1425,7 @@ __global__ void __launch_bounds__(256) entry_prep_kernel(SoapArgs<T> a, long lon
         const int slot = u - local * L + part * L;
         const long long e = e_first + local;
         T* base = records + (size_t)local * lay.per_entry;
-        entry_prep_slot<T>(a, lay, e, slot, NC, base);
+        entry_prep_slot<T>(a, lay, e, slot, NC, base, sha, shb, shc);
     }
     __syncthreads();
     typedef typename Vec4Of<T>::type V;
@@ -1424,7 +1438,8 @@ __global__ void __launch_bounds__(256) entry_prep_kernel(SoapArgs<T> a, long lon
 
 // One slot of an entry's record (see entry_prep_kernel), written at `base`.
 template <typename T>
-__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base) {
+__device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base,
+                                const T* sha, const T* shb, const T* shc) {
     const int L = lay.L;
     if (slot < L) {
         const ShTable tb(L);
@@ -1469,16 +1484,16 @@ __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay
             for (int k = lay.p + lay.leg; k < lay.per_entry; k++) base[k] = 0;
         }
         T value = (T)0.3989422804014327;
-        for (int l = 1; l <= m; l++) value *= -a.sh_c[tb.diag + l] * sin_theta;
+        for (int l = 1; l <= m; l++) value *= -shc[tb.diag + l] * sin_theta;
         T* P = base + lay.p;
         P[leg_index(m, m)] = value;
         if (m + 1 < L) {
             T p2 = value;
-            T p1 = cos_theta * a.sh_c[tb.sq2l1 + m + 1] * value;
+            T p1 = cos_theta * shc[tb.sq2l1 + m + 1] * value;
             P[leg_index(m + 1, m)] = p1;
             for (int l = m + 2; l < L; l++) {
                 const int k = leg_index(l, m);
-                const T p0 = a.sh_a[k] * (cos_theta * p1 + a.sh_b[k] * p2);
+                const T p0 = sha[k] * (cos_theta * p1 + shb[k] * p2);
                 P[k] = p0;
                 p2 = p1;
                 p1 = p0;
