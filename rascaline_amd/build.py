@@ -18,7 +18,9 @@ LIBRARY = os.path.join(HERE, "librascaline_amd.so")
 SOURCES = [
     ("kernels_nl.hip", ["-ffp-contract=off"]),
     ("neighbors.cpp", ["-ffp-contract=off"]),
-    ("kernels_soap.hip", []),
+    # matrix-core accumulators in ordinary VGPRs (none of these kernels needs the extra
+    # accumulation registers; the copies out of them cost occupancy)
+    ("kernels_soap.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("kernels_lode.hip", []),
     ("kernels_mlp.hip", []),
     ("surrogate.cpp", []),

@@ -3158,14 +3158,10 @@ struct ExpansionMfmaLayout {
 // (soap/power_spectrum.rs:340-348): one v_mfma_f32_16x16x4f32 product per (l, 2 x 2
 // block of type pairs, 4 m), rows (t2, n2) and columns (t1, n1), so that lane j holds
 // PS[l][n1][4 h .. 4 h + 3] of the pair (t1, t2), a 16-byte piece of its sample.
-__device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int lane, int L, int S, int n_atoms, int i,
-                                                    int ct, unsigned mask, const float* C, int csp,
-                                                    const long long* __restrict__ sample_flag,
-                                                    const long long* __restrict__ sample_scan,
-                                                    float* __restrict__ out) {
+__device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int lane, int L, int S, const float* C,
+                                                    int csp, const long long* pair_row, float* __restrict__ out) {
     const int nb = (S + 1) / 2;
     const int nbp = nb * (nb + 1) / 2;
-    const int npairs = S * (S + 1) / 2;
     const int P = L * 64;
     const int j16 = lane & 15, jk = lane >> 4;
     for (int u = wave; u < L * nbp; u += n_waves) {
@@ -3190,15 +3186,13 @@ __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int l
         // lane j: rows 4 (j / 16) + v = (t2 = 2 b2 + j / 32, n2 = 4 ((j / 16) % 2) + v),
         // column j % 16 = (t1 = 2 b1 + (j % 16) / 8, n1 = j % 8)
         const int t2 = 2 * b2 + (lane >> 5), t1 = 2 * b1 + (j16 >> 3);
-        if (t2 < S && t1 <= t2 && ((mask >> t1) & 1u) && ((mask >> t2) & 1u)) {
-            const long long idx = (long long)(ct * npairs + pair_index(t1, t2, S)) * n_atoms + i;
-            if (sample_flag[idx]) {
-                const float kappa = t1 != t2 ? 1.4142135623730951f : 1.f;
-                const float norm = (l % 2 == 0 ? 1.f : -1.f) * (float)sqrt((double)(2 * l + 1));
-                float* dst = out + sample_scan[idx] * P + l * 64 + (lane & 7) * 8 + 4 * ((lane >> 4) & 1);
-                store4<float>(dst, (acc[0] * kappa) / norm, (acc[1] * kappa) / norm, (acc[2] * kappa) / norm,
-                              (acc[3] * kappa) / norm);
-            }
+        const long long row = (t2 < S && t1 <= t2) ? pair_row[pair_index(t1, t2, S)] : -1;
+        if (row >= 0) {
+            const float kappa = t1 != t2 ? 1.4142135623730951f : 1.f;
+            const float norm = (l % 2 == 0 ? 1.f : -1.f) * (float)sqrt((double)(2 * l + 1));
+            float* dst = out + row * P + l * 64 + (lane & 7) * 8 + 4 * ((lane >> 4) & 1);
+            store4<float>(dst, (acc[0] * kappa) / norm, (acc[1] * kappa) / norm, (acc[2] * kappa) / norm,
+                          (acc[3] * kappa) / norm);
         }
     }
 }
@@ -3216,7 +3210,8 @@ struct BoolTag {
 };
 
 template <int E, int NH>
-__global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
+__global__ void __launch_bounds__(EXM_THREADS) __attribute__((amdgpu_waves_per_eu(E == 8 && NH <= 2 ? 5 : 1, 8)))
+expansion_mfma_kernel(
     SoapArgs<float> a, const float* __restrict__ entry_data, float* __restrict__ out_coef, float* __restrict__ out_grad,
     const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_flag,
     const long long* __restrict__ sample_scan, float* __restrict__ ps_values) {
@@ -3242,7 +3237,6 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
     const ExpansionMfmaLayout ly(L, E, fused ? S : 0);
     const TileEntryLayout lay(L, 8, 8);
     const ShTable tb(L);
-    const float* shc = a.sh_c;
     const bool grads = out_grad != nullptr;
     const long long e0 = a.offsets[i];
     const int len = (int)(a.offsets[i + 1] - e0);
@@ -3252,6 +3246,23 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
     float* fr = lds + ly.fr;
     float* rab = lds + ly.rab;
     float* geo = lds + ly.geo;
+    // the spherical-harmonics coefficients, and the sample row of each type pair's
+    // power-spectrum values (-1: no sample), in LDS before the first chunk's barrier
+    __shared__ float shc[EXM_MAX_L * (EXM_MAX_L + 1) + 4 * EXM_MAX_L];
+    __shared__ long long pair_row[EXM_MAX_VALUE_TYPES * (EXM_MAX_VALUE_TYPES + 1) / 2];
+    for (int k = tid; k < tb.size; k += EXM_THREADS) shc[k] = a.sh_c[k];
+    if (fused && tid < S * (S + 1) / 2) {
+        int t1 = 0, rem = tid;
+        while (rem >= S - t1) {
+            rem -= S - t1;
+            t1++;
+        }
+        const int t2 = t1 + rem;
+        const unsigned mask = type_mask[i];
+        const long long idx = (long long)(ct * (S * (S + 1) / 2) + tid) * a.n_atoms + i;
+        const long long flag = sample_flag[idx], row = sample_scan[idx];
+        pair_row[tid] = (((mask >> t1) & 1u) && ((mask >> t2) & 1u) && flag) ? row : -1;
+    }
 
     // this lane's blocks: block b = 16 h + lane / 4 enumerates (l, m quad, c quad) with
     // the c quad fastest; the A operand is the radial value c = 4 cq + lane % 4, the B
@@ -3429,6 +3440,9 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
                     for (int k = 0; k < 4; k++) {
                         const int lo = __builtin_amdgcn_readlane(starts, k);
                         const int hi = __builtin_amdgcn_readlane(starts, k + 1);
+                        // (two entries per pass: the second's operand reads issue before the
+                        // first's products)
+#pragma unroll 2
                         for (int q = lo; q < hi; q++) {
                             const int e = __builtin_amdgcn_readlane(ord, q);
 #pragma unroll
@@ -3445,6 +3459,7 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
                     for (int k = 0; k < 4; k++) {
                         const int lo = __builtin_amdgcn_readlane(starts, k);
                         const int hi = __builtin_amdgcn_readlane(starts, k + 1);
+#pragma unroll 2
                         for (int q = lo; q < hi; q++) {
                             const int e = __builtin_amdgcn_readlane(ord, q);
                             const float dx = geo[e * 12 + x], gax = geo[e * 12 + 3 + x], gbx = geo[e * 12 + 6 + x];
@@ -3493,8 +3508,7 @@ __global__ void __launch_bounds__(EXM_THREADS) expansion_mfma_kernel(
     }
     if (fused) {
         lds_barrier();  // the coefficients of every type are in LDS
-        expansion_ps_values(comp, EXM_THREADS / WAVE, lane, L, S, a.n_atoms, i, ct, type_mask[i], lds + ly.cval, ly.csp,
-                            sample_flag, sample_scan, ps_values);
+        expansion_ps_values(comp, EXM_THREADS / WAVE, lane, L, S, lds + ly.cval, ly.csp, pair_row, ps_values);
     }
 }
 
