@@ -110,11 +110,13 @@ __global__ void nl_fill_bins(int n_atoms, const long long* __restrict__ atom_bin
     bin_atoms[bin_start[bin] + slot] = i;
 }
 
-// One wave per center; the lanes take a bin's atoms 64 at a time, so the
-// candidate positions are read coalesced and the lanes never diverge over
-// loop lengths. `emit` = false counts, true writes packed keys: each hit goes
-// to the center's running offset plus its rank among the wave's hits (the
-// per-center order is fixed later by the segment sort).
+// One wave per center. The lanes first take one searched bin each (its shift, first
+// atom and atom count), and a prefix sum over the wave lays the bins' atoms end to
+// end; the lanes then take these candidates 64 at a time, each finding its bin by a
+// binary search over the prefix sums. So the candidate loads of a pass are issued
+// together, and no lane idles on a bin shorter than the wave. `emit` = false counts,
+// true writes packed keys: each hit goes to the center's running offset plus its rank
+// among the wave's hits (the per-center order is fixed later by the segment sort).
 constexpr int NL_WAVES = 4;
 
 template <bool EMIT>
@@ -137,59 +139,83 @@ nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __re
     const int wi0 = atom_wrap[3 * i], wi1 = atom_wrap[3 * i + 1], wi2 = atom_wrap[3 * i + 2];
     const double* pi = positions + 3 * (long long)i;
     const unsigned long long below = (1ull << lane) - 1ull;
+    const int w0 = 2 * sys.n_search[0] + 1, w1 = 2 * sys.n_search[1] + 1, w2 = 2 * sys.n_search[2] + 1;
+    const int n_slots = w0 * w1 * w2;
     long long count = 0;
     long long out = EMIT ? offsets[i] : 0;
-    for (int d0 = -sys.n_search[0]; d0 <= sys.n_search[0]; d0++) {
-        int q0, r0;
-        divmod(b0 + d0, nc0, &q0, &r0);
-        if (!sys.periodic && q0 != 0) continue;
-        for (int d1 = -sys.n_search[1]; d1 <= sys.n_search[1]; d1++) {
-            int q1, r1;
+    for (int slot_base = 0; slot_base < n_slots; slot_base += 64) {
+        // this lane's bin: offsets (d0, d1, d2) in the order of three nested loops
+        int q0 = 0, q1 = 0, q2 = 0, len = 0;
+        long long start = 0;
+        const int slot = slot_base + lane;
+        if (slot < n_slots) {
+            const int d2 = slot % w2 - sys.n_search[2];
+            const int d1 = (slot / w2) % w1 - sys.n_search[1];
+            const int d0 = slot / (w1 * w2) - sys.n_search[0];
+            int r0, r1, r2;
+            divmod(b0 + d0, nc0, &q0, &r0);
             divmod(b1 + d1, nc1, &q1, &r1);
-            if (!sys.periodic && q1 != 0) continue;
-            for (int d2 = -sys.n_search[2]; d2 <= sys.n_search[2]; d2++) {
-                int q2, r2;
-                divmod(b2 + d2, nc2, &q2, &r2);
-                if (!sys.periodic && q2 != 0) continue;
+            divmod(b2 + d2, nc2, &q2, &r2);
+            if (sys.periodic || (q0 == 0 && q1 == 0 && q2 == 0)) {
                 const long long nb = sys.bin_offset + ((long long)r0 * nc1 + r1) * nc2 + r2;
-                const long long start = bin_start[nb];
-                const int len = bin_len[nb];
-                for (int base = 0; base < len; base += 64) {
-                    const int a = base + lane;
-                    bool hit = false, overflow = false;
-                    unsigned long long key = 0;
-                    if (a < len) {
-                        const int j = bin_atoms[start + a];
-                        int s[3];
-                        if (sys.periodic) {
-                            s[0] = q0 + wi0 - atom_wrap[3 * j];
-                            s[1] = q1 + wi1 - atom_wrap[3 * j + 1];
-                            s[2] = q2 + wi2 - atom_wrap[3 * j + 2];
-                        } else {
-                            s[0] = s[1] = s[2] = 0;
-                        }
-                        if (!(j == i && s[0] == 0 && s[1] == 0 && s[2] == 0)) {
-                            double v[3];
-                            pair_vector(pi, positions + 3 * (long long)j, sys.cell, s, v);
-                            const double d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-                            hit = d2v < cutoff2;
-                            if (EMIT && hit) {
-                                overflow = s[0] < -SHIFT_BIAS || s[0] >= SHIFT_BIAS || s[1] < -SHIFT_BIAS ||
-                                           s[1] >= SHIFT_BIAS || s[2] < -SHIFT_BIAS || s[2] >= SHIFT_BIAS;
-                                key = pack_key(j, s[0], s[1], s[2]);
-                            }
-                        }
-                    }
-                    const unsigned long long mask = __ballot(hit);
+                start = bin_start[nb];
+                len = bin_len[nb];
+            }
+        }
+        // exclusive prefix sum of the bins' lengths over the wave
+        int incl = len;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        const int pre = incl - len;
+        const int total = __shfl(incl, 63);
+        for (int base = 0; base < total; base += 64) {
+            const int a = base + lane;
+            // the lane's bin: the last lane whose prefix is at most a
+            int t = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int p = __shfl(pre, t + step);
+                if (p <= a) t += step;
+            }
+            const int off = a - __shfl(pre, t);
+            const long long st = ((long long)__shfl((int)(start >> 32), t) << 32) |
+                                 (unsigned int)__shfl((int)(start & 0xffffffff), t);
+            const int sq0 = __shfl(q0, t), sq1 = __shfl(q1, t), sq2 = __shfl(q2, t);
+            bool hit = false, overflow = false;
+            unsigned long long key = 0;
+            if (a < total) {
+                const int j = bin_atoms[st + off];
+                int s[3];
+                if (sys.periodic) {
+                    s[0] = sq0 + wi0 - atom_wrap[3 * j];
+                    s[1] = sq1 + wi1 - atom_wrap[3 * j + 1];
+                    s[2] = sq2 + wi2 - atom_wrap[3 * j + 2];
+                } else {
+                    s[0] = s[1] = s[2] = 0;
+                }
+                if (!(j == i && s[0] == 0 && s[1] == 0 && s[2] == 0)) {
+                    double v[3];
+                    pair_vector(pi, positions + 3 * (long long)j, sys.cell, s, v);
+                    const double d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    hit = d2v < cutoff2;
                     if (EMIT && hit) {
-                        keys[out + __popcll(mask & below)] = key;
-                        if (overflow) atomicOr(error_flag, 1);
+                        overflow = s[0] < -SHIFT_BIAS || s[0] >= SHIFT_BIAS || s[1] < -SHIFT_BIAS ||
+                                   s[1] >= SHIFT_BIAS || s[2] < -SHIFT_BIAS || s[2] >= SHIFT_BIAS;
+                        key = pack_key(j, s[0], s[1], s[2]);
                     }
-                    const int hits = __popcll(mask);
-                    out += hits;
-                    count += hits;
                 }
             }
+            const unsigned long long mask = __ballot(hit);
+            if (EMIT && hit) {
+                keys[out + __popcll(mask & below)] = key;
+                if (overflow) atomicOr(error_flag, 1);
+            }
+            const int hits = __popcll(mask);
+            out += hits;
+            count += hits;
         }
     }
     if (!EMIT && lane == 0) counts[i] = count;
