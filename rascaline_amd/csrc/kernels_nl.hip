@@ -221,7 +221,37 @@ nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __re
     if (!EMIT && lane == 0) counts[i] = count;
 }
 
-// Bitonic sort of each center's key segment in LDS (one workgroup per center).
+// Bitonic sort of each center's key segment of at most 64 keys in registers, one wave
+// per center (key k in lane k, compare-exchange partners by lane shuffles); longer
+// segments are left to nl_sort_segments.
+constexpr int SORT_WAVES = 4;
+
+__global__ void __launch_bounds__(64 * SORT_WAVES) nl_sort_short_segments(int n_atoms, const long long* __restrict__ offsets,
+                                                                         unsigned long long* __restrict__ keys) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * SORT_WAVES + (threadIdx.x >> 6);
+    if (i >= n_atoms) return;  // uniform over the wave
+    const long long start = offsets[i];
+    const int len = (int)(offsets[i + 1] - start);
+    if (len <= 1 || len > 64) return;
+    unsigned long long key = lane < len ? keys[start + lane] : ~0ull;
+    for (int k = 2; k <= 64; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const unsigned int lo = __shfl_xor((unsigned int)(key & 0xffffffffull), j);
+            const unsigned int hi = __shfl_xor((unsigned int)(key >> 32), j);
+            const unsigned long long other = ((unsigned long long)hi << 32) | lo;
+            const bool ascending = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            const unsigned long long small = key < other ? key : other;
+            const unsigned long long large = key < other ? other : key;
+            key = (lower == ascending) ? small : large;
+        }
+    }
+    if (lane < len) keys[start + lane] = key;
+}
+
+// Bitonic sort of each center's key segment of more than 64 keys in LDS (one workgroup
+// per center).
 constexpr int SORT_CAPACITY = 2048;
 
 __global__ void __launch_bounds__(256) nl_sort_segments(int n_atoms, const long long* __restrict__ offsets,
@@ -231,7 +261,7 @@ __global__ void __launch_bounds__(256) nl_sort_segments(int n_atoms, const long 
     if (i >= n_atoms) return;
     const long long start = offsets[i];
     const int len = (int)(offsets[i + 1] - start);
-    if (len <= 1) return;
+    if (len <= 64) return;  // nl_sort_short_segments
     if (len > SORT_CAPACITY) {
         if (threadIdx.x == 0) atomicOr(error_flag, 2);
         return;
@@ -399,6 +429,8 @@ void launch_nl_candidates(hipStream_t stream, bool emit, int n_atoms, const doub
 void launch_nl_sort_segments(hipStream_t stream, int n_atoms, const long long* offsets, unsigned long long* keys,
                              int* error_flag) {
     if (n_atoms == 0) return;
+    nl_sort_short_segments<<<(n_atoms + SORT_WAVES - 1) / SORT_WAVES, 64 * SORT_WAVES, 0, stream>>>(n_atoms, offsets,
+                                                                                                   keys);
     nl_sort_segments<<<n_atoms, 256, 0, stream>>>(n_atoms, offsets, keys, error_flag);
 }
 

@@ -1186,7 +1186,8 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
         if (lane == 0) windows[e0 + n_win] = make_int4(start, entries, run_start, count);
         n_win++;
     };
-    auto process_run = [&](int first, int count, int j) {
+    // tj, aj: the neighbour's type and its index in its system, loaded with the entries
+    auto process_run = [&](int first, int count, int j, int tj, int aj) {
         if (j == i) {
             // periodic images of the center: no row; close the current window around them
             if (cur_runs > 0) emit(cur_start, cur_entries, cur_run_start, cur_runs);
@@ -1195,7 +1196,6 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
             return;
         }
         const int r = n_runs++;
-        const int tj = species[j];
         if (lane == 0) runs[e0 + r] = make_int4(first, count, tj, 0);
         if (!snapped && j > i) {
             for (int k = lane; k < nloc; k += WAVE) snapshot[k] = counters[k];
@@ -1210,7 +1210,7 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
                 run_rows[(e0 + r) * S] = (int)row;
                 labels[3 * row] = sample_in_block[tj];
                 labels[3 * row + 1] = sys;
-                labels[3 * row + 2] = atom_local[j];
+                labels[3 * row + 2] = aj;
             }
         } else if (lane < n_present) {
             const int t = present[lane];
@@ -1221,7 +1221,7 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
             run_rows[(e0 + r) * S + lane] = (int)row;
             labels[3 * row] = sample_in_block[local];
             labels[3 * row + 1] = sys;
-            labels[3 * row + 2] = atom_local[j];
+            labels[3 * row + 2] = aj;
         }
         __builtin_amdgcn_wave_barrier();
         // windows: up to max_runs runs and max_entries entries; a longer run is split
@@ -1249,11 +1249,13 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
         }
     };
 
-    int open_first = -1, open_j = -1;
+    int open_first = -1, open_j = -1, open_tj = 0, open_aj = 0;
     int prev_last = -1;
     for (int base = 0; base < len; base += WAVE) {
         const int e = base + lane;
         const int j = e < len ? entry_j[e0 + e] : -1;
+        const int tjl = e < len ? species[j] : 0;
+        const int ajl = e < len ? atom_local[j] : 0;
         int prev = __shfl_up(j, 1);
         if (lane == 0) prev = prev_last;
         const bool start = e < len && (e == 0 || j != prev);
@@ -1262,13 +1264,15 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
         while (bits != 0ull) {
             const int b = __ffsll((long long)bits) - 1;
             bits &= bits - 1ull;
-            const int jb = __shfl(j, b);
-            if (open_first >= 0) process_run(open_first, base + b - open_first, open_j);
+            const int jb = __shfl(j, b), tjb = __shfl(tjl, b), ajb = __shfl(ajl, b);
+            if (open_first >= 0) process_run(open_first, base + b - open_first, open_j, open_tj, open_aj);
             open_first = base + b;
             open_j = jb;
+            open_tj = tjb;
+            open_aj = ajb;
         }
     }
-    if (open_first >= 0) process_run(open_first, len - open_first, open_j);
+    if (open_first >= 0) process_run(open_first, len - open_first, open_j, open_tj, open_aj);
     if (cur_runs > 0) emit(cur_start, cur_entries, cur_run_start, cur_runs);
     if (lane == 0) {
         n_windows[i] = n_win;
@@ -1303,13 +1307,6 @@ __global__ void __launch_bounds__(WAVE) ps_grad_plan_kernel(
     }
 }
 
-// Per-entry data of the tiled gradient kernel, one (entry, slot) per thread over
-// the whole batch. Slots [0, L): m = slot, the associated Legendre functions P_lm
-// for l >= m (the recursion of featomic/src/math/spherical_harmonics.rs:228-251
-// run along l for this m, the diagonal chain recomputed per slot); slot 0 also
-// stores the direction, trigonometric factors and the cutoff / scaling factors.
-// Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
-// to NC.
 // radial_integral for n = 8, with the interval's knots and every table value loaded
 // together (16-byte pieces) at the interval of the uniform-grid estimate (from the
 // per-channel grid table), and loaded again only when the stored knots move the
@@ -1386,6 +1383,13 @@ __device__ __forceinline__ double record_index<double>(int j) {
     return (double)j;
 }
 
+// Per-entry data of the tiled gradient kernel, one (entry, slot) per thread over
+// the whole batch. Slots [0, L): m = slot, the associated Legendre functions P_lm
+// for l >= m (the recursion of featomic/src/math/spherical_harmonics.rs:228-251
+// run along l for this m, the diagonal chain recomputed per slot); slot 0 also
+// stores the direction, trigonometric factors and the cutoff / scaling factors.
+// Slots [L, 2L): l = slot - L, the radial spline R_nl and dR_nl/dr, zero-padded
+// to NC.
 template <typename T>
 __device__ void entry_prep_slot(const SoapArgs<T>& a, const TileEntryLayout& lay, long long e, int slot, int NC, T* base,
                                 const T* sha, const T* shb, const T* shc);
