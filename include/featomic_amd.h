@@ -30,13 +30,13 @@ extern "C" {
  * Outputs of featomic_calculator_compute are float64 arrays either way. */
 featomic_status_t featomic_amd_calculator_set_precision(featomic_calculator_t* calculator, int32_t bits);
 
-/* Select the fast form (1) or the plain form (0, default) of the FP32 kernels. Where
- * the matrix-core expansion kernel does not form the power-spectrum values itself
- * (it does for 8 radial functions, max_angular < 9 and at most 16 types, in both
- * forms), the fast form runs the values' m-contraction on the matrix cores with
+/* Select the fast form (1) or the plain form (0, default) of the FP32 kernels. The
+ * fast form runs the power-spectrum values' m-contraction on the matrix cores with
  * bfloat16 hi + lo operand splitting and FP32 accumulation (three products per
  * block, about 16 significant bits per operand; results within 1e-4 of the FP64
- * computation) for n a multiple of 4 and at most 16 types. Its gradient rows take
+ * computation): inside the expansion kernel where it forms the values (8 radial
+ * functions, max_angular < 9, at most 16 types), else in the values kernel for n a
+ * multiple of 4 and at most 16 types. Its gradient rows take
  * the fastest contraction the autotuner finds among the matrix-core ones: the
  * rank-2 kernel (FP32 operands, n = 8, the default of both forms) or the bfloat16
  * hi + lo tiled kernel (n = 8, 12 or 16). Other settings and FP64 use the plain

@@ -602,8 +602,9 @@ mts_tensormap_t* Calculator::run_soap(const HostBatch& batch, const Gradients& g
             if constexpr (std::is_same<T, float>::value) {
                 fused_values = ps && !vjp_grad_ && expansion_mfma_values_supported(S);
                 T* values = fused_values ? ws_.get<T>("out_values", std::max<long long>(total_samples * P, 1)) : nullptr;
+                // the fast form's values: products over m in bfloat16 hi + lo
                 launch_expansion_mfma(lp, asked > 0 ? asked : RA_EXPANSION_MFMA_ENTRIES, entry_data, coef, dself,
-                                      type_mask, sample_flag, sample_scan, values);
+                                      type_mask, sample_flag, sample_scan, values, fast_form != 0);
             }
         } else if (expansion_tiled_shmem<T>(lp, max_entries) + 1024 <= static_cast<size_t>(lds)) {
             launch_expansion_tiled<T>(lp, max_entries, entry_data, coef, dself);

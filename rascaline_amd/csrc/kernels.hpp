@@ -168,13 +168,15 @@ template <typename T>
 void launch_expansion_tiled(const SoapLaunch<T>& p, int max_entries, const T* entry_data, T* coef, T* dself);
 // the same on the matrix cores (FP32, n = 8, l_max < 9), in chunks of 8 or 16 entries;
 // with `ps_values` (at most 16 types), it also writes the power-spectrum values of the
-// samples that `sample_flag` / `sample_scan` place (as launch_ps_values does)
+// samples that `sample_flag` / `sample_scan` place (as launch_ps_values does), with the
+// products over m in FP32 or, with `bf16_values`, in bfloat16 hi + lo
 bool expansion_mfma_supported(int L, int n, int n_species);
 bool expansion_mfma_values_supported(int n_species);
 size_t expansion_mfma_lds(const SoapLaunch<float>& p, int max_entries, bool values);
 void launch_expansion_mfma(const SoapLaunch<float>& p, int max_entries, const float* entry_data, float* coef,
                            float* dself, const unsigned* type_mask = nullptr, const long long* sample_flag = nullptr,
-                           const long long* sample_scan = nullptr, float* ps_values = nullptr);
+                           const long long* sample_scan = nullptr, float* ps_values = nullptr,
+                           bool bf16_values = false);
 template <typename T>
 void launch_ps_gradient_tiled(const SoapLaunch<T>& p, int max_runs, int max_entries, int rows_per_pass, const T* coef,
                               const T* dself, const unsigned* type_mask, const int* runs, const int* run_rows,

@@ -3162,8 +3162,12 @@ struct ExpansionMfmaLayout {
 // (soap/power_spectrum.rs:340-348): one v_mfma_f32_16x16x4f32 product per (l, 2 x 2
 // block of type pairs, 4 m), rows (t2, n2) and columns (t1, n1), so that lane j holds
 // PS[l][n1][4 h .. 4 h + 3] of the pair (t1, t2), a 16-byte piece of its sample.
+// With `bf16`, the products over m are split into bfloat16 hi + lo parts
+// (hi hi + hi lo + lo hi on v_mfma_f32_16x16x16bf16_1k, all of m in one product, FP32
+// accumulation), as the fast form's values kernel forms them.
 __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int lane, int L, int S, const float* C,
-                                                    int csp, const long long* pair_row, float* __restrict__ out) {
+                                                    int csp, const long long* pair_row, bool bf16,
+                                                    float* __restrict__ out) {
     const int nb = (S + 1) / 2;
     const int nbp = nb * (nb + 1) / 2;
     const int P = L * 64;
@@ -3181,11 +3185,32 @@ __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int l
         const float* C2 = C + (kv2 ? k2 : 0) * csp + l * l * 8 + (lane & 7);
         const float* C1 = C + (kv1 ? k1 : 0) * csp + l * l * 8 + (lane & 7);
         float4_t acc = {0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < (2 * l + 1 + 3) / 4; ks++) {
-            const int mp = 4 * ks + jk;
-            const bool mv = mp <= 2 * l;
-            const int off = (mv ? mp : 0) * 8;
-            acc = mfma16x16((mv && kv2) ? C2[off] : 0.f, (mv && kv1) ? C1[off] : 0.f, acc);
+        if (bf16) {
+            // lane i + 16 g holds m = 4 g .. 4 g + 3 of row / column i
+            short4_t ah, al, bh, bl;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int mp = 4 * jk + q;
+                const bool mv = mp <= 2 * l;
+                const int off = (mv ? mp : 0) * 8;
+                short h, lo;
+                bf16_split((mv && kv2) ? C2[off] : 0.f, h, lo);
+                ah[q] = h;
+                al[q] = lo;
+                bf16_split((mv && kv1) ? C1[off] : 0.f, h, lo);
+                bh[q] = h;
+                bl[q] = lo;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+        } else {
+            for (int ks = 0; ks < (2 * l + 1 + 3) / 4; ks++) {
+                const int mp = 4 * ks + jk;
+                const bool mv = mp <= 2 * l;
+                const int off = (mv ? mp : 0) * 8;
+                acc = mfma16x16((mv && kv2) ? C2[off] : 0.f, (mv && kv1) ? C1[off] : 0.f, acc);
+            }
         }
         // lane j: rows 4 (j / 16) + v = (t2 = 2 b2 + j / 32, n2 = 4 ((j / 16) % 2) + v),
         // column j % 16 = (t1 = 2 b1 + (j % 16) / 8, n1 = j % 8)
@@ -3218,7 +3243,7 @@ __global__ void __launch_bounds__(EXM_THREADS) __attribute__((amdgpu_waves_per_e
 expansion_mfma_kernel(
     SoapArgs<float> a, const float* __restrict__ entry_data, float* __restrict__ out_coef, float* __restrict__ out_grad,
     const unsigned* __restrict__ type_mask, const long long* __restrict__ sample_flag,
-    const long long* __restrict__ sample_scan, float* __restrict__ ps_values) {
+    const long long* __restrict__ sample_scan, float* __restrict__ ps_values, int bf16_values) {
     extern __shared__ __align__(16) unsigned char smem[];
     // the chunk's entries grouped by type within the group of four types (values), and
     // the same without the center's own periodic images (gradients)
@@ -3512,7 +3537,8 @@ expansion_mfma_kernel(
     }
     if (fused) {
         lds_barrier();  // the coefficients of every type are in LDS
-        expansion_ps_values(comp, EXM_THREADS / WAVE, lane, L, S, lds + ly.cval, ly.csp, pair_row, ps_values);
+        expansion_ps_values(comp, EXM_THREADS / WAVE, lane, L, S, lds + ly.cval, ly.csp, pair_row, bf16_values != 0,
+                            ps_values);
     }
 }
 
@@ -4213,7 +4239,7 @@ size_t expansion_mfma_lds(const SoapLaunch<float>& p, int E, bool values) {
 
 void launch_expansion_mfma(const SoapLaunch<float>& p, int E, const float* entry_data, float* coef, float* dself,
                            const unsigned* type_mask, const long long* sample_flag, const long long* sample_scan,
-                           float* ps_values) {
+                           float* ps_values, bool bf16_values) {
     if (p.n_atoms == 0) return;
     if (!expansion_mfma_supported(p.L, p.n, p.n_species) || (E != 8 && E != 16)) {
         throw std::invalid_argument("the matrix-core expansion kernel takes n = 8, l < 9 and chunks of 8 or 16 entries");
@@ -4230,7 +4256,8 @@ void launch_expansion_mfma(const SoapLaunch<float>& p, int E, const float* entry
 #define RA_LAUNCH_EXM(EV, NHV)                                                                        \
     expansion_mfma_kernel<EV, NHV><<<p.n_atoms, EXM_THREADS, shmem, p.stream>>>(a, entry_data, coef, dself, \
                                                                                 type_mask, sample_flag,   \
-                                                                                sample_scan, ps_values)
+                                                                                sample_scan, ps_values, \
+                                                                                bf16_values ? 1 : 0)
 #define RA_LAUNCH_EXM_E(NHV)  \
     if (E == 8) {             \
         RA_LAUNCH_EXM(8, NHV); \

@@ -12,7 +12,8 @@ vector-ALU expansion kernel:
 
 The spherical expansion's values and position gradients are checked element-wise at
 1e-4 with a floor of 1e-5 of the block's largest value; the power spectrum built on
-them likewise.
+them likewise. The fast form's values (bfloat16 hi + lo products in the same kernel) are
+checked against the oracle at 1e-4 of each block's largest value.
 """
 
 import sys
@@ -108,3 +109,26 @@ def test_expansion_mfma_against_vector_alu_kernel():
         assert gm.shape == gv.shape
         scale = max(np.abs(gv).max(), 1e-30)
         assert np.abs(gm - gv).max() <= 2e-6 * scale
+
+
+def test_power_spectrum_values_bf16_split_in_expansion_kernel():
+    """The fast form's values (products over m in bfloat16 hi + lo, formed in the expansion
+    kernel) stay within 1e-4 of the FP64 oracle per block, and agree with the FP32 products."""
+    say("power spectrum values, bfloat16 hi + lo in the expansion kernel")
+    reference = oracle_to_tensormap(orc.soap_power_spectrum(systems(), hypers(6)))
+    syst = [ra.System(s.types, s.positions, s.cell) for s in systems()]
+    out = {}
+    for fast in (False, True):
+        calc = ra.SoapPowerSpectrum(**hypers(6))
+        calc.set_precision(32)
+        calc.set_fast_form(fast)
+        calc.tuning = {"expansion_entries": -8, "gradient_window": -1}
+        out[fast] = calc.compute(syst)
+    worst = 0.0
+    for bf, bp, be in zip(out[True].blocks(), out[False].blocks(), reference.blocks()):
+        scale = max(np.abs(be.values).max(), 1e-30)
+        err = np.abs(bf.values - be.values).max() / scale
+        worst = max(worst, err)
+        assert err <= 1e-4
+        assert np.abs(bf.values - bp.values).max() <= 1e-4 * scale
+    say(f"  block-relative error of the split products: {worst:.2e}")
