@@ -7,7 +7,7 @@ vector-ALU expansion kernel:
 * a cell of a few atoms, whose centers see their own periodic images (values only,
   no self gradient), and one atom whose only neighbours are its own images;
 * an isolated atom of a cluster (no entries: the self contribution alone);
-* l_max 2, 6, 7 and 8 (one to four halves of 16 blocks per instruction);
+* l_max 0, 1, 2, 6, 7 and 8 (one to four halves of 16 blocks per instruction);
 * chunks of 8 and 16 entries, with centers of more entries than a chunk.
 
 The spherical expansion's values and position gradients are checked element-wise at
@@ -82,7 +82,7 @@ def check_against(tm, reference):
     return worst
 
 
-@pytest.mark.parametrize("lmax", [2, 6, 7, 8])
+@pytest.mark.parametrize("lmax", [0, 1, 2, 6, 7, 8])
 @pytest.mark.parametrize("entries", [-8, -16])
 def test_expansion_mfma_against_oracle(lmax, entries):
     say("spherical expansion, l_max", lmax, "entries", entries)
