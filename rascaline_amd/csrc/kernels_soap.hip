@@ -3135,7 +3135,8 @@ __global__ void __launch_bounds__(TILE_THREADS) expansion_tiled_kernel(SoapArgs<
 // angular values, and lane 4 b + j holds out[l^2 + 4 mq + j][4 cq .. 4 cq + 3], one
 // 16-byte piece of the output. The chunk's entries are visited grouped by type, one
 // set of accumulators per type of a group of four; more types take one pass over the
-// entries per group.
+// entries per group. For the power spectrum (at most 16 types), the coefficients also
+// stay in LDS and the center's power-spectrum values follow (expansion_ps_values).
 constexpr int EXM_THREADS = 256;
 constexpr int EXM_MAX_L = 9;
 constexpr int EXM_MAX_VALUE_TYPES = 16;
