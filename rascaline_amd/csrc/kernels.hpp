@@ -14,10 +14,15 @@ void launch_nl_assign_bins(hipStream_t stream, int n_atoms, const double* positi
                            const NLSystem* systems, long long* atom_bin, int* atom_wrap, int* bin_count);
 void launch_nl_fill_bins(hipStream_t stream, int n_atoms, const long long* atom_bin, const long long* bin_start,
                          int* bin_fill, int* bin_atoms);
+// counting pass (emit = false) and emitting pass (emit = true); with `stash`
+// (nl_stash_keys(n_atoms) keys), the counting pass keeps each center's first keys and
+// the emitting pass copies them instead of searching again
+size_t nl_stash_keys(int n_atoms);
 void launch_nl_candidates(hipStream_t stream, bool emit, int n_atoms, const double* positions, const int* atom_system,
                           const NLSystem* systems, const long long* atom_bin, const int* atom_wrap,
                           const long long* bin_start, const int* bin_len, const int* bin_atoms, double cutoff2,
-                          long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag);
+                          long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag,
+                          unsigned long long* stash = nullptr);
 void launch_nl_sort_segments(hipStream_t stream, int n_atoms, const long long* offsets, unsigned long long* keys,
                              int* error_flag);
 void launch_nl_finalize(hipStream_t stream, long long n_entries, int n_atoms, const long long* offsets,

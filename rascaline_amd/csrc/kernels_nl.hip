@@ -117,6 +117,9 @@ __global__ void nl_fill_bins(int n_atoms, const long long* __restrict__ atom_bin
 // together, and no lane idles on a bin shorter than the wave. `emit` = false counts,
 // true writes packed keys: each hit goes to the center's running offset plus its rank
 // among the wave's hits (the per-center order is fixed later by the segment sort).
+// keys per center that the counting pass keeps for the emitting pass (centers with
+// more are searched again)
+constexpr int NL_STASH = 128;
 constexpr int NL_WAVES = 4;
 
 template <bool EMIT>
@@ -126,10 +129,20 @@ nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __re
               const int* __restrict__ atom_wrap, const long long* __restrict__ bin_start,
               const int* __restrict__ bin_len, const int* __restrict__ bin_atoms, double cutoff2,
               long long* __restrict__ counts, const long long* __restrict__ offsets,
-              unsigned long long* __restrict__ keys, int* __restrict__ error_flag) {
+              unsigned long long* __restrict__ keys, unsigned long long* __restrict__ stash,
+              int* __restrict__ error_flag) {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
     if (i >= n_atoms) return;  // uniform over the wave
+    if (EMIT && stash) {
+        // the counting pass kept this center's keys: copy them
+        const long long first = offsets[i];
+        const long long n = offsets[i + 1] - first;
+        if (n <= NL_STASH) {
+            for (int k = lane; k < n; k += 64) keys[first + k] = stash[(long long)i * NL_STASH + k];
+            return;
+        }
+    }
     const NLSystem& sys = systems[atom_system[i]];
     const long long local_bin = atom_bin[i] - sys.bin_offset;
     const int nc0 = sys.n_cells[0], nc1 = sys.n_cells[1], nc2 = sys.n_cells[2];
@@ -201,7 +214,7 @@ nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __re
                     pair_vector(pi, positions + 3 * (long long)j, sys.cell, s, v);
                     const double d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
                     hit = d2v < cutoff2;
-                    if (EMIT && hit) {
+                    if ((EMIT || stash) && hit) {
                         overflow = s[0] < -SHIFT_BIAS || s[0] >= SHIFT_BIAS || s[1] < -SHIFT_BIAS ||
                                    s[1] >= SHIFT_BIAS || s[2] < -SHIFT_BIAS || s[2] >= SHIFT_BIAS;
                         key = pack_key(j, s[0], s[1], s[2]);
@@ -211,6 +224,12 @@ nl_candidates(int n_atoms, const double* __restrict__ positions, const int* __re
             const unsigned long long mask = __ballot(hit);
             if (EMIT && hit) {
                 keys[out + __popcll(mask & below)] = key;
+                if (overflow) atomicOr(error_flag, 1);
+            }
+            if (!EMIT && stash && hit) {
+                // the first NL_STASH keys, for the emitting pass to copy
+                const long long rank = count + __popcll(mask & below);
+                if (rank < NL_STASH) stash[(long long)i * NL_STASH + rank] = key;
                 if (overflow) atomicOr(error_flag, 1);
             }
             const int hits = __popcll(mask);
@@ -408,21 +427,24 @@ void launch_nl_fill_bins(hipStream_t stream, int n_atoms, const long long* atom_
     nl_fill_bins<<<(n_atoms + 255) / 256, 256, 0, stream>>>(n_atoms, atom_bin, bin_start, bin_fill, bin_atoms);
 }
 
+size_t nl_stash_keys(int n_atoms) { return (size_t)n_atoms * NL_STASH; }
+
 void launch_nl_candidates(hipStream_t stream, bool emit, int n_atoms, const double* positions, const int* atom_system,
                           const NLSystem* systems, const long long* atom_bin, const int* atom_wrap,
                           const long long* bin_start, const int* bin_len, const int* bin_atoms, double cutoff2,
-                          long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag) {
+                          long long* counts, const long long* offsets, unsigned long long* keys, int* error_flag,
+                          unsigned long long* stash) {
     if (n_atoms == 0) return;
     const int threads = 64 * NL_WAVES;
     const int blocks = (n_atoms + NL_WAVES - 1) / NL_WAVES;
     if (emit) {
         nl_candidates<true><<<blocks, threads, 0, stream>>>(n_atoms, positions, atom_system, systems, atom_bin, atom_wrap,
                                                             bin_start, bin_len, bin_atoms, cutoff2, counts, offsets,
-                                                            keys, error_flag);
+                                                            keys, stash, error_flag);
     } else {
         nl_candidates<false><<<blocks, threads, 0, stream>>>(n_atoms, positions, atom_system, systems, atom_bin, atom_wrap,
                                                              bin_start, bin_len, bin_atoms, cutoff2, counts, offsets,
-                                                             keys, error_flag);
+                                                             keys, stash, error_flag);
     }
 }
 

@@ -201,8 +201,9 @@ DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cu
     exclusive_scan(stream, bin_count64, bin_start, n_bins, ws);
     launch_nl_fill_bins(stream, n_atoms, atom_bin, bin_start, bin_fill, bin_atoms);
     const double cutoff2 = cutoff * cutoff;
+    unsigned long long* stash = ws.get<unsigned long long>("nl_stash", std::max<size_t>(nl_stash_keys(n_atoms), 1));
     launch_nl_candidates(stream, false, n_atoms, d_positions, d_atom_system, d_systems, atom_bin, atom_wrap, bin_start,
-                         bin_count, bin_atoms, cutoff2, counts, nullptr, nullptr, error_flag);
+                         bin_count, bin_atoms, cutoff2, counts, nullptr, nullptr, error_flag, stash);
     exclusive_scan(stream, counts, nl.offsets, n_atoms, ws);
     long long n_entries = 0;
     if (pass.mode == PassMode::Replay) {
@@ -218,7 +219,7 @@ DeviceNeighbors build_neighbors(Workspace& ws, const HostBatch& batch, double cu
     const size_t ne = static_cast<size_t>(n_entries);
     unsigned long long* keys = ws.get<unsigned long long>("nl_keys", ne);
     launch_nl_candidates(stream, true, n_atoms, d_positions, d_atom_system, d_systems, atom_bin, atom_wrap, bin_start,
-                         bin_count, bin_atoms, cutoff2, counts, nl.offsets, keys, error_flag);
+                         bin_count, bin_atoms, cutoff2, counts, nl.offsets, keys, error_flag, stash);
     launch_nl_sort_segments(stream, n_atoms, nl.offsets, keys, error_flag);
     nl.entry_j = ws.get<int>("nl_entry_j", ne);
     nl.entry_shift = ws.get<int>("nl_entry_shift", 3 * ne);
