@@ -3187,24 +3187,27 @@ __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int l
         const float* C1 = C + (kv1 ? k1 : 0) * csp + l * l * 8 + (lane & 7);
         float4_t acc = {0.f, 0.f, 0.f, 0.f};
         if (bf16) {
-            // lane i + 16 g holds m = 4 g .. 4 g + 3 of row / column i
-            short4_t ah, al, bh, bl;
+            // per 16 values of m (two passes at l = 8): lane i + 16 g holds m = 16 kb + 4 g ..
+            // 16 kb + 4 g + 3 of row / column i
+            for (int kb = 0; kb < (2 * l + 1 + 15) / 16; kb++) {
+                short4_t ah, al, bh, bl;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int mp = 4 * jk + q;
-                const bool mv = mp <= 2 * l;
-                const int off = (mv ? mp : 0) * 8;
-                short h, lo;
-                bf16_split((mv && kv2) ? C2[off] : 0.f, h, lo);
-                ah[q] = h;
-                al[q] = lo;
-                bf16_split((mv && kv1) ? C1[off] : 0.f, h, lo);
-                bh[q] = h;
-                bl[q] = lo;
+                for (int q = 0; q < 4; q++) {
+                    const int mp = 16 * kb + 4 * jk + q;
+                    const bool mv = mp <= 2 * l;
+                    const int off = (mv ? mp : 0) * 8;
+                    short h, lo;
+                    bf16_split((mv && kv2) ? C2[off] : 0.f, h, lo);
+                    ah[q] = h;
+                    al[q] = lo;
+                    bf16_split((mv && kv1) ? C1[off] : 0.f, h, lo);
+                    bh[q] = h;
+                    bl[q] = lo;
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
         } else {
             for (int ks = 0; ks < (2 * l + 1 + 3) / 4; ks++) {
                 const int mp = 4 * ks + jk;

@@ -111,15 +111,17 @@ def test_expansion_mfma_against_vector_alu_kernel():
         assert np.abs(gm - gv).max() <= 2e-6 * scale
 
 
-def test_power_spectrum_values_bf16_split_in_expansion_kernel():
+@pytest.mark.parametrize("lmax", [6, 8])
+def test_power_spectrum_values_bf16_split_in_expansion_kernel(lmax):
     """The fast form's values (products over m in bfloat16 hi + lo, formed in the expansion
-    kernel) stay within 1e-4 of the FP64 oracle per block, and agree with the FP32 products."""
-    say("power spectrum values, bfloat16 hi + lo in the expansion kernel")
-    reference = oracle_to_tensormap(orc.soap_power_spectrum(systems(), hypers(6)))
+    kernel; at l = 8 the 17 values of m take two products of 16) stay within 1e-4 of the FP64
+    oracle per block, and agree with the FP32 products."""
+    say("power spectrum values, bfloat16 hi + lo in the expansion kernel, l_max", lmax)
+    reference = oracle_to_tensormap(orc.soap_power_spectrum(systems(), hypers(lmax)))
     syst = [ra.System(s.types, s.positions, s.cell) for s in systems()]
     out = {}
     for fast in (False, True):
-        calc = ra.SoapPowerSpectrum(**hypers(6))
+        calc = ra.SoapPowerSpectrum(**hypers(lmax))
         calc.set_precision(32)
         calc.set_fast_form(fast)
         calc.tuning = {"expansion_entries": -8, "gradient_window": -1}
