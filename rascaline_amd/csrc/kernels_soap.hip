@@ -1643,6 +1643,8 @@ __device__ __forceinline__ void bf16_split(float x, short& hi, short& lo) {
 }
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Gradient rows of the complete runs of one window (the contraction phase of the
@@ -3164,8 +3166,8 @@ struct ExpansionMfmaLayout {
 // block of type pairs, 4 m), rows (t2, n2) and columns (t1, n1), so that lane j holds
 // PS[l][n1][4 h .. 4 h + 3] of the pair (t1, t2), a 16-byte piece of its sample.
 // With `bf16`, the products over m are split into bfloat16 hi + lo parts
-// (hi hi + hi lo + lo hi on v_mfma_f32_16x16x16bf16_1k, all of m in one product, FP32
-// accumulation), as the fast form's values kernel forms them.
+// (hi hi + lo hi + hi lo, FP32 accumulation) on gfx950's v_mfma_f32_16x16x32_bf16: two
+// products of K = 32 per 16 values of m, the hi and lo parts of A side by side in K.
 __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int lane, int L, int S, const float* C,
                                                     int csp, const long long* pair_row, bool bf16,
                                                     float* __restrict__ out) {
@@ -3187,26 +3189,31 @@ __device__ __forceinline__ void expansion_ps_values(int wave, int n_waves, int l
         const float* C1 = C + (kv1 ? k1 : 0) * csp + l * l * 8 + (lane & 7);
         float4_t acc = {0.f, 0.f, 0.f, 0.f};
         if (bf16) {
-            // per 16 values of m (two passes at l = 8): lane i + 16 g holds m = 16 kb + 4 g ..
-            // 16 kb + 4 g + 3 of row / column i
+            // v_mfma_f32_16x16x32_bf16, per 16 values of m (two passes at l = 8): lane i + 16 g
+            // holds K values 8 g .. 8 g + 7 of row / column i. K < 16 carries the hi parts of
+            // m = 16 kb + K, K >= 16 the lo parts (A) or again the hi parts (B) of m = 16 kb + K
+            // - 16, so that one product gives hi hi + lo hi; a second, with the lo parts of B
+            // against the hi parts of A on K < 16, gives hi lo.
+            const bool upper = jk >= 2;
             for (int kb = 0; kb < (2 * l + 1 + 15) / 16; kb++) {
-                short4_t ah, al, bh, bl;
+                short8_t a1, b1, a2, b2;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int mp = 16 * kb + 4 * jk + q;
+                for (int q = 0; q < 8; q++) {
+                    const int mp = 16 * kb + 8 * (jk & 1) + q;
                     const bool mv = mp <= 2 * l;
                     const int off = (mv ? mp : 0) * 8;
-                    short h, lo;
-                    bf16_split((mv && kv2) ? C2[off] : 0.f, h, lo);
-                    ah[q] = h;
-                    al[q] = lo;
-                    bf16_split((mv && kv1) ? C1[off] : 0.f, h, lo);
-                    bh[q] = h;
-                    bl[q] = lo;
+                    short ah, al, bh, bl;
+                    bf16_split((mv && kv2) ? C2[off] : 0.f, ah, al);
+                    bf16_split((mv && kv1) ? C1[off] : 0.f, bh, bl);
+                    a1[q] = upper ? al : ah;
+                    b1[q] = bh;
+                    a2[q] = upper ? (short)0 : ah;
+                    b2[q] = upper ? (short)0 : bl;
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a1),
+                                                              __builtin_bit_cast(bf16x8_t, b1), acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a2),
+                                                              __builtin_bit_cast(bf16x8_t, b2), acc, 0, 0, 0);
             }
         } else {
             for (int ks = 0; ks < (2 * l + 1 + 3) / 4; ks++) {
