@@ -2858,24 +2858,30 @@ __global__ void __launch_bounds__(PSV_THREADS) ps_values_mfma_kernel(
             }
             const int bj = bi + rem;
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            // v_mfma_f32_16x16x32_bf16, per 16 values of m: lane i + 16 g holds K values
+            // 8 g .. 8 g + 7; K < 16 carries the hi parts of A, K >= 16 its lo parts, both
+            // against the hi parts of B (hi hi + lo hi), then the hi parts of A against the
+            // lo parts of B on K < 16 (hi lo)
+            const bool upper = lane >= 32;
             for (int kb = 0; kb < kcount; kb += 16) {
-                short4_t ah, al, bh, bl;
+                short8_t a1, b1, a2, b2;
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int mi = kb + kq + e;
+                for (int e = 0; e < 8; e++) {
+                    const int mi = kb + 8 * ((lane >> 4) & 1) + e;
                     const float av = mi < kcount ? Cl[mi * Wp + 16 * bi + i16] : 0.f;
                     const float bv = mi < kcount ? Cl[mi * Wp + 16 * bj + i16] : 0.f;
-                    short h, lo;
-                    bf16_split(av, h, lo);
-                    ah[e] = h;
-                    al[e] = lo;
-                    bf16_split(bv, h, lo);
-                    bh[e] = h;
-                    bl[e] = lo;
+                    short ah, al, bh, bl;
+                    bf16_split(av, ah, al);
+                    bf16_split(bv, bh, bl);
+                    a1[e] = upper ? al : ah;
+                    b1[e] = bh;
+                    a2[e] = upper ? (short)0 : ah;
+                    b2[e] = upper ? (short)0 : bl;
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a1),
+                                                              __builtin_bit_cast(bf16x8_t, b1), acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a2),
+                                                              __builtin_bit_cast(bf16x8_t, b2), acc, 0, 0, 0);
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) G[(size_t)(16 * bi + kq + v) * Wp + 16 * bj + i16] = acc[v];
